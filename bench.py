@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Throughput of the polar-grid PM step on one MI355X.
+
+Workload: 256 x 512 log-polar grid, N = 10M particles of the seeded
+exponential disc (Q = 1.5, rigid flat-rotation halo), harmonics m <= 16,
+three time-step zones (nstep 1, 2, 4; dt = 0.025, 0.05, 0.1), synthetic
+initial conditions (galaxy_amd/configs.py C2).  One step = galaxy's loop
+body: findf (zone combination, azimuthal transform, radial Green's
+contraction, synthesis, polcat) then step (interpolation, halo force,
+leap-frog, zone changes, mass assignment), all on the device; particles are
+re-sorted by cell every --sort-interval steps inside the timed region.
+
+value = particle pushes per second of the mixed form (FP32 particles and
+grids, FP32 contraction): one push = one stpgrp application; with zones,
+zone-k particles are pushed every nstep(k) steps.  The plain FP64 form runs
+the same steps from the same particles; the JSON line carries its rate, the
+ratio, and the m = 2 amplitude of both end states.
+
+roofline: the push kernel, algorithmic bytes = pushes x 32 B (x, y, vx, vy
+read and written in FP32) over its summed HIP-event time on the library's
+stream, against 8 TB/s.  cpu_baseline: the oracle's FP32 restatement on one
+host core over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("particle-steps/s @10M (mixed vs FP64 GPU, m=2 err<1e-4); "
+          "% HBM roofline")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def a2(x, y, lscale, r1=1.0, r2=3.0):
+    """|A_2| = |sum e^{-2i phi}| / N over particles with r1 <= R <= r2."""
+    x = x.astype(np.float64)
+    y = y.astype(np.float64)
+    R = np.hypot(x, y) / lscale
+    sel = (R >= r1) & (R <= r2)
+    ph = np.arctan2(y[sel], x[sel])
+    c = np.exp(-2j * ph).sum() / max(sel.sum(), 1)
+    return complex(c)
+
+
+def run_gpu(w, precision, n, steps, warmup, sort_interval):
+    from galaxy_amd import configs
+    t0 = time.time()
+    g = configs.make_grid(w, precision=precision, n=n)
+    g.set_sort_interval(sort_interval)
+    log("[bench] %s set-up %.1f s (N=%d)" % (precision, time.time() - t0, n))
+    if warmup:
+        g.run(warmup)
+    g.sync()
+    g.set_profile(True)
+    t0 = time.perf_counter()
+    g.run(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    push_ms, launches, pushes = g.get_profile()
+    g.set_profile(False)
+    out = g.unload()
+    res = dict(seconds=t1 - t0, pushes=pushes, push_ms=push_ms,
+               launches=launches, a2=a2(out["x"], out["y"], w.lscale()),
+               noff=g.noff)
+    log("[bench] %s: %d steps in %.3f s, %d pushes, push kernel %.3f ms total"
+        % (precision, steps, t1 - t0, pushes, push_ms))
+    g.close()
+    return res
+
+
+def cpu_baseline(w, n, steps):
+    """The oracle (FP32 restatement, one core) on n particles of workload w."""
+    from galaxy_amd import configs
+    from oracle import oracle as orc
+    kw = configs.grid_kwargs(w, n)
+    m = orc.P2D(kw["nr"], kw["na"], kw["hash_"], kw["softl"], tsoft=1,
+                nsect=1, nzones=kw["nzones"], nstep=kw["nstep"], zbr=kw["zbr"],
+                pmass=kw["pmass"], offanal=True, prec="f", use_fx=True)
+    m.fixforce = configs.fixforce(w, float(m.rgrid))
+    m.load(*configs.particles(w, n, None, float(m.rgrid)))
+    m.masset()
+    pushes = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.findf()
+        m.step()
+        pushes += int(m.moved.sum())
+    dt = time.perf_counter() - t0
+    return dict(value=pushes / dt, unit="particle-steps/s", cores=1,
+                kind="port",
+                sample="oracle FP32, %s geometry, N=%d, %d steps (%d pushes, %.1f s)"
+                       % (w.name, n, steps, pushes, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=None, help="particles (default 10M)")
+    ap.add_argument("--sort-interval", type=int, default=20)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
+    ap.add_argument("--cpu-n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the step's "
+              "multi-GPU form is a design, not measured); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    from galaxy_amd import configs
+    w = configs.C2
+    n = args.n or w.n
+    mixed = run_gpu(w, "fp32", n, args.steps, args.warmup, args.sort_interval)
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_gpu(w, "fp64", n, args.steps, args.warmup, args.sort_interval)
+    value = mixed["pushes"] / mixed["seconds"]
+    ms_per_step = 1e3 * mixed["seconds"] / args.steps
+    bytes_per_push = 32.0
+    achieved = mixed["pushes"] * bytes_per_push / (mixed["push_ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="push_kernel<float>",
+                bytes_per_push=bytes_per_push,
+                avg_launch_us=round(1e3 * mixed["push_ms"] / max(mixed["launches"], 1), 2))
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, args.cpu_n, args.cpu_steps)
+        log("[bench] cpu baseline: %.3g pushes/s (%s)" % (cpu["value"], cpu["sample"]))
+    extra = dict(
+        n_steps_rate=n * args.steps / mixed["seconds"],
+        pushes_per_step=mixed["pushes"] / args.steps,
+        sort_interval=args.sort_interval,
+    )
+    if fp64:
+        v64 = fp64["pushes"] / fp64["seconds"]
+        extra.update(fp64_value=v64, mixed_over_fp64=value / v64,
+                     fp64_ms_per_step=1e3 * fp64["seconds"] / args.steps,
+                     a2_mixed=abs(mixed["a2"]), a2_fp64=abs(fp64["a2"]),
+                     m2_err=abs(mixed["a2"] - fp64["a2"]))
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded exponential disc)",
+        "config": {"workload": w.name, "grid": [w.nr, w.na], "n": n,
+                   "harmonics": "m<=%d" % w.hash, "zones": list(w.nstep),
+                   "dt": [w.dt * s for s in w.nstep], "parallelism": "none"},
+        "roofline": roof, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,44 @@
+"""Builds libgalhip.so in-tree for gfx950 with hipcc.
+
+Flags:
+  -ffp-contract=off       every product and sum rounded separately, as the
+                          reference's FP32 arithmetic (cell indices and CIC
+                          weights must be bit-identical)
+  -fhip-fp32-correctly-rounded-divide-sqrt
+                          IEEE FP32 division and square root
+  -target-feature -packed-fp32-ops
+                          no packed FP32 instructions
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libgalhip.so")
+SOURCES = ["galhip_api.hip", "galhip_push.hip", "galhip_solve.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-I", os.path.join(ROOT, "include"),
+]
+
+
+def build(verbose=False) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    print("built", LIB)

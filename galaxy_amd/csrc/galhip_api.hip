@@ -1,0 +1,895 @@
+// galhip_api.hip -- C ABI of libgalhip (include/galhip.h): context set-up,
+// device storage, the reference's zone/time bookkeeping on the host, and
+// the launch sequence of findf and step.
+//
+// Host-side state mirrors the reference's /zones/ and /point/ commons
+// (SRC/lib15/inc/admin.f:36-71): istep, mzone, lstep, mstep are advanced by
+// the host exactly as masscl/zoncmb advance them (masscl.f:31-98,
+// zoncmb.f:40-125); jrad/krad live on the device because the particle kernel
+// produces them, and the solve kernels read them there.
+#include "galhip_kern.h"
+#include "../../include/galhip.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace gh {
+template <typename R> hipError_t launch_push(const PushArgs<R> &, hipStream_t);
+template <typename R> hipError_t launch_massgn(const PushArgs<R> &, hipStream_t);
+template <typename R> hipError_t launch_gather(const GatherArgs<R> &, hipStream_t);
+template <typename R>
+hipError_t launch_key(const R *, const R *, const uint8_t *, long long, int,
+                      int, int, R, unsigned *, unsigned *, hipStream_t);
+template <typename T>
+hipError_t launch_permute(const T *, T *, const unsigned *, long long,
+                          hipStream_t);
+template <typename R> hipError_t launch_analyse(const AnaArgs<R> &, hipStream_t);
+template <typename R> hipError_t launch_synth(const SynArgs<R> &, hipStream_t);
+hipError_t launch_green(double *, const double *, int, int, int, int, double,
+                        double, int, hipStream_t);
+hipError_t launch_d2f(const double *, float *, long long, hipStream_t);
+hipError_t launch_fill_i64(long long *, long long, long long, hipStream_t);
+hipError_t launch_fill_i32(int *, int, long long, hipStream_t);
+hipError_t launch_masscl(const ZonePtrs &, int, long long, int *, int *, int,
+                         hipStream_t);
+}  // namespace gh
+
+using namespace gh;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return fail(GALHIP_EHIP, "%s: %s (%s:%d)", #expr,                 \
+                        hipGetErrorString(e_), __FILE__, __LINE__);           \
+    } while (0)
+
+struct galhip_ctx {
+    int prec;
+    int nr, na, ng, mmax, nsect, mesh, tsoft, nz, offanal;
+    int nstep[GH_MAXZ];
+    double zbr[GH_MAXZ];
+    double softl, pmass;
+    std::vector<int> lg;
+    double alpha, rgrid, thmax;   // values in the context's precision
+    // host time/zone bookkeeping
+    int istep, mzone;
+    int lstep[2][GH_MAXZ], mstep[GH_MAXZ];
+    double fnew[GH_MAXZ], fold[GH_MAXZ];
+    int use_new[GH_MAXZ], use_old[GH_MAXZ];
+    int masscl_done;              // masscl of this istep already on the device
+    int have_green, have_field;
+    // device storage
+    hipStream_t s;
+    long long n, cap;
+    void *x, *y, *vx, *vy;
+    uint8_t *zone;
+    unsigned *id;
+    void *x2, *y2, *vx2, *vy2;
+    uint8_t *zone2;
+    unsigned *id2;
+    unsigned *key, *key2, *perm, *perm2;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
+    long long *mfx[GH_MAXZ], *moz[GH_MAXZ];
+    int *jrad, *krad, *ranges, *lg_d;
+    long long *counters;
+    void *rho, *coef, *fld2, *pot, *G, *tw;
+    double *Gd, *twd;
+    double *fr_val, *fr_der, fr_dr;
+    int nfr, qbits, ncoef;
+    int sort_interval, steps_since_sort;
+    // profiling
+    int prof;
+    std::vector<hipEvent_t> ev_free, ev_used;
+    double prof_ms;
+    long long prof_launch, prof_push0;
+};
+
+static size_t esize(const galhip_ctx *c) { return c->prec == GALHIP_FP64 ? 8 : 4; }
+
+template <typename T> static int dalloc(T **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return 0;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess)
+        return fail(GALHIP_EHIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return 0;
+}
+
+template <typename T> static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+static void free_particles(galhip_ctx *c) {
+    dfree(c->x); dfree(c->y); dfree(c->vx); dfree(c->vy); dfree(c->zone); dfree(c->id);
+    dfree(c->x2); dfree(c->y2); dfree(c->vx2); dfree(c->vy2); dfree(c->zone2); dfree(c->id2);
+    dfree(c->key); dfree(c->key2); dfree(c->perm); dfree(c->perm2); dfree(c->sort_tmp);
+    c->sort_tmp_bytes = 0;
+    c->n = c->cap = 0;
+}
+
+extern "C" const char *galhip_last_error(void) { return g_err.c_str(); }
+
+// ---- set-up ------------------------------------------------------------------
+
+static int qbits_for(long long n) {
+    int b = 0;
+    while (b < 62 && (1LL << b) < n + 1) b++;
+    if (b < 1) b = 1;
+    return 62 - b;
+}
+
+extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
+    if (!p || !out) return fail(GALHIP_EINVAL, "galhip_p2dset: null argument");
+    *out = nullptr;
+    if (p->precision != GALHIP_FP32 && p->precision != GALHIP_FP64)
+        return fail(GALHIP_EINVAL, "precision must be GALHIP_FP32 or GALHIP_FP64");
+    int na = 2 * (p->na / 2);                       // setgrd.f:58
+    if (p->nr < 3 || na < 4 || na > 2048)
+        return fail(GALHIP_EINVAL, "grid %d x %d out of range", p->nr, p->na);
+    if (p->nzones < 1 || p->nzones > GALHIP_MAXZONES)
+        return fail(GALHIP_EINVAL, "nzones %d out of range", p->nzones);
+    if (p->nsect < 1) return fail(GALHIP_EINVAL, "nsect not set");
+    if (!(p->softl > 0.0)) return fail(GALHIP_EINVAL, "Softening length must be greater than 0");
+    if (p->tsoft != 1 && p->tsoft != 2) return fail(GALHIP_EINVAL, "Unrecognized softening rule");
+    if (p->nskip < 0 || p->nskip > GALHIP_MAXSKIP) return fail(GALHIP_EINVAL, "nskip out of range");
+    galhip_ctx *c = new galhip_ctx();   // value-initialised: scalars zeroed
+    c->prec = p->precision;
+    c->nr = p->nr;
+    c->na = na;
+    c->mmax = na / 2 + 1;
+    c->ng = p->hash + 1 < c->mmax ? p->hash + 1 : c->mmax;   // p2dset.f:40, setgrd.f:89
+    if (c->ng < 1) c->ng = 1;
+    if (c->ng > 64) { delete c; return fail(GALHIP_EINVAL, "at most 64 harmonics supported"); }
+    c->nsect = p->nsect;
+    c->mesh = c->nr * na;
+    c->tsoft = p->tsoft;
+    c->nz = p->nzones;
+    c->offanal = p->offanal != 0;
+    c->softl = p->softl;
+    c->pmass = p->pmass;
+    for (int i = 0; i < c->nz; i++) {
+        c->nstep[i] = p->nstep[i] > 0 ? p->nstep[i] : 1;
+        c->zbr[i] = p->zbr[i];
+    }
+    // filter (filset.f:118-141): lg[m] != 0 skips harmonic m
+    c->lg.assign(c->mmax, 1);
+    c->lg[0] = 0;
+    for (int m = 1; m < c->ng; m++) c->lg[m] = (c->nsect == 1 || m % c->nsect == 0) ? 0 : 1;
+    for (int i = 0; i < p->nskip; i++)
+        if (p->skip[i] >= 0 && p->skip[i] < c->mmax) c->lg[p->skip[i]] = 1;
+    // alpha, rgrid = grofu(nr - 1), thmax (setgrd.f:88,127-128,172-176)
+    const double two_pi = 2.0 * 3.141592653589793238462643;
+    if (c->prec == GALHIP_FP32) {
+        float a = (float)(two_pi / (double)na);
+        c->alpha = a;
+        c->rgrid = expf(a * (float)(c->nr - 1)) - 1.0f;
+        c->thmax = (float)na / (float)c->nsect;
+    } else {
+        c->alpha = two_pi / (double)na;
+        c->rgrid = exp(c->alpha * (double)(c->nr - 1)) - 1.0;
+        c->thmax = (double)na / (double)c->nsect;
+    }
+    c->ncoef = 2 * c->ng;
+    c->istep = 0;
+    c->mzone = c->nz;
+    int rc = 0;
+    // a blocking stream: the synchronous hipMemcpy calls of the load/unload
+    // and parity entry points (null stream) are ordered with its kernels
+    hipError_t e = hipStreamCreate(&c->s);
+    if (e != hipSuccess) { delete c; return fail(GALHIP_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    size_t es = esize(c);
+    for (int z = 0; z < c->nz; z++) {
+        rc |= dalloc(&c->mfx[z], sizeof(long long) * c->mesh);
+        rc |= dalloc(&c->moz[z], sizeof(long long) * c->mesh);
+    }
+    rc |= dalloc(&c->jrad, sizeof(int) * GH_MAXZ);
+    rc |= dalloc(&c->krad, sizeof(int) * GH_MAXZ);
+    rc |= dalloc(&c->ranges, sizeof(int) * 4);
+    rc |= dalloc(&c->lg_d, sizeof(int) * c->mmax);
+    rc |= dalloc(&c->counters, sizeof(long long) * 4);
+    rc |= dalloc(&c->rho, es * c->mesh);
+    rc |= dalloc(&c->coef, es * c->ncoef * c->nr);
+    rc |= dalloc(&c->fld2, 2 * es * c->mesh);
+    rc |= dalloc(&c->pot, es * c->mesh);
+    rc |= dalloc(&c->tw, 2 * es * na);
+    rc |= dalloc(&c->twd, 2 * sizeof(double) * na);
+    rc |= dalloc(&c->Gd, sizeof(double) * 3 * (size_t)c->ng * c->nr * c->nr);
+    if (c->prec == GALHIP_FP32)
+        rc |= dalloc(&c->G, sizeof(float) * 3 * (size_t)c->ng * c->nr * c->nr);
+    else
+        c->G = c->Gd;
+    if (rc) { galhip_free(c); return rc; }
+    // twiddles: cos, sin of 2 pi k / na, k reduced exactly
+    std::vector<double> twd(2 * na);
+    for (int k = 0; k < na; k++) {
+        twd[2 * k] = cos(two_pi * (double)k / (double)na);
+        twd[2 * k + 1] = sin(two_pi * (double)k / (double)na);
+    }
+    HIPCHK(hipMemcpy(c->twd, twd.data(), sizeof(double) * 2 * na, hipMemcpyHostToDevice));
+    if (c->prec == GALHIP_FP32) {
+        std::vector<float> twf(2 * na);
+        for (int i = 0; i < 2 * na; i++) twf[i] = (float)twd[i];
+        HIPCHK(hipMemcpy(c->tw, twf.data(), sizeof(float) * 2 * na, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(c->tw, twd.data(), sizeof(double) * 2 * na, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(c->lg_d, c->lg.data(), sizeof(int) * c->mmax, hipMemcpyHostToDevice));
+    std::vector<int> jr(GH_MAXZ, c->nr), kr(GH_MAXZ, 0);
+    HIPCHK(hipMemcpy(c->jrad, jr.data(), sizeof(int) * GH_MAXZ, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->krad, kr.data(), sizeof(int) * GH_MAXZ, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->counters, 0, sizeof(long long) * 4));
+    HIPCHK(hipMemset(c->fld2, 0, 2 * es * c->mesh));
+    HIPCHK(hipMemset(c->pot, 0, es * c->mesh));
+    HIPCHK(hipMemset(c->rho, 0, es * c->mesh));
+    for (int z = 0; z < c->nz; z++) {
+        HIPCHK(hipMemset(c->mfx[z], 0, sizeof(long long) * c->mesh));
+        HIPCHK(hipMemset(c->moz[z], 0, sizeof(long long) * c->mesh));
+    }
+    c->sort_interval = 0;
+    c->qbits = qbits_for(1);
+    *out = c;
+    return 0;
+}
+
+extern "C" int galhip_free(galhip_ctx *c) {
+    if (!c) return 0;
+    (void)hipStreamSynchronize(c->s);
+    free_particles(c);
+    for (int z = 0; z < GH_MAXZ; z++) { dfree(c->mfx[z]); dfree(c->moz[z]); }
+    dfree(c->jrad); dfree(c->krad); dfree(c->ranges); dfree(c->lg_d); dfree(c->counters);
+    dfree(c->rho); dfree(c->coef); dfree(c->fld2); dfree(c->pot); dfree(c->tw); dfree(c->twd);
+    if (c->G != (void *)c->Gd) dfree(c->G);
+    dfree(c->Gd);
+    dfree(c->fr_val); dfree(c->fr_der);
+    for (auto e : c->ev_free) (void)hipEventDestroy(e);
+    for (auto e : c->ev_used) (void)hipEventDestroy(e);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+    return 0;
+}
+
+extern "C" int galhip_geometry(const galhip_ctx *c, double *out) {
+    if (!c || !out) return fail(GALHIP_EINVAL, "null argument");
+    out[0] = c->alpha;
+    out[1] = c->rgrid;
+    out[2] = c->thmax;
+    out[3] = c->ng;
+    out[4] = c->mmax;
+    out[5] = c->mesh;
+    return 0;
+}
+
+// ---- Green's tables ------------------------------------------------------------
+
+static int green_to_work(galhip_ctx *c) {
+    if (c->prec == GALHIP_FP32) {
+        long long n = 3LL * c->ng * c->nr * c->nr;
+        HIPCHK(launch_d2f(c->Gd, (float *)c->G, n, c->s));
+    }
+    c->have_green = 1;
+    return 0;
+}
+
+extern "C" int galhip_greenm(galhip_ctx *c) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    double alpha_d = 2.0 * 3.141592653589793238462643 / (double)c->na;
+    HIPCHK(launch_green(c->Gd, c->twd, c->nr, c->na, c->ng, c->nsect, alpha_d,
+                        c->softl, c->tsoft, c->s));
+    int rc = green_to_work(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int galhip_get_green(const galhip_ctx *c, double *G) {
+    if (!c || !G) return fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(G, c->Gd, sizeof(double) * 3 * (size_t)c->ng * c->nr * c->nr,
+                     hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int galhip_set_green(galhip_ctx *c, const double *G) {
+    if (!c || !G) return fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipMemcpy(c->Gd, G, sizeof(double) * 3 * (size_t)c->ng * c->nr * c->nr,
+                     hipMemcpyHostToDevice));
+    int rc = green_to_work(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int galhip_set_fixforce(galhip_ctx *c, int32_t n, double dr,
+                                   const double *val, const double *der) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    dfree(c->fr_val);
+    dfree(c->fr_der);
+    c->nfr = 0;
+    if (n <= 0) return 0;
+    if (n < 2 || !(dr > 0.0) || !val || !der)
+        return fail(GALHIP_EINVAL, "fixed force table needs n >= 2, dr > 0");
+    int rc = dalloc(&c->fr_val, sizeof(double) * n) | dalloc(&c->fr_der, sizeof(double) * n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(c->fr_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fr_der, der, sizeof(double) * n, hipMemcpyHostToDevice));
+    c->nfr = n;
+    c->fr_dr = dr;
+    return 0;
+}
+
+// ---- particles -------------------------------------------------------------------
+
+static int ensure_particles(galhip_ctx *c, long long n) {
+    if (n <= c->cap) { c->n = n; return 0; }
+    free_particles(c);
+    size_t es = esize(c);
+    int rc = 0;
+    rc |= dalloc(&c->x, es * n); rc |= dalloc(&c->y, es * n);
+    rc |= dalloc(&c->vx, es * n); rc |= dalloc(&c->vy, es * n);
+    rc |= dalloc(&c->zone, n); rc |= dalloc(&c->id, sizeof(unsigned) * n);
+    rc |= dalloc(&c->x2, es * n); rc |= dalloc(&c->y2, es * n);
+    rc |= dalloc(&c->vx2, es * n); rc |= dalloc(&c->vy2, es * n);
+    rc |= dalloc(&c->zone2, n); rc |= dalloc(&c->id2, sizeof(unsigned) * n);
+    rc |= dalloc(&c->key, sizeof(unsigned) * n); rc |= dalloc(&c->key2, sizeof(unsigned) * n);
+    rc |= dalloc(&c->perm, sizeof(unsigned) * n); rc |= dalloc(&c->perm2, sizeof(unsigned) * n);
+    if (rc) { free_particles(c); return rc; }
+    size_t tmp = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+        nullptr, tmp, c->key, c->key2, c->perm, c->perm2, (int)n, 0, 32, c->s);
+    if (e != hipSuccess) return fail(GALHIP_EHIP, "radix sort sizing: %s", hipGetErrorString(e));
+    rc = dalloc(&c->sort_tmp, tmp);
+    if (rc) { free_particles(c); return rc; }
+    c->sort_tmp_bytes = tmp;
+    c->cap = c->n = n;
+    return 0;
+}
+
+template <typename R> static int sort_impl(galhip_ctx *c) {
+    long long n = c->n;
+    if (n <= 1) return 0;
+    HIPCHK(launch_key<R>((const R *)c->x, (const R *)c->y, c->zone, n, c->nr,
+                         c->na, c->nz, (R)c->alpha, c->key, c->perm, c->s));
+    int end_bit = 1;
+    while ((1LL << end_bit) <= (long long)c->mesh) end_bit++;
+    size_t tmp = c->sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->key, c->key2,
+                                              c->perm, c->perm2, (int)n, 0,
+                                              end_bit, c->s));
+    HIPCHK(launch_permute<R>((const R *)c->x, (R *)c->x2, c->perm2, n, c->s));
+    HIPCHK(launch_permute<R>((const R *)c->y, (R *)c->y2, c->perm2, n, c->s));
+    HIPCHK(launch_permute<R>((const R *)c->vx, (R *)c->vx2, c->perm2, n, c->s));
+    HIPCHK(launch_permute<R>((const R *)c->vy, (R *)c->vy2, c->perm2, n, c->s));
+    HIPCHK(launch_permute<uint8_t>(c->zone, c->zone2, c->perm2, n, c->s));
+    HIPCHK(launch_permute<unsigned>(c->id, c->id2, c->perm2, n, c->s));
+    std::swap(c->x, c->x2);
+    std::swap(c->y, c->y2);
+    std::swap(c->vx, c->vx2);
+    std::swap(c->vy, c->vy2);
+    std::swap(c->zone, c->zone2);
+    std::swap(c->id, c->id2);
+    c->steps_since_sort = 0;
+    return 0;
+}
+
+extern "C" int galhip_sort(galhip_ctx *c) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    return c->prec == GALHIP_FP32 ? sort_impl<float>(c) : sort_impl<double>(c);
+}
+
+extern "C" int galhip_set_sort_interval(galhip_ctx *c, int32_t k) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    c->sort_interval = k < 0 ? 0 : k;
+    return 0;
+}
+
+extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
+                             const void *y, const void *vx, const void *vy,
+                             const int32_t *iz) {
+    if (!c || n < 0 || (n > 0 && (!x || !y || !vx || !vy || !iz)))
+        return fail(GALHIP_EINVAL, "galhip_restrt: bad arguments");
+    if (n >= (1LL << 31)) return fail(GALHIP_EINVAL, "at most 2^31-1 particles");
+    HIPCHK(hipStreamSynchronize(c->s));
+    int rc = ensure_particles(c, n);
+    if (rc) return rc;
+    size_t es = esize(c);
+    std::vector<uint8_t> z(n);
+    std::vector<unsigned> ids(n);
+    for (long long i = 0; i < n; i++) {
+        int v = iz[i];
+        if (v < 1 || v > c->nz + 1)
+            return fail(GALHIP_EINVAL, "particle %lld has zone %d outside 1..%d", i, v, c->nz + 1);
+        z[i] = (uint8_t)v;
+        ids[i] = (unsigned)i;
+    }
+    HIPCHK(hipMemcpy(c->x, x, es * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->y, y, es * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->vx, vx, es * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->vy, vy, es * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->zone, z.data(), n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+    c->qbits = qbits_for(n);
+    rc = galhip_sort(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+template <typename R>
+static void scatter_back(const std::vector<R> &src, const std::vector<unsigned> &ids,
+                         void *dst) {
+    R *d = (R *)dst;
+    for (size_t i = 0; i < ids.size(); i++) d[ids[i]] = src[i];
+}
+
+extern "C" int galhip_unload(galhip_ctx *c, void *x, void *y, void *vx,
+                             void *vy, int32_t *iz) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    long long n = c->n;
+    std::vector<unsigned> ids(n);
+    HIPCHK(hipMemcpy(ids.data(), c->id, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    void *src[4] = {c->x, c->y, c->vx, c->vy};
+    void *dst[4] = {x, y, vx, vy};
+    for (int k = 0; k < 4; k++) {
+        if (!dst[k]) continue;
+        if (c->prec == GALHIP_FP32) {
+            std::vector<float> h(n);
+            HIPCHK(hipMemcpy(h.data(), src[k], 4 * n, hipMemcpyDeviceToHost));
+            scatter_back(h, ids, dst[k]);
+        } else {
+            std::vector<double> h(n);
+            HIPCHK(hipMemcpy(h.data(), src[k], 8 * n, hipMemcpyDeviceToHost));
+            scatter_back(h, ids, dst[k]);
+        }
+    }
+    if (iz) {
+        std::vector<uint8_t> z(n);
+        HIPCHK(hipMemcpy(z.data(), c->zone, n, hipMemcpyDeviceToHost));
+        for (long long i = 0; i < n; i++) iz[ids[i]] = z[i];
+    }
+    return 0;
+}
+
+// ---- zone bookkeeping (host) -----------------------------------------------------
+
+static void set_mzone(galhip_ctx *c) {
+    int m = 1;
+    if (c->nz > 1)
+        for (int i = 1; i <= c->nz; i++)
+            if (c->istep % c->nstep[i - 1] == 0) m = i;
+    c->mzone = m;
+}
+
+// masscl's time stamps (masscl.f:31-40,92-96)
+static void masscl_host(galhip_ctx *c, bool normal) {
+    for (int z = 1; z <= c->mzone; z++) {
+        int ns = c->nstep[z - 1];
+        if (normal) c->mstep[z - 1] = c->istep + ns;
+        else c->mstep[z - 1] = c->istep <= 0 ? 0 : ((c->istep - 1) / ns + 1) * ns;
+    }
+    for (int z = 2; z <= c->mzone; z++) {
+        c->lstep[0][z - 1] = c->lstep[1][z - 1];
+        c->lstep[1][z - 1] = c->mstep[z - 1];
+    }
+}
+
+// zoncmb's fractions for the current istep (zoncmb.f:40-125)
+static int zoncmb_host(galhip_ctx *c) {
+    set_mzone(c);
+    for (int z = 2; z <= c->nz; z++) {
+        int ns = c->nstep[z - 1];
+        int i = c->lstep[1][z - 1] - c->istep;
+        if (i < 0 || i > ns)
+            return fail(GALHIP_ERANGE, "ZONCMB: New mass array at a useless time (istep %d, lstep %d)",
+                        c->istep, c->lstep[1][z - 1]);
+        c->use_new[z - 1] = i != ns;
+        c->fnew[z - 1] = 1.0 - (double)i / (double)ns;
+        c->use_old[z - 1] = 0;
+        c->fold[z - 1] = 0.0;
+        if (i != 0) {
+            int j = c->istep - c->lstep[0][z - 1];
+            if (c->lstep[0][z - 1] < 0 || j < 0 || j > ns)
+                return fail(GALHIP_ERANGE, "ZONCMB: Old mass array at a useless time (istep %d, lstep %d)",
+                            c->istep, c->lstep[0][z - 1]);
+            c->use_old[z - 1] = 1;
+            c->fold[z - 1] = 1.0 - (double)j / (double)ns;
+        }
+    }
+    return 0;
+}
+
+extern "C" int galhip_set_step(galhip_ctx *c, int32_t istep, const int32_t *lstep) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    c->istep = istep;
+    if (lstep)
+        for (int z = 0; z < c->nz; z++) {
+            c->lstep[0][z] = lstep[z];
+            c->lstep[1][z] = lstep[c->nz + z];
+        }
+    set_mzone(c);
+    c->masscl_done = 0;
+    return 0;
+}
+
+extern "C" int galhip_get_step(const galhip_ctx *c, int32_t *istep, int32_t *mzone,
+                               int32_t *lstep) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (istep) *istep = c->istep;
+    if (mzone) *mzone = c->mzone;
+    if (lstep)
+        for (int z = 0; z < c->nz; z++) {
+            lstep[z] = c->lstep[0][z];
+            lstep[c->nz + z] = c->lstep[1][z];
+        }
+    return 0;
+}
+
+// ---- kernel argument assembly -----------------------------------------------------
+
+template <typename R> static PushArgs<R> push_args(galhip_ctx *c) {
+    PushArgs<R> a;
+    memset(&a, 0, sizeof a);
+    a.x = (R *)c->x; a.y = (R *)c->y; a.vx = (R *)c->vx; a.vy = (R *)c->vy;
+    a.zone = c->zone;
+    a.n = c->n;
+    a.fld = (const typename Vec2<R>::T *)c->fld2;
+    for (int z = 0; z < c->nz; z++) a.mfx[z] = c->mfx[z];
+    a.jrad = c->jrad;
+    a.krad = c->krad;
+    a.counters = c->counters;
+    a.fr_val = c->fr_val;
+    a.fr_der = c->fr_der;
+    a.fr_dr = c->fr_dr;
+    a.nfr = c->nfr;
+    a.fixforce = c->nfr > 0;
+    a.nr = c->nr; a.na = c->na; a.mesh = c->mesh; a.nz = c->nz;
+    a.mzone = c->mzone; a.offanal = c->offanal; a.nsect = c->nsect; a.qbits = c->qbits;
+    a.alpha = (R)c->alpha; a.rgrid = (R)c->rgrid; a.thmax = (R)c->thmax;
+    for (int z = 0; z < c->nz; z++) {
+        a.zbr[z] = (R)c->zbr[z];
+        a.nstep[z] = c->nstep[z];
+    }
+    return a;
+}
+
+template <typename R>
+static AnaArgs<R> ana_args(galhip_ctx *c, int lpot, int explicit_zones,
+                           int jzone, int kzone, int masscl_z) {
+    AnaArgs<R> a;
+    memset(&a, 0, sizeof a);
+    for (int z = 0; z < c->nz; z++) {
+        a.mfx[z] = c->mfx[z];
+        a.moz[z] = c->moz[z];
+        a.mfx_w[z] = c->mfx[z];
+        a.moz_w[z] = c->moz[z];
+        a.fnew[z] = c->fnew[z];
+        a.fold[z] = c->fold[z];
+        a.use_new[z] = c->use_new[z];
+        a.use_old[z] = c->use_old[z];
+    }
+    a.jrad = c->jrad; a.krad = c->krad; a.ranges = c->ranges;
+    a.rho = (R *)c->rho; a.coef = (R *)c->coef; a.tw = (const R *)c->tw; a.lg = c->lg_d;
+    a.q = ldexp(1.0, -c->qbits);
+    a.pmass = c->pmass;
+    a.nr = c->nr; a.na = c->na; a.ng = c->ng; a.ncoef = c->ncoef; a.nz = c->nz;
+    a.mzone = c->mzone; a.lpot = lpot; a.jzone = jzone; a.kzone = kzone;
+    a.explicit_zones = explicit_zones;
+    a.masscl_z = masscl_z;
+    return a;
+}
+
+template <typename R> static SynArgs<R> syn_args(galhip_ctx *c, int lpot) {
+    SynArgs<R> a;
+    a.coef = (const R *)c->coef;
+    a.G = (const R *)c->G;
+    a.tw = (const R *)c->tw;
+    a.lg = c->lg_d;
+    a.ranges = c->ranges;
+    a.fld = (typename Vec2<R>::T *)c->fld2;
+    a.pot = (R *)c->pot;
+    a.nr = c->nr; a.na = c->na; a.ng = c->ng; a.ncoef = c->ncoef; a.mmax = c->mmax;
+    a.lpot = lpot;
+    return a;
+}
+
+// ---- masset / findf / step -----------------------------------------------------------
+
+template <typename R> static int masset_impl(galhip_ctx *c) {
+    c->mzone = c->nz;
+    masscl_host(c, false);
+    ZonePtrs zp;
+    memset(&zp, 0, sizeof zp);
+    for (int z = 0; z < c->nz; z++) { zp.mfx[z] = c->mfx[z]; zp.moz[z] = c->moz[z]; }
+    HIPCHK(launch_masscl(zp, c->nz, c->mesh, c->jrad, c->krad, c->nr, c->s));
+    PushArgs<R> a = push_args<R>(c);
+    HIPCHK(launch_massgn<R>(a, c->s));
+    int rc = zoncmb_host(c);
+    if (rc) return rc;
+    c->masscl_done = 0;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int galhip_masset(galhip_ctx *c) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    return c->prec == GALHIP_FP32 ? masset_impl<float>(c) : masset_impl<double>(c);
+}
+
+template <typename R>
+static int solve_impl(galhip_ctx *c, int lpot, int explicit_zones, int jzone,
+                      int kzone, int masscl_z) {
+    if (!c->have_green) return fail(GALHIP_EINVAL, "P2CONV: Green function not set (call galhip_greenm)");
+    AnaArgs<R> a = ana_args<R>(c, lpot, explicit_zones, jzone, kzone, masscl_z);
+    HIPCHK(launch_analyse<R>(a, c->s));
+    SynArgs<R> b = syn_args<R>(c, lpot);
+    HIPCHK(launch_synth<R>(b, c->s));
+    c->have_field = 1;
+    return 0;
+}
+
+extern "C" int galhip_findf(galhip_ctx *c, int32_t lpot) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (c->mzone < 1 || c->mzone > c->nz) return fail(GALHIP_ERANGE, "FINDF: invalid mzone");
+    return c->prec == GALHIP_FP32 ? solve_impl<float>(c, lpot != 0, 0, 0, 0, 0)
+                                  : solve_impl<double>(c, lpot != 0, 0, 0, 0, 0);
+}
+
+extern "C" int galhip_p2fndf(galhip_ctx *c, int32_t jzone, int32_t kzone, int32_t lpot) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    return c->prec == GALHIP_FP32 ? solve_impl<float>(c, lpot != 0, 1, jzone, kzone, 0)
+                                  : solve_impl<double>(c, lpot != 0, 1, jzone, kzone, 0);
+}
+
+static hipEvent_t take_event(galhip_ctx *c) {
+    hipEvent_t e;
+    if (!c->ev_free.empty()) {
+        e = c->ev_free.back();
+        c->ev_free.pop_back();
+    } else {
+        (void)hipEventCreate(&e);
+    }
+    c->ev_used.push_back(e);
+    return e;
+}
+
+template <typename R> static int step_impl(galhip_ctx *c) {
+    set_mzone(c);
+    masscl_host(c, true);
+    if (!c->masscl_done) {
+        ZonePtrs zp;
+        memset(&zp, 0, sizeof zp);
+        for (int z = 0; z < c->nz; z++) { zp.mfx[z] = c->mfx[z]; zp.moz[z] = c->moz[z]; }
+        HIPCHK(launch_masscl(zp, c->mzone, c->mesh, c->jrad, c->krad, c->nr, c->s));
+    }
+    c->masscl_done = 0;
+    PushArgs<R> a = push_args<R>(c);
+    if (c->prof) HIPCHK(hipEventRecord(take_event(c), c->s));
+    HIPCHK(launch_push<R>(a, c->s));
+    if (c->prof) {
+        HIPCHK(hipEventRecord(take_event(c), c->s));
+        c->prof_launch++;
+    }
+    c->istep += 1;
+    return zoncmb_host(c);
+}
+
+extern "C" int galhip_step(galhip_ctx *c) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c->have_field) return fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    return c->prec == GALHIP_FP32 ? step_impl<float>(c) : step_impl<double>(c);
+}
+
+extern "C" int galhip_run(galhip_ctx *c, int32_t nsteps) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    for (int k = 0; k < nsteps; k++) {
+        set_mzone(c);
+        // findf with the masscl of this step fused into the analysis kernel
+        int rc = c->prec == GALHIP_FP32 ? solve_impl<float>(c, 0, 0, 0, 0, c->mzone)
+                                        : solve_impl<double>(c, 0, 0, 0, 0, c->mzone);
+        if (rc) return rc;
+        // the fused masscl also resets jrad/krad of the moving zones
+        HIPCHK(launch_fill_i32(c->jrad, c->nr, c->mzone, c->s));
+        HIPCHK(launch_fill_i32(c->krad, 0, c->mzone, c->s));
+        c->masscl_done = 1;
+        rc = galhip_step(c);
+        if (rc) return rc;
+        if (c->sort_interval > 0 && ++c->steps_since_sort >= c->sort_interval) {
+            rc = galhip_sort(c);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+// ---- group-level parity entry points ------------------------------------------------
+
+template <typename R>
+static int gather_impl(galhip_ctx *c, int64_t n, const void *x, const void *y,
+                       int fold, int32_t *cell, void *wt, void *acc) {
+    if (n <= 0) return 0;
+    size_t es = sizeof(R);
+    R *dx, *dy, *dwt = nullptr, *dacc = nullptr;
+    int *dcell = nullptr;
+    int rc = dalloc(&dx, es * n) | dalloc(&dy, es * n);
+    if (cell) rc |= dalloc(&dcell, sizeof(int) * n);
+    if (wt) rc |= dalloc(&dwt, 4 * es * n);
+    if (acc) rc |= dalloc(&dacc, 2 * es * n);
+    if (!rc) {
+        (void)hipMemcpy(dx, x, es * n, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dy, y, es * n, hipMemcpyHostToDevice);
+        GatherArgs<R> a;
+        a.x = dx; a.y = dy; a.n = n;
+        a.fld = (const typename Vec2<R>::T *)c->fld2;
+        a.acc = dacc; a.cell = dcell; a.wt = dwt;
+        a.nr = c->nr; a.na = c->na; a.fold = fold && c->nsect > 1;
+        a.alpha = (R)c->alpha; a.thmax = (R)c->thmax;
+        hipError_t e = launch_gather<R>(a, c->s);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+        if (e != hipSuccess) rc = fail(GALHIP_EHIP, "gather kernel: %s", hipGetErrorString(e));
+        if (!rc && cell) (void)hipMemcpy(cell, dcell, sizeof(int) * n, hipMemcpyDeviceToHost);
+        if (!rc && wt) (void)hipMemcpy(wt, dwt, 4 * es * n, hipMemcpyDeviceToHost);
+        if (!rc && acc) (void)hipMemcpy(acc, dacc, 2 * es * n, hipMemcpyDeviceToHost);
+    }
+    dfree(dx); dfree(dy); dfree(dcell); dfree(dwt); dfree(dacc);
+    return rc;
+}
+
+extern "C" int galhip_weight(galhip_ctx *c, int64_t n, const void *x, const void *y,
+                             int32_t old, int32_t *cell, void *wt) {
+    if (!c || n < 0) return fail(GALHIP_EINVAL, "bad arguments");
+    HIPCHK(hipStreamSynchronize(c->s));
+    return c->prec == GALHIP_FP32 ? gather_impl<float>(c, n, x, y, !old, cell, wt, nullptr)
+                                  : gather_impl<double>(c, n, x, y, !old, cell, wt, nullptr);
+}
+
+extern "C" int galhip_grdacc(galhip_ctx *c, int64_t n, const void *x, const void *y,
+                             void *acc) {
+    if (!c || n < 0 || !acc) return fail(GALHIP_EINVAL, "bad arguments");
+    HIPCHK(hipStreamSynchronize(c->s));
+    return c->prec == GALHIP_FP32 ? gather_impl<float>(c, n, x, y, 0, nullptr, nullptr, acc)
+                                  : gather_impl<double>(c, n, x, y, 0, nullptr, nullptr, acc);
+}
+
+extern "C" int galhip_get_mass(const galhip_ctx *c, int64_t *mfx, int64_t *mozfx,
+                               int32_t *jrad, int32_t *krad, int32_t *qbits) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int z = 0; z < c->nz; z++) {
+        if (mfx) HIPCHK(hipMemcpy(mfx + (size_t)z * c->mesh, c->mfx[z], 8 * c->mesh, hipMemcpyDeviceToHost));
+        if (mozfx) HIPCHK(hipMemcpy(mozfx + (size_t)z * c->mesh, c->moz[z], 8 * c->mesh, hipMemcpyDeviceToHost));
+    }
+    if (jrad) HIPCHK(hipMemcpy(jrad, c->jrad, sizeof(int) * c->nz, hipMemcpyDeviceToHost));
+    if (krad) HIPCHK(hipMemcpy(krad, c->krad, sizeof(int) * c->nz, hipMemcpyDeviceToHost));
+    if (qbits) *qbits = c->qbits;
+    return 0;
+}
+
+extern "C" int galhip_set_mass(galhip_ctx *c, const int64_t *mfx,
+                               const int64_t *mozfx, const int32_t *jrad,
+                               const int32_t *krad) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int z = 0; z < c->nz; z++) {
+        if (mfx) HIPCHK(hipMemcpy(c->mfx[z], mfx + (size_t)z * c->mesh, 8 * c->mesh, hipMemcpyHostToDevice));
+        if (mozfx) HIPCHK(hipMemcpy(c->moz[z], mozfx + (size_t)z * c->mesh, 8 * c->mesh, hipMemcpyHostToDevice));
+    }
+    if (jrad) HIPCHK(hipMemcpy(c->jrad, jrad, sizeof(int) * c->nz, hipMemcpyHostToDevice));
+    if (krad) HIPCHK(hipMemcpy(c->krad, krad, sizeof(int) * c->nz, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int galhip_get_rho(const galhip_ctx *c, void *rho) {
+    if (!c || !rho) return fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(rho, c->rho, esize(c) * c->mesh, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int galhip_get_field(const galhip_ctx *c, void *fld) {
+    if (!c || !fld) return fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    size_t es = esize(c), m = c->mesh;
+    std::vector<unsigned char> tmp(2 * es * m);
+    HIPCHK(hipMemcpy(tmp.data(), c->fld2, 2 * es * m, hipMemcpyDeviceToHost));
+    unsigned char *o = (unsigned char *)fld;
+    for (size_t i = 0; i < m; i++) {
+        memcpy(o + i * es, tmp.data() + 2 * i * es, es);
+        memcpy(o + (m + i) * es, tmp.data() + (2 * i + 1) * es, es);
+    }
+    HIPCHK(hipMemcpy(o + 2 * m * es, c->pot, es * m, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int galhip_set_field(galhip_ctx *c, const void *fld) {
+    if (!c || !fld) return fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    size_t es = esize(c), m = c->mesh;
+    std::vector<unsigned char> tmp(2 * es * m);
+    const unsigned char *in = (const unsigned char *)fld;
+    for (size_t i = 0; i < m; i++) {
+        memcpy(tmp.data() + 2 * i * es, in + i * es, es);
+        memcpy(tmp.data() + (2 * i + 1) * es, in + (m + i) * es, es);
+    }
+    HIPCHK(hipMemcpy(c->fld2, tmp.data(), 2 * es * m, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->pot, in + 2 * m * es, es * m, hipMemcpyHostToDevice));
+    c->have_field = 1;
+    return 0;
+}
+
+extern "C" int galhip_get_ranges(const galhip_ctx *c, int32_t *r4) {
+    if (!c || !r4) return fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(r4, c->ranges, sizeof(int) * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int galhip_get_noff(const galhip_ctx *c, int64_t *noff) {
+    if (!c || !noff) return fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    long long v[4];
+    HIPCHK(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
+    *noff = v[0];
+    return 0;
+}
+
+// ---- profiling ----------------------------------------------------------------------
+
+extern "C" int galhip_set_profile(galhip_ctx *c, int32_t on) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (auto e : c->ev_used) c->ev_free.push_back(e);
+    c->ev_used.clear();
+    c->prof = on != 0;
+    c->prof_ms = 0.0;
+    c->prof_launch = 0;
+    long long v[4];
+    HIPCHK(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
+    c->prof_push0 = v[1];
+    return 0;
+}
+
+extern "C" int galhip_get_profile(galhip_ctx *c, double *push_ms, int64_t *launches,
+                                  int64_t *pushes) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    double ms = 0.0;
+    for (size_t i = 0; i + 1 < c->ev_used.size(); i += 2) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, c->ev_used[i], c->ev_used[i + 1]));
+        ms += t;
+    }
+    long long v[4];
+    HIPCHK(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
+    if (push_ms) *push_ms = ms;
+    if (launches) *launches = c->prof_launch;
+    if (pushes) *pushes = v[1] - c->prof_push0;
+    return 0;
+}
+
+extern "C" int galhip_sync(galhip_ctx *c) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}

@@ -1,0 +1,386 @@
+// galhip_solve.hip -- field solve of the p2d polar grid.
+//
+// analyse_kernel (one block per ring): zoncmb + scaled (SRC/lib15/zoncmb.f:
+//   40-125, scaled.f:91) from the fixed-point zone sums, then the forward
+//   azimuthal transform of p2manl (p2manl.f:35-43) restricted to the active
+//   harmonics m < ng, as a direct real DFT against a twiddle table held in
+//   L2: for the harmonic counts used here (ng <= 33 of na/2+1) this costs
+//   about as much as a radix FFT and keeps exact FFTPACK conventions
+//   (half-complex a_m = sum x cos, b_m = -sum x sin).  Optionally performs
+//   the masscl of the next push (masscl.f:67-146) on its ring.
+// synth_kernel (one block per field ring): p2conv's radial contraction with
+//   the per-harmonic Green's matrices (p2conv.f:95-186), p2fsyn's inverse
+//   transform and 1/na (p2fsyn.f:40-52) and polcat (polcat.f:26-40), writing
+//   interleaved (F_x, F_y) and the potential.
+// green_kernel: p2grnm's table construction in FP64 (p2grnm.f:98-203):
+//   kernel samples (radfor/azifor/grnfun), DCT-I / DST-I over the half
+//   circle, harmonic filter, p2gwrt's [m][field ring][source ring] layout.
+#include "galhip_kern.h"
+
+namespace gh {
+
+constexpr int MAX_NA = 2048;
+constexpr int MAX_NG = 64;
+
+template <typename R> __device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Radial ranges of p2fndf (p2fndf.f:26-55, findf.f:42-54).  Returns false
+// if no mass is on the grid.  kr receives krad after p2fndf's cumulative
+// update.
+__device__ bool solve_ranges(const int *jrad, const int *krad, int nz, int nr,
+                             int mzone, int lpot, int explicit_zones,
+                             int jzone_in, int kzone_in, int kr[GH_MAXZ],
+                             int out[4]) {
+    int jrs = nr, krs = 0;
+    for (int ipm = 1; ipm <= nz; ipm++) {
+        jrs = min(jrs, jrad[ipm - 1]);
+        krs = max(krs, krad[ipm - 1]);
+        kr[ipm - 1] = krad[ipm - 1];
+        if (ipm > 1) kr[ipm - 1] = max(kr[ipm - 1], krs);
+    }
+    if (jrs > krs || krs == 0) return false;
+    int jzone, kzone;
+    if (explicit_zones) {
+        jzone = jzone_in;
+        kzone = kzone_in;
+    } else if (lpot || mzone == nz) {
+        jzone = 0;
+        kzone = nz + 1;
+    } else {
+        jzone = 1;
+        kzone = mzone;
+    }
+    int jrf = 1;
+    if (jzone > 0 && jrad[jzone - 1] < nr) jrf = jrad[jzone - 1];
+    int krf = nr;
+    if (kzone <= nz) {
+        krf = kr[kzone - 1];
+        while (krf < jrf) {
+            kzone++;
+            if (kzone > nz) return false;
+            krf = kr[kzone - 1];
+        }
+    }
+    out[0] = jrs;
+    out[1] = krs;
+    out[2] = jrf;
+    out[3] = krf;
+    return true;
+}
+
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void analyse_kernel(AnaArgs<R> a) {
+    __shared__ R rho_s[MAX_NA];
+    __shared__ int s_rng[4], s_ok;
+    const int tid = threadIdx.x, na = a.na, ir = blockIdx.x + 1;
+    if (tid == 0) {
+        int kr[GH_MAXZ], rg[4] = {0, 0, 0, 0};
+        bool ok = solve_ranges(a.jrad, a.krad, a.nz, a.nr, a.mzone, a.lpot,
+                               a.explicit_zones, a.jzone, a.kzone, kr, rg);
+        s_ok = ok;
+        for (int q = 0; q < 4; q++) s_rng[q] = rg[q];
+        if (blockIdx.x == 0) {
+            for (int q = 0; q < 4; q++) a.ranges[q] = ok ? rg[q] : -1;
+            if (ok)
+                for (int z = 1; z < a.nz; z++) a.krad[z] = kr[z];
+        }
+    }
+    __syncthreads();
+    // zoncmb + scaled for this ring, then the fused masscl
+    const size_t row = (size_t)(ir - 1) * na;
+    for (int j = tid; j < na; j += GH_BLOCK) {
+        const size_t idx = row + j;
+        long long m0 = a.mfx[0][idx];
+        double acc = (double)m0 * a.q;
+        long long mz[GH_MAXZ];
+        mz[0] = m0;
+        for (int z = 2; z <= a.nz; z++) {
+            mz[z - 1] = a.mfx[z - 1][idx];
+            if (a.use_new[z - 1])
+                acc = acc + a.fnew[z - 1] * ((double)mz[z - 1] * a.q);
+            if (a.use_old[z - 1])
+                acc = acc + a.fold[z - 1] * ((double)a.moz[z - 1][idx] * a.q);
+        }
+        R v = (R)(acc * a.pmass);
+        rho_s[j] = v;
+        a.rho[idx] = v;
+        for (int z = 1; z <= a.masscl_z; z++) {
+            if (z >= 2) a.moz_w[z - 1][idx] = mz[z - 1];
+            a.mfx_w[z - 1][idx] = 0;
+        }
+    }
+    __syncthreads();
+    if (!s_ok || ir < s_rng[0] || ir > s_rng[1]) return;
+    // forward transform of the active harmonics: one wave per harmonic
+    const int wave = tid >> 6, lane = tid & 63, nw = GH_BLOCK / 64;
+    for (int m = wave; m < a.ng; m += nw) {
+        R sa = 0, sb = 0;
+        if (!a.lg[m]) {
+            int k = (m * lane) % na;
+            const int step = (m * 64) % na;
+            for (int j = lane; j < na; j += 64) {
+                R rv = rho_s[j];
+                sa += rv * a.tw[2 * k];
+                sb += rv * a.tw[2 * k + 1];
+                k += step;
+                if (k >= na) k -= na;
+            }
+            sa = wave_sum(sa);
+            sb = wave_sum(sb);
+        }
+        if (lane == 0) {
+            if (m == 0) {
+                a.coef[ir - 1] = sa;
+            } else if (2 * m == na) {
+                a.coef[(size_t)(2 * m - 1) * a.nr + ir - 1] = sa;
+            } else {
+                a.coef[(size_t)(2 * m - 1) * a.nr + ir - 1] = sa;
+                a.coef[(size_t)(2 * m) * a.nr + ir - 1] = -sb;
+            }
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void synth_kernel(SynArgs<R> a) {
+    typedef typename Vec2<R>::T R2;
+    __shared__ R W[3][2][MAX_NG];
+    const int tid = threadIdx.x, na = a.na, nr = a.nr, ir = blockIdx.x + 1;
+    const int jrs = a.ranges[0], krs = a.ranges[1];
+    const int jrf = a.ranges[2], krf = a.ranges[3];
+    if (jrs < 1 || ir < jrf || ir > krf) return;
+    const int ntypes = a.lpot ? 3 : 2;
+    const int wave = tid >> 6, lane = tid & 63, nw = GH_BLOCK / 64;
+    const int mmax = a.mmax;
+    const int maz = min(mmax - 2, a.ng - 1);
+    for (int task = wave; task < ntypes * a.ng; task += nw) {
+        const int type = task / a.ng, m = task - type * a.ng;
+        const bool azim = type == 1;
+        const bool valid = !a.lg[m] && (!azim || (m >= 1 && m <= maz));
+        R sc = 0, ss = 0;
+        if (valid) {
+            const bool pair = m >= 1 && m < mmax - 1;
+            const int off = m == 0 ? 0 : 2 * m - 1;
+            const R *g = a.G + (((size_t)type * a.ng + m) * nr + (ir - 1)) * nr;
+            const R *mc = a.coef + (size_t)off * nr;
+            const R *ms = a.coef + (size_t)(off + 1) * nr;
+            for (int is = jrs + lane; is <= krs; is += 64) {
+                R gv = g[is - 1];
+                R c = mc[is - 1];
+                R s = pair ? ms[is - 1] : (R)0;
+                if (!azim) {
+                    sc += gv * c;
+                    ss += gv * s;
+                } else {
+                    sc += gv * s;
+                    ss -= gv * c;
+                }
+            }
+            sc = wave_sum(sc);
+            ss = wave_sum(ss);
+        }
+        if (lane == 0) {
+            W[type][0][m] = sc;
+            W[type][1][m] = ss;
+        }
+    }
+    __syncthreads();
+    const R inv = (R)1 / (R)na;
+    for (int j = tid; j < na; j += GH_BLOCK) {
+        R F[3] = {0, 0, 0};
+        for (int type = 0; type < ntypes; type++) {
+            R s = W[type][0][0];
+            int k = j % na;
+            for (int m = 1; m < a.ng; m++) {
+                if (2 * m < na) {
+                    R c = a.tw[2 * k], sn = a.tw[2 * k + 1];
+                    s += (R)2 * (W[type][0][m] * c - W[type][1][m] * sn);
+                } else {
+                    s += (j & 1) ? -W[type][0][m] : W[type][0][m];
+                }
+                k += j;
+                if (k >= na) k -= na;
+            }
+            F[type] = s * inv;
+        }
+        const R cp = a.tw[2 * j], sp = a.tw[2 * j + 1];
+        R2 out;
+        out.x = F[0] * cp - F[1] * sp;
+        out.y = F[0] * sp + F[1] * cp;
+        const size_t idx = (size_t)(ir - 1) * na + j;
+        a.fld[idx] = out;
+        if (a.lpot) a.pot[idx] = F[2];
+    }
+}
+
+// ---- Green's tables (FP64) ----
+
+__device__ double softm(double ds) {
+    if (ds <= 0.0) return 0.0;
+    if (ds < 1.0)
+        return 4.0 * ds * ds * ds * (1.0 / 3.0 - 0.3 * ds * ds + 0.125 * ds * ds * ds);
+    if (ds < 2.0)
+        return -1.0 / 15.0 + ds * ds * ds *
+               (8.0 / 3.0 - 3.0 * ds + 1.2 * ds * ds - ds * ds * ds / 6.0);
+    return 1.0;
+}
+
+__device__ double kernel_value(int type, double rs, double rf, double cth,
+                               double sth, double eps, int tsoft) {
+    double d2 = rs * rs + rf * rf - 2.0 * rs * rf * cth;
+    if (type == 0) {       // radfor (radfor.f:25-38)
+        if (tsoft == 1) return (rs * cth - rf) / pow(d2 + eps * eps, 1.5);
+        return d2 > 0.0 ? (rs * cth - rf) * softm(sqrt(d2) / eps) / pow(d2, 1.5) : 0.0;
+    }
+    if (type == 1) {       // azifor (azifor.f:25-40)
+        if (tsoft == 1) return -rs * sth / pow(d2 + eps * eps, 1.5);
+        return d2 > 0.0 ? -rs * sth * softm(sqrt(d2) / eps) / pow(d2, 1.5) : 0.0;
+    }
+    // grnfun -> sftpot (sftpot.f:20-40)
+    if (tsoft == 1) return -1.0 / sqrt(d2 + eps * eps);
+    double x = sqrt(d2) / eps;
+    if (x <= 0.0) return -1.4 / eps;
+    if (x < 1.0) return (-1.4 + x * x * (2.0 / 3.0 - 0.3 * x * x + 0.1 * x * x * x)) / eps;
+    if (x < 2.0)
+        return (-1.6 + 1.0 / (15.0 * x) +
+                x * x * (4.0 / 3.0 - x + 0.3 * x * x - x * x * x / 30.0)) / eps;
+    return -1.0 / sqrt(d2);
+}
+
+// block (is, ir, type): samples over the half circle, then the transform
+// of every kept harmonic.  twd = cos, sin of 2 pi k / na in double.
+__global__ __launch_bounds__(GH_BLOCK) void green_kernel(
+    double *__restrict__ G, const double *__restrict__ twd, int nr, int na,
+    int ng, int nsect, double alpha, double eps, int tsoft) {
+    __shared__ double f[MAX_NA / 2 + 1];
+    const int is = blockIdx.x + 1, ir = blockIdx.y + 1, type = blockIdx.z;
+    const int mmax = na / 2 + 1;
+    const bool azim = type == 1;
+    const double rs = exp(alpha * (double)(is - 1)) - 1.0;
+    const double rf = exp(alpha * (double)(ir - 1)) - 1.0;
+    for (int k = threadIdx.x; k < mmax; k += GH_BLOCK)
+        f[k] = kernel_value(type, rs, rf, twd[2 * k], twd[2 * k + 1], eps, tsoft);
+    __syncthreads();
+    const int maz = min(mmax - 2, ng - 1);
+    for (int m = threadIdx.x; m < ng; m += GH_BLOCK) {
+        double v = 0.0;
+        if (m % nsect == 0) {
+            if (!azim) {
+                // DCT-I over mmax points (rcost): f0 + (-1)^m f_n/2 + 2 sum
+                v = f[0] + ((m & 1) ? -f[mmax - 1] : f[mmax - 1]);
+                int k = m;
+                for (int j = 1; j < mmax - 1; j++) {
+                    v += 2.0 * f[j] * twd[2 * k];
+                    k += m;
+                    if (k >= na) k -= na;
+                }
+            } else if (m >= 1 && m <= maz) {
+                // DST-I over the na/2-1 interior points (rsint)
+                int k = m;
+                for (int j = 1; j < mmax - 1; j++) {
+                    v += 2.0 * f[j] * twd[2 * k + 1];
+                    k += m;
+                    if (k >= na) k -= na;
+                }
+            }
+        }
+        G[(((size_t)type * ng + m) * nr + (ir - 1)) * nr + (is - 1)] = v;
+    }
+}
+
+template <typename S, typename D>
+__global__ void convert_kernel(const S *__restrict__ in, D *__restrict__ out,
+                               long long n) {
+    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i < n) out[i] = (D)in[i];
+}
+
+template <typename T>
+__global__ void fill_kernel(T *__restrict__ p, T v, long long n) {
+    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// masscl (masscl.f:67-146) as a kernel: save zones 2..mz, clear zones 1..mz,
+// reset jrad/krad of those zones.
+__global__ void masscl_kernel(ZonePtrs zp, int mz, long long mesh, int *jrad,
+                              int *krad, int nr) {
+    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i < mesh) {
+        for (int z = 1; z <= mz; z++) {
+            if (z >= 2) zp.moz[z - 1][i] = zp.mfx[z - 1][i];
+            zp.mfx[z - 1][i] = 0;
+        }
+    }
+    if (i < mz) {
+        jrad[i] = nr;
+        krad[i] = 0;
+    }
+}
+
+// ---- host launchers ----
+
+template <typename R>
+hipError_t launch_analyse(const AnaArgs<R> &a, hipStream_t s) {
+    if (a.na > MAX_NA) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(analyse_kernel<R>, dim3(a.nr), dim3(GH_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_synth(const SynArgs<R> &a, hipStream_t s) {
+    if (a.ng > MAX_NG) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(synth_kernel<R>, dim3(a.nr), dim3(GH_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_green(double *G, const double *twd, int nr, int na, int ng,
+                        int nsect, double alpha, double eps, int tsoft,
+                        hipStream_t s) {
+    if (na > MAX_NA) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(green_kernel, dim3(nr, nr, 3), dim3(GH_BLOCK), 0, s, G,
+                       twd, nr, na, ng, nsect, alpha, eps, tsoft);
+    return hipGetLastError();
+}
+
+hipError_t launch_d2f(const double *in, float *out, long long n, hipStream_t s) {
+    hipLaunchKernelGGL((convert_kernel<double, float>),
+                       dim3((unsigned)((n + GH_BLOCK - 1) / GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i64(long long *p, long long v, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel<long long>,
+                       dim3((unsigned)((n + GH_BLOCK - 1) / GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int *p, int v, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel<int>,
+                       dim3((unsigned)((n + GH_BLOCK - 1) / GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_masscl(const ZonePtrs &zp, int mz, long long mesh, int *jrad,
+                         int *krad, int nr, hipStream_t s) {
+    long long n = mesh > mz ? mesh : mz;
+    hipLaunchKernelGGL(masscl_kernel, dim3((unsigned)((n + GH_BLOCK - 1) / GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, zp, mz, mesh, jrad, krad, nr);
+    return hipGetLastError();
+}
+
+template hipError_t launch_analyse<float>(const AnaArgs<float> &, hipStream_t);
+template hipError_t launch_analyse<double>(const AnaArgs<double> &, hipStream_t);
+template hipError_t launch_synth<float>(const SynArgs<float> &, hipStream_t);
+template hipError_t launch_synth<double>(const SynArgs<double> &, hipStream_t);
+
+}  // namespace gh

@@ -1,0 +1,239 @@
+"""Host-side mirror of GALAXY's lib15 interface for the p2d polar grid.
+
+``P2DGrid`` owns one libgalhip context.  Its methods carry the names of the
+reference routines they replace and take the same meaning of arguments
+(SRC/lib15/<name>.f):
+
+  ``P2DGrid(...)``  p2dset + setgrd + filset + setspc (grid and zones)
+  ``greenm()``      greenm -> p2grnm (Green's tables, built on the GPU in FP64)
+  ``restrt()``      restrt, particle part (upload)
+  ``unload()``      unload (download in load order)
+  ``masset()``      masset
+  ``findf(lpot)``   findf -> p2fndf -> polcat
+  ``p2fndf(j, k)``  p2fndf with an explicit zone range
+  ``step()``        step
+  ``run(n)``        galaxy's loop of findf + step
+  ``weight()``, ``grdacc()``  group-level routines for parity checks
+
+Errors raise ``GalhipError`` where the reference would call crash().
+Arrays are numpy; coordinates in grid units, velocities in displacement per
+step (SRC/lib15/setrun.f:858-873).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (GALHIP_FP32, GALHIP_FP64, MAXSKIP, MAXZONES, P2DParams,
+                   check, load)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class P2DGrid:
+    """A 2-D polar grid with its particles on one GPU."""
+
+    def __init__(self, nr, na, hash_, softl, tsoft=1, nsect=1, skip=(),
+                 nzones=1, nstep=(1,), zbr=(0.0,), offanal=True, pmass=1.0,
+                 precision="fp32"):
+        lib = load()
+        p = P2DParams()
+        p.nr, p.na, p.hash, p.nsect = nr, na, hash_, nsect
+        skip = list(skip)
+        if len(skip) > MAXSKIP:
+            raise ValueError("at most %d skipped harmonics" % MAXSKIP)
+        p.nskip = len(skip)
+        for i, m in enumerate(skip):
+            p.skip[i] = m
+        p.softl, p.tsoft = float(softl), int(tsoft)
+        if nzones > MAXZONES:
+            raise ValueError("at most %d zones" % MAXZONES)
+        p.nzones = nzones
+        for i in range(nzones):
+            p.nstep[i] = int(nstep[i])
+            p.zbr[i] = float(zbr[i])
+        p.offanal = int(bool(offanal))
+        p.pmass = float(pmass)
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("precision must be 'fp32' or 'fp64'")
+        p.precision = GALHIP_FP32 if precision == "fp32" else GALHIP_FP64
+        self.dtype = np.float32 if precision == "fp32" else np.float64
+        self.precision = precision
+        h = C.c_void_p()
+        check(lib.galhip_p2dset(C.byref(p), C.byref(h)), "p2dset")
+        self._lib, self._h = lib, h
+        g = np.zeros(6)
+        check(lib.galhip_geometry(h, _ptr(g)), "geometry")
+        self.alpha, self.rgrid, self.thmax = g[0], g[1], g[2]
+        self.ng, self.mmax, self.mesh = int(g[3]), int(g[4]), int(g[5])
+        self.nr, self.na, self.nsect, self.nzones = nr, 2 * (na // 2), nsect, nzones
+        self.n = 0
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.galhip_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        check(rc, what)
+
+    # -- set-up ---------------------------------------------------------------
+    def greenm(self):
+        self._c(self._lib.galhip_greenm(self._h), "greenm")
+
+    def get_green(self):
+        G = np.zeros((3, self.ng, self.nr, self.nr), np.float64)
+        self._c(self._lib.galhip_get_green(self._h, _ptr(G)), "get_green")
+        return G
+
+    def set_green(self, G):
+        G = np.ascontiguousarray(G, np.float64)
+        assert G.shape == (3, self.ng, self.nr, self.nr)
+        self._c(self._lib.galhip_set_green(self._h, _ptr(G)), "set_green")
+
+    def set_fixforce(self, dr, val, der):
+        val = np.ascontiguousarray(val, np.float64)
+        der = np.ascontiguousarray(der, np.float64)
+        self._c(self._lib.galhip_set_fixforce(self._h, len(val), float(dr),
+                                              _ptr(val), _ptr(der)), "set_fixforce")
+
+    # -- particles ----------------------------------------------------------------
+    def restrt(self, x, y, vx, vy, iz=None):
+        dt = self.dtype
+        arrs = [np.ascontiguousarray(a, dt) for a in (x, y, vx, vy)]
+        n = len(arrs[0])
+        iz = np.ones(n, np.int32) if iz is None else np.ascontiguousarray(iz, np.int32)
+        self._c(self._lib.galhip_restrt(self._h, n, *[_ptr(a) for a in arrs],
+                                        _ptr(iz)), "restrt")
+        self.n = n
+
+    def unload(self):
+        n = self.n
+        out = {k: np.empty(n, self.dtype) for k in ("x", "y", "vx", "vy")}
+        out["iz"] = np.empty(n, np.int32)
+        self._c(self._lib.galhip_unload(self._h, _ptr(out["x"]), _ptr(out["y"]),
+                                        _ptr(out["vx"]), _ptr(out["vy"]),
+                                        _ptr(out["iz"])), "unload")
+        return out
+
+    # -- time bookkeeping ---------------------------------------------------------
+    def set_step(self, istep, lstep=None):
+        ls = None
+        if lstep is not None:
+            ls = np.ascontiguousarray(lstep, np.int32).reshape(2, self.nzones)
+        self._c(self._lib.galhip_set_step(self._h, int(istep), _ptr(ls)), "set_step")
+
+    def get_step(self):
+        i, m = C.c_int32(), C.c_int32()
+        ls = np.zeros((2, self.nzones), np.int32)
+        self._c(self._lib.galhip_get_step(self._h, C.byref(i), C.byref(m), _ptr(ls)),
+                "get_step")
+        return i.value, m.value, ls
+
+    # -- the step -----------------------------------------------------------------
+    def masset(self):
+        self._c(self._lib.galhip_masset(self._h), "masset")
+
+    def findf(self, lpot=False):
+        self._c(self._lib.galhip_findf(self._h, int(bool(lpot))), "findf")
+
+    def p2fndf(self, jzone, kzone, lpot=False):
+        self._c(self._lib.galhip_p2fndf(self._h, int(jzone), int(kzone),
+                                        int(bool(lpot))), "p2fndf")
+
+    def step(self):
+        self._c(self._lib.galhip_step(self._h), "step")
+
+    def run(self, nsteps):
+        self._c(self._lib.galhip_run(self._h, int(nsteps)), "run")
+
+    def sync(self):
+        self._c(self._lib.galhip_sync(self._h), "sync")
+
+    def sort(self):
+        self._c(self._lib.galhip_sort(self._h), "sort")
+
+    def set_sort_interval(self, k):
+        self._c(self._lib.galhip_set_sort_interval(self._h, int(k)), "set_sort_interval")
+
+    # -- group level ---------------------------------------------------------------
+    def weight(self, x, y, old=True):
+        x = np.ascontiguousarray(x, self.dtype)
+        y = np.ascontiguousarray(y, self.dtype)
+        n = len(x)
+        cell = np.empty(n, np.int32)
+        wt = np.empty((n, 4), self.dtype)
+        self._c(self._lib.galhip_weight(self._h, n, _ptr(x), _ptr(y), int(bool(old)),
+                                        _ptr(cell), _ptr(wt)), "weight")
+        return cell, wt
+
+    def grdacc(self, x, y):
+        x = np.ascontiguousarray(x, self.dtype)
+        y = np.ascontiguousarray(y, self.dtype)
+        acc = np.empty((len(x), 2), self.dtype)
+        self._c(self._lib.galhip_grdacc(self._h, len(x), _ptr(x), _ptr(y), _ptr(acc)),
+                "grdacc")
+        return acc
+
+    # -- grids -----------------------------------------------------------------------
+    def get_mass(self):
+        mfx = np.zeros((self.nzones, self.mesh), np.int64)
+        moz = np.zeros((self.nzones, self.mesh), np.int64)
+        jr = np.zeros(self.nzones, np.int32)
+        kr = np.zeros(self.nzones, np.int32)
+        qb = C.c_int32()
+        self._c(self._lib.galhip_get_mass(self._h, _ptr(mfx), _ptr(moz), _ptr(jr),
+                                          _ptr(kr), C.byref(qb)), "get_mass")
+        return mfx, moz, jr, kr, qb.value
+
+    def set_mass(self, mfx=None, moz=None, jrad=None, krad=None):
+        a = [None if v is None else np.ascontiguousarray(v, t) for v, t in
+             ((mfx, np.int64), (moz, np.int64), (jrad, np.int32), (krad, np.int32))]
+        self._c(self._lib.galhip_set_mass(self._h, *[_ptr(v) for v in a]), "set_mass")
+
+    def get_rho(self):
+        rho = np.zeros(self.mesh, self.dtype)
+        self._c(self._lib.galhip_get_rho(self._h, _ptr(rho)), "get_rho")
+        return rho
+
+    def get_field(self):
+        f = np.zeros((3, self.mesh), self.dtype)
+        self._c(self._lib.galhip_get_field(self._h, _ptr(f)), "get_field")
+        return f
+
+    def set_field(self, fld):
+        f = np.ascontiguousarray(fld, self.dtype)
+        if f.shape[0] == 2:
+            f = np.concatenate([f, np.zeros((1, self.mesh), self.dtype)])
+        self._c(self._lib.galhip_set_field(self._h, _ptr(f)), "set_field")
+
+    def ranges(self):
+        r = np.zeros(4, np.int32)
+        self._c(self._lib.galhip_get_ranges(self._h, _ptr(r)), "get_ranges")
+        return tuple(int(v) for v in r)
+
+    @property
+    def noff(self):
+        v = C.c_int64()
+        self._c(self._lib.galhip_get_noff(self._h, C.byref(v)), "get_noff")
+        return v.value
+
+    # -- profiling ----------------------------------------------------------------
+    def set_profile(self, on=True):
+        self._c(self._lib.galhip_set_profile(self._h, int(bool(on))), "set_profile")
+
+    def get_profile(self):
+        ms, nl, npush = C.c_double(), C.c_int64(), C.c_int64()
+        self._c(self._lib.galhip_get_profile(self._h, C.byref(ms), C.byref(nl),
+                                             C.byref(npush)), "get_profile")
+        return ms.value, nl.value, npush.value

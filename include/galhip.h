@@ -1,0 +1,184 @@
+/*
+ * galhip.h -- C ABI of libgalhip.so, the MI355X (gfx950) implementation of
+ * GALAXY's 2-D polar-grid particle-mesh step.
+ *
+ * Every entry point takes plain scalars and pointers and returns 0 on
+ * success or a negative GALHIP_E* code; galhip_last_error() returns the
+ * message of the last failure on the calling thread.  The reference has no
+ * return codes (its crash() STOPs, SRC/lib15/crash.f:23-37); a caller that
+ * wants the reference's fatal semantics stops on any non-zero return.
+ *
+ * The reference keeps its state in COMMON blocks and module arrays
+ * (SRC/lib15/inc, SRC/lib15/aarrays.f); here that state lives in an
+ * opaque context owned by the library, with device-resident particles and
+ * grids.  Host buffers cross the boundary only at load/unload and in the
+ * parity entry points.  All work of one context goes on one HIP stream in
+ * submission order; a context must be used from one host thread at a time
+ * (the reference is single-threaded too).
+ *
+ * Array conventions (as the reference's): rings ir = 1..nr, spokes
+ * it = 1..na, cell index (ir-1)*na + (it-1); zones 1..nzones, zone
+ * nzones+1 = the off-grid list.  Coordinates are grid units, velocities are
+ * displacements per step and accelerations velocity changes per step
+ * (internal units, SRC/lib15/setrun.f:858-873).  Floating point buffers are
+ * float for a GALHIP_FP32 context and double for a GALHIP_FP64 context.
+ */
+#ifndef GALHIP_H
+#define GALHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GALHIP_FP32 4
+#define GALHIP_FP64 8
+#define GALHIP_MAXZONES 10
+#define GALHIP_MAXSKIP 16
+
+#define GALHIP_OK 0
+#define GALHIP_EINVAL (-1)   /* invalid argument or state */
+#define GALHIP_EHIP (-2)     /* HIP runtime failure */
+#define GALHIP_ERANGE (-3)   /* reference invariant violated (crash() cases) */
+
+typedef struct galhip_ctx galhip_ctx;
+
+/* Grid, filter, zone and scale parameters: the values the reference reads
+ * from its .dat cards into /gridsi/, /gridsr/, /filter/, /zones/ and
+ * /scalesr/ (p2dset.f:24-80, filset.f:42-141, setrun.f:200-220,770-873). */
+typedef struct {
+    int32_t nr, na;            /* GRID card: rings, spokes (na forced even) */
+    int32_t hash;              /* HASH card: highest active harmonic */
+    int32_t nsect;             /* SECT card: sectors (1 = whole plane) */
+    int32_t nskip;             /* SKIP card: number of harmonics listed */
+    int32_t skip[GALHIP_MAXSKIP];
+    double softl;              /* SOFT card: softening length, grid units */
+    int32_t tsoft;             /* 1 Plummer, 2 cubic spline */
+    int32_t nzones;            /* ZONE cards */
+    int32_t nstep[GALHIP_MAXZONES];  /* step multiple of zone i (nstep[0] = 1) */
+    double zbr[GALHIP_MAXZONES];     /* inner radius of zone i, grid units */
+    int32_t offanal;           /* OFFG card, first flag: move off-grid particles */
+    double pmass;              /* particle mass, internal units (setrun.f:866) */
+    int32_t precision;         /* GALHIP_FP32 or GALHIP_FP64 */
+} galhip_p2d_params;
+
+/* p2dset + setgrd + filset + setspc: create a context for one p2d grid.
+ * Replaces SRC/lib15/p2dset.f:1, SRC/lib15/setgrd.f:1, SRC/lib15/filset.f:1,
+ * SRC/lib15/setspc.f:1 (grid and zone set-up; particle storage is sized by
+ * galhip_restrt). */
+int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out);
+int galhip_free(galhip_ctx *ctx);
+const char *galhip_last_error(void);
+
+/* Geometry constants as the reference computes them (setgrd.f:52-200):
+ * out[0] = alpha, out[1] = rgrid, out[2] = thmax, out[3] = ng, out[4] = mmax,
+ * out[5] = mesh. */
+int galhip_geometry(const galhip_ctx *ctx, double *out);
+
+/* greenm -> p2grnm: build the Fourier-transformed Green's tables on the GPU
+ * in FP64 (SRC/lib15/greenm.f:1, SRC/lib15/p2grnm.f:1).  The tables are kept
+ * on the device; nothing is written to disk. */
+int galhip_greenm(galhip_ctx *ctx);
+/* Copy the tables in or out as G[3][ng][nr][nr] doubles (type 0 = F_R,
+ * 1 = F_phi, 2 = potential; [field ring][source ring]), the content of the
+ * reference's .grt records (SRC/lib15/p2gwrt.f:1). */
+int galhip_get_green(const galhip_ctx *ctx, double *G);
+int galhip_set_green(galhip_ctx *ctx, const double *G);
+
+/* Tabulated fixed radial force F(r) (internal units), cubic Hermite between
+ * r_k = k*dr, k = 0..n-1, from values and derivatives.  Stands in for the
+ * rigid-halo / axisymmetric force functions halfrc and axifrc
+ * (SRC/lib15/halfrc.f:1, SRC/lib15/axifrc.f:1) called from getacc
+ * (SRC/lib15/getacc.f:115-135).  n = 0 removes the force. */
+int galhip_set_fixforce(galhip_ctx *ctx, int32_t n, double dr,
+                        const double *val, const double *der);
+
+/* restrt (particle part, SRC/lib15/restrt.f:1): load n particles from host
+ * arrays; iz[i] = zone (1..nzones) or nzones+1 if off the grid.  Particles
+ * are stored on the device as structure of arrays, ordered by cell. */
+int galhip_restrt(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
+                  const void *vx, const void *vy, const int32_t *iz);
+/* unload (SRC/lib15/unload.f:1): copy the particles back to host arrays in
+ * their load order.  Any pointer may be NULL. */
+int galhip_unload(galhip_ctx *ctx, void *x, void *y, void *vx, void *vy,
+                  int32_t *iz);
+
+/* Step counter and zone time stamps (/admin/ istep, /point/ lstep,
+ * SRC/lib15/inc/admin.f:36-37).  lstep is [2][nzones] with zone 1 unused;
+ * either pointer may be NULL on get. */
+int galhip_set_step(galhip_ctx *ctx, int32_t istep, const int32_t *lstep);
+int galhip_get_step(const galhip_ctx *ctx, int32_t *istep, int32_t *mzone,
+                    int32_t *lstep);
+
+/* masset: assign every on-grid particle, then combine zones and scale
+ * (SRC/lib15/masset.f:1). */
+int galhip_masset(galhip_ctx *ctx);
+
+/* findf(lpot) -> p2fndf -> polcat: solve for the Cartesian accelerations
+ * (and the potential when lpot != 0) over the reference's radial ranges
+ * (SRC/lib15/findf.f:1, SRC/lib15/p2fndf.f:1, SRC/lib15/polcat.f:1). */
+int galhip_findf(galhip_ctx *ctx, int32_t lpot);
+/* p2fndf(jzone, kzone) with an explicit zone range, then polcat
+ * (SRC/lib15/p2fndf.f:1). */
+int galhip_p2fndf(galhip_ctx *ctx, int32_t jzone, int32_t kzone, int32_t lpot);
+
+/* step: advance the particles of the zones due at this step, assign their
+ * mass, combine zones and scale (SRC/lib15/step.f:1 with masscl, getacc,
+ * grdacc, stpgrp, rezgrp, massgn, mascmb/zoncmb and scaled). */
+int galhip_step(galhip_ctx *ctx);
+
+/* galaxy's inner loop: nsteps x (findf(.false.); step) on the device
+ * (SRC/progs/galaxy.f:107-148 without analysis and I/O). */
+int galhip_run(galhip_ctx *ctx, int32_t nsteps);
+
+/* ---- group-level entry points (parity checks; host buffers) ---- */
+
+/* weight(jst, old) for n positions (SRC/lib15/weight.f:1): 0-based cells
+ * (ncl-1) and weights wt[n][4].  old = 0 applies the sector fold used for
+ * mass assignment. */
+int galhip_weight(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
+                  int32_t old, int32_t *cell, void *wt);
+/* grdacc(jst) for n positions (SRC/lib15/grdacc.f:1): grid accelerations
+ * acc[n][2] interpolated from the current Cartesian field. */
+int galhip_grdacc(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
+                  void *acc);
+
+/* Read back grids: mass sums as fixed-point int64 [nzones][mesh] (weight x
+ * 2^qbits), the saved zone masses grdmoz, and the zone ring ranges jrad,
+ * krad [nzones] (/zones/).  Any pointer may be NULL. */
+int galhip_get_mass(const galhip_ctx *ctx, int64_t *mfx, int64_t *mozfx,
+                    int32_t *jrad, int32_t *krad, int32_t *qbits);
+/* Load the zone mass arrays and ring ranges, as restrt does from a dump's
+ * mass records (SRC/lib15/restrt.f:1, records of SRC/lib15/dumpp.f:57-198);
+ * mfx/mozfx in the fixed-point units of galhip_get_mass.  Any pointer may be
+ * NULL to leave that array unchanged. */
+int galhip_set_mass(galhip_ctx *ctx, const int64_t *mfx, const int64_t *mozfx,
+                    const int32_t *jrad, const int32_t *krad);
+/* The combined, scaled mass of zone 1 (grdmss(:,1) before p2manl). */
+int galhip_get_rho(const galhip_ctx *ctx, void *rho);
+/* Field as fld[3][mesh]: F_x, F_y (after polcat) and potential. */
+int galhip_get_field(const galhip_ctx *ctx, void *fld);
+int galhip_set_field(galhip_ctx *ctx, const void *fld);
+/* Radial ranges used by the last solve: jrs, krs, jrf, krf (1-based). */
+int galhip_get_ranges(const galhip_ctx *ctx, int32_t *r4);
+/* Off-grid particle count (/admin/ noff). */
+int galhip_get_noff(const galhip_ctx *ctx, int64_t *noff);
+
+/* ---- performance controls ---- */
+
+/* Re-sort particles by cell every k steps of galhip_run (0 = never). */
+int galhip_set_sort_interval(galhip_ctx *ctx, int32_t k);
+int galhip_sort(galhip_ctx *ctx);
+/* Record HIP events around every particle-push launch on the context's
+ * stream; galhip_get_profile returns the summed push time (ms), the number
+ * of push launches and of particles pushed since profiling was enabled. */
+int galhip_set_profile(galhip_ctx *ctx, int32_t on);
+int galhip_get_profile(galhip_ctx *ctx, double *push_ms, int64_t *launches,
+                       int64_t *pushes);
+int galhip_sync(galhip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -248,9 +248,46 @@ void orc_green(int nr, int na, int ng, int nsect, double alpha, double softl,
 
 /* ---------------- field solve (FP64) ---------------- */
 
-/* Forward transform of one ring, active harmonics only is not enough for
- * the reference's in-place semantics, so the full half-complex row is
- * formed (p2manl.f:35-43). */
+/* Half-complex forward transform of one ring restricted to harmonics
+ * m < ng (the only ones p2conv reads); other slots are set to zero. */
+static void rfftf_lim(int na, int ng, const double *tw, const double *x,
+                      double *y)
+{
+    for (int i = 0; i < na; i++) y[i] = 0.0;
+    for (int m = 0; m < ng; m++) {
+        double a = 0.0, b = 0.0;
+        for (int j = 0; j < na; j++) {
+            int k = (int)(((long)m * j) % na);
+            a += x[j] * tw[2 * k];
+            b += x[j] * tw[2 * k + 1];
+        }
+        if (m == 0) y[0] = a;
+        else if (2 * m == na) y[na - 1] = a;
+        else { y[2 * m - 1] = a; y[2 * m] = -b; }
+    }
+}
+
+/* Unnormalised backward transform of one ring from harmonics m < ng. */
+static void rfftb_lim(int na, int ng, const double *tw, const double *x,
+                      double *y)
+{
+    for (int j = 0; j < na; j++) {
+        double s = x[0];
+        for (int m = 1; m < ng; m++) {
+            int k = (int)(((long)m * j) % na);
+            if (2 * m < na)
+                s += 2.0 * (x[2 * m - 1] * tw[2 * k] - x[2 * m] * tw[2 * k + 1]);
+            else
+                s += (j % 2 ? -x[na - 1] : x[na - 1]);
+        }
+        y[j] = s;
+    }
+}
+
+/* p2manl + p2conv + p2fsyn.  The forward transform of p2manl is formed for
+ * the harmonics m < ng only (p2conv never reads the others); the rings of
+ * [jrs, krs] outside [jrf, krf] therefore hold zeros in the inactive slots
+ * where the reference leaves all coefficients. */
 void orc_fndf(int nr, int na, int ng, const int32_t *lg, const double *G,
               double *rho, int jrs, int krs, int jrf, int krf, int ntypes,
               double *fld)
@@ -259,10 +296,15 @@ void orc_fndf(int nr, int na, int ng, const int32_t *lg, const double *G,
     int mesh = nr * na;
     size_t nn = (size_t)nr * nr;
     double *row = malloc(sizeof(double) * na);
+    double *tw = malloc(sizeof(double) * 2 * na);
+    for (int k = 0; k < na; k++) {
+        tw[2 * k] = cos2pi(k, na);
+        tw[2 * k + 1] = sin2pi(k, na);
+    }
     /* p2manl on rings jrs..krs in place */
     for (int ir = jrs; ir <= krs; ir++) {
         double *p = rho + (size_t)(ir - 1) * na;
-        orc_rfftf(na, p, row);
+        rfftf_lim(na, ng, tw, p, row);
         memcpy(p, row, sizeof(double) * na);
     }
     double *w = malloc(sizeof(double) * mesh);
@@ -304,12 +346,13 @@ void orc_fndf(int nr, int na, int ng, const int32_t *lg, const double *G,
         /* p2fsyn on rings jrf..krf, times 1/na */
         for (int ir = jrf; ir <= krf; ir++) {
             double *p = F + (size_t)(ir - 1) * na;
-            orc_rfftb(na, p, row);
+            rfftb_lim(na, ng, tw, p, row);
             for (int j = 0; j < na; j++) p[j] = row[j] / (double)na;
         }
     }
     free(w);
     free(row);
+    free(tw);
 }
 
 /* polcat (SRC/lib15/polcat.f:26-40), FP64 */
