@@ -23,12 +23,25 @@ namespace gh {
 template <typename R> hipError_t launch_push(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_massgn(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_gather(const GatherArgs<R> &, hipStream_t);
+template <typename R> hipError_t launch_ut(const PushArgs<R> &, hipStream_t);
 template <typename R>
-hipError_t launch_key(const R *, const R *, const uint8_t *, long long, int,
-                      int, int, R, unsigned *, unsigned *, hipStream_t);
-template <typename T>
-hipError_t launch_permute(const T *, T *, const unsigned *, long long,
-                          hipStream_t);
+hipError_t launch_key(const typename Vec2<R>::T *, const uint8_t *, long long,
+                      int, int, int, int, unsigned, unsigned *, unsigned *,
+                      hipStream_t);
+template <typename R> struct PermArgs {
+    const typename Vec4<R>::T *xv_in;
+    typename Vec4<R>::T *xv_out;
+    const typename Vec2<R>::T *ut_in;
+    typename Vec2<R>::T *ut_out;
+    const uint8_t *zin;
+    uint8_t *zout;
+    const unsigned *iin;
+    unsigned *iout;
+    const unsigned *perm;
+    long long n;
+};
+template <typename R>
+hipError_t launch_permute_all(const PermArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_analyse(const AnaArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_synth(const SynArgs<R> &, hipStream_t);
 hipError_t launch_green(double *, const double *, int, int, int, int, double,
@@ -80,10 +93,10 @@ struct galhip_ctx {
     // device storage
     hipStream_t s;
     long long n, cap;
-    void *x, *y, *vx, *vy;
+    void *xv, *ut;                     // (x, y, vx, vy) and (u, t) per particle
     uint8_t *zone;
     unsigned *id;
-    void *x2, *y2, *vx2, *vy2;
+    void *xv2, *ut2;
     uint8_t *zone2;
     unsigned *id2;
     unsigned *key, *key2, *perm, *perm2;
@@ -121,8 +134,8 @@ template <typename T> static void dfree(T *&p) {
 }
 
 static void free_particles(galhip_ctx *c) {
-    dfree(c->x); dfree(c->y); dfree(c->vx); dfree(c->vy); dfree(c->zone); dfree(c->id);
-    dfree(c->x2); dfree(c->y2); dfree(c->vx2); dfree(c->vy2); dfree(c->zone2); dfree(c->id2);
+    dfree(c->xv); dfree(c->ut); dfree(c->zone); dfree(c->id);
+    dfree(c->xv2); dfree(c->ut2); dfree(c->zone2); dfree(c->id2);
     dfree(c->key); dfree(c->key2); dfree(c->perm); dfree(c->perm2); dfree(c->sort_tmp);
     c->sort_tmp_bytes = 0;
     c->n = c->cap = 0;
@@ -346,11 +359,9 @@ static int ensure_particles(galhip_ctx *c, long long n) {
     free_particles(c);
     size_t es = esize(c);
     int rc = 0;
-    rc |= dalloc(&c->x, es * n); rc |= dalloc(&c->y, es * n);
-    rc |= dalloc(&c->vx, es * n); rc |= dalloc(&c->vy, es * n);
+    rc |= dalloc(&c->xv, 4 * es * n); rc |= dalloc(&c->ut, 2 * es * n);
+    rc |= dalloc(&c->xv2, 4 * es * n); rc |= dalloc(&c->ut2, 2 * es * n);
     rc |= dalloc(&c->zone, n); rc |= dalloc(&c->id, sizeof(unsigned) * n);
-    rc |= dalloc(&c->x2, es * n); rc |= dalloc(&c->y2, es * n);
-    rc |= dalloc(&c->vx2, es * n); rc |= dalloc(&c->vy2, es * n);
     rc |= dalloc(&c->zone2, n); rc |= dalloc(&c->id2, sizeof(unsigned) * n);
     rc |= dalloc(&c->key, sizeof(unsigned) * n); rc |= dalloc(&c->key2, sizeof(unsigned) * n);
     rc |= dalloc(&c->perm, sizeof(unsigned) * n); rc |= dalloc(&c->perm2, sizeof(unsigned) * n);
@@ -369,24 +380,32 @@ static int ensure_particles(galhip_ctx *c, long long n) {
 template <typename R> static int sort_impl(galhip_ctx *c) {
     long long n = c->n;
     if (n <= 1) return 0;
-    HIPCHK(launch_key<R>((const R *)c->x, (const R *)c->y, c->zone, n, c->nr,
-                         c->na, c->nz, (R)c->alpha, c->key, c->perm, c->s));
+    // keys of at most 16 bits: cells shifted right until they fit, off-grid last
+    int kshift = 0;
+    while (((long long)c->mesh >> kshift) > 65534) kshift++;
+    unsigned kmax = (unsigned)((c->mesh - 1) >> kshift) + 1;
+    HIPCHK(launch_key<R>((const typename Vec2<R>::T *)c->ut, c->zone, n, c->nr,
+                         c->na, c->nz, kshift, kmax, c->key, c->perm, c->s));
     int end_bit = 1;
-    while ((1LL << end_bit) <= (long long)c->mesh) end_bit++;
+    while ((1LL << end_bit) <= (long long)kmax) end_bit++;
     size_t tmp = c->sort_tmp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->key, c->key2,
                                               c->perm, c->perm2, (int)n, 0,
                                               end_bit, c->s));
-    HIPCHK(launch_permute<R>((const R *)c->x, (R *)c->x2, c->perm2, n, c->s));
-    HIPCHK(launch_permute<R>((const R *)c->y, (R *)c->y2, c->perm2, n, c->s));
-    HIPCHK(launch_permute<R>((const R *)c->vx, (R *)c->vx2, c->perm2, n, c->s));
-    HIPCHK(launch_permute<R>((const R *)c->vy, (R *)c->vy2, c->perm2, n, c->s));
-    HIPCHK(launch_permute<uint8_t>(c->zone, c->zone2, c->perm2, n, c->s));
-    HIPCHK(launch_permute<unsigned>(c->id, c->id2, c->perm2, n, c->s));
-    std::swap(c->x, c->x2);
-    std::swap(c->y, c->y2);
-    std::swap(c->vx, c->vx2);
-    std::swap(c->vy, c->vy2);
+    PermArgs<R> p;
+    p.xv_in = (const typename Vec4<R>::T *)c->xv;
+    p.xv_out = (typename Vec4<R>::T *)c->xv2;
+    p.ut_in = (const typename Vec2<R>::T *)c->ut;
+    p.ut_out = (typename Vec2<R>::T *)c->ut2;
+    p.zin = c->zone;
+    p.zout = c->zone2;
+    p.iin = c->id;
+    p.iout = c->id2;
+    p.perm = c->perm2;
+    p.n = n;
+    HIPCHK(launch_permute_all<R>(p, c->s));
+    std::swap(c->xv, c->xv2);
+    std::swap(c->ut, c->ut2);
     std::swap(c->zone, c->zone2);
     std::swap(c->id, c->id2);
     c->steps_since_sort = 0;
@@ -403,6 +422,8 @@ extern "C" int galhip_set_sort_interval(galhip_ctx *c, int32_t k) {
     c->sort_interval = k < 0 ? 0 : k;
     return 0;
 }
+
+static int compute_ut(galhip_ctx *c);
 
 extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
                              const void *y, const void *vx, const void *vy,
@@ -423,24 +444,25 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
         z[i] = (uint8_t)v;
         ids[i] = (unsigned)i;
     }
-    HIPCHK(hipMemcpy(c->x, x, es * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->y, y, es * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->vx, vx, es * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->vy, vy, es * n, hipMemcpyHostToDevice));
+    {
+        // interleave (x, y, vx, vy) per particle
+        std::vector<unsigned char> buf(4 * es * n);
+        const unsigned char *src[4] = {(const unsigned char *)x, (const unsigned char *)y,
+                                       (const unsigned char *)vx, (const unsigned char *)vy};
+        for (long long i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++)
+                memcpy(buf.data() + (4 * i + k) * es, src[k] + i * es, es);
+        HIPCHK(hipMemcpy(c->xv, buf.data(), 4 * es * n, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(c->zone, z.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
     c->qbits = qbits_for(n);
+    rc = compute_ut(c);
+    if (rc) return rc;
     rc = galhip_sort(c);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->s));
     return 0;
-}
-
-template <typename R>
-static void scatter_back(const std::vector<R> &src, const std::vector<unsigned> &ids,
-                         void *dst) {
-    R *d = (R *)dst;
-    for (size_t i = 0; i < ids.size(); i++) d[ids[i]] = src[i];
 }
 
 extern "C" int galhip_unload(galhip_ctx *c, void *x, void *y, void *vx,
@@ -450,19 +472,15 @@ extern "C" int galhip_unload(galhip_ctx *c, void *x, void *y, void *vx,
     long long n = c->n;
     std::vector<unsigned> ids(n);
     HIPCHK(hipMemcpy(ids.data(), c->id, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
-    void *src[4] = {c->x, c->y, c->vx, c->vy};
     void *dst[4] = {x, y, vx, vy};
+    size_t es = esize(c);
+    std::vector<unsigned char> buf(4 * es * n);
+    HIPCHK(hipMemcpy(buf.data(), c->xv, 4 * es * n, hipMemcpyDeviceToHost));
     for (int k = 0; k < 4; k++) {
         if (!dst[k]) continue;
-        if (c->prec == GALHIP_FP32) {
-            std::vector<float> h(n);
-            HIPCHK(hipMemcpy(h.data(), src[k], 4 * n, hipMemcpyDeviceToHost));
-            scatter_back(h, ids, dst[k]);
-        } else {
-            std::vector<double> h(n);
-            HIPCHK(hipMemcpy(h.data(), src[k], 8 * n, hipMemcpyDeviceToHost));
-            scatter_back(h, ids, dst[k]);
-        }
+        unsigned char *d = (unsigned char *)dst[k];
+        for (long long i = 0; i < n; i++)
+            memcpy(d + (size_t)ids[i] * es, buf.data() + (4 * i + k) * es, es);
     }
     if (iz) {
         std::vector<uint8_t> z(n);
@@ -548,10 +566,20 @@ extern "C" int galhip_get_step(const galhip_ctx *c, int32_t *istep, int32_t *mzo
 
 // ---- kernel argument assembly -----------------------------------------------------
 
+static UtConst ut_const(const galhip_ctx *c) {
+    UtConst k;
+    k.alpha_d = c->alpha;                 // alpha in the context precision
+    k.inv_alpha = 1.0 / c->alpha;
+    k.tscale = (double)c->na / (2.0 * 3.141592653589793238462643);
+    k.na = c->na;
+    return k;
+}
+
 template <typename R> static PushArgs<R> push_args(galhip_ctx *c) {
     PushArgs<R> a;
     memset(&a, 0, sizeof a);
-    a.x = (R *)c->x; a.y = (R *)c->y; a.vx = (R *)c->vx; a.vy = (R *)c->vy;
+    a.xv = (typename Vec4<R>::T *)c->xv;
+    a.ut = (typename Vec2<R>::T *)c->ut;
     a.zone = c->zone;
     a.n = c->n;
     a.fld = (const typename Vec2<R>::T *)c->fld2;
@@ -562,11 +590,13 @@ template <typename R> static PushArgs<R> push_args(galhip_ctx *c) {
     a.fr_val = c->fr_val;
     a.fr_der = c->fr_der;
     a.fr_dr = c->fr_dr;
+    a.fr_inv_dr = c->nfr > 0 ? 1.0 / c->fr_dr : 0.0;
     a.nfr = c->nfr;
     a.fixforce = c->nfr > 0;
     a.nr = c->nr; a.na = c->na; a.mesh = c->mesh; a.nz = c->nz;
     a.mzone = c->mzone; a.offanal = c->offanal; a.nsect = c->nsect; a.qbits = c->qbits;
-    a.alpha = (R)c->alpha; a.rgrid = (R)c->rgrid; a.thmax = (R)c->thmax;
+    a.k = ut_const(c);
+    a.rgrid = (R)c->rgrid; a.thmax = (R)c->thmax;
     for (int z = 0; z < c->nz; z++) {
         a.zbr[z] = (R)c->zbr[z];
         a.nstep[z] = c->nstep[z];
@@ -574,16 +604,20 @@ template <typename R> static PushArgs<R> push_args(galhip_ctx *c) {
     return a;
 }
 
+static int compute_ut(galhip_ctx *c) {
+    if (c->prec == GALHIP_FP32) HIPCHK(launch_ut<float>(push_args<float>(c), c->s));
+    else HIPCHK(launch_ut<double>(push_args<double>(c), c->s));
+    return 0;
+}
+
 template <typename R>
 static AnaArgs<R> ana_args(galhip_ctx *c, int lpot, int explicit_zones,
-                           int jzone, int kzone, int masscl_z) {
+                           int jzone, int kzone) {
     AnaArgs<R> a;
     memset(&a, 0, sizeof a);
     for (int z = 0; z < c->nz; z++) {
         a.mfx[z] = c->mfx[z];
         a.moz[z] = c->moz[z];
-        a.mfx_w[z] = c->mfx[z];
-        a.moz_w[z] = c->moz[z];
         a.fnew[z] = c->fnew[z];
         a.fold[z] = c->fold[z];
         a.use_new[z] = c->use_new[z];
@@ -596,12 +630,20 @@ static AnaArgs<R> ana_args(galhip_ctx *c, int lpot, int explicit_zones,
     a.nr = c->nr; a.na = c->na; a.ng = c->ng; a.ncoef = c->ncoef; a.nz = c->nz;
     a.mzone = c->mzone; a.lpot = lpot; a.jzone = jzone; a.kzone = kzone;
     a.explicit_zones = explicit_zones;
-    a.masscl_z = masscl_z;
     return a;
 }
 
-template <typename R> static SynArgs<R> syn_args(galhip_ctx *c, int lpot) {
+template <typename R> static SynArgs<R> syn_args(galhip_ctx *c, int lpot,
+                                                 int masscl_z) {
     SynArgs<R> a;
+    memset(&a, 0, sizeof a);
+    for (int z = 0; z < c->nz; z++) {
+        a.mfx[z] = c->mfx[z];
+        a.moz[z] = c->moz[z];
+    }
+    a.jrad = c->jrad;
+    a.krad = c->krad;
+    a.masscl_z = masscl_z;
     a.coef = (const R *)c->coef;
     a.G = (const R *)c->G;
     a.tw = (const R *)c->tw;
@@ -641,9 +683,9 @@ template <typename R>
 static int solve_impl(galhip_ctx *c, int lpot, int explicit_zones, int jzone,
                       int kzone, int masscl_z) {
     if (!c->have_green) return fail(GALHIP_EINVAL, "P2CONV: Green function not set (call galhip_greenm)");
-    AnaArgs<R> a = ana_args<R>(c, lpot, explicit_zones, jzone, kzone, masscl_z);
+    AnaArgs<R> a = ana_args<R>(c, lpot, explicit_zones, jzone, kzone);
     HIPCHK(launch_analyse<R>(a, c->s));
-    SynArgs<R> b = syn_args<R>(c, lpot);
+    SynArgs<R> b = syn_args<R>(c, lpot, masscl_z);
     HIPCHK(launch_synth<R>(b, c->s));
     c->have_field = 1;
     return 0;
@@ -705,13 +747,11 @@ extern "C" int galhip_run(galhip_ctx *c, int32_t nsteps) {
     if (!c) return fail(GALHIP_EINVAL, "null context");
     for (int k = 0; k < nsteps; k++) {
         set_mzone(c);
-        // findf with the masscl of this step fused into the analysis kernel
+        // findf with the masscl of this step (zone sums saved and cleared,
+        // jrad/krad reset) fused into the synthesis kernel
         int rc = c->prec == GALHIP_FP32 ? solve_impl<float>(c, 0, 0, 0, 0, c->mzone)
                                         : solve_impl<double>(c, 0, 0, 0, 0, c->mzone);
         if (rc) return rc;
-        // the fused masscl also resets jrad/krad of the moving zones
-        HIPCHK(launch_fill_i32(c->jrad, c->nr, c->mzone, c->s));
-        HIPCHK(launch_fill_i32(c->krad, 0, c->mzone, c->s));
         c->masscl_done = 1;
         rc = galhip_step(c);
         if (rc) return rc;
@@ -743,8 +783,9 @@ static int gather_impl(galhip_ctx *c, int64_t n, const void *x, const void *y,
         a.x = dx; a.y = dy; a.n = n;
         a.fld = (const typename Vec2<R>::T *)c->fld2;
         a.acc = dacc; a.cell = dcell; a.wt = dwt;
-        a.nr = c->nr; a.na = c->na; a.fold = fold && c->nsect > 1;
-        a.alpha = (R)c->alpha; a.thmax = (R)c->thmax;
+        a.nr = c->nr; a.fold = fold && c->nsect > 1;
+        a.k = ut_const(c);
+        a.thmax = (R)c->thmax;
         hipError_t e = launch_gather<R>(a, c->s);
         if (e == hipSuccess) e = hipStreamSynchronize(c->s);
         if (e != hipSuccess) rc = fail(GALHIP_EHIP, "gather kernel: %s", hipGetErrorString(e));

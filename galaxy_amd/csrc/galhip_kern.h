@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "galhip_math.h"
+
 #define GH_MAXZ 10
 #define GH_BLOCK 256
 // 2.d0 * pi of SRC/lib15/inc/pi.f
@@ -19,31 +21,56 @@ namespace gh {
 template <typename R> struct Vec2;
 template <> struct Vec2<float> { typedef float2 T; };
 template <> struct Vec2<double> { typedef double2 T; };
+template <typename R> struct Vec4;
+template <> struct Vec4<float> { typedef float4 T; };
+template <> struct Vec4<double> { typedef double4 T; };
+
+__device__ __forceinline__ float2 mk2(float a, float b) { return make_float2(a, b); }
+__device__ __forceinline__ double2 mk2(double a, double b) { return make_double2(a, b); }
+__device__ __forceinline__ float4 mk4(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+__device__ __forceinline__ double4 mk4(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
 
 // ---- per-particle geometry ------------------------------------------------
 
-// uofgr stdpol rule u = log(1 + r) / alpha (SRC/lib15/uofgr.f:24).  The FP32
-// logarithm is the correctly rounded one, taken through double.
-__device__ __forceinline__ float uofgr(float r, float alpha) {
-    float one_r = 1.0f + r;
-    float l = (float)log((double)one_r);
-    return l / alpha;
+// Grid coordinates of a position as weight() forms them
+// (SRC/lib15/weight.f:72-79, uofgr.f:24): u = log(1 + r) / alpha and
+// t = atan2(y, x) na / (2 pi), plus na when negative (t is the unfolded
+// angle).  FP32: the reference's FP32 logarithm and quotient, correctly
+// rounded -- log through gh_log_ge1 rounded once to float, and l / alpha as
+// (double)l * (1/alpha) rounded once (equal to the IEEE float quotient
+// because a float quotient is never within 2^-49 of a float midpoint); the
+// angle in double, rounded once.  FP64: the same formulas in double.
+struct UtConst {
+    double inv_alpha;   // 1 / (double)alpha
+    double tscale;      // na / (2 pi)
+    double alpha_d;     // alpha in the context precision, as double
+    int na;
+};
+
+__device__ __forceinline__ void ut_of(float x, float y, float rr,
+                                      const UtConst &k, float &u, float &t) {
+    float v = 1.0f + rr;
+    float l = (float)gh_log_ge1((double)v);
+    u = (float)((double)l * k.inv_alpha);
+    t = (float)(gh_atan2((double)y, (double)x) * k.tscale);
+    if (t < 0.0f) t = t + (float)k.na;
 }
-__device__ __forceinline__ double uofgr(double r, double alpha) {
-    return log(1.0 + r) / alpha;
+__device__ __forceinline__ void ut_of(double x, double y, double rr,
+                                      const UtConst &k, double &u, double &t) {
+    double l = gh_log_ge1(1.0 + rr);
+    u = l / k.alpha_d;
+    t = gh_atan2(y, x) * (double)k.na / GH_TWO_PI;
+    if (t < 0.0) t = t + (double)k.na;
 }
 
-// weight() for one p2d particle (SRC/lib15/weight.f:61-97).  Returns the
-// 0-based cell (ncl - 1); w[0..3] = CIC weights of corners (c, c+1, c+na,
-// c+1+na).  fold applies the mass-assignment sector fold t = mod(t, thmax).
+// Cell and CIC weights from (u, t) (weight.f:84-95).  The reference's cell
+// is c = ju*na + jt with jt in [0, na]; (ring, spoke) = (c div na, c mod na)
+// without a division.  w[0..3] belong to the corners (ring, spoke),
+// (ring, spoke+1), (ring+1, spoke), (ring+1, spoke+1), spoke+1 wrapping
+// (massgn.f:79-85, grdacc.f:45-50).
 template <typename R>
-__device__ __forceinline__ int weight1(R x, R y, int nr, int na, R alpha,
-                                       bool fold, R thmax, R w[4], R &rr) {
-    rr = sqrt(x * x + y * y);
-    R u = uofgr(rr, alpha);
-    R t = (R)(atan2((double)y, (double)x) * (double)na / GH_TWO_PI);
-    if (t < (R)0) t = t + (R)na;
-    if (fold) t = fmod(t, thmax);
+__device__ __forceinline__ void cic(R u, R t, int nr, int na, int &ring,
+                                    int &spoke, R w[4]) {
     int ju = (int)u;
     if (ju > nr - 2) ju = nr - 2;
     int jt = (int)t;
@@ -54,16 +81,35 @@ __device__ __forceinline__ int weight1(R x, R y, int nr, int na, R alpha,
     w[1] = dt - dd;
     w[2] = du - dd;
     w[3] = dd;
-    return ju * na + jt;
+    ring = ju;
+    spoke = jt;
+    if (jt >= na) {
+        ring = ju + 1;
+        spoke = jt - na;
+    }
 }
 
-// The four corners with the azimuthal wrap of massgn.f:79-85 / grdacc.f:45-50.
-__device__ __forceinline__ void corners(int c, int na, int ic[4]) {
-    ic[0] = c;
-    ic[1] = c + 1;
-    if ((c + 1) % na == 0) ic[1] -= na;
-    ic[2] = c + na;
+// Flat corner indices (row-major cell = ring*na + spoke) of a cell.
+__device__ __forceinline__ void corners_rs(int ring, int spoke, int na,
+                                           int ic[4]) {
+    int s2 = spoke + 1 == na ? 0 : spoke + 1;
+    ic[0] = ring * na + spoke;
+    ic[1] = ring * na + s2;
+    ic[2] = ic[0] + na;
     ic[3] = ic[1] + na;
+}
+
+// weight() for one position: cell (c = ring*na + spoke), weights, radius.
+template <typename R>
+__device__ __forceinline__ int weight1(R x, R y, int nr, const UtConst &k,
+                                       bool fold, R thmax, R w[4], R &rr,
+                                       R &u, R &t) {
+    rr = sqrt(x * x + y * y);
+    ut_of(x, y, rr, k, u, t);
+    R tf = fold ? (R)fmod(t, thmax) : t;
+    int ring, spoke;
+    cic(u, tf, nr, k.na, ring, spoke, w);
+    return ring * k.na + spoke;
 }
 
 // Fixed-point weight: w * 2^qbits, rounded to nearest even.
@@ -72,11 +118,13 @@ __device__ __forceinline__ long long quant(R w, int qbits) {
     return __double2ll_rn(ldexp((double)w, qbits));
 }
 
-// Cubic Hermite interpolation of the tabulated fixed radial force.
+// Cubic Hermite interpolation of the tabulated fixed radial force on
+// r_k = k dr (s = r * (1/dr), the same formula as the oracle's).
 __device__ __forceinline__ double fixforce(double r, int nfr, double dr,
+                                           double inv_dr,
                                            const double *__restrict__ val,
                                            const double *__restrict__ der) {
-    double s = r / dr;
+    double s = r * inv_dr;
     int k = (int)s;
     if (k >= nfr - 1) return val[nfr - 1];
     if (k < 0) k = 0;
@@ -104,7 +152,8 @@ template <> __device__ __forceinline__ float2 zero2<float>() { return make_float
 template <> __device__ __forceinline__ double2 zero2<double>() { return make_double2(0.0, 0.0); }
 
 template <typename R> struct PushArgs {
-    R *x, *y, *vx, *vy;
+    typename Vec4<R>::T *xv;          // (x, y, vx, vy) per particle
+    typename Vec2<R>::T *ut;          // (u, t) of the current position
     uint8_t *zone;
     long long n;
     const typename Vec2<R>::T *fld;   // [mesh] (F_x, F_y) after polcat
@@ -112,10 +161,11 @@ template <typename R> struct PushArgs {
     int *jrad, *krad;                 // [nz] 1-based ring ranges
     long long *counters;              // [0] off-grid delta, [1] pushes
     const double *fr_val, *fr_der;
-    double fr_dr;
+    double fr_dr, fr_inv_dr;
     int nfr, fixforce;
     int nr, na, mesh, nz, mzone, offanal, nsect, qbits;
-    R alpha, rgrid, thmax;
+    UtConst k;
+    R rgrid, thmax;
     R zbr[GH_MAXZ];
     int nstep[GH_MAXZ];
 };
@@ -128,26 +178,24 @@ template <typename R> struct GatherArgs {
     R *acc;        // [n][2] or NULL
     int *cell;     // [n] or NULL
     R *wt;         // [n][4] or NULL
-    int nr, na, fold;
-    R alpha, thmax;
+    int nr, fold;
+    UtConst k;
+    R thmax;
 };
 
 template <typename R> struct AnaArgs {
     const long long *mfx[GH_MAXZ];
     const long long *moz[GH_MAXZ];
-    long long *mfx_w[GH_MAXZ];        // fused masscl: zone arrays to clear
-    long long *moz_w[GH_MAXZ];        // fused masscl: saved copies
     double fnew[GH_MAXZ], fold[GH_MAXZ];
     int use_new[GH_MAXZ], use_old[GH_MAXZ];
     int *jrad, *krad;
     int *ranges;                      // out: jrs, krs, jrf, krf
     R *rho;                           // [mesh]
-    R *coef;                          // [nr][ncoef]
+    R *coef;                          // [ncoef][nr]
     const R *tw;                      // [na][2] cos, sin of 2 pi k / na
     const int *lg;                    // [mmax] skip flags
     double q, pmass;
     int nr, na, ng, ncoef, nz, mzone, lpot, jzone, kzone, explicit_zones;
-    int masscl_z;                     // fused masscl for zones 1..masscl_z (0 = none)
 };
 
 template <typename R> struct SynArgs {
@@ -159,6 +207,13 @@ template <typename R> struct SynArgs {
     typename Vec2<R>::T *fld;
     R *pot;
     int nr, na, ng, ncoef, mmax, lpot;
+    // masscl of the coming push (masscl.f:67-146), done here because the
+    // analysis kernel has consumed the zone sums: zones 1..masscl_z cleared,
+    // zones 2..masscl_z saved, their jrad/krad reset (0 = none)
+    long long *mfx[GH_MAXZ];
+    long long *moz[GH_MAXZ];
+    int *jrad, *krad;
+    int masscl_z;
 };
 
 }  // namespace gh

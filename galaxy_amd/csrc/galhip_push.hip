@@ -1,30 +1,42 @@
-// galhip_push.hip -- the fused particle kernel of the p2d step and the
-// group-level gather kernels.
+// galhip_push.hip -- the fused particle kernel of the p2d step, the mass
+// assignment of masset, the group-level gather kernel and the sort helpers.
 //
 // push_kernel does, for each particle of the zones due at this step, what
 // the reference's per-group pipeline does (SRC/lib15/step.f:186-234):
-//   getacc/grdacc  weights at the old position, 4-corner interpolation of
-//                  the Cartesian field, fixed radial force, zone scaling
+//   getacc/grdacc  CIC weights at the old position, 4-corner interpolation
+//                  of the Cartesian field, fixed radial force, zone scaling
 //                  (getacc.f:35-163, grdacc.f:40-66)
 //   stpgrp/offgrd  RVLF leap-frog and the off-grid test (stpgrp.f:27-56,
 //                  offgrd.f:55-59, addoff.f:30-70)
 //   rezgrp/chstep  zone changes (rezgrp.f:30-50, zoneno.f, chstep.f:22-31)
 //   massgn         weights at the new position, scatter into the zone's
 //                  mass array (massgn.f:68-89)
-// in one pass over structure-of-arrays particle storage.  Mass sums are
-// 64-bit fixed point (weight x 2^qbits): integer adds commute, so the
-// result is bit-identical for any processing order and needs no float
-// atomics.  Each block first accumulates into an LDS tile covering the
-// rings its particles touch (particles are kept ordered by cell, so a
-// block's particles share one or two rings) and flushes the tile with one
-// 64-bit global atomic per touched cell.
+// in one pass over structure-of-arrays particle storage.
+//
+// The old position's weights are not recomputed: every particle carries the
+// (u, t) grid coordinates that its last mass assignment formed from the
+// same position with the same formulas (weight.f forms them twice per step,
+// once for the forces and once for the masses), so the transcendental part
+// runs once per push.
+//
+// Mass sums are 64-bit fixed point (weight x 2^qbits): integer adds
+// commute, so the sums are bit-identical for any processing order and need
+// no float atomics.  The deposit is reduced in three levels: lanes of a
+// wavefront that target the same cell are summed with shuffles (particles
+// are kept ordered by cell, so a wavefront holds few distinct cells); the
+// wavefront's leader adds the group's four corner sums into an LDS tile of
+// the rings the block touches; the block flushes the tile's non-zero cells
+// with one 64-bit global atomic each.  Global 64-bit atomics are the scarce
+// resource on MI355X (about 24 G/s device-wide) and LDS atomics the next
+// (about 230 G/s), so each level divides the count of the next.
 #include "galhip_kern.h"
 #include <climits>
 
 namespace gh {
 
 constexpr int PPT = 4;            // particles per thread
-constexpr int TILE_CAP = 4096;    // LDS tile, cells (32 KiB of int64)
+constexpr int TILE_CAP = 1024;    // LDS tile, cells of a ring x spoke window
+constexpr int TILE_REP = 4;       // replicas (lane & 3): 4 x 8 KiB of int64
 
 template <typename R>
 __device__ __forceinline__ R tsfac(const PushArgs<R> &a, int iz) {
@@ -32,9 +44,9 @@ __device__ __forceinline__ R tsfac(const PushArgs<R> &a, int iz) {
     return (R)a.nstep[i - 1];
 }
 
+// zoneno for a position of known radius (zoneno.f:40-55)
 template <typename R>
-__device__ __forceinline__ int zoneno(const PushArgs<R> &a, R x, R y) {
-    R rad = sqrt(x * x + y * y);
+__device__ __forceinline__ int zone_of(const PushArgs<R> &a, R rad) {
     int z = 1;
     if (a.nz > 1)
         for (int i = 1; i <= a.nz; i++)
@@ -57,24 +69,62 @@ __device__ __forceinline__ void chstep(R tfac, R ox, R oy, R &ovx, R &ovy,
     ny = oy + nvy;
 }
 
+__device__ __forceinline__ long long wave_sum64(long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Global deposit of one corner value with Fortran's flat indexing over the
+// [nz][mesh] zone arrays (grdmss(j, jz)); past the last array it is dropped.
+__device__ __forceinline__ void deposit_global(long long *const *mfx, int nz,
+                                               int mesh, int zone, int idx,
+                                               long long v) {
+    long long f = (long long)(zone - 1) * mesh + idx;
+    if (f < (long long)nz * mesh) {
+        int zf = zone - 1, cf = idx;
+        while (cf >= mesh) {
+            cf -= mesh;
+            zf++;
+        }
+        atomicAdd((unsigned long long *)&mfx[zf][cf], (unsigned long long)v);
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
     typedef typename Vec2<R>::T R2;
-    __shared__ long long tile[TILE_CAP];
-    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_cnt[GH_MAXZ];
-    __shared__ int s_dnoff, s_pushes, s_zt, s_base, s_len;
+    typedef typename Vec4<R>::T R4;
+    __shared__ long long tile[TILE_REP][TILE_CAP];
+    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ];
+    __shared__ int s_dnoff, s_pushes, s_zt, s_sref;
+    __shared__ int s_rmin, s_rmax, s_dmin, s_dmax;
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     const long long base = (long long)blockIdx.x * (GH_BLOCK * PPT);
     const int nz = a.nz, nlists = a.nz + 1, na = a.na, nr = a.nr;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
         s_kr[tid] = -1;
-        s_cnt[tid] = 0;
     }
     if (tid == 0) {
         s_dnoff = 0;
         s_pushes = 0;
+        s_zt = 0;
+        s_rmin = INT_MAX;
+        s_rmax = -1;
+        s_dmin = INT_MAX;
+        s_dmax = INT_MIN;
     }
 
     int zin[PPT];
@@ -92,154 +142,201 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
     // every thread reaches this barrier; a block with nothing to move exits
     if (!__syncthreads_or(any)) return;
 
-    int dcell[PPT], dzone[PPT];
+    // deposit state kept per particle: ring, spoke, zone (0 = none), weights
+    int dring[PPT], dspoke[PPT], dzone[PPT];
     R dw[PPT][4];
     int my_pushes = 0, my_dnoff = 0;
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        dcell[k] = -1;
         dzone[k] = 0;
-        if (!mv[k]) continue;
-        const long long i = base + k * GH_BLOCK + tid;
-        const int z0 = zin[k];
-        const bool offlist = z0 == nlists;
-        R ox = a.x[i], oy = a.y[i], ovx = a.vx[i], ovy = a.vy[i];
-        R ax = 0, ay = 0;
-        my_pushes++;
-        if (!offlist) {
-            R w[4], rr;
-            int c = weight1<R>(ox, oy, nr, na, a.alpha, false, a.thmax, w, rr);
-            int ic[4];
-            corners(c, na, ic);
+        dring[k] = 0;
+        dspoke[k] = 0;
+        if (mv[k]) {
+            const long long i = base + k * GH_BLOCK + tid;
+            const int z0 = zin[k];
+            const bool offlist = z0 == nlists;
+            const R4 st = a.xv[i];
+            R ox = st.x, oy = st.y, ovx = st.z, ovy = st.w;
+            R ax = 0, ay = 0;
+            my_pushes++;
+            R rr = sqrt(ox * ox + oy * oy);
+            if (!offlist) {
+                // grdacc: weights of the old position from its (u, t)
+                R w[4];
+                int ring, spoke, ic[4];
+                const R2 cu = a.ut[i];
+                cic(cu.x, cu.y, nr, na, ring, spoke, w);
+                corners_rs(ring, spoke, na, ic);
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                // the last ring with t == na points past the field: read 0
-                R2 f = ic[q] < a.mesh ? a.fld[ic[q]] : zero2<R>();
-                ax = ax + f.x * w[q];
-                ay = ay + f.y * w[q];
-            }
-            if (a.fixforce) {
-                R fr = (R)fixforce((double)rr, a.nfr, a.fr_dr, a.fr_val, a.fr_der);
-                if (fr != (R)0) {
-                    fr = fr / rr;
-                    ax = ax + fr * ox;
-                    ay = ay + fr * oy;
+                for (int q = 0; q < 4; q++) {
+                    // the last ring with t == na points past the field: 0
+                    R2 f = ic[q] < a.mesh ? a.fld[ic[q]] : zero2<R>();
+                    ax = ax + f.x * w[q];
+                    ay = ay + f.y * w[q];
+                }
+                if (a.fixforce) {
+                    R fr = (R)fixforce((double)rr, a.nfr, a.fr_dr, a.fr_inv_dr,
+                                       a.fr_val, a.fr_der);
+                    if (fr != (R)0) {
+                        fr = fr / rr;
+                        ax = ax + fr * ox;
+                        ay = ay + fr * oy;
+                    }
                 }
             }
-        }
-        if (nz > 1 && z0 > 1) {
-            R t = tsfac(a, z0);
-            t = t * t;
-            ax = t * ax;
-            ay = t * ay;
-        }
-        R nvx = ovx + ax, nvy = ovy + ay;
-        R nx = ox + nvx, ny = oy + nvy;
-        int znew = z0;
-        R rnew = sqrt(nx * nx + ny * ny);
-        bool off = rnew > a.rgrid;
-        if (!offlist) {
-            if (off) {
-                znew = nlists;
-                my_dnoff++;
-                if (a.offanal && z0 < nz) {
-                    R tfac = tsfac(a, nz) / tsfac(a, z0);
+            if (nz > 1 && z0 > 1) {
+                R t = tsfac(a, z0);
+                t = t * t;
+                ax = t * ax;
+                ay = t * ay;
+            }
+            R nvx = ovx + ax, nvy = ovy + ay;
+            R nx = ox + nvx, ny = oy + nvy;
+            int znew = z0;
+            R rnew = sqrt(nx * nx + ny * ny);
+            bool off = rnew > a.rgrid;
+            if (!offlist) {
+                if (off) {
+                    znew = nlists;
+                    my_dnoff++;
+                    if (a.offanal && z0 < nz) {
+                        R tfac = tsfac(a, nz) / tsfac(a, z0);
+                        chstep(tfac, ox, oy, ovx, ovy, ax, ay, nx, ny, nvx, nvy);
+                        rnew = sqrt(nx * nx + ny * ny);
+                    }
+                }
+            } else if (!off) {
+                my_dnoff--;
+                znew = nz;
+            }
+            if (a.mzone > 1 && !offlist && znew <= a.mzone) {
+                int kz = zone_of(a, rnew);
+                if (kz > a.mzone) kz = a.mzone;
+                if (kz > znew) kz = zone_of(a, rr);
+                if (kz != znew) {
+                    R tfac = tsfac(a, kz) / tsfac(a, znew);
                     chstep(tfac, ox, oy, ovx, ovy, ax, ay, nx, ny, nvx, nvy);
+                    rnew = sqrt(nx * nx + ny * ny);
+                    znew = kz;
                 }
             }
-        } else if (!off) {
-            my_dnoff--;
-            znew = nz;
-        }
-        if (a.mzone > 1 && !offlist && znew <= a.mzone) {
-            int kz = zoneno(a, nx, ny);
-            if (kz > a.mzone) kz = a.mzone;
-            if (kz > znew) kz = zoneno(a, ox, oy);
-            if (kz != znew) {
-                R tfac = tsfac(a, kz) / tsfac(a, znew);
-                chstep(tfac, ox, oy, ovx, ovy, ax, ay, nx, ny, nvx, nvy);
-                znew = kz;
+            a.xv[i] = mk4(nx, ny, nvx, nvy);
+            if (znew != z0) a.zone[i] = (uint8_t)znew;
+            if (znew >= 1 && znew <= nz) {
+                R u, t;
+                ut_of(nx, ny, rnew, a.k, u, t);
+                a.ut[i] = mk2(u, t);
+                R tf = a.nsect > 1 ? (R)fmod(t, a.thmax) : t;
+                cic(u, tf, nr, na, dring[k], dspoke[k], dw[k]);
+                dzone[k] = znew;
             }
-        }
-        a.x[i] = nx;
-        a.y[i] = ny;
-        a.vx[i] = nvx;
-        a.vy[i] = nvy;
-        if (znew != z0) a.zone[i] = (uint8_t)znew;
-        if (znew >= 1 && znew <= nz) {
-            R rr;
-            int c = weight1<R>(nx, ny, nr, na, a.alpha, a.nsect > 1, a.thmax,
-                               dw[k], rr);
-            dcell[k] = c;
-            dzone[k] = znew;
-            int ring = c / na;
-            atomicMin(&s_jr[znew - 1], ring);
-            atomicMax(&s_kr[znew - 1], ring);
-            atomicAdd(&s_cnt[znew - 1], 1);
         }
     }
     if (my_pushes) atomicAdd(&s_pushes, my_pushes);
     if (my_dnoff) atomicAdd(&s_dnoff, my_dnoff);
-    __syncthreads();
-
-    // choose the tile: the zone with most depositing particles, rings
-    // [jr, kr + 1] (corners reach the next ring)
+    // the tile's zone and reference spoke: thread 0's first deposit
     if (tid == 0) {
-        int zt = -1, best = 0;
-        for (int z = 0; z < nz; z++)
-            if (s_cnt[z] > best) {
-                best = s_cnt[z];
-                zt = z;
+#pragma unroll
+        for (int k = PPT - 1; k >= 0; k--)
+            if (dzone[k] > 0) {
+                s_zt = dzone[k];
+                s_sref = dspoke[k];
             }
-        int len = 0;
-        if (zt >= 0) {
-            len = (s_kr[zt] - s_jr[zt] + 2) * na;
-            if (len > TILE_CAP || s_kr[zt] + 1 >= nr) zt = -1;
-        }
-        s_zt = zt;
-        s_base = zt >= 0 ? s_jr[zt] * na : 0;
-        s_len = zt >= 0 ? len : 0;
     }
     __syncthreads();
-    const int zt = s_zt, tbase = s_base, tlen = s_len;
-    for (int j = tid; j < tlen; j += GH_BLOCK) tile[j] = 0;
-    __syncthreads();
-
-    const long long total = (long long)nz * a.mesh;
+    const int zt = s_zt, sref = s_sref;        // zt == 0: no tile
+    const int half = na >> 1;
+    // ring and spoke-offset ranges of the tile zone's deposits: registers,
+    // then one wavefront reduction; other zones' ring ranges go to LDS
+    int rmin = INT_MAX, rmax = -1, dmin = INT_MAX, dmax = INT_MIN;
+    int dd[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        if (dcell[k] < 0) continue;
-        int ic[4];
-        corners(dcell[k], na, ic);
-        const bool in_tile = (dzone[k] - 1) == zt;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            long long v = quant<R>(dw[k][q], a.qbits);
-            if (v == 0) continue;
-            if (in_tile) {
-                atomicAdd((unsigned long long *)&tile[ic[q] - tbase],
-                          (unsigned long long)v);
+        dd[k] = 0;
+        if (dzone[k] > 0) {
+            if (dzone[k] == zt) {
+                int d = dspoke[k] - sref;
+                if (d < -half) d += na;
+                else if (d >= half) d -= na;
+                dd[k] = d;
+                rmin = min(rmin, dring[k]);
+                rmax = max(rmax, dring[k]);
+                dmin = min(dmin, d);
+                dmax = max(dmax, d);
             } else {
-                // flat index over the [nz][mesh] zone arrays, as Fortran's
-                // grdmss(j, jz); past the last array the weight is dropped
-                long long f = (long long)(dzone[k] - 1) * a.mesh + ic[q];
-                if (f < total) {
-                    int zf = (int)(f / a.mesh);
-                    int cf = (int)(f - (long long)zf * a.mesh);
-                    atomicAdd((unsigned long long *)&a.mfx[zf][cf],
-                              (unsigned long long)v);
-                }
+                atomicMin(&s_jr[dzone[k] - 1], dring[k]);
+                atomicMax(&s_kr[dzone[k] - 1], dring[k]);
             }
         }
     }
-    __syncthreads();
-    for (int j = tid; j < tlen; j += GH_BLOCK) {
-        long long v = tile[j];
-        if (v != 0)
-            atomicAdd((unsigned long long *)&a.mfx[zt][tbase + j],
-                      (unsigned long long)v);
+    rmin = wave_min(rmin);
+    rmax = wave_max(rmax);
+    dmin = wave_min(dmin);
+    dmax = wave_max(dmax);
+    if (lane == 0 && rmax >= 0) {
+        atomicMin(&s_rmin, rmin);
+        atomicMax(&s_rmax, rmax);
+        atomicMin(&s_dmin, dmin);
+        atomicMax(&s_dmax, dmax);
     }
-    if (tid < nz && s_cnt[tid] > 0) {
+    __syncthreads();
+    // tile: rings [r0, r1 + 1] x spoke offsets [d0, d1 + 1] of zone zt
+    const int r0 = s_rmin, r1 = s_rmax, d0 = s_dmin, d1 = s_dmax;
+    const int tw = d1 - d0 + 2, th = r1 - r0 + 2;
+    const bool use_tile = zt > 0 && r1 >= 0 && r1 + 1 < nr && th * tw <= TILE_CAP;
+    const int tlen = use_tile ? th * tw : 0;
+    if (tid == 0 && zt > 0 && r1 >= 0) {
+        atomicMin(&s_jr[zt - 1], r0);
+        atomicMax(&s_kr[zt - 1], r1);
+    }
+    for (int j = tid; j < tlen; j += GH_BLOCK) {
+#pragma unroll
+        for (int rep = 0; rep < TILE_REP; rep++) tile[rep][j] = 0;
+    }
+    __syncthreads();
+
+    long long *trep = tile[lane & (TILE_REP - 1)];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        if (dzone[k] == 0) continue;
+        long long qv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) qv[q] = quant<R>(dw[k][q], a.qbits);
+        if (use_tile && dzone[k] == zt) {
+            const int b = (dring[k] - r0) * tw + (dd[k] - d0);
+            const int ti[4] = {b, b + 1, b + tw, b + tw + 1};
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (qv[q] != 0)
+                    atomicAdd((unsigned long long *)&trep[ti[q]], (unsigned long long)qv[q]);
+        } else {
+            int ic[4];
+            corners_rs(dring[k], dspoke[k], na, ic);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (qv[q] != 0) deposit_global(a.mfx, nz, a.mesh, dzone[k], ic[q], qv[q]);
+        }
+    }
+    __syncthreads();
+    // flush: one global atomic per touched cell
+    for (int row = 0; row < th && use_tile; row++) {
+        const int ring = r0 + row;
+        for (int col = tid; col < tw; col += GH_BLOCK) {
+            const int j = row * tw + col;
+            long long v = 0;
+#pragma unroll
+            for (int rep = 0; rep < TILE_REP; rep++) v += tile[rep][j];
+            if (v != 0) {
+                int spoke = sref + d0 + col;
+                if (spoke < 0) spoke += na;
+                else if (spoke >= na) spoke -= na;
+                if (spoke >= na) spoke -= na;
+                atomicAdd((unsigned long long *)&a.mfx[zt - 1][ring * na + spoke],
+                          (unsigned long long)v);
+            }
+        }
+    }
+    if (tid < nz && s_kr[tid] >= 0) {
         atomicMin(&a.jrad[tid], s_jr[tid] + 1);
         int k = s_kr[tid] + 2;
         atomicMax(&a.krad[tid], k < nr ? k : nr);
@@ -252,34 +349,73 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
     }
 }
 
-// masset / massgn for all on-grid particles (no push): the deposit half of
-// push_kernel, through global atomics.
+// masset / massgn for every on-grid particle without a push: also fills the
+// particles' (u, t) so the next push finds them.  Global atomics after the
+// wavefront reduction.
 template <typename R>
 __global__ __launch_bounds__(GH_BLOCK) void massgn_kernel(PushArgs<R> a) {
+    typedef typename Vec4<R>::T R4;
+    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ];
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid < GH_MAXZ) {
+        s_jr[tid] = INT_MAX;
+        s_kr[tid] = -1;
+    }
+    __syncthreads();
+    long long i = (long long)blockIdx.x * GH_BLOCK + tid;
+    int z = i < a.n ? a.zone[i] : 0;
+    bool on = z >= 1 && z <= a.nz;
+    R w[4];
+    int ring = 0, spoke = 0;
+    if (on) {
+        R rr, u, t;
+        const R4 st = a.xv[i];
+        (void)weight1<R>(st.x, st.y, a.nr, a.k, false, a.thmax, w, rr, u, t);
+        a.ut[i] = mk2(u, t);
+        R tf = a.nsect > 1 ? (R)fmod(t, a.thmax) : t;
+        cic(u, tf, a.nr, a.na, ring, spoke, w);
+        atomicMin(&s_jr[z - 1], ring);
+        atomicMax(&s_kr[z - 1], ring);
+    }
+    __syncthreads();
+    // one global min/max per zone and block (ring ranges are 1-based)
+    if (tid < a.nz && s_kr[tid] >= 0) {
+        atomicMin(&a.jrad[tid], s_jr[tid] + 1);
+        int k = s_kr[tid] + 2;
+        atomicMax(&a.krad[tid], k < a.nr ? k : a.nr);
+    }
+    const int key = on ? ((z - 1) * (a.nr + 1) + ring) * a.na + spoke : -1;
+    long long qv[4];
+    for (int q = 0; q < 4; q++) qv[q] = on ? quant<R>(w[q], a.qbits) : 0;
+    unsigned long long pend = __ballot(key >= 0);
+    while (pend) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const int lk = __shfl(key, leader, 64);
+        const bool mine = key == lk;
+        const unsigned long long mm = __ballot(mine);
+        long long s[4];
+        for (int q = 0; q < 4; q++) s[q] = wave_sum64(mine ? qv[q] : 0);
+        if (lane == leader) {
+            int ic[4];
+            corners_rs(ring, spoke, a.na, ic);
+            for (int q = 0; q < 4; q++)
+                if (s[q] != 0) deposit_global(a.mfx, a.nz, a.mesh, z, ic[q], s[q]);
+        }
+        pend &= ~mm;
+    }
+}
+
+// (u, t) of every particle's position (after a load)
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void ut_kernel(PushArgs<R> a) {
+    typedef typename Vec4<R>::T R4;
     long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
     if (i >= a.n) return;
-    int z = a.zone[i];
-    if (z < 1 || z > a.nz) return;
-    R w[4], rr;
-    int c = weight1<R>(a.x[i], a.y[i], a.nr, a.na, a.alpha, a.nsect > 1,
-                       a.thmax, w, rr);
-    int ring = c / a.na;
-    atomicMin(&a.jrad[z - 1], ring + 1);
-    int k = ring + 2;
-    atomicMax(&a.krad[z - 1], k < a.nr ? k : a.nr);
-    int ic[4];
-    corners(c, a.na, ic);
-    const long long total = (long long)a.nz * a.mesh;
-    for (int q = 0; q < 4; q++) {
-        long long v = quant<R>(w[q], a.qbits);
-        if (v == 0) continue;
-        long long f = (long long)(z - 1) * a.mesh + ic[q];
-        if (f < total) {
-            int zf = (int)(f / a.mesh);
-            int cf = (int)(f - (long long)zf * a.mesh);
-            atomicAdd((unsigned long long *)&a.mfx[zf][cf], (unsigned long long)v);
-        }
-    }
+    const R4 st = a.xv[i];
+    R x = st.x, y = st.y;
+    R rr = sqrt(x * x + y * y), u, t;
+    ut_of(x, y, rr, a.k, u, t);
+    a.ut[i] = mk2(u, t);
 }
 
 // weight(jst, old) and grdacc(jst) for arbitrary positions.
@@ -288,16 +424,16 @@ __global__ __launch_bounds__(GH_BLOCK) void gather_kernel(GatherArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
     if (i >= a.n) return;
-    R w[4], rr;
-    int c = weight1<R>(a.x[i], a.y[i], a.nr, a.na, a.alpha, a.fold != 0,
-                       a.thmax, w, rr);
+    R w[4], rr, u, t;
+    int c = weight1<R>(a.x[i], a.y[i], a.nr, a.k, a.fold != 0, a.thmax, w, rr,
+                       u, t);
     if (a.cell) a.cell[i] = c;
     if (a.wt)
         for (int q = 0; q < 4; q++) a.wt[4 * i + q] = w[q];
     if (a.acc) {
         int ic[4];
-        corners(c, a.na, ic);
-        int mesh = a.nr * a.na;
+        corners_rs(c / a.k.na, c % a.k.na, a.k.na, ic);
+        int mesh = a.nr * a.k.na;
         R ax = 0, ay = 0;
         for (int q = 0; q < 4; q++) {
             R2 f = ic[q] < mesh ? a.fld[ic[q]] : zero2<R>();
@@ -309,30 +445,53 @@ __global__ __launch_bounds__(GH_BLOCK) void gather_kernel(GatherArgs<R> a) {
     }
 }
 
-// sort key: cell of the (unfolded) position; off-grid particles last.
+// sort key: the cell of the particle's (u, t) shifted right by kshift so
+// that keys fit 16 bits (two radix passes); off-grid particles last.
 template <typename R>
 __global__ __launch_bounds__(GH_BLOCK) void key_kernel(
-    const R *__restrict__ x, const R *__restrict__ y,
+    const typename Vec2<R>::T *__restrict__ ut,
     const uint8_t *__restrict__ zone, long long n, int nr, int na, int nz,
-    R alpha, unsigned *__restrict__ key, unsigned *__restrict__ idx) {
+    int kshift, unsigned kmax, unsigned *__restrict__ key,
+    unsigned *__restrict__ idx) {
     long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
     if (i >= n) return;
-    unsigned k = (unsigned)(nr * na);
+    unsigned k = kmax;
     int z = zone[i];
     if (z >= 1 && z <= nz) {
-        R w[4], rr;
-        k = (unsigned)weight1<R>(x[i], y[i], nr, na, alpha, false, (R)na, w, rr);
+        R w[4];
+        int ring, spoke;
+        typename Vec2<R>::T c = ut[i];
+        cic(c.x, c.y, nr, na, ring, spoke, w);
+        k = (unsigned)(ring * na + spoke) >> kshift;
+        if (k >= kmax) k = kmax - 1;
     }
     key[i] = k;
     idx[i] = (unsigned)i;
 }
 
-template <typename T>
-__global__ __launch_bounds__(GH_BLOCK) void permute_kernel(
-    const T *__restrict__ in, T *__restrict__ out,
-    const unsigned *__restrict__ perm, long long n) {
+// one pass moving every particle array to sorted order
+template <typename R> struct PermArgs {
+    const typename Vec4<R>::T *xv_in;
+    typename Vec4<R>::T *xv_out;
+    const typename Vec2<R>::T *ut_in;
+    typename Vec2<R>::T *ut_out;
+    const uint8_t *zin;
+    uint8_t *zout;
+    const unsigned *iin;
+    unsigned *iout;
+    const unsigned *perm;
+    long long n;
+};
+
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void permute_all_kernel(PermArgs<R> p) {
     long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
-    if (i < n) out[i] = in[perm[i]];
+    if (i >= p.n) return;
+    unsigned s = p.perm[i];
+    p.xv_out[i] = p.xv_in[s];
+    p.ut_out[i] = p.ut_in[s];
+    p.zout[i] = p.zin[s];
+    p.iout[i] = p.iin[s];
 }
 
 // ---- host launchers ---------------------------------------------------------
@@ -358,6 +517,14 @@ hipError_t launch_massgn(const PushArgs<R> &a, hipStream_t s) {
 }
 
 template <typename R>
+hipError_t launch_ut(const PushArgs<R> &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ut_kernel<R>, dim3(nblocks(a.n, GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_gather(const GatherArgs<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(gather_kernel<R>, dim3(nblocks(a.n, GH_BLOCK)),
@@ -366,49 +533,35 @@ hipError_t launch_gather(const GatherArgs<R> &a, hipStream_t s) {
 }
 
 template <typename R>
-hipError_t launch_key(const R *x, const R *y, const uint8_t *zone, long long n,
-                      int nr, int na, int nz, R alpha, unsigned *key,
-                      unsigned *idx, hipStream_t s) {
+hipError_t launch_key(const typename Vec2<R>::T *ut, const uint8_t *zone,
+                      long long n, int nr, int na, int nz, int kshift,
+                      unsigned kmax, unsigned *key, unsigned *idx,
+                      hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(key_kernel<R>, dim3(nblocks(n, GH_BLOCK)), dim3(GH_BLOCK),
-                       0, s, x, y, zone, n, nr, na, nz, alpha, key, idx);
+                       0, s, ut, zone, n, nr, na, nz, kshift, kmax, key, idx);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_permute(const T *in, T *out, const unsigned *perm,
-                          long long n, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(permute_kernel<T>, dim3(nblocks(n, GH_BLOCK)),
-                       dim3(GH_BLOCK), 0, s, in, out, perm, n);
+template <typename R>
+hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
+    if (p.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(permute_all_kernel<R>, dim3(nblocks(p.n, GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, p);
     return hipGetLastError();
 }
 
-template hipError_t launch_push<float>(const PushArgs<float> &, hipStream_t);
-template hipError_t launch_push<double>(const PushArgs<double> &, hipStream_t);
-template hipError_t launch_massgn<float>(const PushArgs<float> &, hipStream_t);
-template hipError_t launch_massgn<double>(const PushArgs<double> &, hipStream_t);
-template hipError_t launch_gather<float>(const GatherArgs<float> &, hipStream_t);
-template hipError_t launch_gather<double>(const GatherArgs<double> &, hipStream_t);
-template hipError_t launch_key<float>(const float *, const float *,
-                                      const uint8_t *, long long, int, int,
-                                      int, float, unsigned *, unsigned *,
-                                      hipStream_t);
-template hipError_t launch_key<double>(const double *, const double *,
-                                       const uint8_t *, long long, int, int,
-                                       int, double, unsigned *, unsigned *,
-                                       hipStream_t);
-template hipError_t launch_permute<float>(const float *, float *,
-                                          const unsigned *, long long,
-                                          hipStream_t);
-template hipError_t launch_permute<double>(const double *, double *,
-                                           const unsigned *, long long,
-                                           hipStream_t);
-template hipError_t launch_permute<uint8_t>(const uint8_t *, uint8_t *,
-                                            const unsigned *, long long,
-                                            hipStream_t);
-template hipError_t launch_permute<unsigned>(const unsigned *, unsigned *,
-                                             const unsigned *, long long,
-                                             hipStream_t);
+#define GH_INST(R)                                                            \
+    template hipError_t launch_push<R>(const PushArgs<R> &, hipStream_t);     \
+    template hipError_t launch_massgn<R>(const PushArgs<R> &, hipStream_t);   \
+    template hipError_t launch_ut<R>(const PushArgs<R> &, hipStream_t);       \
+    template hipError_t launch_gather<R>(const GatherArgs<R> &, hipStream_t); \
+    template hipError_t launch_key<R>(const typename Vec2<R>::T *,           \
+                                      const uint8_t *, long long, int, int,   \
+                                      int, int, unsigned, unsigned *,         \
+                                      unsigned *, hipStream_t);               \
+    template hipError_t launch_permute_all<R>(const PermArgs<R> &, hipStream_t);
+GH_INST(float)
+GH_INST(double)
 
 }  // namespace gh

@@ -72,60 +72,64 @@ __device__ bool solve_ranges(const int *jrad, const int *krad, int nz, int nr,
     return true;
 }
 
+// One block of SOLVE_BLOCK threads (16 wavefronts) per ring: the dot
+// products over rings and spokes are short, so many wavefronts per CU are
+// what hides the L2 latency of their loads.
+constexpr int SOLVE_BLOCK = 1024;
+constexpr int ANA_SPLIT = 1;   // blocks per ring in the analysis (harmonics)
+constexpr int SYN_SPLIT = 1;   // blocks per ring in the synthesis (spokes)
+
+// grid (nr, ANA_SPLIT): block (ir, h) forms the ring's combined mass and
+// the harmonics m = h, h + ANA_SPLIT, ...
 template <typename R>
-__global__ __launch_bounds__(GH_BLOCK) void analyse_kernel(AnaArgs<R> a) {
+__global__ __launch_bounds__(SOLVE_BLOCK) void analyse_kernel(AnaArgs<R> a) {
     __shared__ R rho_s[MAX_NA];
+    __shared__ R tws[2 * MAX_NA];
     __shared__ int s_rng[4], s_ok;
     const int tid = threadIdx.x, na = a.na, ir = blockIdx.x + 1;
+    const int h = blockIdx.y;
     if (tid == 0) {
         int kr[GH_MAXZ], rg[4] = {0, 0, 0, 0};
         bool ok = solve_ranges(a.jrad, a.krad, a.nz, a.nr, a.mzone, a.lpot,
                                a.explicit_zones, a.jzone, a.kzone, kr, rg);
         s_ok = ok;
         for (int q = 0; q < 4; q++) s_rng[q] = rg[q];
-        if (blockIdx.x == 0) {
+        if (blockIdx.x == 0 && h == 0) {
             for (int q = 0; q < 4; q++) a.ranges[q] = ok ? rg[q] : -1;
             if (ok)
                 for (int z = 1; z < a.nz; z++) a.krad[z] = kr[z];
         }
     }
     __syncthreads();
-    // zoncmb + scaled for this ring, then the fused masscl
+    // zoncmb + scaled for this ring
     const size_t row = (size_t)(ir - 1) * na;
-    for (int j = tid; j < na; j += GH_BLOCK) {
+    for (int j = tid; j < na; j += SOLVE_BLOCK) {
         const size_t idx = row + j;
-        long long m0 = a.mfx[0][idx];
-        double acc = (double)m0 * a.q;
-        long long mz[GH_MAXZ];
-        mz[0] = m0;
+        double acc = (double)a.mfx[0][idx] * a.q;
         for (int z = 2; z <= a.nz; z++) {
-            mz[z - 1] = a.mfx[z - 1][idx];
             if (a.use_new[z - 1])
-                acc = acc + a.fnew[z - 1] * ((double)mz[z - 1] * a.q);
+                acc = acc + a.fnew[z - 1] * ((double)a.mfx[z - 1][idx] * a.q);
             if (a.use_old[z - 1])
                 acc = acc + a.fold[z - 1] * ((double)a.moz[z - 1][idx] * a.q);
         }
         R v = (R)(acc * a.pmass);
         rho_s[j] = v;
-        a.rho[idx] = v;
-        for (int z = 1; z <= a.masscl_z; z++) {
-            if (z >= 2) a.moz_w[z - 1][idx] = mz[z - 1];
-            a.mfx_w[z - 1][idx] = 0;
-        }
+        if (h == 0) a.rho[idx] = v;
     }
+    for (int j = tid; j < 2 * na; j += SOLVE_BLOCK) tws[j] = a.tw[j];
     __syncthreads();
     if (!s_ok || ir < s_rng[0] || ir > s_rng[1]) return;
     // forward transform of the active harmonics: one wave per harmonic
-    const int wave = tid >> 6, lane = tid & 63, nw = GH_BLOCK / 64;
-    for (int m = wave; m < a.ng; m += nw) {
+    const int wave = tid >> 6, lane = tid & 63, nw = SOLVE_BLOCK / 64;
+    for (int m = h + ANA_SPLIT * wave; m < a.ng; m += ANA_SPLIT * nw) {
         R sa = 0, sb = 0;
         if (!a.lg[m]) {
             int k = (m * lane) % na;
             const int step = (m * 64) % na;
             for (int j = lane; j < na; j += 64) {
                 R rv = rho_s[j];
-                sa += rv * a.tw[2 * k];
-                sb += rv * a.tw[2 * k + 1];
+                sa += rv * tws[2 * k];
+                sb += rv * tws[2 * k + 1];
                 k += step;
                 if (k >= na) k -= na;
             }
@@ -145,16 +149,37 @@ __global__ __launch_bounds__(GH_BLOCK) void analyse_kernel(AnaArgs<R> a) {
     }
 }
 
+// grid (nr, SYN_SPLIT): block (ir, q) forms the ring's contraction (small)
+// and synthesises spokes [q na/SYN_SPLIT, (q+1) na/SYN_SPLIT).
 template <typename R>
-__global__ __launch_bounds__(GH_BLOCK) void synth_kernel(SynArgs<R> a) {
+__global__ __launch_bounds__(SOLVE_BLOCK) void synth_kernel(SynArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     __shared__ R W[3][2][MAX_NG];
+    __shared__ R tws[2 * MAX_NA];
     const int tid = threadIdx.x, na = a.na, nr = a.nr, ir = blockIdx.x + 1;
+    const int q = blockIdx.y;
+    const int jlo = (q * na) / SYN_SPLIT, jhi = ((q + 1) * na) / SYN_SPLIT;
+    // masscl of the coming push, on this block's cells
+    if (a.masscl_z > 0) {
+        const size_t row = (size_t)(ir - 1) * na;
+        for (int j = jlo + tid; j < jhi; j += SOLVE_BLOCK) {
+            const size_t idx = row + j;
+            for (int z = 1; z <= a.masscl_z; z++) {
+                if (z >= 2) a.moz[z - 1][idx] = a.mfx[z - 1][idx];
+                a.mfx[z - 1][idx] = 0;
+            }
+        }
+        if (blockIdx.x == 0 && q == 0 && tid < a.masscl_z) {
+            a.jrad[tid] = nr;
+            a.krad[tid] = 0;
+        }
+    }
     const int jrs = a.ranges[0], krs = a.ranges[1];
     const int jrf = a.ranges[2], krf = a.ranges[3];
     if (jrs < 1 || ir < jrf || ir > krf) return;
+    for (int j = tid; j < 2 * na; j += SOLVE_BLOCK) tws[j] = a.tw[j];
     const int ntypes = a.lpot ? 3 : 2;
-    const int wave = tid >> 6, lane = tid & 63, nw = GH_BLOCK / 64;
+    const int wave = tid >> 6, lane = tid & 63, nw = SOLVE_BLOCK / 64;
     const int mmax = a.mmax;
     const int maz = min(mmax - 2, a.ng - 1);
     for (int task = wave; task < ntypes * a.ng; task += nw) {
@@ -190,15 +215,14 @@ __global__ __launch_bounds__(GH_BLOCK) void synth_kernel(SynArgs<R> a) {
     }
     __syncthreads();
     const R inv = (R)1 / (R)na;
-    for (int j = tid; j < na; j += GH_BLOCK) {
+    for (int j = jlo + tid; j < jhi; j += SOLVE_BLOCK) {
         R F[3] = {0, 0, 0};
         for (int type = 0; type < ntypes; type++) {
             R s = W[type][0][0];
-            int k = j % na;
+            int k = j;
             for (int m = 1; m < a.ng; m++) {
                 if (2 * m < na) {
-                    R c = a.tw[2 * k], sn = a.tw[2 * k + 1];
-                    s += (R)2 * (W[type][0][m] * c - W[type][1][m] * sn);
+                    s += (R)2 * (W[type][0][m] * tws[2 * k] - W[type][1][m] * tws[2 * k + 1]);
                 } else {
                     s += (j & 1) ? -W[type][0][m] : W[type][0][m];
                 }
@@ -207,7 +231,7 @@ __global__ __launch_bounds__(GH_BLOCK) void synth_kernel(SynArgs<R> a) {
             }
             F[type] = s * inv;
         }
-        const R cp = a.tw[2 * j], sp = a.tw[2 * j + 1];
+        const R cp = tws[2 * j], sp = tws[2 * j + 1];
         R2 out;
         out.x = F[0] * cp - F[1] * sp;
         out.y = F[0] * sp + F[1] * cp;
@@ -327,14 +351,14 @@ __global__ void masscl_kernel(ZonePtrs zp, int mz, long long mesh, int *jrad,
 template <typename R>
 hipError_t launch_analyse(const AnaArgs<R> &a, hipStream_t s) {
     if (a.na > MAX_NA) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(analyse_kernel<R>, dim3(a.nr), dim3(GH_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(analyse_kernel<R>, dim3(a.nr, ANA_SPLIT), dim3(SOLVE_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_synth(const SynArgs<R> &a, hipStream_t s) {
     if (a.ng > MAX_NG) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(synth_kernel<R>, dim3(a.nr), dim3(GH_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(synth_kernel<R>, dim3(a.nr, SYN_SPLIT), dim3(SOLVE_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

@@ -13,10 +13,15 @@
 /* 2.d0 * pi of SRC/lib15/inc/pi.f */
 #define TWO_PI_D (2.0 * 3.141592653589793238462643)
 
-/* ---------------- REAL-templated particle routines ---------------- */
+/* ---------------- REAL-templated particle routines ----------------
+ * The FP32 logarithm of uofgr is the correctly rounded logf, taken through
+ * the double logarithm: the reference's logf comes from whatever libm its
+ * compiler links (unpinned; glibc's logf is within 0.82 ulp, not correctly
+ * rounded), so the oracle fixes the one value every libm aims at. */
+static inline float logf_cr(float v) { return (float)log((double)v); }
 #define REAL float
 #define SFX(name) name##_f
-#define RLOG logf
+#define RLOG logf_cr
 #define REXP expf
 #define RSQRT sqrtf
 #define RFMOD fmodf
@@ -42,11 +47,13 @@
 float orc_alpha_f(int na) { return (float)(TWO_PI_D / (double)na); }
 double orc_alpha_d(int na) { return TWO_PI_D / (double)na; }
 
-/* Cubic Hermite interpolation of a tabulated radial force. */
+/* Cubic Hermite interpolation of a tabulated radial force on r_k = k dr,
+ * with s = r * (1/dr). */
 double orc_fixforce(double r, int nfr, double dr, const double *val,
                     const double *der)
 {
-    double s = r / dr;
+    double inv_dr = 1.0 / dr;
+    double s = r * inv_dr;
     int k = (int)s;
     if (k >= nfr - 1) return val[nfr - 1];
     if (k < 0) k = 0;
