@@ -103,18 +103,38 @@ def cpu_baseline(w, n, steps):
                        % (w.name, n, steps, pushes, dt))
 
 
+def copy_calibration(nbytes):
+    """A plain elementwise copy of known size (FP32 x 1.0 into a second
+    buffer: nbytes read, nbytes written), run three times; its FETCH_SIZE and
+    WRITE_SIZE per dispatch scale the same counters of the push kernel."""
+    import torch
+    n = nbytes // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 4 * n, "dispatches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=None, help="particles (default 10M)")
-    ap.add_argument("--sort-interval", type=int, default=20)
+    ap.add_argument("--sort-interval", type=int, default=10)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
-    ap.add_argument("--cpu-n", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-n", type=int, default=2_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain copy kernel of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the step's "
               "multi-GPU form is a design, not measured); use --gpus 1",

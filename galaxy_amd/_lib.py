@@ -73,11 +73,14 @@ SIGNATURES = {
 _lib = None
 
 
-def load(path: str = LIBPATH) -> C.CDLL:
-    """Load libgalhip.so once; raises if it has not been built."""
+def load(path: str = None) -> C.CDLL:
+    """Load libgalhip.so once; raises if it has not been built.  The
+    environment variable GALHIP_LIBRARY selects another build of the same
+    library (for launch-shape tuning)."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("GALHIP_LIBRARY") or LIBPATH
     if not os.path.exists(path):
         raise RuntimeError(
             "libgalhip.so not found at %s: build it with "

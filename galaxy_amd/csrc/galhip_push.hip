@@ -34,9 +34,19 @@
 
 namespace gh {
 
-constexpr int PPT = 4;            // particles per thread
-constexpr int TILE_CAP = 1024;    // LDS tile, cells of a ring x spoke window
-constexpr int TILE_REP = 4;       // replicas (lane & 3): 4 x 8 KiB of int64
+// Launch shape (tuned on MI355X; see the DESIGN notes of the deposit):
+#ifndef GH_PPT
+#define GH_PPT 4
+#endif
+#ifndef GH_TILE_CAP
+#define GH_TILE_CAP 1024
+#endif
+#ifndef GH_TILE_REP
+#define GH_TILE_REP 2
+#endif
+constexpr int PPT = GH_PPT;            // particles per thread
+constexpr int TILE_CAP = GH_TILE_CAP;  // LDS tile, cells of a ring x spoke window
+constexpr int TILE_REP = GH_TILE_REP;  // replicas (lane & (REP-1)) against conflicts
 
 template <typename R>
 __device__ __forceinline__ R tsfac(const PushArgs<R> &a, int iz) {

@@ -123,27 +123,32 @@ void orc_rfftb(int n, const double *x, double *y)
 
 /* DCT-I of n points (rcost, stest.f:169-195):
  * y_i = x_0 + (-1)^i x_{n-1} + 2 sum_{k=1}^{n-2} x_k cos(pi k i/(n-1)) */
-void orc_cost(int n, const double *x, double *y)
+/* first nout outputs of the DCT-I below */
+static void cost_n(int n, const double *x, double *y, int nout)
 {
     int m = n - 1;
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < nout; i++) {
         double s = x[0] + (i % 2 ? -x[m] : x[m]);
         for (int k = 1; k < m; k++) s += 2.0 * x[k] * cos2pi((long)k * i, 2 * m);
         y[i] = s;
     }
 }
 
+void orc_cost(int n, const double *x, double *y) { cost_n(n, x, y, n); }
+
 /* DST-I of n points (rsint, stest.f:138-165):
  * y_i = 2 sum_{k=1}^{n} x_k sin(pi k i/(n+1)), i = 1..n */
-void orc_sint(int n, const double *x, double *y)
+static void sint_n(int n, const double *x, double *y, int nout)
 {
-    for (int i = 1; i <= n; i++) {
+    for (int i = 1; i <= nout; i++) {
         double s = 0.0;
         for (int k = 1; k <= n; k++)
             s += 2.0 * x[k - 1] * sin2pi((long)k * i, 2 * (n + 1));
         y[i - 1] = s;
     }
 }
+
+void orc_sint(int n, const double *x, double *y) { sint_n(n, x, y, n); }
 
 /* ---------------- Green's function kernels (FP64) ---------------- */
 
@@ -217,8 +222,12 @@ void orc_green(int nr, int na, int ng, int nsect, double alpha, double softl,
                                          : grnfun(rs, rf, th, softl, tsoft);
                     f[it - it1] = v;
                 }
-                if (azim) orc_sint(npt, f, g);
-                else orc_cost(npt, f, g);
+                /* without the direct kernels only the harmonics m < ng are
+                 * needed: the rest are filtered out below */
+                int nout = D ? npt : (ng - (azim ? 1 : 0) < npt ? ng - (azim ? 1 : 0) : npt);
+                for (int q = nout; q < npt; q++) g[q] = 0.0;
+                if (azim) sint_n(npt, f, g, nout);
+                else cost_n(npt, f, g, nout);
                 /* filter: harmonic m = it - 1 kept iff m < ng and m mod nsect == 0 */
                 for (int it = it1; it <= it2; it++) {
                     int m = it - 1;
