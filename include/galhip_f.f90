@@ -1,0 +1,204 @@
+! galhip_f.f90 -- Fortran 2003 (ISO_C_BINDING) interfaces to libgalhip.so,
+! one for each entry point of galhip.h, for GALAXY's Fortran host code.
+!
+! The reference swaps implementations of its subroutines by link order
+! (SRC/progs/Makefile:22,25,55 put lib15_mpi ahead of main15.a).  A shim that
+! replaces findf, step and masset (SRC/lib15/findf.f:1, step.f:1,
+! masset.f:1) uses this module and links ahead of main15.a with
+! -L<dir> -lgalhip.  Floating point arguments are passed as type(c_ptr)
+! (c_loc of a real or real*8 array), because their kind follows the
+! context's precision.
+module galhip_f
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: GALHIP_FP32 = 4, GALHIP_FP64 = 8
+  integer(c_int32_t), parameter :: GALHIP_MAXZONES = 10, GALHIP_MAXSKIP = 16
+  integer(c_int), parameter :: GALHIP_OK = 0, GALHIP_EINVAL = -1, &
+       GALHIP_EHIP = -2, GALHIP_ERANGE = -3
+
+  type, bind(c) :: galhip_p2d_params
+     integer(c_int32_t) :: nr, na, hash, nsect, nskip
+     integer(c_int32_t) :: skip(GALHIP_MAXSKIP)
+     real(c_double) :: softl
+     integer(c_int32_t) :: tsoft, nzones
+     integer(c_int32_t) :: nstep(GALHIP_MAXZONES)
+     real(c_double) :: zbr(GALHIP_MAXZONES)
+     integer(c_int32_t) :: offanal
+     real(c_double) :: pmass
+     integer(c_int32_t) :: precision
+  end type galhip_p2d_params
+
+  interface
+     integer(c_int) function galhip_p2dset(p, ctx) bind(c, name='galhip_p2dset')
+       import :: c_int, c_ptr, galhip_p2d_params
+       type(galhip_p2d_params), intent(in) :: p
+       type(c_ptr), intent(out) :: ctx
+     end function galhip_p2dset
+     integer(c_int) function galhip_free(ctx) bind(c, name='galhip_free')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function galhip_free
+     type(c_ptr) function galhip_last_error() bind(c, name='galhip_last_error')
+       import :: c_ptr
+     end function galhip_last_error
+     integer(c_int) function galhip_geometry(ctx, out) bind(c, name='galhip_geometry')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out(*)
+     end function galhip_geometry
+     integer(c_int) function galhip_greenm(ctx) bind(c, name='galhip_greenm')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function galhip_greenm
+     integer(c_int) function galhip_get_green(ctx, g) bind(c, name='galhip_get_green')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: g(*)
+     end function galhip_get_green
+     integer(c_int) function galhip_set_green(ctx, g) bind(c, name='galhip_set_green')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: g(*)
+     end function galhip_set_green
+     integer(c_int) function galhip_set_fixforce(ctx, n, dr, val, der) &
+          bind(c, name='galhip_set_fixforce')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       real(c_double), value :: dr
+       real(c_double), intent(in) :: val(*), der(*)
+     end function galhip_set_fixforce
+     integer(c_int) function galhip_restrt(ctx, n, x, y, vx, vy, iz) &
+          bind(c, name='galhip_restrt')
+       import :: c_int, c_ptr, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y, vx, vy
+       integer(c_int32_t), intent(in) :: iz(*)
+     end function galhip_restrt
+     integer(c_int) function galhip_unload(ctx, x, y, vx, vy, iz) &
+          bind(c, name='galhip_unload')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: x, y, vx, vy, iz
+     end function galhip_unload
+     integer(c_int) function galhip_set_step(ctx, istep, lstep) &
+          bind(c, name='galhip_set_step')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: istep
+       type(c_ptr), value :: lstep
+     end function galhip_set_step
+     integer(c_int) function galhip_get_step(ctx, istep, mzone, lstep) &
+          bind(c, name='galhip_get_step')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: istep, mzone
+       type(c_ptr), value :: lstep
+     end function galhip_get_step
+     integer(c_int) function galhip_masset(ctx) bind(c, name='galhip_masset')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function galhip_masset
+     integer(c_int) function galhip_findf(ctx, lpot) bind(c, name='galhip_findf')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: lpot
+     end function galhip_findf
+     integer(c_int) function galhip_p2fndf(ctx, jzone, kzone, lpot) &
+          bind(c, name='galhip_p2fndf')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: jzone, kzone, lpot
+     end function galhip_p2fndf
+     integer(c_int) function galhip_step(ctx) bind(c, name='galhip_step')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function galhip_step
+     integer(c_int) function galhip_run(ctx, nsteps) bind(c, name='galhip_run')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nsteps
+     end function galhip_run
+     integer(c_int) function galhip_weight(ctx, n, x, y, old, cell, wt) &
+          bind(c, name='galhip_weight')
+       import :: c_int, c_ptr, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y
+       integer(c_int32_t), value :: old
+       integer(c_int32_t), intent(out) :: cell(*)
+       type(c_ptr), value :: wt
+     end function galhip_weight
+     integer(c_int) function galhip_grdacc(ctx, n, x, y, acc) &
+          bind(c, name='galhip_grdacc')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y, acc
+     end function galhip_grdacc
+     integer(c_int) function galhip_get_mass(ctx, mfx, mozfx, jrad, krad, qbits) &
+          bind(c, name='galhip_get_mass')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: mfx, mozfx
+       integer(c_int32_t), intent(out) :: jrad(*), krad(*), qbits
+     end function galhip_get_mass
+     integer(c_int) function galhip_set_mass(ctx, mfx, mozfx, jrad, krad) &
+          bind(c, name='galhip_set_mass')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: mfx, mozfx
+       integer(c_int32_t), intent(in) :: jrad(*), krad(*)
+     end function galhip_set_mass
+     integer(c_int) function galhip_get_rho(ctx, rho) bind(c, name='galhip_get_rho')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, rho
+     end function galhip_get_rho
+     integer(c_int) function galhip_get_field(ctx, fld) bind(c, name='galhip_get_field')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, fld
+     end function galhip_get_field
+     integer(c_int) function galhip_set_field(ctx, fld) bind(c, name='galhip_set_field')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, fld
+     end function galhip_set_field
+     integer(c_int) function galhip_get_ranges(ctx, r4) bind(c, name='galhip_get_ranges')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: r4(4)
+     end function galhip_get_ranges
+     integer(c_int) function galhip_get_noff(ctx, noff) bind(c, name='galhip_get_noff')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: noff
+     end function galhip_get_noff
+     integer(c_int) function galhip_set_sort_interval(ctx, k) &
+          bind(c, name='galhip_set_sort_interval')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: k
+     end function galhip_set_sort_interval
+     integer(c_int) function galhip_sort(ctx) bind(c, name='galhip_sort')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function galhip_sort
+     integer(c_int) function galhip_set_profile(ctx, on) bind(c, name='galhip_set_profile')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on
+     end function galhip_set_profile
+     integer(c_int) function galhip_get_profile(ctx, push_ms, launches, pushes) &
+          bind(c, name='galhip_get_profile')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: push_ms
+       integer(c_int64_t), intent(out) :: launches, pushes
+     end function galhip_get_profile
+     integer(c_int) function galhip_sync(ctx) bind(c, name='galhip_sync')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function galhip_sync
+  end interface
+end module galhip_f
