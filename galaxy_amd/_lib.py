@@ -54,6 +54,7 @@ SIGNATURES = {
     "galhip_p2fndf": [_vp, _i32, _i32, _i32],
     "galhip_step": [_vp],
     "galhip_run": [_vp, _i32],
+    "galhip_tmcent": [_vp, _i32],
     "galhip_weight": [_vp, _i64, _vp, _vp, _i32, _vp, _vp],
     "galhip_grdacc": [_vp, _i64, _vp, _vp, _vp],
     "galhip_get_mass": [_vp, _vp, _vp, _vp, _vp, _vp],
@@ -65,6 +66,7 @@ SIGNATURES = {
     "galhip_get_noff": [_vp, _vp],
     "galhip_set_sort_interval": [_vp, _i32],
     "galhip_sort": [_vp],
+    "galhip_get_order": [_vp, _vp],
     "galhip_set_profile": [_vp, _i32],
     "galhip_get_profile": [_vp, _vp, _vp, _vp],
     "galhip_sync": [_vp],

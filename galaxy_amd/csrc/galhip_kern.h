@@ -124,11 +124,15 @@ __device__ __forceinline__ double fixforce(double r, int nfr, double dr,
                                            double inv_dr,
                                            const double *__restrict__ val,
                                            const double *__restrict__ der) {
+    // branch-free, so that the table reads of several particles overlap:
+    // past the table end t = 1 selects val[nfr - 1] exactly (the other
+    // Hermite basis values are then exact zeros)
     double s = r * inv_dr;
     int k = (int)s;
-    if (k >= nfr - 1) return val[nfr - 1];
+    const bool past = k >= nfr - 1;
     if (k < 0) k = 0;
-    double t = s - (double)k;
+    if (past) k = nfr - 2;
+    double t = past ? 1.0 : s - (double)k;
     double t2 = t * t;
     double omt = 1.0 - t;
     double omt2 = omt * omt;
@@ -214,6 +218,22 @@ template <typename R> struct SynArgs {
     long long *moz[GH_MAXZ];
     int *jrad, *krad;
     int masscl_z;
+};
+
+// counting sort of the particle arrays by bin (bin_count / bin_scatter)
+template <typename R> struct BinArgs {
+    const typename Vec4<R>::T *xv_in;
+    typename Vec4<R>::T *xv_out;
+    const typename Vec2<R>::T *ut_in;
+    typename Vec2<R>::T *ut_out;
+    const uint8_t *zin;
+    uint8_t *zout;
+    const unsigned *iin;
+    unsigned *iout;
+    unsigned *ctr;          // counts (count pass) or cursors (scatter pass)
+    long long n;
+    int nr, na, nz, shift;
+    unsigned koff;          // the off-grid bin (= number of cell bins)
 };
 
 }  // namespace gh

@@ -11,7 +11,7 @@
 //   rezgrp/chstep  zone changes (rezgrp.f:30-50, zoneno.f, chstep.f:22-31)
 //   massgn         weights at the new position, scatter into the zone's
 //                  mass array (massgn.f:68-89)
-// in one pass over structure-of-arrays particle storage.
+// in one pass over the particle arrays.
 //
 // The old position's weights are not recomputed: every particle carries the
 // (u, t) grid coordinates that its last mass assignment formed from the
@@ -21,14 +21,15 @@
 //
 // Mass sums are 64-bit fixed point (weight x 2^qbits): integer adds
 // commute, so the sums are bit-identical for any processing order and need
-// no float atomics.  The deposit is reduced in three levels: lanes of a
-// wavefront that target the same cell are summed with shuffles (particles
-// are kept ordered by cell, so a wavefront holds few distinct cells); the
-// wavefront's leader adds the group's four corner sums into an LDS tile of
-// the rings the block touches; the block flushes the tile's non-zero cells
-// with one 64-bit global atomic each.  Global 64-bit atomics are the scarce
-// resource on MI355X (about 24 G/s device-wide) and LDS atomics the next
-// (about 230 G/s), so each level divides the count of the next.
+// no float atomics.  Each block deposits into an LDS tile that covers the
+// rings x spokes window of its particles (particles are kept ordered by
+// cell, so the window is small), with TILE_REP copies chosen by lane so
+// that lanes hitting the same cell are spread over copies; the block then
+// flushes every non-zero tile cell with one 64-bit global atomic.  Blocks
+// whose window exceeds the tile, and deposits into another zone than the
+// tile's, go straight to global atomics.  Global 64-bit atomics are the
+// scarce resource on MI355X (about 24 G/s device-wide), LDS atomics far
+// less so.
 #include "galhip_kern.h"
 #include <climits>
 
@@ -152,6 +153,56 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
     // every thread reaches this barrier; a block with nothing to move exits
     if (!__syncthreads_or(any)) return;
 
+    // The loads of all PPT particles are issued before any is consumed:
+    // state and (u, t) first, then the 4 x PPT field gathers, then the
+    // fixed-force table reads.  These phases are straight-line code (indices
+    // clamped, results selected), so each phase's reads are in flight
+    // together instead of one particle's latency chain after another's.
+    R4 st[PPT];
+    R2 cu[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        long long i = base + k * GH_BLOCK + tid;
+        if (i >= a.n) i = a.n - 1;
+        st[k] = a.xv[i];
+        cu[k] = a.ut[i];
+    }
+    // grdacc: weights of the old position from its (u, t); off-grid and
+    // idle particles read cell 0 and weigh nothing
+    R ax[PPT], ay[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const bool grid = mv[k] && zin[k] != nlists;
+        R w[4];
+        int ring, spoke, ic[4];
+        cic(grid ? cu[k].x : (R)0, grid ? cu[k].y : (R)0, nr, na, ring, spoke, w);
+        corners_rs(ring, spoke, na, ic);
+        R sx = 0, sy = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            // the last ring with t == na points past the field: 0
+            const bool in = ic[q] < a.mesh;
+            R2 f = a.fld[in ? ic[q] : 0];
+            R wq = (grid && in) ? w[q] : (R)0;
+            sx = sx + f.x * wq;
+            sy = sy + f.y * wq;
+        }
+        ax[k] = sx;
+        ay[k] = sy;
+    }
+    R rr[PPT], fr[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        rr[k] = sqrt(st[k].x * st[k].x + st[k].y * st[k].y);
+        fr[k] = 0;
+    }
+    if (a.fixforce) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+            fr[k] = (R)fixforce((double)rr[k], a.nfr, a.fr_dr, a.fr_inv_dr,
+                                a.fr_val, a.fr_der);
+    }
+
     // deposit state kept per particle: ring, spoke, zone (0 = none), weights
     int dring[PPT], dspoke[PPT], dzone[PPT];
     R dw[PPT][4];
@@ -165,42 +216,21 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
             const long long i = base + k * GH_BLOCK + tid;
             const int z0 = zin[k];
             const bool offlist = z0 == nlists;
-            const R4 st = a.xv[i];
-            R ox = st.x, oy = st.y, ovx = st.z, ovy = st.w;
-            R ax = 0, ay = 0;
+            R ox = st[k].x, oy = st[k].y, ovx = st[k].z, ovy = st[k].w;
+            R ax_ = ax[k], ay_ = ay[k];
             my_pushes++;
-            R rr = sqrt(ox * ox + oy * oy);
-            if (!offlist) {
-                // grdacc: weights of the old position from its (u, t)
-                R w[4];
-                int ring, spoke, ic[4];
-                const R2 cu = a.ut[i];
-                cic(cu.x, cu.y, nr, na, ring, spoke, w);
-                corners_rs(ring, spoke, na, ic);
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    // the last ring with t == na points past the field: 0
-                    R2 f = ic[q] < a.mesh ? a.fld[ic[q]] : zero2<R>();
-                    ax = ax + f.x * w[q];
-                    ay = ay + f.y * w[q];
-                }
-                if (a.fixforce) {
-                    R fr = (R)fixforce((double)rr, a.nfr, a.fr_dr, a.fr_inv_dr,
-                                       a.fr_val, a.fr_der);
-                    if (fr != (R)0) {
-                        fr = fr / rr;
-                        ax = ax + fr * ox;
-                        ay = ay + fr * oy;
-                    }
-                }
+            if (!offlist && fr[k] != (R)0) {
+                R f = fr[k] / rr[k];
+                ax_ = ax_ + f * ox;
+                ay_ = ay_ + f * oy;
             }
             if (nz > 1 && z0 > 1) {
                 R t = tsfac(a, z0);
                 t = t * t;
-                ax = t * ax;
-                ay = t * ay;
+                ax_ = t * ax_;
+                ay_ = t * ay_;
             }
-            R nvx = ovx + ax, nvy = ovy + ay;
+            R nvx = ovx + ax_, nvy = ovy + ay_;
             R nx = ox + nvx, ny = oy + nvy;
             int znew = z0;
             R rnew = sqrt(nx * nx + ny * ny);
@@ -211,7 +241,7 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
                     my_dnoff++;
                     if (a.offanal && z0 < nz) {
                         R tfac = tsfac(a, nz) / tsfac(a, z0);
-                        chstep(tfac, ox, oy, ovx, ovy, ax, ay, nx, ny, nvx, nvy);
+                        chstep(tfac, ox, oy, ovx, ovy, ax_, ay_, nx, ny, nvx, nvy);
                         rnew = sqrt(nx * nx + ny * ny);
                     }
                 }
@@ -222,10 +252,10 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
             if (a.mzone > 1 && !offlist && znew <= a.mzone) {
                 int kz = zone_of(a, rnew);
                 if (kz > a.mzone) kz = a.mzone;
-                if (kz > znew) kz = zone_of(a, rr);
+                if (kz > znew) kz = zone_of(a, rr[k]);
                 if (kz != znew) {
                     R tfac = tsfac(a, kz) / tsfac(a, znew);
-                    chstep(tfac, ox, oy, ovx, ovy, ax, ay, nx, ny, nvx, nvy);
+                    chstep(tfac, ox, oy, ovx, ovy, ax_, ay_, nx, ny, nvx, nvy);
                     rnew = sqrt(nx * nx + ny * ny);
                     znew = kz;
                 }
@@ -455,53 +485,182 @@ __global__ __launch_bounds__(GH_BLOCK) void gather_kernel(GatherArgs<R> a) {
     }
 }
 
-// sort key: the cell of the particle's (u, t) shifted right by kshift so
-// that keys fit 16 bits (two radix passes); off-grid particles last.
+// tmcent / tmcgrp for the retarded-velocity leap-frog (tmcent.f,
+// tmcgrp.f:26-36): every particle's velocity minus fac times its getacc
+// acceleration (grid interpolation from its cached (u, t), fixed radial
+// force, zone rescaling; none off the grid), in the push's operation order.
 template <typename R>
-__global__ __launch_bounds__(GH_BLOCK) void key_kernel(
-    const typename Vec2<R>::T *__restrict__ ut,
-    const uint8_t *__restrict__ zone, long long n, int nr, int na, int nz,
-    int kshift, unsigned kmax, unsigned *__restrict__ key,
-    unsigned *__restrict__ idx) {
-    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    unsigned k = kmax;
-    int z = zone[i];
-    if (z >= 1 && z <= nz) {
+__global__ __launch_bounds__(GH_BLOCK) void tmcent_kernel(PushArgs<R> a, R fac) {
+    typedef typename Vec2<R>::T R2;
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const int z0 = a.zone[i];
+    typename Vec4<R>::T st = a.xv[i];
+    R ax = 0, ay = 0;
+    if (z0 != a.nz + 1) {
         R w[4];
-        int ring, spoke;
-        typename Vec2<R>::T c = ut[i];
-        cic(c.x, c.y, nr, na, ring, spoke, w);
-        k = (unsigned)(ring * na + spoke) >> kshift;
-        if (k >= kmax) k = kmax - 1;
+        int ring, spoke, ic[4];
+        const R2 cu = a.ut[i];
+        cic(cu.x, cu.y, a.nr, a.na, ring, spoke, w);
+        corners_rs(ring, spoke, a.na, ic);
+        for (int q = 0; q < 4; q++) {
+            R2 f = ic[q] < a.mesh ? a.fld[ic[q]] : zero2<R>();
+            ax = ax + f.x * w[q];
+            ay = ay + f.y * w[q];
+        }
+        if (a.fixforce) {
+            const R rr = sqrt(st.x * st.x + st.y * st.y);
+            R fr = (R)fixforce((double)rr, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val,
+                               a.fr_der);
+            if (fr != (R)0) {
+                fr = fr / rr;
+                ax = ax + fr * st.x;
+                ay = ay + fr * st.y;
+            }
+        }
     }
-    key[i] = k;
-    idx[i] = (unsigned)i;
+    if (a.nz > 1 && z0 > 1) {
+        R t = tsfac(a, z0);
+        t = t * t;
+        ax = t * ax;
+        ay = t * ay;
+    }
+    st.z = st.z - fac * ax;
+    st.w = st.w - fac * ay;
+    a.xv[i] = st;
 }
 
-// one pass moving every particle array to sorted order
-template <typename R> struct PermArgs {
-    const typename Vec4<R>::T *xv_in;
-    typename Vec4<R>::T *xv_out;
-    const typename Vec2<R>::T *ut_in;
-    typename Vec2<R>::T *ut_out;
-    const uint8_t *zin;
-    uint8_t *zout;
-    const unsigned *iin;
-    unsigned *iout;
-    const unsigned *perm;
-    long long n;
-};
+// ---- bin sort --------------------------------------------------------------
+//
+// Particles are kept grouped by cell only for locality (small deposit
+// tiles, cached field reads).  No result depends on their order inside a
+// cell: each particle's step is its own and the mass sums are integer sums.
+// So a counting sort does the regrouping: a pass that counts the particles
+// of every bin, an exclusive scan of the counts, and a pass that claims a
+// slot in the particle's bin atomically and moves all its arrays there.
+// The order inside a bin is then arbitrary.  Bin = cell >> shift of the
+// particle's (u, t) for zones 1..nz; off-grid particles share the last bin.
 
 template <typename R>
-__global__ __launch_bounds__(GH_BLOCK) void permute_all_kernel(PermArgs<R> p) {
-    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
-    if (i >= p.n) return;
-    unsigned s = p.perm[i];
-    p.xv_out[i] = p.xv_in[s];
-    p.ut_out[i] = p.ut_in[s];
-    p.zout[i] = p.zin[s];
-    p.iout[i] = p.iin[s];
+__device__ __forceinline__ unsigned bin_of(const BinArgs<R> &b, long long i) {
+    const int z = b.zin[i];
+    if (z < 1 || z > b.nz) return b.koff;
+    R w[4];
+    int ring, spoke;
+    const typename Vec2<R>::T c = b.ut_in[i];
+    cic(c.x, c.y, b.nr, b.na, ring, spoke, w);
+    unsigned k = (unsigned)(ring * b.na + spoke) >> b.shift;
+    return k < b.koff ? k : b.koff - 1;
+}
+
+// Lanes of a wavefront with equal bins take consecutive slots of that bin:
+// one atomic per distinct bin of the wavefront (particles arrive nearly
+// grouped, so few).  Returns the lane's slot (the counter's old value plus
+// the lane's rank among the equal lanes).
+__device__ __forceinline__ unsigned wave_claim(unsigned key, bool act,
+                                               unsigned *ctr) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pend = __ballot(act);
+    unsigned slot = 0;
+    while (pend) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const unsigned lk = __builtin_amdgcn_readlane(key, leader);
+        const bool mine = act && key == lk;
+        const unsigned long long mm = __ballot(mine);
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lk], (unsigned)__popcll(mm));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (mine) slot = base + (unsigned)__popcll(mm & ((1ull << lane) - 1ull));
+        pend &= ~mm;
+    }
+    return slot;
+}
+
+// One pass of the counting sort over PPT particles per thread.  The bins
+// of a block's particles span a short range (the particles are still
+// nearly grouped), so the block counts them in an LDS histogram over that
+// range and adds each non-zero count to the global counter with one atomic;
+// in the scatter pass that atomic returns the block's first slot in the
+// bin and the LDS atomic's return value is the particle's rank among the
+// block's particles of the bin.  Blocks whose range exceeds the histogram
+// (the first, random-order sort) claim slots per wavefront instead.
+constexpr int BIN_CAP = 4096;
+
+template <typename R, bool SCATTER>
+__global__ __launch_bounds__(GH_BLOCK) void bin_pass_kernel(BinArgs<R> b) {
+    __shared__ unsigned hist[BIN_CAP];
+    __shared__ unsigned sbase[SCATTER ? BIN_CAP : 1];
+    __shared__ unsigned s_kmin, s_kmax;
+    const int tid = threadIdx.x;
+    const long long base = (long long)blockIdx.x * (GH_BLOCK * PPT);
+    if (tid == 0) {
+        s_kmin = 0xffffffffu;
+        s_kmax = 0u;
+    }
+    unsigned key[PPT];
+    bool act[PPT];
+    unsigned kmin = 0xffffffffu, kmax = 0u;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const long long i = base + k * GH_BLOCK + tid;
+        act[k] = i < b.n;
+        key[k] = act[k] ? bin_of(b, i) : 0u;
+        if (act[k]) {
+            kmin = min(kmin, key[k]);
+            kmax = max(kmax, key[k]);
+        }
+    }
+    __syncthreads();
+    kmin = (unsigned)wave_min((int)min(kmin, 0x7fffffffu));
+    kmax = (unsigned)wave_max((int)kmax);
+    if ((tid & 63) == 0) {
+        atomicMin(&s_kmin, kmin);
+        atomicMax(&s_kmax, kmax);
+    }
+    __syncthreads();
+    const unsigned k0 = s_kmin, k1 = s_kmax;
+    if (k0 <= k1 && k1 - k0 < (unsigned)BIN_CAP) {
+        const int span = (int)(k1 - k0) + 1;
+        for (int j = tid; j < span; j += GH_BLOCK) hist[j] = 0u;
+        __syncthreads();
+        unsigned rank[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+            rank[k] = act[k] ? atomicAdd(&hist[key[k] - k0], 1u) : 0u;
+        __syncthreads();
+        for (int j = tid; j < span; j += GH_BLOCK) {
+            const unsigned c = hist[j];
+            if (c) {
+                const unsigned s0 = atomicAdd(&b.ctr[k0 + j], c);
+                if (SCATTER) sbase[j] = s0;
+            }
+        }
+        if (SCATTER) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                if (!act[k]) continue;
+                const long long i = base + k * GH_BLOCK + tid;
+                const unsigned slot = sbase[key[k] - k0] + rank[k];
+                b.xv_out[slot] = b.xv_in[i];
+                b.ut_out[slot] = b.ut_in[i];
+                b.zout[slot] = b.zin[i];
+                b.iout[slot] = b.iin[i];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const unsigned slot = wave_claim(key[k], act[k], b.ctr);
+            if (SCATTER && act[k]) {
+                const long long i = base + k * GH_BLOCK + tid;
+                b.xv_out[slot] = b.xv_in[i];
+                b.ut_out[slot] = b.ut_in[i];
+                b.zout[slot] = b.zin[i];
+                b.iout[slot] = b.iin[i];
+            }
+        }
+    }
 }
 
 // ---- host launchers ---------------------------------------------------------
@@ -543,21 +702,26 @@ hipError_t launch_gather(const GatherArgs<R> &a, hipStream_t s) {
 }
 
 template <typename R>
-hipError_t launch_key(const typename Vec2<R>::T *ut, const uint8_t *zone,
-                      long long n, int nr, int na, int nz, int kshift,
-                      unsigned kmax, unsigned *key, unsigned *idx,
-                      hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(key_kernel<R>, dim3(nblocks(n, GH_BLOCK)), dim3(GH_BLOCK),
-                       0, s, ut, zone, n, nr, na, nz, kshift, kmax, key, idx);
+hipError_t launch_tmcent(const PushArgs<R> &a, R fac, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tmcent_kernel<R>, dim3(nblocks(a.n, GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, a, fac);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
-    if (p.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(permute_all_kernel<R>, dim3(nblocks(p.n, GH_BLOCK)),
-                       dim3(GH_BLOCK), 0, s, p);
+hipError_t launch_bin_count(const BinArgs<R> &b, hipStream_t s) {
+    if (b.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((bin_pass_kernel<R, false>), dim3(nblocks(b.n, GH_BLOCK * PPT)),
+                       dim3(GH_BLOCK), 0, s, b);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_bin_scatter(const BinArgs<R> &b, hipStream_t s) {
+    if (b.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((bin_pass_kernel<R, true>), dim3(nblocks(b.n, GH_BLOCK * PPT)),
+                       dim3(GH_BLOCK), 0, s, b);
     return hipGetLastError();
 }
 
@@ -566,11 +730,9 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
     template hipError_t launch_massgn<R>(const PushArgs<R> &, hipStream_t);   \
     template hipError_t launch_ut<R>(const PushArgs<R> &, hipStream_t);       \
     template hipError_t launch_gather<R>(const GatherArgs<R> &, hipStream_t); \
-    template hipError_t launch_key<R>(const typename Vec2<R>::T *,           \
-                                      const uint8_t *, long long, int, int,   \
-                                      int, int, unsigned, unsigned *,         \
-                                      unsigned *, hipStream_t);               \
-    template hipError_t launch_permute_all<R>(const PermArgs<R> &, hipStream_t);
+    template hipError_t launch_tmcent<R>(const PushArgs<R> &, R, hipStream_t); \
+    template hipError_t launch_bin_count<R>(const BinArgs<R> &, hipStream_t); \
+    template hipError_t launch_bin_scatter<R>(const BinArgs<R> &, hipStream_t);
 GH_INST(float)
 GH_INST(double)
 

@@ -83,6 +83,22 @@ def halo_table(rgrid, lscale, ts, md=0.5, rcore=0.1, n=4097):
     return dr, val, der
 
 
+def soft_kt_table(rgrid, lscale, ts, rstar, mass=1.0, n=4097):
+    """Fixed radial force of a softened Kuz'min-Toomre disc (disc type SOFT:
+    F(R) = -M R / (rstar^2 + R^2)^1.5 with rstar = 1 + epsilon,
+    SRC/lib15/frdisc.f:34, inimod.f:74) in internal units, the reference's
+    axifrc = frtot(r / lscale) ts / gvfac (SRC/lib15/axifrc.f, gvfac =
+    1/(lscale ts), setrun.f:873), tabulated like halo_table."""
+    dr = rgrid / (n - 2)
+    r = dr * np.arange(n)
+    R = r / lscale
+    a2 = rstar * rstar
+    s = a2 + R * R
+    val = -mass * R / s ** 1.5 * lscale * ts * ts
+    der = -mass * (a2 - 2.0 * R * R) / s ** 2.5 * ts * ts
+    return dr, val, der
+
+
 def _enclosed_fraction_inverse(u, rtrunc):
     """Solve 1 - (1 + R) e^-R = u (1 - (1 + rt) e^-rt) for R by Newton."""
     mt = 1.0 - (1.0 + rtrunc) * math.exp(-rtrunc)

@@ -154,6 +154,10 @@ class P2DGrid:
     def step(self):
         self._c(self._lib.galhip_step(self._h), "step")
 
+    def tmcent(self, start=True):
+        """Time-centre (start) or un-centre the velocities (tmcent)."""
+        self._c(self._lib.galhip_tmcent(self._h, int(bool(start))), "tmcent")
+
     def run(self, nsteps):
         self._c(self._lib.galhip_run(self._h, int(nsteps)), "run")
 
@@ -162,6 +166,12 @@ class P2DGrid:
 
     def sort(self):
         self._c(self._lib.galhip_sort(self._h), "sort")
+
+    def get_order(self):
+        """Load-order index of the particle in each storage slot."""
+        ids = np.empty(self.n, np.int32)
+        self._c(self._lib.galhip_get_order(self._h, _ptr(ids)), "get_order")
+        return ids
 
     def set_sort_interval(self, k):
         self._c(self._lib.galhip_set_sort_interval(self._h, int(k)), "set_sort_interval")

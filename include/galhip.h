@@ -128,6 +128,15 @@ int galhip_p2fndf(galhip_ctx *ctx, int32_t jzone, int32_t kzone, int32_t lpot);
  * grdacc, stpgrp, rezgrp, massgn, mascmb/zoncmb and scaled). */
 int galhip_step(galhip_ctx *ctx);
 
+/* tmcent(start) for the retarded-velocity leap-frog (SRC/lib15/tmcent.f:1,
+ * tmcgrp.f:26-36): every particle's velocity minus (start != 0) or plus
+ * half its getacc acceleration from the current field, turning loaded
+ * velocities into the half-step-back ones the leap-frog carries (as pcs2dmp
+ * does before the first step) or back.  Needs a field (masset + findf);
+ * calling it twice in the same direction is the reference's logic error
+ * (GALHIP_ERANGE). */
+int galhip_tmcent(galhip_ctx *ctx, int32_t start);
+
 /* galaxy's inner loop: nsteps x (findf(.false.); step) on the device
  * (SRC/progs/galaxy.f:107-148 without analysis and I/O). */
 int galhip_run(galhip_ctx *ctx, int32_t nsteps);
@@ -167,9 +176,14 @@ int galhip_get_noff(const galhip_ctx *ctx, int64_t *noff);
 
 /* ---- performance controls ---- */
 
-/* Re-sort particles by cell every k steps of galhip_run (0 = never). */
+/* Regroup particles by cell every k steps of galhip_run (0 = never), and
+ * now.  No result depends on the particle order (the reference keeps its
+ * particles in zone lists in load order, SRC/lib15/gather.f:1); the order
+ * only sets the locality of the device passes.  galhip_get_order copies the
+ * load-order index (0-based) of the particle in each storage slot. */
 int galhip_set_sort_interval(galhip_ctx *ctx, int32_t k);
 int galhip_sort(galhip_ctx *ctx);
+int galhip_get_order(const galhip_ctx *ctx, int32_t *ids);
 /* Record HIP events around every particle-push launch on the context's
  * stream; galhip_get_profile returns the summed push time (ms), the number
  * of push launches and of particles pushed since profiling was enabled. */

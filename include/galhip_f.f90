@@ -116,6 +116,11 @@ module galhip_f
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function galhip_step
+     integer(c_int) function galhip_tmcent(ctx, start) bind(c, name='galhip_tmcent')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: start
+     end function galhip_tmcent
      integer(c_int) function galhip_run(ctx, nsteps) bind(c, name='galhip_run')
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx
@@ -184,6 +189,11 @@ module galhip_f
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function galhip_sort
+     integer(c_int) function galhip_get_order(ctx, ids) bind(c, name='galhip_get_order')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: ids(*)
+     end function galhip_get_order
      integer(c_int) function galhip_set_profile(ctx, on) bind(c, name='galhip_set_profile')
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx

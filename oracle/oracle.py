@@ -90,6 +90,8 @@ for _s, _p, _t in (("f", _f32p, C.c_float), ("d", _f64p, C.c_double)):
     _sig("orc_push_" + _s, C.c_int, [C.POINTER(StepCfg), C.c_int, _p, _p, _p,
                                      _p, _i32p, _p, _p, _p, _i64p, C.c_int,
                                      _i32p, _i32p, _u8p])
+    _sig("orc_tmcent_" + _s, None, [C.POINTER(StepCfg), C.c_int, _p, _p, _p, _p,
+                                    _i32p, _p, _p, C.c_int])
 _sig("orc_fixforce", C.c_double, [C.c_double, C.c_int, C.c_double, _f64p, _f64p])
 for _n in ("orc_rfftf", "orc_rfftb", "orc_cost", "orc_sint"):
     _sig(_n, None, [C.c_int, _f64p, _f64p])
@@ -431,3 +433,16 @@ class P2D:
         self.istep += 1
         self._zoncmb_scaled()
         return dn
+
+    def tmcent(self, start=True, field=None):
+        """tmcent (SRC/lib15/tmcent.f, tmcgrp.f:26-36, retarded-velocity
+        leap-frog): every particle's velocity minus (start) or plus half its
+        getacc acceleration from the current field."""
+        if field is None:
+            field = self.fld[:2].astype(self.dt)
+        fx = np.ascontiguousarray(field[0], self.dt)
+        fy = np.ascontiguousarray(field[1], self.dt)
+        c = self.stepcfg()
+        fn = _lib.orc_tmcent_f if self.prec == "f" else _lib.orc_tmcent_d
+        fn(C.byref(c), len(self.x), self.x, self.y, self.vx, self.vy, self.iz,
+           fx, fy, int(bool(start)))

@@ -83,6 +83,12 @@ int orc_push_d(const orc_stepcfg *c, int n, double *x, double *y, double *vx,
                double *vy, int32_t *iz, const double *fx, const double *fy,
                double *grdmss, int64_t *mfx, int qbits,
                int32_t *jrad, int32_t *krad, uint8_t *moved);
+void orc_tmcent_f(const orc_stepcfg *c, int n, const float *x, const float *y,
+                  float *vx, float *vy, const int32_t *iz, const float *fx,
+                  const float *fy, int start);
+void orc_tmcent_d(const orc_stepcfg *c, int n, const double *x, const double *y,
+                  double *vx, double *vy, const int32_t *iz, const double *fx,
+                  const double *fy, int start);
 
 /* ---- FFTPACK-convention transforms, direct sums in double ----
  * rfftf: forward real DFT, half-complex output (stest.f:84-103)

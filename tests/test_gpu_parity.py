@@ -277,3 +277,22 @@ def test_sort_preserves_particles():
     after = g.unload()
     for k in before:
         assert np.array_equal(before[k], after[k])
+
+
+def test_sort_groups_by_cell(orc):
+    """After galhip_sort the storage order runs through the cells in
+    ring-major order, off-grid particles last (the cells recomputed by the
+    oracle from the unloaded positions)."""
+    _say("sort grouping")
+    n = 20000
+    g = gpu_model(precision="fp32", n=n)
+    g.run(3)
+    g.sort()
+    ids = g.get_order()
+    out = g.unload()
+    assert np.array_equal(np.sort(ids), np.arange(n))
+    nr, na, nz = SMALL.nr, SMALL.na, len(SMALL.nstep)
+    cell, _, _ = orc.weight(out["x"], out["y"], nr, na, "f")
+    on = (out["iz"] >= 1) & (out["iz"] <= nz)
+    key = np.where(on, cell, nr * na)
+    assert np.all(np.diff(key[ids]) >= 0)

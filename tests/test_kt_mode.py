@@ -1,0 +1,60 @@
+"""The reference's standard p2d test (EXAMPLES/p2d/README): the cold
+softened Kuz'min-Toomre disc of run100 grows a bar-like m = 2 mode whose
+eigenfrequency linear theory puts at 0.540 + 0.158i.  Starting from the
+reference's own run100.dmp0 particles, the step (the oracle on the CPU, the
+HIP path on the GPU) is run to t = 56 with the run's grid, filter, zones and
+fixed axisymmetric force, and one growing mode is fitted to the m = 2
+density transform over 21 <= t <= 56 (the window of the example's modefit
+script, EXAMPLES/p2d/modefit.s).  This checks the whole step at once:
+mass assignment, the m = 2 Green's-function solve, the sector filter, the
+fixed force, interpolation, the leap-frog and the zones.  The tolerance,
+2 % in pattern speed and 5 % in growth rate, covers the run-to-run spread
+of a mode grown from rounding noise (the example quotes ~1 %)."""
+import numpy as np
+import pytest
+
+from tests._model import KT_OMEGA, kt_mode, kt_setup
+
+
+def _check(om, res, amp):
+    assert abs(om.real - KT_OMEGA.real) < 0.02 * KT_OMEGA.real, om
+    assert abs(om.imag - KT_OMEGA.imag) < 0.05 * KT_OMEGA.imag, om
+    assert res < 0.5 and amp > 1e-3, (res, amp)
+
+
+def test_kt_mode_oracle(orc):
+    kw, ff, parts, _ = kt_setup()
+    m = orc.P2D(fixforce=ff, prec="f", use_fx=True, **kw)
+    x, y, vx, vy, iz = parts
+    m.load(x, y, vx, vy, iz)
+    m.masset()
+
+    def advance(an):
+        m.findf(False)
+        rho = m.rho.copy() if an else None
+        m.step()
+        return rho
+
+    _check(*kt_mode(advance))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_kt_mode_gpu(precision):
+    from galaxy_amd.lib15 import P2DGrid
+    print("[kt mode] %s" % precision, flush=True)
+    kw, ff, parts, _ = kt_setup()
+    g = P2DGrid(precision=precision, **kw)
+    g.greenm()
+    g.set_fixforce(*ff)
+    g.restrt(*parts)
+    g.masset()
+
+    def advance(an):
+        g.findf(False)
+        rho = g.get_rho() if an else None
+        g.step()
+        return rho
+
+    _check(*kt_mode(advance))
+    g.close()
