@@ -18,7 +18,8 @@ ratio, and the m = 2 amplitude of both end states.
 
 roofline: the push kernel, algorithmic bytes = pushes x 32 B (x, y, vx, vy
 read and written in FP32) over its summed HIP-event time on the library's
-stream, against 8 TB/s.  cpu_baseline: the oracle's FP32 restatement on one
+stream, against 8 TB/s; traffic = measured HBM bytes per launch (per-push
+bytes from rocprofv3 counters times the pushes per launch).  cpu_baseline: the oracle's FP32 restatement on one
 host core over a bounded sample of the same workload.
 """
 import argparse
@@ -36,6 +37,12 @@ sys.path.insert(0, ROOT)
 METRIC = ("particle-steps/s @10M (mixed vs FP64 GPU, m=2 err<1e-4); "
           "% HBM roofline")
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per push of push_kernel<float> on this workload: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE of the all-zones launch (10M pushes), each scaled
+# by what the same counter reads for a 1 GiB copy kernel (FETCH_SIZE x 2,
+# WRITE_SIZE x 1); profiles/push_kernel_p2d-256x512-n1e7-3zones_traffic.txt.
+PUSH_HBM_READ_B = 25.5
+PUSH_HBM_WRITE_B = 25.3
 
 
 def log(*a):
@@ -159,8 +166,11 @@ def main():
     ms_per_step = 1e3 * mixed["seconds"] / args.steps
     bytes_per_push = 32.0
     achieved = mixed["pushes_prof"] * bytes_per_push / (mixed["push_ms"] * 1e-3) / 1e9
+    per_launch = mixed["pushes_prof"] / max(mixed["launches"], 1)
+    traffic = (PUSH_HBM_READ_B + PUSH_HBM_WRITE_B) * per_launch
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=round(traffic),
                 kernel="push_kernel<float>",
                 bytes_per_push=bytes_per_push,
                 avg_launch_us=round(1e3 * mixed["push_ms"] / max(mixed["launches"], 1), 2))

@@ -24,9 +24,25 @@ template <typename R> hipError_t launch_push(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_massgn(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_gather(const GatherArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_ut(const PushArgs<R> &, hipStream_t);
-template <typename R> hipError_t launch_bin_count(const BinArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_tmcent(const PushArgs<R> &, R, hipStream_t);
-template <typename R> hipError_t launch_bin_scatter(const BinArgs<R> &, hipStream_t);
+template <typename R>
+hipError_t launch_key(const typename Vec2<R>::T *, const uint8_t *, long long,
+                      int, int, int, int, unsigned, unsigned *, unsigned *,
+                      hipStream_t);
+template <typename R> struct PermArgs {
+    const typename Vec4<R>::T *xv_in;
+    typename Vec4<R>::T *xv_out;
+    const typename Vec2<R>::T *ut_in;
+    typename Vec2<R>::T *ut_out;
+    const uint8_t *zin;
+    uint8_t *zout;
+    const unsigned *iin;
+    unsigned *iout;
+    const unsigned *perm;
+    long long n;
+};
+template <typename R>
+hipError_t launch_permute_all(const PermArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_analyse(const AnaArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_synth(const SynArgs<R> &, hipStream_t);
 hipError_t launch_green(double *, const double *, int, int, int, int, double,
@@ -85,10 +101,9 @@ struct galhip_ctx {
     void *xv2, *ut2;
     uint8_t *zone2;
     unsigned *id2;
-    unsigned *bins, *cursor;           // counting sort: per-bin counts, slots
-    void *scan_tmp;
-    size_t scan_tmp_bytes;
-    int nbins;
+    unsigned *key, *key2, *perm, *perm2;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
     long long *mfx[GH_MAXZ], *moz[GH_MAXZ];
     int *jrad, *krad, *ranges, *lg_d;
     long long *counters;
@@ -123,6 +138,8 @@ template <typename T> static void dfree(T *&p) {
 static void free_particles(galhip_ctx *c) {
     dfree(c->xv); dfree(c->ut); dfree(c->zone); dfree(c->id);
     dfree(c->xv2); dfree(c->ut2); dfree(c->zone2); dfree(c->id2);
+    dfree(c->key); dfree(c->key2); dfree(c->perm); dfree(c->perm2); dfree(c->sort_tmp);
+    c->sort_tmp_bytes = 0;
     c->n = c->cap = 0;
 }
 
@@ -256,7 +273,6 @@ extern "C" int galhip_free(galhip_ctx *c) {
     free_particles(c);
     for (int z = 0; z < GH_MAXZ; z++) { dfree(c->mfx[z]); dfree(c->moz[z]); }
     dfree(c->jrad); dfree(c->krad); dfree(c->ranges); dfree(c->lg_d); dfree(c->counters);
-    dfree(c->bins); dfree(c->cursor); dfree(c->scan_tmp);
     dfree(c->rho); dfree(c->coef); dfree(c->fld2); dfree(c->pot); dfree(c->tw); dfree(c->twd);
     if (c->G != (void *)c->Gd) dfree(c->G);
     dfree(c->Gd);
@@ -349,64 +365,47 @@ static int ensure_particles(galhip_ctx *c, long long n) {
     rc |= dalloc(&c->xv2, 4 * es * n); rc |= dalloc(&c->ut2, 2 * es * n);
     rc |= dalloc(&c->zone, n); rc |= dalloc(&c->id, sizeof(unsigned) * n);
     rc |= dalloc(&c->zone2, n); rc |= dalloc(&c->id2, sizeof(unsigned) * n);
+    rc |= dalloc(&c->key, sizeof(unsigned) * n); rc |= dalloc(&c->key2, sizeof(unsigned) * n);
+    rc |= dalloc(&c->perm, sizeof(unsigned) * n); rc |= dalloc(&c->perm2, sizeof(unsigned) * n);
     if (rc) { free_particles(c); return rc; }
+    size_t tmp = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+        nullptr, tmp, c->key, c->key2, c->perm, c->perm2, (int)n, 0, 32, c->s);
+    if (e != hipSuccess) return fail(GALHIP_EHIP, "radix sort sizing: %s", hipGetErrorString(e));
+    rc = dalloc(&c->sort_tmp, tmp);
+    if (rc) { free_particles(c); return rc; }
+    c->sort_tmp_bytes = tmp;
     c->cap = c->n = n;
     return 0;
 }
 
-// Bins of the counting sort: cells (ring-major) shifted right by
-// GH_BIN_SHIFT, then one bin for the off-grid particles.
-#ifndef GH_BIN_SHIFT
-#define GH_BIN_SHIFT 0
-#endif
-
-static int ensure_bins(galhip_ctx *c) {
-    if (c->bins) return 0;
-    const int ncell = ((c->mesh - 1) >> GH_BIN_SHIFT) + 1;
-    c->nbins = ncell + 1;
-    int rc = dalloc(&c->bins, sizeof(unsigned) * c->nbins) |
-             dalloc(&c->cursor, sizeof(unsigned) * c->nbins);
-    if (rc) return rc;
-    size_t tmp = 0;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, c->bins, c->cursor,
-                                                    c->nbins, c->s);
-    if (e != hipSuccess) return fail(GALHIP_EHIP, "scan sizing: %s", hipGetErrorString(e));
-    rc = dalloc(&c->scan_tmp, tmp ? tmp : 1);
-    if (rc) return rc;
-    c->scan_tmp_bytes = tmp;
-    return 0;
-}
-
-// Regroup the particles by bin (counting sort; the order inside a bin is
-// arbitrary and no result depends on it).
 template <typename R> static int sort_impl(galhip_ctx *c) {
     long long n = c->n;
     if (n <= 1) return 0;
-    int rc = ensure_bins(c);
-    if (rc) return rc;
-    BinArgs<R> b;
-    b.xv_in = (const typename Vec4<R>::T *)c->xv;
-    b.xv_out = (typename Vec4<R>::T *)c->xv2;
-    b.ut_in = (const typename Vec2<R>::T *)c->ut;
-    b.ut_out = (typename Vec2<R>::T *)c->ut2;
-    b.zin = c->zone;
-    b.zout = c->zone2;
-    b.iin = c->id;
-    b.iout = c->id2;
-    b.n = n;
-    b.nr = c->nr;
-    b.na = c->na;
-    b.nz = c->nz;
-    b.shift = GH_BIN_SHIFT;
-    b.koff = (unsigned)(c->nbins - 1);
-    HIPCHK(hipMemsetAsync(c->bins, 0, sizeof(unsigned) * c->nbins, c->s));
-    b.ctr = c->bins;
-    HIPCHK(launch_bin_count<R>(b, c->s));
-    size_t tmp = c->scan_tmp_bytes;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp, tmp, c->bins, c->cursor,
-                                            c->nbins, c->s));
-    b.ctr = c->cursor;
-    HIPCHK(launch_bin_scatter<R>(b, c->s));
+    // keys of at most 16 bits: cells shifted right until they fit, off-grid last
+    int kshift = 0;
+    while (((long long)c->mesh >> kshift) > 65534) kshift++;
+    unsigned kmax = (unsigned)((c->mesh - 1) >> kshift) + 1;
+    HIPCHK(launch_key<R>((const typename Vec2<R>::T *)c->ut, c->zone, n, c->nr,
+                         c->na, c->nz, kshift, kmax, c->key, c->perm, c->s));
+    int end_bit = 1;
+    while ((1LL << end_bit) <= (long long)kmax) end_bit++;
+    size_t tmp = c->sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->key, c->key2,
+                                              c->perm, c->perm2, (int)n, 0,
+                                              end_bit, c->s));
+    PermArgs<R> p;
+    p.xv_in = (const typename Vec4<R>::T *)c->xv;
+    p.xv_out = (typename Vec4<R>::T *)c->xv2;
+    p.ut_in = (const typename Vec2<R>::T *)c->ut;
+    p.ut_out = (typename Vec2<R>::T *)c->ut2;
+    p.zin = c->zone;
+    p.zout = c->zone2;
+    p.iin = c->id;
+    p.iout = c->id2;
+    p.perm = c->perm2;
+    p.n = n;
+    HIPCHK(launch_permute_all<R>(p, c->s));
     std::swap(c->xv, c->xv2);
     std::swap(c->ut, c->ut2);
     std::swap(c->zone, c->zone2);

@@ -47,19 +47,22 @@ struct UtConst {
     int na;
 };
 
+// tab: the math tables (GH_MATH_TAB, or a copy of it staged in LDS).
 __device__ __forceinline__ void ut_of(float x, float y, float rr,
-                                      const UtConst &k, float &u, float &t) {
+                                      const UtConst &k, float &u, float &t,
+                                      const double *tab = GH_MATH_TAB) {
     float v = 1.0f + rr;
-    float l = (float)gh_log_ge1((double)v);
+    float l = (float)gh_log_ge1((double)v, tab);
     u = (float)((double)l * k.inv_alpha);
-    t = (float)(gh_atan2((double)y, (double)x) * k.tscale);
+    t = (float)(gh_atan2((double)y, (double)x, tab) * k.tscale);
     if (t < 0.0f) t = t + (float)k.na;
 }
 __device__ __forceinline__ void ut_of(double x, double y, double rr,
-                                      const UtConst &k, double &u, double &t) {
-    double l = gh_log_ge1(1.0 + rr);
+                                      const UtConst &k, double &u, double &t,
+                                      const double *tab = GH_MATH_TAB) {
+    double l = gh_log_ge1(1.0 + rr, tab);
     u = l / k.alpha_d;
-    t = gh_atan2(y, x) * (double)k.na / GH_TWO_PI;
+    t = gh_atan2(y, x, tab) * (double)k.na / GH_TWO_PI;
     if (t < 0.0) t = t + (double)k.na;
 }
 
@@ -118,20 +121,30 @@ __device__ __forceinline__ long long quant(R w, int qbits) {
     return __double2ll_rn(ldexp((double)w, qbits));
 }
 
-// Cubic Hermite interpolation of the tabulated fixed radial force on
-// r_k = k dr (s = r * (1/dr), the same formula as the oracle's).
-__device__ __forceinline__ double fixforce(double r, int nfr, double dr,
-                                           double inv_dr,
-                                           const double *__restrict__ val,
-                                           const double *__restrict__ der) {
-    // branch-free, so that the table reads of several particles overlap:
-    // past the table end t = 1 selects val[nfr - 1] exactly (the other
-    // Hermite basis values are then exact zeros)
+// Table interval of the fixed radial force at radius r: k with
+// r_k <= r < r_k+1 (s = r * (1/dr)), clamped to the last interval.
+__device__ __forceinline__ int fixforce_index(double r, int nfr, double inv_dr,
+                                              bool &past) {
     double s = r * inv_dr;
     int k = (int)s;
-    const bool past = k >= nfr - 1;
+    past = k >= nfr - 1;
     if (k < 0) k = 0;
     if (past) k = nfr - 2;
+    return k;
+}
+
+// Cubic Hermite interpolation of the tabulated fixed radial force on
+// r_k = k dr (s = r * (1/dr), the same formula as the oracle's) in the
+// interval k that fixforce_index gives.  Branch-free, so that the table
+// reads of several particles overlap: past the table end t = 1 selects
+// val[nfr - 1] exactly (the other Hermite basis values are then exact
+// zeros).  val and der hold the table from entry k0 on (a slice of it).
+__device__ __forceinline__ double fixforce_at(double r, int k, bool past, double dr,
+                                              double inv_dr,
+                                              const double *__restrict__ val,
+                                              const double *__restrict__ der,
+                                              int k0) {
+    double s = r * inv_dr;
     double t = past ? 1.0 : s - (double)k;
     double t2 = t * t;
     double omt = 1.0 - t;
@@ -140,8 +153,18 @@ __device__ __forceinline__ double fixforce(double r, int nfr, double dr,
     double h10 = t * omt2;
     double h01 = t2 * (3.0 - 2.0 * t);
     double h11 = t2 * (t - 1.0);
-    return h00 * val[k] + h10 * dr * der[k] + h01 * val[k + 1] +
-           h11 * dr * der[k + 1];
+    const int j = k - k0;
+    return h00 * val[j] + h10 * dr * der[j] + h01 * val[j + 1] +
+           h11 * dr * der[j + 1];
+}
+
+__device__ __forceinline__ double fixforce(double r, int nfr, double dr,
+                                           double inv_dr,
+                                           const double *__restrict__ val,
+                                           const double *__restrict__ der) {
+    bool past;
+    const int k = fixforce_index(r, nfr, inv_dr, past);
+    return fixforce_at(r, k, past, dr, inv_dr, val, der, 0);
 }
 
 // ---- kernel argument blocks -------------------------------------------------
@@ -218,22 +241,6 @@ template <typename R> struct SynArgs {
     long long *moz[GH_MAXZ];
     int *jrad, *krad;
     int masscl_z;
-};
-
-// counting sort of the particle arrays by bin (bin_count / bin_scatter)
-template <typename R> struct BinArgs {
-    const typename Vec4<R>::T *xv_in;
-    typename Vec4<R>::T *xv_out;
-    const typename Vec2<R>::T *ut_in;
-    typename Vec2<R>::T *ut_out;
-    const uint8_t *zin;
-    uint8_t *zout;
-    const unsigned *iin;
-    unsigned *iout;
-    unsigned *ctr;          // counts (count pass) or cursors (scatter pass)
-    long long n;
-    int nr, na, nz, shift;
-    unsigned koff;          // the off-grid bin (= number of cell bins)
 };
 
 }  // namespace gh

@@ -45,9 +45,11 @@ namespace gh {
 #ifndef GH_TILE_REP
 #define GH_TILE_REP 2
 #endif
+
 constexpr int PPT = GH_PPT;            // particles per thread
 constexpr int TILE_CAP = GH_TILE_CAP;  // LDS tile, cells of a ring x spoke window
 constexpr int TILE_REP = GH_TILE_REP;  // replicas (lane & (REP-1)) against conflicts
+constexpr int FF_CAP = 256;            // staged fixed-force table entries
 
 template <typename R>
 __device__ __forceinline__ R tsfac(const PushArgs<R> &a, int iz) {
@@ -113,13 +115,19 @@ __device__ __forceinline__ void deposit_global(long long *const *mfx, int nz,
 }
 
 template <typename R>
-__global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
+// At least 5 wavefronts per SIMD: caps the kernel at 96 VGPRs without
+// spilling (measured: 4 waves at the unconstrained 102 VGPRs is 8 % slower,
+// 6 waves spill).
+__global__ __launch_bounds__(GH_BLOCK, 5) void push_kernel(PushArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     typedef typename Vec4<R>::T R4;
     __shared__ long long tile[TILE_REP][TILE_CAP];
     __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ];
     __shared__ int s_dnoff, s_pushes, s_zt, s_sref;
     __shared__ int s_rmin, s_rmax, s_dmin, s_dmax;
+    __shared__ int s_fref, s_frmin, s_frmax, s_fdmin, s_fdmax, s_kmin, s_kmax;
+    // slice of the fixed-force table that the block's radii use
+    __shared__ double s_ffv[FF_CAP], s_ffd[FF_CAP];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const long long base = (long long)blockIdx.x * (GH_BLOCK * PPT);
@@ -136,6 +144,12 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
         s_rmax = -1;
         s_dmin = INT_MAX;
         s_dmax = INT_MIN;
+        s_frmin = INT_MAX;
+        s_frmax = -1;
+        s_fdmin = INT_MAX;
+        s_fdmax = INT_MIN;
+        s_kmin = INT_MAX;
+        s_kmax = -1;
     }
 
     int zin[PPT];
@@ -167,40 +181,149 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
         st[k] = a.xv[i];
         cu[k] = a.ut[i];
     }
-    // grdacc: weights of the old position from its (u, t); off-grid and
-    // idle particles read cell 0 and weigh nothing
-    R ax[PPT], ay[PPT];
+    // grdacc: weights of the old position from its (u, t) (off-grid and
+    // idle particles weigh nothing), the field read from an LDS copy of the
+    // window of rings x spokes that the block's old cells span -- the same
+    // windowing as the deposit tile below, whose LDS it borrows.  A window
+    // larger than that space falls back to reads from global memory.
+    const int half = na >> 1;
+    int gring[PPT], gspoke[PPT], gd[PPT];
+    R gw[PPT][4];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
         const bool grid = mv[k] && zin[k] != nlists;
-        R w[4];
-        int ring, spoke, ic[4];
-        cic(grid ? cu[k].x : (R)0, grid ? cu[k].y : (R)0, nr, na, ring, spoke, w);
-        corners_rs(ring, spoke, na, ic);
-        R sx = 0, sy = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            // the last ring with t == na points past the field: 0
-            const bool in = ic[q] < a.mesh;
-            R2 f = a.fld[in ? ic[q] : 0];
-            R wq = (grid && in) ? w[q] : (R)0;
-            sx = sx + f.x * wq;
-            sy = sy + f.y * wq;
-        }
-        ax[k] = sx;
-        ay[k] = sy;
+        cic(grid ? cu[k].x : (R)0, grid ? cu[k].y : (R)0, nr, na, gring[k],
+            gspoke[k], gw[k]);
+        if (!grid) gring[k] = -1;
     }
     R rr[PPT], fr[PPT];
+    int kf[PPT];
+    bool kpast[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
         rr[k] = sqrt(st[k].x * st[k].x + st[k].y * st[k].y);
         fr[k] = 0;
+        kpast[k] = false;
+        kf[k] = a.fixforce ? fixforce_index((double)rr[k], a.nfr, a.fr_inv_dr, kpast[k]) : 0;
     }
-    if (a.fixforce) {
+    if (tid == 0) {
+        int r = 0;
+#pragma unroll
+        for (int k = PPT - 1; k >= 0; k--)
+            if (gring[k] >= 0) r = gspoke[k];
+        s_fref = r;
+    }
+    __syncthreads();
+    const int fref = s_fref;
+    {
+        int rmin = INT_MAX, rmax = -1, dmin = INT_MAX, dmax = INT_MIN;
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            int d = gspoke[k] - fref;
+            if (d < -half) d += na;
+            else if (d >= half) d -= na;
+            gd[k] = d;
+            if (gring[k] >= 0) {
+                rmin = min(rmin, gring[k]);
+                rmax = max(rmax, gring[k]);
+                dmin = min(dmin, d);
+                dmax = max(dmax, d);
+            }
+        }
+        int kmin = INT_MAX, kmax = -1;
 #pragma unroll
         for (int k = 0; k < PPT; k++)
-            fr[k] = (R)fixforce((double)rr[k], a.nfr, a.fr_dr, a.fr_inv_dr,
-                                a.fr_val, a.fr_der);
+            if (gring[k] >= 0) {
+                kmin = min(kmin, kf[k]);
+                kmax = max(kmax, kf[k]);
+            }
+        rmin = wave_min(rmin);
+        rmax = wave_max(rmax);
+        dmin = wave_min(dmin);
+        dmax = wave_max(dmax);
+        kmin = wave_min(kmin);
+        kmax = wave_max(kmax);
+        if (lane == 0 && rmax >= 0) {
+            atomicMin(&s_frmin, rmin);
+            atomicMax(&s_frmax, rmax);
+            atomicMin(&s_fdmin, dmin);
+            atomicMax(&s_fdmax, dmax);
+            atomicMin(&s_kmin, kmin);
+            atomicMax(&s_kmax, kmax);
+        }
+    }
+    __syncthreads();
+    R2 *ftile = reinterpret_cast<R2 *>(&tile[0][0]);
+    constexpr int FT_CAP = (int)(sizeof(tile) / sizeof(R2));
+    const int fr0 = s_frmin, fr1 = s_frmax, fd0 = s_fdmin, fd1 = s_fdmax;
+    const int ftw = fd1 - fd0 + 2, fth = fr1 - fr0 + 2;
+    const bool use_ft = fr1 >= 0 && fth * ftw <= FT_CAP;
+    if (use_ft) {
+        // rows past the last ring read as zero, as the global path does
+        for (int j = tid; j < fth * ftw; j += GH_BLOCK) {
+            const int row = j / ftw, col = j - row * ftw;
+            const int ring = fr0 + row;
+            int spoke = fref + fd0 + col;
+            if (spoke < 0) spoke += na;
+            else if (spoke >= na) spoke -= na;
+            if (spoke >= na) spoke -= na;
+            ftile[j] = ring < nr ? a.fld[ring * na + spoke] : zero2<R>();
+        }
+    }
+    // the table entries k0 .. k1 + 1 that the block's grid particles read
+    const int k0 = s_kmin, nk = s_kmax + 2 - s_kmin;
+    const bool use_ff = a.fixforce && s_kmax >= 0 && nk <= FF_CAP;
+    if (use_ff)
+        for (int j = tid; j < nk; j += GH_BLOCK) {
+            s_ffv[j] = a.fr_val[k0 + j];
+            s_ffd[j] = a.fr_der[k0 + j];
+        }
+    __syncthreads();
+    R ax[PPT], ay[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const bool grid = gring[k] >= 0;
+        R sx = 0, sy = 0;
+        if (use_ft) {
+            const int b = grid ? (gring[k] - fr0) * ftw + (gd[k] - fd0) : 0;
+            const int ti[4] = {b, b + 1, b + ftw, b + ftw + 1};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const R2 f = ftile[ti[q]];
+                const R wq = grid ? gw[k][q] : (R)0;
+                sx = sx + f.x * wq;
+                sy = sy + f.y * wq;
+            }
+        } else {
+            int ic[4];
+            corners_rs(grid ? gring[k] : 0, gspoke[k], na, ic);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                // the last ring with t == na points past the field: 0
+                const bool in = ic[q] < a.mesh;
+                R2 f = a.fld[in ? ic[q] : 0];
+                R wq = (grid && in) ? gw[k][q] : (R)0;
+                sx = sx + f.x * wq;
+                sy = sy + f.y * wq;
+            }
+        }
+        ax[k] = sx;
+        ay[k] = sy;
+    }
+    if (use_ff) {
+        // the staged slice; particles off the grid or idle (their force is
+        // not used) read its first interval
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const bool grid = gring[k] >= 0;
+            fr[k] = (R)fixforce_at((double)rr[k], grid ? kf[k] : k0, grid && kpast[k],
+                                   a.fr_dr, a.fr_inv_dr, s_ffv, s_ffd, k0);
+        }
+    } else if (a.fixforce) {
+#pragma unroll
+        for (int k = 0; k < PPT; k++)
+            fr[k] = (R)fixforce_at((double)rr[k], kf[k], kpast[k], a.fr_dr,
+                                   a.fr_inv_dr, a.fr_val, a.fr_der, 0);
     }
 
     // deposit state kept per particle: ring, spoke, zone (0 = none), weights
@@ -285,7 +408,6 @@ __global__ __launch_bounds__(GH_BLOCK) void push_kernel(PushArgs<R> a) {
     }
     __syncthreads();
     const int zt = s_zt, sref = s_sref;        // zt == 0: no tile
-    const int half = na >> 1;
     // ring and spoke-offset ranges of the tile zone's deposits: registers,
     // then one wavefront reduction; other zones' ring ranges go to LDS
     int rmin = INT_MAX, rmax = -1, dmin = INT_MAX, dmax = INT_MIN;
@@ -530,137 +652,53 @@ __global__ __launch_bounds__(GH_BLOCK) void tmcent_kernel(PushArgs<R> a, R fac) 
     a.xv[i] = st;
 }
 
-// ---- bin sort --------------------------------------------------------------
-//
-// Particles are kept grouped by cell only for locality (small deposit
-// tiles, cached field reads).  No result depends on their order inside a
-// cell: each particle's step is its own and the mass sums are integer sums.
-// So a counting sort does the regrouping: a pass that counts the particles
-// of every bin, an exclusive scan of the counts, and a pass that claims a
-// slot in the particle's bin atomically and moves all its arrays there.
-// The order inside a bin is then arbitrary.  Bin = cell >> shift of the
-// particle's (u, t) for zones 1..nz; off-grid particles share the last bin.
+// sort key: the cell of the particle's (u, t) shifted right by kshift so
+// that keys fit 16 bits (two radix passes); off-grid particles last.
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void key_kernel(
+    const typename Vec2<R>::T *__restrict__ ut,
+    const uint8_t *__restrict__ zone, long long n, int nr, int na, int nz,
+    int kshift, unsigned kmax, unsigned *__restrict__ key,
+    unsigned *__restrict__ idx) {
+    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    unsigned k = kmax;
+    int z = zone[i];
+    if (z >= 1 && z <= nz) {
+        R w[4];
+        int ring, spoke;
+        typename Vec2<R>::T c = ut[i];
+        cic(c.x, c.y, nr, na, ring, spoke, w);
+        k = (unsigned)(ring * na + spoke) >> kshift;
+        if (k >= kmax) k = kmax - 1;
+    }
+    key[i] = k;
+    idx[i] = (unsigned)i;
+}
+
+// one pass moving every particle array to sorted order
+template <typename R> struct PermArgs {
+    const typename Vec4<R>::T *xv_in;
+    typename Vec4<R>::T *xv_out;
+    const typename Vec2<R>::T *ut_in;
+    typename Vec2<R>::T *ut_out;
+    const uint8_t *zin;
+    uint8_t *zout;
+    const unsigned *iin;
+    unsigned *iout;
+    const unsigned *perm;
+    long long n;
+};
 
 template <typename R>
-__device__ __forceinline__ unsigned bin_of(const BinArgs<R> &b, long long i) {
-    const int z = b.zin[i];
-    if (z < 1 || z > b.nz) return b.koff;
-    R w[4];
-    int ring, spoke;
-    const typename Vec2<R>::T c = b.ut_in[i];
-    cic(c.x, c.y, b.nr, b.na, ring, spoke, w);
-    unsigned k = (unsigned)(ring * b.na + spoke) >> b.shift;
-    return k < b.koff ? k : b.koff - 1;
-}
-
-// Lanes of a wavefront with equal bins take consecutive slots of that bin:
-// one atomic per distinct bin of the wavefront (particles arrive nearly
-// grouped, so few).  Returns the lane's slot (the counter's old value plus
-// the lane's rank among the equal lanes).
-__device__ __forceinline__ unsigned wave_claim(unsigned key, bool act,
-                                               unsigned *ctr) {
-    const int lane = threadIdx.x & 63;
-    unsigned long long pend = __ballot(act);
-    unsigned slot = 0;
-    while (pend) {
-        const int leader = __ffsll((long long)pend) - 1;
-        const unsigned lk = __builtin_amdgcn_readlane(key, leader);
-        const bool mine = act && key == lk;
-        const unsigned long long mm = __ballot(mine);
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(&ctr[lk], (unsigned)__popcll(mm));
-        base = __builtin_amdgcn_readlane(base, leader);
-        if (mine) slot = base + (unsigned)__popcll(mm & ((1ull << lane) - 1ull));
-        pend &= ~mm;
-    }
-    return slot;
-}
-
-// One pass of the counting sort over PPT particles per thread.  The bins
-// of a block's particles span a short range (the particles are still
-// nearly grouped), so the block counts them in an LDS histogram over that
-// range and adds each non-zero count to the global counter with one atomic;
-// in the scatter pass that atomic returns the block's first slot in the
-// bin and the LDS atomic's return value is the particle's rank among the
-// block's particles of the bin.  Blocks whose range exceeds the histogram
-// (the first, random-order sort) claim slots per wavefront instead.
-constexpr int BIN_CAP = 4096;
-
-template <typename R, bool SCATTER>
-__global__ __launch_bounds__(GH_BLOCK) void bin_pass_kernel(BinArgs<R> b) {
-    __shared__ unsigned hist[BIN_CAP];
-    __shared__ unsigned sbase[SCATTER ? BIN_CAP : 1];
-    __shared__ unsigned s_kmin, s_kmax;
-    const int tid = threadIdx.x;
-    const long long base = (long long)blockIdx.x * (GH_BLOCK * PPT);
-    if (tid == 0) {
-        s_kmin = 0xffffffffu;
-        s_kmax = 0u;
-    }
-    unsigned key[PPT];
-    bool act[PPT];
-    unsigned kmin = 0xffffffffu, kmax = 0u;
-#pragma unroll
-    for (int k = 0; k < PPT; k++) {
-        const long long i = base + k * GH_BLOCK + tid;
-        act[k] = i < b.n;
-        key[k] = act[k] ? bin_of(b, i) : 0u;
-        if (act[k]) {
-            kmin = min(kmin, key[k]);
-            kmax = max(kmax, key[k]);
-        }
-    }
-    __syncthreads();
-    kmin = (unsigned)wave_min((int)min(kmin, 0x7fffffffu));
-    kmax = (unsigned)wave_max((int)kmax);
-    if ((tid & 63) == 0) {
-        atomicMin(&s_kmin, kmin);
-        atomicMax(&s_kmax, kmax);
-    }
-    __syncthreads();
-    const unsigned k0 = s_kmin, k1 = s_kmax;
-    if (k0 <= k1 && k1 - k0 < (unsigned)BIN_CAP) {
-        const int span = (int)(k1 - k0) + 1;
-        for (int j = tid; j < span; j += GH_BLOCK) hist[j] = 0u;
-        __syncthreads();
-        unsigned rank[PPT];
-#pragma unroll
-        for (int k = 0; k < PPT; k++)
-            rank[k] = act[k] ? atomicAdd(&hist[key[k] - k0], 1u) : 0u;
-        __syncthreads();
-        for (int j = tid; j < span; j += GH_BLOCK) {
-            const unsigned c = hist[j];
-            if (c) {
-                const unsigned s0 = atomicAdd(&b.ctr[k0 + j], c);
-                if (SCATTER) sbase[j] = s0;
-            }
-        }
-        if (SCATTER) {
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < PPT; k++) {
-                if (!act[k]) continue;
-                const long long i = base + k * GH_BLOCK + tid;
-                const unsigned slot = sbase[key[k] - k0] + rank[k];
-                b.xv_out[slot] = b.xv_in[i];
-                b.ut_out[slot] = b.ut_in[i];
-                b.zout[slot] = b.zin[i];
-                b.iout[slot] = b.iin[i];
-            }
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < PPT; k++) {
-            const unsigned slot = wave_claim(key[k], act[k], b.ctr);
-            if (SCATTER && act[k]) {
-                const long long i = base + k * GH_BLOCK + tid;
-                b.xv_out[slot] = b.xv_in[i];
-                b.ut_out[slot] = b.ut_in[i];
-                b.zout[slot] = b.zin[i];
-                b.iout[slot] = b.iin[i];
-            }
-        }
-    }
+__global__ __launch_bounds__(GH_BLOCK) void permute_all_kernel(PermArgs<R> p) {
+    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    unsigned s = p.perm[i];
+    p.xv_out[i] = p.xv_in[s];
+    p.ut_out[i] = p.ut_in[s];
+    p.zout[i] = p.zin[s];
+    p.iout[i] = p.iin[s];
 }
 
 // ---- host launchers ---------------------------------------------------------
@@ -710,18 +748,21 @@ hipError_t launch_tmcent(const PushArgs<R> &a, R fac, hipStream_t s) {
 }
 
 template <typename R>
-hipError_t launch_bin_count(const BinArgs<R> &b, hipStream_t s) {
-    if (b.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((bin_pass_kernel<R, false>), dim3(nblocks(b.n, GH_BLOCK * PPT)),
-                       dim3(GH_BLOCK), 0, s, b);
+hipError_t launch_key(const typename Vec2<R>::T *ut, const uint8_t *zone,
+                      long long n, int nr, int na, int nz, int kshift,
+                      unsigned kmax, unsigned *key, unsigned *idx,
+                      hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(key_kernel<R>, dim3(nblocks(n, GH_BLOCK)), dim3(GH_BLOCK),
+                       0, s, ut, zone, n, nr, na, nz, kshift, kmax, key, idx);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_bin_scatter(const BinArgs<R> &b, hipStream_t s) {
-    if (b.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((bin_pass_kernel<R, true>), dim3(nblocks(b.n, GH_BLOCK * PPT)),
-                       dim3(GH_BLOCK), 0, s, b);
+hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
+    if (p.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(permute_all_kernel<R>, dim3(nblocks(p.n, GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, p);
     return hipGetLastError();
 }
 
@@ -731,8 +772,11 @@ hipError_t launch_bin_scatter(const BinArgs<R> &b, hipStream_t s) {
     template hipError_t launch_ut<R>(const PushArgs<R> &, hipStream_t);       \
     template hipError_t launch_gather<R>(const GatherArgs<R> &, hipStream_t); \
     template hipError_t launch_tmcent<R>(const PushArgs<R> &, R, hipStream_t); \
-    template hipError_t launch_bin_count<R>(const BinArgs<R> &, hipStream_t); \
-    template hipError_t launch_bin_scatter<R>(const BinArgs<R> &, hipStream_t);
+    template hipError_t launch_key<R>(const typename Vec2<R>::T *,           \
+                                      const uint8_t *, long long, int, int,   \
+                                      int, int, unsigned, unsigned *,         \
+                                      unsigned *, hipStream_t);               \
+    template hipError_t launch_permute_all<R>(const PermArgs<R> &, hipStream_t);
 GH_INST(float)
 GH_INST(double)
 
