@@ -115,10 +115,10 @@ __device__ __forceinline__ void deposit_global(long long *const *mfx, int nz,
 }
 
 template <typename R>
-// At least 5 wavefronts per SIMD: caps the kernel at 96 VGPRs without
-// spilling (measured: 4 waves at the unconstrained 102 VGPRs is 8 % slower,
-// 6 waves spill).
-__global__ __launch_bounds__(GH_BLOCK, 5) void push_kernel(PushArgs<R> a) {
+// Wavefronts per SIMD, measured: FP32 at least 5 (96 VGPRs, no spill; 4
+// waves at the unconstrained 102 VGPRs is 8 % slower, 6 waves spill); FP64
+// at least 4 (128 VGPRs, a few spills, 4 % faster than 3 waves without).
+__global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(PushArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     typedef typename Vec4<R>::T R4;
     __shared__ long long tile[TILE_REP][TILE_CAP];
@@ -260,10 +260,19 @@ __global__ __launch_bounds__(GH_BLOCK, 5) void push_kernel(PushArgs<R> a) {
     const bool use_ft = fr1 >= 0 && fth * ftw <= FT_CAP;
     if (use_ft) {
         // rows past the last ring read as zero, as the global path does
+        // (row, col) of element j advanced incrementally: one division per
+        // thread instead of one per element
+        const int drow = GH_BLOCK / ftw, dcol = GH_BLOCK - drow * ftw;
+        int row = tid / ftw, col = tid - row * ftw;
         for (int j = tid; j < fth * ftw; j += GH_BLOCK) {
-            const int row = j / ftw, col = j - row * ftw;
             const int ring = fr0 + row;
             int spoke = fref + fd0 + col;
+            row += drow;
+            col += dcol;
+            if (col >= ftw) {
+                col -= ftw;
+                row++;
+            }
             if (spoke < 0) spoke += na;
             else if (spoke >= na) spoke -= na;
             if (spoke >= na) spoke -= na;
