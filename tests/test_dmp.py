@@ -1,0 +1,97 @@
+"""Restart dumps (galaxy_amd/dmp.py) and the step against the reference's
+own run: EXAMPLES/p2d/run100.dmp0 (copied verbatim as tests/golden/
+run100.dmp0) is the reference's start state of its standard test and
+run100.dmp10 (tests/golden/run100_dmp10.npz) the state its galaxy program
+dumped 500 steps later (t = 10).
+
+  - read_dmp decodes the particles, lists, step and mass arrays exactly as
+    the fixture script did, and write_dmp writes the file back byte for
+    byte;
+  - 500 steps from dmp0, by the oracle (CPU) and by the HIP path (GPU, FP32,
+    with periodic re-sorts), land on the reference's dmp10: every particle
+    on the same list, positions within 2e-6 and velocities within 4e-6 of
+    the largest value (the disc is still in its quiet linear phase at
+    t = 10, so rounding differences do not grow; FP32 summation and FFT
+    rounding give the measured 4e-7 and 1e-6).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from galaxy_amd import dmp
+from tests._model import kt_setup
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DMP0 = os.path.join(HERE, "golden", "run100.dmp0")
+
+
+def _fixture(tag):
+    return np.load(os.path.join(HERE, "golden", "run100_%s.npz" % tag))
+
+
+def test_read_matches_fixture_and_roundtrip(tmp_path):
+    d = dmp.read_dmp(DMP0)
+    f = _fixture("dmp0")
+    for k in ("x", "y", "vx", "vy", "iz"):
+        assert np.array_equal(d[k], f[k]), k
+    assert np.array_equal(d["grdmss"][0], f["grdmss1"])
+    assert np.array_equal(d["grdmoz"][1], f["grdmoz2"])
+    assert np.array_equal(d["grdmss"][1], f["grdmss2"])
+    assert d["istep"] == 0 and d["nz"] == 2 and d["nbod"] == 15000
+    out = tmp_path / "copy.dmp"
+    dmp.write_dmp(out, d)
+    assert out.read_bytes() == open(DMP0, "rb").read()
+
+
+def test_write_relinks_lists(tmp_path):
+    d = dmp.read_dmp(DMP0)
+    rng = np.random.default_rng(5)
+    iz = rng.integers(1, d["nz"] + 2, d["nbod"]).astype(np.int32)
+    x = d["x"] + 1.0
+    out = tmp_path / "mod.dmp"
+    dmp.write_dmp(out, d, x=x, iz=iz, istep=40)
+    e = dmp.read_dmp(out)
+    assert np.array_equal(e["iz"], iz) and np.array_equal(e["x"], x.astype(np.float32))
+    assert e["istep"] == 40 and e["noff"] == int(np.sum(iz == d["nz"] + 1))
+
+
+def _against_dmp10(x, y, vx, vy, iz):
+    f = _fixture("dmp10")
+    assert np.array_equal(iz, f["iz"])
+    for k, v, tol in (("x", x, 2e-6), ("y", y, 2e-6), ("vx", vx, 4e-6), ("vy", vy, 4e-6)):
+        ref = f[k].astype(np.float64)
+        err = np.abs(np.asarray(v, np.float64) - ref).max() / np.abs(ref).max()
+        assert err < tol, (k, err)
+
+
+def test_oracle_500_steps_reach_reference_dump(orc):
+    kw, ff, _, _ = kt_setup()
+    d = dmp.read_dmp(DMP0)
+    m = orc.P2D(fixforce=ff, prec="f", use_fx=True, **kw)
+    m.load(d["x"], d["y"], d["vx"], d["vy"], d["iz"])
+    m.masset()
+    for _ in range(500):
+        m.findf(False)
+        m.step()
+    assert m.istep == 500
+    _against_dmp10(m.x, m.y, m.vx, m.vy, m.iz)
+
+
+@pytest.mark.gpu
+def test_gpu_500_steps_reach_reference_dump():
+    from galaxy_amd.lib15 import P2DGrid
+    print("[dmp] 500 steps from run100.dmp0", flush=True)
+    kw, ff, _, _ = kt_setup()
+    g = P2DGrid(precision="fp32", **kw)
+    g.greenm()
+    g.set_fixforce(*ff)
+    dmp.restart(g, dmp.read_dmp(DMP0))
+    g.set_sort_interval(10)
+    g.run(500)
+    istep, _, lstep = g.get_step()
+    assert istep == 500
+    assert np.array_equal(lstep[:, 1], _fixture("dmp10")["lstep"])
+    out = g.unload()
+    _against_dmp10(out["x"], out["y"], out["vx"], out["vy"], out["iz"])
+    g.close()
