@@ -205,6 +205,22 @@ extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
         c->rgrid = exp(c->alpha * (double)(c->nr - 1)) - 1.0;
         c->thmax = (double)na / (double)c->nsect;
     }
+    // outermost cell against the softening length (setgrd.f:181-199): the
+    // reference stops when the radial size of the last cell exceeds 5 softl
+    {
+        double aw;
+        if (c->prec == GALHIP_FP32) {
+            float a = (float)c->alpha;
+            aw = (double)((float)c->rgrid - (expf(a * (float)(c->nr - 2)) - 1.0f));
+        } else {
+            aw = c->rgrid - (exp(c->alpha * (double)(c->nr - 2)) - 1.0);
+        }
+        if (aw / c->softl > 5.0) {
+            delete c;
+            return fail(GALHIP_ERANGE, "SETGRD: Unsuitable choice of grid parameters "
+                        "(outermost cell %.4g times the softening length)", aw / p->softl);
+        }
+    }
     c->ncoef = 2 * c->ng;
     c->istep = 0;
     c->mzone = c->nz;
@@ -448,13 +464,16 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
     size_t es = esize(c);
     std::vector<uint8_t> z(n);
     std::vector<unsigned> ids(n);
+    long long cnt[4] = {0, 0, 0, 0};   // counters: [0] noff, [1] pushes
     for (long long i = 0; i < n; i++) {
         int v = iz[i];
         if (v < 1 || v > c->nz + 1)
             return fail(GALHIP_EINVAL, "particle %lld has zone %d outside 1..%d", i, v, c->nz + 1);
         z[i] = (uint8_t)v;
         ids[i] = (unsigned)i;
+        if (v == c->nz + 1) cnt[0]++;    // /admin/ noff: the off-grid list
     }
+    HIPCHK(hipMemcpy(c->counters, cnt, sizeof cnt, hipMemcpyHostToDevice));
     {
         // interleave (x, y, vx, vy) per particle
         std::vector<unsigned char> buf(4 * es * n);

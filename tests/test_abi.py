@@ -45,3 +45,25 @@ def test_params_struct_layout():
     off = {f[0]: getattr(P2DParams, f[0]).offset for f in P2DParams._fields_}
     assert off["skip"] == 20 and off["softl"] == 88
     assert off["zbr"] == 144 and ctypes.sizeof(P2DParams) == 248
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgalhip.so not built")
+def test_p2dset_rejects_grids_coarse_against_softening():
+    """setgrd.f:181-199: the reference stops when the outermost cell is more
+    than 5 softening lengths wide; galhip_p2dset returns GALHIP_ERANGE there
+    (before any device call, so this runs without a GPU).  Widths here: the
+    last ring of a 24 x 32 grid is rgrid - grofu(22) = 2.0 grid units."""
+    from galaxy_amd import _lib
+    lib = _lib.load()
+    p = _lib.P2DParams()
+    p.nr, p.na, p.hash, p.nsect, p.nskip = 24, 32, 2, 1, 0
+    p.tsoft, p.nzones, p.offanal, p.pmass = 1, 1, 1, 1.0
+    p.nstep[0] = 1
+    p.precision = _lib.GALHIP_FP64
+    import math
+    a = 2 * math.pi / 32
+    width = (math.exp(a * 23) - 1) - (math.exp(a * 22) - 1)
+    ctx = ctypes.c_void_p()
+    p.softl = width / 5.5
+    rc = lib.galhip_p2dset(ctypes.byref(p), ctypes.byref(ctx))
+    assert rc == -3 and b"SETGRD" in lib.galhip_last_error()

@@ -180,12 +180,18 @@ def test_step_chain_bit_exact(orc, prec, op):
     _say("step chain", prec)
     m, parts = oracle_model(orc, prec=op)
     g = gpu_model(precision=prec, parts=parts)
+    _chain(m, g, 8, prec == "fp32")
+
+
+def _chain(m, g, nsteps, exact):
+    """nsteps of findf + step on both sides with the oracle's field injected
+    into the GPU each step; everything compared after every step."""
     m.masset()
     g.masset()
     dt = m.dt
-    exact = prec == "fp32"
-    vmax = max(np.abs(m.vx).max(), np.abs(m.vy).max())
-    for k in range(8):
+    vmax = max(np.abs(m.vx).max() if len(m.vx) else 0.0,
+               np.abs(m.vy).max() if len(m.vy) else 0.0)
+    for k in range(nsteps):
         g.findf()                       # combined mass + ranges on the GPU
         m.findf()
         if exact:
@@ -218,6 +224,65 @@ def test_step_chain_bit_exact(orc, prec, op):
         assert i == m.istep and mz == m.mzone
         assert np.array_equal(ls[:, 1:], m.lstep[:, 1:])
         assert g.noff == m.noff
+
+
+# Grid, filter and zone configurations beyond the default model, each run
+# six steps bit-exactly in FP32 (the reference's option space on the path:
+# sectors and skipped harmonics, many zones, the largest supported na and
+# harmonic count, particles off the grid with and without offanal, a
+# single particle and none).
+CONFIGS = {
+    "sectors2-skip0-1zone-nooff": dict(nr=48, na=96, hash=4, nsect=2, skip=(0,),
+                                       nstep=(1,), zbr=(0.0,), offanal=False),
+    "4zones": dict(nr=48, na=96, hash=6, nstep=(1, 2, 4, 8), zbr=(0.0, 1.0, 2.0, 3.0)),
+    "na2048": dict(nr=24, na=2048, hash=8, n=6000),
+    "ng64": dict(nr=32, na=256, hash=63),
+    "offgrid": dict(nr=40, na=96, hash=4, nstep=(1, 2), zbr=(0.0, 2.0), scale=1.4),
+    "offgrid-nooff": dict(nr=40, na=96, hash=4, nstep=(1, 2), zbr=(0.0, 2.0), scale=1.4,
+                          offanal=False),
+    "one-particle": dict(nr=24, na=96, hash=2, n=1),
+    "no-particles": dict(nr=24, na=96, hash=2, n=0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CONFIGS))
+def test_config_chain_fp32(orc, name):
+    from galaxy_amd import configs
+    from galaxy_amd.lib15 import P2DGrid
+    _say("config chain", name)
+    c = dict(CONFIGS[name])
+    n = c.pop("n", 4000)
+    scale = c.pop("scale", 1.0)
+    w = configs.Workload(name, nr=c["nr"], na=c["na"], hash=c["hash"], n=max(n, 1),
+                         steps=6, dt=0.05, nstep=c.get("nstep", (1, 2, 4)),
+                         zbr_model=c.get("zbr", (0.0, 1.5, 3.5)), seed=17)
+    kw = configs.grid_kwargs(w, max(n, 1))
+    kw.update(nsect=c.get("nsect", 1), offanal=c.get("offanal", True))
+    skip = c.get("skip", ())
+    m = orc.P2D(kw["nr"], kw["na"], kw["hash_"], kw["softl"], tsoft=1, nsect=kw["nsect"],
+                skip=skip, nzones=kw["nzones"], nstep=kw["nstep"], zbr=kw["zbr"],
+                pmass=kw["pmass"], offanal=kw["offanal"], prec="f", use_fx=True)
+    m.fixforce = configs.fixforce(w, float(m.rgrid))
+    x, y, vx, vy, iz = configs.particles(w, max(n, 1), None, float(m.rgrid))
+    if scale != 1.0:
+        x, y = x * scale, y * scale
+        r = np.hypot(x, y)
+        iz = np.where(r > float(m.rgrid), len(kw["nstep"]) + 1, iz).astype(np.int32)
+    x, y, vx, vy, iz = (a[:n] for a in (x, y, vx, vy, iz))
+    m.load(x, y, vx, vy, iz)
+    g = P2DGrid(precision="fp32", skip=skip, **kw)
+    g.greenm()
+    g.set_fixforce(*m.fixforce)
+    g.restrt(x, y, vx, vy, iz)
+    if n == 0:
+        # nothing to assign or move: the calls succeed and the grid stays empty
+        g.masset()
+        g.findf()
+        g.step()
+        mfx, _, _, _, _ = g.get_mass()
+        assert not mfx.any() and g.noff == 0
+        return
+    _chain(m, g, 6, True)
 
 
 def test_ftest_fp64(orc):
