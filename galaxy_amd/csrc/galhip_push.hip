@@ -152,13 +152,19 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
         s_kmax = -1;
     }
 
+    // the block's particles through block-based pointers and 32-bit
+    // offsets (uniform base, per-lane offset: no 64-bit address arithmetic)
+    R4 *const xvb = a.xv + base;
+    R2 *const utb = a.ut + base;
+    uint8_t *const zb = a.zone + base;
+    const int nblk = (int)min((long long)(GH_BLOCK * PPT), a.n - base);
     int zin[PPT];
     bool mv[PPT];
     int any = 0;
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        long long i = base + k * GH_BLOCK + tid;
-        zin[k] = i < a.n ? (int)a.zone[i] : 0;
+        const int j = k * GH_BLOCK + tid;
+        zin[k] = j < nblk ? (int)zb[j] : 0;
         bool off = zin[k] == nlists;
         mv[k] = zin[k] > 0 &&
                 (off ? (a.offanal && a.mzone == nz) : (zin[k] <= a.mzone));
@@ -176,10 +182,9 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     R2 cu[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        long long i = base + k * GH_BLOCK + tid;
-        if (i >= a.n) i = a.n - 1;
-        st[k] = a.xv[i];
-        cu[k] = a.ut[i];
+        const int j = min(k * GH_BLOCK + tid, nblk - 1);
+        st[k] = xvb[j];
+        cu[k] = utb[j];
     }
     // grdacc: weights of the old position from its (u, t) (off-grid and
     // idle particles weigh nothing), the field read from an LDS copy of the
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
         dring[k] = 0;
         dspoke[k] = 0;
         if (mv[k]) {
-            const long long i = base + k * GH_BLOCK + tid;
+            const int j = k * GH_BLOCK + tid;
             const int z0 = zin[k];
             const bool offlist = z0 == nlists;
             R ox = st[k].x, oy = st[k].y, ovx = st[k].z, ovy = st[k].w;
@@ -392,12 +397,12 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
                     znew = kz;
                 }
             }
-            a.xv[i] = mk4(nx, ny, nvx, nvy);
-            if (znew != z0) a.zone[i] = (uint8_t)znew;
+            xvb[j] = mk4(nx, ny, nvx, nvy);
+            if (znew != z0) zb[j] = (uint8_t)znew;
             if (znew >= 1 && znew <= nz) {
                 R u, t;
                 ut_of(nx, ny, rnew, a.k, u, t);
-                a.ut[i] = mk2(u, t);
+                utb[j] = mk2(u, t);
                 R tf = a.nsect > 1 ? (R)fmod(t, a.thmax) : t;
                 cic(u, tf, nr, na, dring[k], dspoke[k], dw[k]);
                 dzone[k] = znew;
