@@ -43,6 +43,13 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE x 1); profiles/push_kernel_p2d-256x512-n1e7-3zones_traffic.txt.
 PUSH_HBM_READ_B = 25.5
 PUSH_HBM_WRITE_B = 25.3
+# VALU instructions per push of push_kernel<float> (rocprofv3 SQ_INSTS_VALU of
+# the all-zones launch x 64 lanes / 10M pushes;
+# profiles/push_kernel_p2d-256x512-n1e7-3zones_sq_counters.txt) and MI355X's
+# VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz (one wave64 instruction per
+# CU per cycle, non-packed).
+PUSH_VALU_PER_PUSH = 521.0
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 
 def log(*a):
@@ -71,6 +78,8 @@ def run_gpu(w, precision, n, steps, warmup, sort_interval):
     log("[bench] %s set-up %.1f s (N=%d)" % (precision, time.time() - t0, n))
     if warmup:
         g.run(warmup)
+    g.findf(True)                      # energy at the start (outside the timing)
+    e0 = g.integrals()
     g.sync()
     g.set_profile(False)               # zero the push counter, no events
     t0 = time.perf_counter()
@@ -83,15 +92,23 @@ def run_gpu(w, precision, n, steps, warmup, sort_interval):
     g.sync()
     push_ms, launches, pushes_p = g.get_profile()
     g.set_profile(False)
+    g.findf(True)
+    e1 = g.integrals()
     out = g.unload()
     res = dict(seconds=t1 - t0, pushes=pushes, push_ms=push_ms,
                launches=launches, pushes_prof=pushes_p,
-               a2=a2(out["x"], out["y"], w.lscale()), noff=g.noff)
+               a2=a2(out["x"], out["y"], w.lscale()), noff=g.noff,
+               e0=e0["energy"], e1=e1["energy"], lz0=e0["lz"], lz1=e1["lz"])
     log("[bench] %s: %d steps in %.4f s, %d pushes; profiled pass: push kernel "
         "%.3f ms over %d launches, %d pushes" % (precision, steps, t1 - t0, pushes,
                                                 push_ms, launches, pushes_p))
     g.close()
     return res
+
+
+def drift(r):
+    """Relative energy change over the run (icheck's integrals, FP64 sums)."""
+    return (r["e1"] - r["e0"]) / abs(r["e0"])
 
 
 def cpu_baseline(w, n, steps):
@@ -178,8 +195,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_n, args.cpu_steps)
         log("[bench] cpu baseline: %.3g pushes/s (%s)" % (cpu["value"], cpu["sample"]))
+    in_kernel = mixed["pushes_prof"] / (mixed["push_ms"] * 1e-3)
     extra = dict(
         n_steps_rate=n * args.steps / mixed["seconds"],
+        push_valu_frac=round(in_kernel * PUSH_VALU_PER_PUSH / VALU_PEAK_LANE_OPS, 3),
         pushes_per_step=mixed["pushes"] / args.steps,
         sort_interval=args.sort_interval,
     )
@@ -188,7 +207,11 @@ def main():
         extra.update(fp64_value=v64, mixed_over_fp64=value / v64,
                      fp64_ms_per_step=1e3 * fp64["seconds"] / args.steps,
                      a2_mixed=abs(mixed["a2"]), a2_fp64=abs(fp64["a2"]),
-                     m2_err=abs(mixed["a2"] - fp64["a2"]))
+                     m2_err=abs(mixed["a2"] - fp64["a2"]),
+                     energy_drift_mixed=drift(mixed), energy_drift_fp64=drift(fp64),
+                     energy_drift_err=abs(drift(mixed) - drift(fp64)),
+                     lz_drift_mixed=(mixed["lz1"] - mixed["lz0"]) / abs(mixed["lz0"]),
+                     lz_drift_fp64=(fp64["lz1"] - fp64["lz0"]) / abs(fp64["lz0"]))
     line = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

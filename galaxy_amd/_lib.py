@@ -64,6 +64,8 @@ SIGNATURES = {
     "galhip_set_field": [_vp, _vp],
     "galhip_get_ranges": [_vp, _vp],
     "galhip_get_noff": [_vp, _vp],
+    "galhip_set_fixpot": [_vp, _i32, _dbl, _vp, _vp],
+    "galhip_integrals": [_vp, _vp],
     "galhip_set_sort_interval": [_vp, _i32],
     "galhip_sort": [_vp],
     "galhip_get_order": [_vp, _vp],

@@ -91,6 +91,12 @@ def fixforce(w: Workload, rgrid=None):
     return disc.halo_table(rg, w.lscale(), w.dt, md=w.md, rcore=w.rcore)
 
 
+def fixpot(w: Workload, rgrid=None):
+    """The halo's potential (energy diagnostics), on fixforce's grid."""
+    rg = w.rgrid_d() if rgrid is None else rgrid
+    return disc.halo_potential_table(rg, w.lscale(), w.dt, md=w.md, rcore=w.rcore)
+
+
 def make_grid(w: Workload, precision="fp32", n=None, seed=None, green=True):
     """A ready P2DGrid for workload w: Green's tables, halo table, particles
     loaded and assigned (masset)."""
@@ -100,6 +106,7 @@ def make_grid(w: Workload, precision="fp32", n=None, seed=None, green=True):
         g.greenm()
     dr, val, der = fixforce(w, g.rgrid)
     g.set_fixforce(dr, val, der)
+    g.set_fixpot(*fixpot(w, g.rgrid))
     x, y, vx, vy, iz = particles(w, n, seed, g.rgrid)
     g.restrt(x, y, vx, vy, iz)
     g.masset()

@@ -26,6 +26,9 @@ template <typename R> hipError_t launch_gather(const GatherArgs<R> &, hipStream_
 template <typename R> hipError_t launch_ut(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_tmcent(const PushArgs<R> &, R, hipStream_t);
 template <typename R>
+hipError_t launch_integrals(const PushArgs<R> &, const R *, const double *, const double *,
+                            int, double, double *, double *, hipStream_t);
+template <typename R>
 hipError_t launch_key(const typename Vec2<R>::T *, const uint8_t *, long long,
                       int, int, int, int, unsigned, unsigned *, unsigned *,
                       hipStream_t);
@@ -110,6 +113,9 @@ struct galhip_ctx {
     void *rho, *coef, *fld2, *pot, *G, *tw;
     double *Gd, *twd;
     double *fr_val, *fr_der, fr_dr;
+    double *fp_val, *fp_der, fp_dr;   // fixed potential table (diagnostics)
+    int nfp;
+    int pot_valid;                    // the last solve formed the potential
     int nfr, qbits, ncoef;
     int sort_interval, steps_since_sort;
     // profiling
@@ -289,6 +295,7 @@ extern "C" int galhip_free(galhip_ctx *c) {
     free_particles(c);
     for (int z = 0; z < GH_MAXZ; z++) { dfree(c->mfx[z]); dfree(c->moz[z]); }
     dfree(c->jrad); dfree(c->krad); dfree(c->ranges); dfree(c->lg_d); dfree(c->counters);
+    dfree(c->fp_val); dfree(c->fp_der);
     dfree(c->rho); dfree(c->coef); dfree(c->fld2); dfree(c->pot); dfree(c->tw); dfree(c->twd);
     if (c->G != (void *)c->Gd) dfree(c->G);
     dfree(c->Gd);
@@ -367,6 +374,26 @@ extern "C" int galhip_set_fixforce(galhip_ctx *c, int32_t n, double dr,
     HIPCHK(hipMemcpy(c->fr_der, der, sizeof(double) * n, hipMemcpyHostToDevice));
     c->nfr = n;
     c->fr_dr = dr;
+    return 0;
+}
+
+// Fixed potential table for the energy diagnostics, as galhip_set_fixforce.
+extern "C" int galhip_set_fixpot(galhip_ctx *c, int32_t n, double dr,
+                                 const double *val, const double *der) {
+    if (!c) return fail(GALHIP_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    dfree(c->fp_val);
+    dfree(c->fp_der);
+    c->nfp = 0;
+    if (n <= 0) return 0;
+    if (n < 2 || !(dr > 0.0) || !val || !der)
+        return fail(GALHIP_EINVAL, "fixed potential table needs n >= 2, dr > 0");
+    int rc = dalloc(&c->fp_val, sizeof(double) * n) | dalloc(&c->fp_der, sizeof(double) * n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(c->fp_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fp_der, der, sizeof(double) * n, hipMemcpyHostToDevice));
+    c->nfp = n;
+    c->fp_dr = dr;
     return 0;
 }
 
@@ -718,6 +745,7 @@ static int solve_impl(galhip_ctx *c, int lpot, int explicit_zones, int jzone,
     SynArgs<R> b = syn_args<R>(c, lpot, masscl_z);
     HIPCHK(launch_synth<R>(b, c->s));
     c->have_field = 1;
+    c->pot_valid = lpot != 0;
     return 0;
 }
 
@@ -788,6 +816,35 @@ extern "C" int galhip_tmcent(galhip_ctx *c, int32_t start) {
                     c->tcent ? "up" : "down");
     if (!c->have_field) return fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
     return c->prec == GALHIP_FP32 ? tmcent_impl<float>(c, start) : tmcent_impl<double>(c, start);
+}
+
+template <typename R> static int integrals_impl(galhip_ctx *c, double *out) {
+    const long long nb = c->n > 0 ? (c->n + GH_BLOCK - 1) / GH_BLOCK : 0;
+    double *part = nullptr, *dout = nullptr;
+    int rc = dalloc(&part, sizeof(double) * 9 * (nb > 0 ? nb : 1)) |
+             dalloc(&dout, sizeof(double) * 9);
+    if (!rc) {
+        PushArgs<R> a = push_args<R>(c);
+        hipError_t e = launch_integrals<R>(a, c->pot_valid ? (const R *)c->pot : nullptr,
+                                           c->fp_val, c->fp_der, c->nfp, c->fp_dr, part,
+                                           dout, c->s);
+        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 9,
+                                                hipMemcpyDeviceToHost, c->s);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+        if (e != hipSuccess) rc = fail(GALHIP_EHIP, "integrals: %s", hipGetErrorString(e));
+    }
+    dfree(part);
+    dfree(dout);
+    if (rc) return rc;
+    // particle sums times the particle mass; no potential: NaN
+    for (int q = 0; q < 9; q++) out[q] *= c->pmass;
+    if (!c->pot_valid) out[7] = NAN;
+    return 0;
+}
+
+extern "C" int galhip_integrals(galhip_ctx *c, double *out) {
+    if (!c || !out) return fail(GALHIP_EINVAL, "null argument");
+    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, out) : integrals_impl<double>(c, out);
 }
 
 extern "C" int galhip_run(galhip_ctx *c, int32_t nsteps) {

@@ -666,6 +666,96 @@ __global__ __launch_bounds__(GH_BLOCK) void tmcent_kernel(PushArgs<R> a, R fac) 
     a.xv[i] = st;
 }
 
+// ---- global integrals (icheck, SRC/lib15/icheck.f:24-70) ------------------
+//
+// Per particle, in FP64: 1, x, y, vx, vy, v^2, x vy - y vx, the grid
+// potential interpolated with the CIC weights of its cached (u, t) (zones
+// 1..nz, when a potential is given), the fixed potential (Hermite table,
+// when given).  Velocities are per zone-1 step: the stored ones carry the
+// step factor of their zone (the off-grid list that of the top zone).  Each
+// block writes its sums; a second pass adds the blocks in a fixed order, so
+// the result does not depend on scheduling.
+constexpr int NINT = 9;
+
+__device__ __forceinline__ double wave_sumd(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void integrals_kernel(PushArgs<R> a, const R *pot,
+                                                             const double *fp_val,
+                                                             const double *fp_der, int nfp,
+                                                             double fp_dr, double *part) {
+    __shared__ double red[GH_BLOCK / 64][NINT];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long i = (long long)blockIdx.x * GH_BLOCK + tid;
+    double v[NINT];
+#pragma unroll
+    for (int q = 0; q < NINT; q++) v[q] = 0.0;
+    if (i < a.n) {
+        const int z = a.zone[i];
+        if (z >= 1) {
+            const typename Vec4<R>::T st = a.xv[i];
+            const int zs = z <= a.nz ? z : a.nz;
+            const double sf = 1.0 / (double)a.nstep[zs - 1];
+            const double x = st.x, y = st.y, vx = st.z * sf, vy = st.w * sf;
+            v[0] = 1.0;
+            v[1] = x;
+            v[2] = y;
+            v[3] = vx;
+            v[4] = vy;
+            v[5] = vx * vx + vy * vy;
+            v[6] = x * vy - y * vx;
+            if (pot && z <= a.nz) {
+                R w[4];
+                int ring, spoke, ic[4];
+                const typename Vec2<R>::T cu = a.ut[i];
+                cic(cu.x, cu.y, a.nr, a.na, ring, spoke, w);
+                corners_rs(ring, spoke, a.na, ic);
+                double ph = 0.0;
+                for (int q = 0; q < 4; q++)
+                    if (ic[q] < a.mesh) ph += (double)w[q] * (double)pot[ic[q]];
+                v[7] = ph;
+            }
+            if (nfp > 0)
+                v[8] = fixforce(sqrt(x * x + y * y), nfp, fp_dr, 1.0 / fp_dr, fp_val, fp_der);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NINT; q++) {
+        const double t = wave_sumd(v[q]);
+        if (lane == 0) red[wave][q] = t;
+    }
+    __syncthreads();
+    if (tid < NINT) {
+        double t = 0.0;
+        for (int w = 0; w < GH_BLOCK / 64; w++) t += red[w][tid];
+        part[(long long)blockIdx.x * NINT + tid] = t;
+    }
+}
+
+// out[q] = sum over blocks of part[b][q], blocks in order (one block)
+__global__ __launch_bounds__(GH_BLOCK) void integrals_sum_kernel(const double *part, int nb,
+                                                                 double *out) {
+    __shared__ double red[GH_BLOCK / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int q = 0; q < NINT; q++) {
+        double t = 0.0;
+        for (int b = tid; b < nb; b += GH_BLOCK) t += part[(long long)b * NINT + q];
+        t = wave_sumd(t);
+        if (lane == 0) red[wave] = t;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < GH_BLOCK / 64; w++) s += red[w];
+            out[q] = s;
+        }
+        __syncthreads();
+    }
+}
+
 // sort key: the cell of the particle's (u, t) shifted right by kshift so
 // that keys fit 16 bits (two radix passes); off-grid particles last.
 template <typename R>
@@ -754,6 +844,19 @@ hipError_t launch_gather(const GatherArgs<R> &a, hipStream_t s) {
 }
 
 template <typename R>
+hipError_t launch_integrals(const PushArgs<R> &a, const R *pot, const double *fp_val,
+                            const double *fp_der, int nfp, double fp_dr, double *part,
+                            double *out, hipStream_t s) {
+    const unsigned nb = a.n > 0 ? nblocks(a.n, GH_BLOCK) : 0;
+    if (nb)
+        hipLaunchKernelGGL(integrals_kernel<R>, dim3(nb), dim3(GH_BLOCK), 0, s, a, pot,
+                           fp_val, fp_der, nfp, fp_dr, part);
+    hipLaunchKernelGGL(integrals_sum_kernel, dim3(1), dim3(GH_BLOCK), 0, s,
+                       (const double *)part, (int)nb, out);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_tmcent(const PushArgs<R> &a, R fac, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(tmcent_kernel<R>, dim3(nblocks(a.n, GH_BLOCK)),
@@ -786,6 +889,9 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
     template hipError_t launch_ut<R>(const PushArgs<R> &, hipStream_t);       \
     template hipError_t launch_gather<R>(const GatherArgs<R> &, hipStream_t); \
     template hipError_t launch_tmcent<R>(const PushArgs<R> &, R, hipStream_t); \
+    template hipError_t launch_integrals<R>(const PushArgs<R> &, const R *,     \
+                                            const double *, const double *, int, \
+                                            double, double *, double *, hipStream_t); \
     template hipError_t launch_key<R>(const typename Vec2<R>::T *,           \
                                       const uint8_t *, long long, int, int,   \
                                       int, int, unsigned, unsigned *,         \

@@ -83,6 +83,34 @@ def halo_table(rgrid, lscale, ts, md=0.5, rcore=0.1, n=4097):
     return dr, val, der
 
 
+def disc_potential(R, md=0.5):
+    """Potential of the untruncated exponential disc (Freeman):
+    -(md/2) R [I0(R/2) K1(R/2) - I1(R/2) K0(R/2)], -md at R = 0."""
+    R = np.asarray(R, np.float64)
+    y = 0.5 * np.abs(R)
+    yy = np.where(y > 0, y, 1.0)
+    _, _, i1k0, i0k1 = _bessel_terms(yy)
+    return np.where(y > 0, -0.5 * md * np.abs(R) * (i0k1 - i1k0), -md)
+
+
+def halo_potential(R, md=0.5, rcore=0.1):
+    """Potential of the rigid halo of halo_force: (1/2) ln(R^2 + rcore^2)
+    (the flat-curve total) minus the disc's."""
+    R = np.asarray(R, np.float64)
+    return 0.5 * np.log(R * R + rcore * rcore) - disc_potential(R, md)
+
+
+def halo_potential_table(rgrid, lscale, ts, md=0.5, rcore=0.1, n=4097):
+    """The halo potential in internal units (velocity^2: (lscale ts)^2 times
+    the model value) on the halo_table grid, derivative = -force."""
+    dr = rgrid / (n - 2)
+    r = dr * np.arange(n)
+    R = r / lscale
+    val = halo_potential(R, md, rcore) * (lscale * ts) ** 2
+    der = -halo_force(R, md, rcore) * lscale * ts * ts
+    return dr, val, der
+
+
 def soft_kt_table(rgrid, lscale, ts, rstar, mass=1.0, n=4097):
     """Fixed radial force of a softened Kuz'min-Toomre disc (disc type SOFT:
     F(R) = -M R / (rstar^2 + R^2)^1.5 with rstar = 1 + epsilon,
@@ -96,6 +124,18 @@ def soft_kt_table(rgrid, lscale, ts, rstar, mass=1.0, n=4097):
     s = a2 + R * R
     val = -mass * R / s ** 1.5 * lscale * ts * ts
     der = -mass * (a2 - 2.0 * R * R) / s ** 2.5 * ts * ts
+    return dr, val, der
+
+
+def soft_kt_potential_table(rgrid, lscale, ts, rstar, mass=1.0, n=4097):
+    """Potential -M / sqrt(rstar^2 + R^2) of soft_kt_table's disc (phidsc
+    'SOFT', SRC/lib15/phidsc.f:38) in internal units, derivative = -force."""
+    dr = rgrid / (n - 2)
+    r = dr * np.arange(n)
+    R = r / lscale
+    s = rstar * rstar + R * R
+    val = -mass / np.sqrt(s) * (lscale * ts) ** 2
+    der = mass * R / s ** 1.5 * lscale * ts * ts
     return dr, val, der
 
 

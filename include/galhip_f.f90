@@ -179,6 +179,19 @@ module galhip_f
        type(c_ptr), value :: ctx
        integer(c_int64_t), intent(out) :: noff
      end function galhip_get_noff
+     integer(c_int) function galhip_set_fixpot(ctx, n, dr, val, der) &
+          bind(c, name='galhip_set_fixpot')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       real(c_double), value :: dr
+       real(c_double), intent(in) :: val(*), der(*)
+     end function galhip_set_fixpot
+     integer(c_int) function galhip_integrals(ctx, out) bind(c, name='galhip_integrals')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out(9)
+     end function galhip_integrals
      integer(c_int) function galhip_set_sort_interval(ctx, k) &
           bind(c, name='galhip_set_sort_interval')
        import :: c_int, c_ptr, c_int32_t

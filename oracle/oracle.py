@@ -434,6 +434,42 @@ class P2D:
         self._zoncmb_scaled()
         return dn
 
+    def integrals(self, fixpot=None):
+        """icheck's global integrals (SRC/lib15/icheck.f:24-70) in FP64 numpy,
+        as galhip_integrals defines them: mass, cm (mass x position),
+        momentum, sum m v^2, Lz with velocities per zone-1 step; sum m Phi
+        over the grid's particles from the potential of the last
+        findf(lpot=True), CIC-interpolated at the particle's own cell and
+        weights; sum m Phi_fixed from a (dr, val, der) Hermite table."""
+        x = self.x.astype(np.float64)
+        y = self.y.astype(np.float64)
+        iz = self.iz
+        zs = np.minimum(iz, self.nzones)
+        sf = 1.0 / np.asarray(self.nstep, np.float64)[np.maximum(zs, 1) - 1]
+        vx = self.vx.astype(np.float64) * sf
+        vy = self.vy.astype(np.float64) * sf
+        on = iz >= 1
+        pm = float(self.pmass)
+        out = np.zeros(9)
+        out[0] = pm * on.sum()
+        out[1], out[2] = pm * x[on].sum(), pm * y[on].sum()
+        out[3], out[4] = pm * vx[on].sum(), pm * vy[on].sum()
+        out[5] = pm * (vx[on] ** 2 + vy[on] ** 2).sum()
+        out[6] = pm * (x[on] * vy[on] - y[on] * vx[on]).sum()
+        grid = (iz >= 1) & (iz <= self.nzones)
+        cell, w, _ = weight(self.x[grid], self.y[grid], self.nr, self.na, self.prec)
+        ring, spoke = cell // self.na, cell % self.na
+        s2 = np.where(spoke + 1 == self.na, 0, spoke + 1)
+        ic = np.stack([ring * self.na + spoke, ring * self.na + s2,
+                       (ring + 1) * self.na + spoke, (ring + 1) * self.na + s2], 1)
+        pot = np.append(self.fld[2].astype(np.float64), 0.0)
+        ic = np.where(ic < self.mesh, ic, self.mesh)
+        out[7] = pm * (w.astype(np.float64) * pot[ic]).sum()
+        if fixpot is not None:
+            dr, val, der = fixpot
+            out[8] = pm * fixforce(np.hypot(x[on], y[on]), dr, val, der).sum()
+        return out
+
     def tmcent(self, start=True, field=None):
         """tmcent (SRC/lib15/tmcent.f, tmcgrp.f:26-36, retarded-velocity
         leap-frog): every particle's velocity minus (start) or plus half its

@@ -361,3 +361,40 @@ def test_sort_groups_by_cell(orc):
     on = (out["iz"] >= 1) & (out["iz"] <= nz)
     key = np.where(on, cell, nr * na)
     assert np.all(np.diff(key[ids]) >= 0)
+
+
+@pytest.mark.parametrize("prec,op", PRECS)
+def test_integrals_match_oracle(orc, prec, op):
+    """galhip_integrals (icheck's global integrals, FP64 sums) against the
+    oracle's numpy sums over the same particles: mass, centre of mass,
+    momentum, kinetic energy and Lz; the grid potential energy to
+    the solve's own accuracy; the fixed-potential energy from the same
+    table.  Without a potential the grid term
+    is NaN."""
+    from galaxy_amd import disc
+    _say("integrals", prec)
+    m, parts = oracle_model(orc, prec=op)
+    g = gpu_model(precision=prec, parts=parts)
+    fp = disc.halo_potential_table(float(g.rgrid), SMALL.lscale(), SMALL.dt)
+    g.set_fixpot(*fp)
+    m.masset()
+    g.masset()
+    g.findf(False)
+    assert np.isnan(g.integrals()["w_grid"])
+    for _ in range(3):
+        m.findf()
+        m.step()
+    g.run(3)
+    m.findf(lpot=True)
+    g.findf(True)
+    ref = m.integrals(fixpot=fp)
+    got = g.integrals()
+    vec = np.concatenate([[got["mass"]], got["cm"] * got["mass"], got["momentum"],
+                          [2 * got["ke"], got["lz"], got["w_grid"], got["w_fixed"]]])
+    scale = np.abs(ref).max()
+    # the three steps ran on each side's own solve: positions agree to the
+    # solves' accuracy (FP64 1e-10, FP32 1e-5 in the force), hence the bounds
+    tol_kin = 1e-10 if prec == "fp64" else 1e-6
+    assert np.abs(vec[:7] - ref[:7]).max() <= tol_kin * scale, (vec[:7], ref[:7])
+    assert abs(vec[7] - ref[7]) <= (1e-9 if prec == "fp64" else 2e-5) * abs(ref[7])
+    assert abs(vec[8] - ref[8]) <= (1e-10 if prec == "fp64" else 1e-6) * abs(ref[8])

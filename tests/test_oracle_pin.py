@@ -292,3 +292,48 @@ def test_chstep_identity_and_zone_scaling(orc):
     assert float(m.y[0]) == y1
     m.step(field=zero)
     assert np.isclose(m.y[0], 0.4)
+
+
+def test_fixed_potentials_match_their_forces():
+    """The diagnostics' fixed potentials: their derivative is minus the
+    fixed force the step applies (central differences on the tables'
+    grids), for the exponential-disc workload's halo and the softened
+    Kuz'min-Toomre disc; the Freeman disc potential is -md at the centre."""
+    from galaxy_amd import disc
+    for pot, force in (
+            (lambda: disc.halo_potential_table(20.0, 3.3, 0.025, n=2049),
+             lambda: disc.halo_table(20.0, 3.3, 0.025, n=2049)),
+            (lambda: disc.soft_kt_potential_table(90.0, 10.0, 0.02, 1.367879, n=2049),
+             lambda: disc.soft_kt_table(90.0, 10.0, 0.02, 1.367879, n=2049))):
+        dr, val, der = pot()
+        dr2, fval, _ = force()
+        assert dr == dr2
+        num = (val[2:] - val[:-2]) / (2 * dr)
+        scale = np.abs(fval).max()
+        # central differences: O(dr^2) error, largest inside the halo core
+        assert np.abs(num - der[1:-1]).max() < 1e-3 * scale
+        assert np.abs(der + fval).max() <= 1e-15 * scale
+    assert abs(disc.disc_potential(np.array([0.0]))[0] + 0.5) < 1e-15
+    R = np.linspace(0.05, 8, 400)
+    h = 1e-5
+    dphi = (disc.disc_potential(R + h) - disc.disc_potential(R - h)) / (2 * h)
+    assert np.abs(dphi + disc.disc_force(R)).max() < 1e-8
+
+
+def test_oracle_integrals_closed_form(orc):
+    """Zone scaling and the kinematic sums of the integrals on hand-placed
+    particles: a zone-2 particle stores twice its per-step velocity."""
+    m = orc.P2D(16, 32, 2, 0.5, nzones=2, nstep=(1, 2), zbr=(0.0, 3.0), pmass=0.25,
+                prec="d")
+    x = np.array([1.0, 5.0, -2.0])
+    y = np.array([0.0, 0.0, 1.0])
+    vx = np.array([0.0, 0.0, 0.3])
+    vy = np.array([0.5, 2.0, 0.0])             # particle 2 is in zone 2
+    m.load(x, y, vx, vy, np.array([1, 2, 1], np.int32))
+    out = m.integrals()
+    vy1 = np.array([0.5, 1.0, 0.0])
+    assert np.isclose(out[0], 0.75)
+    assert np.allclose(out[1:3], 0.25 * np.array([4.0, 1.0]))
+    assert np.allclose(out[3:5], 0.25 * np.array([0.3, 1.5]))
+    assert np.isclose(out[5], 0.25 * (0.25 + 1.0 + 0.09))
+    assert np.isclose(out[6], 0.25 * (1.0 * 0.5 + 5.0 * 1.0 - 1.0 * 0.3))
