@@ -65,7 +65,7 @@ SIGNATURES = {
     "galhip_get_ranges": [_vp, _vp],
     "galhip_get_noff": [_vp, _vp],
     "galhip_set_fixpot": [_vp, _i32, _dbl, _vp, _vp],
-    "galhip_integrals": [_vp, _vp],
+    "galhip_integrals": [_vp, _i32, _vp],
     "galhip_set_sort_interval": [_vp, _i32],
     "galhip_sort": [_vp],
     "galhip_get_order": [_vp, _vp],

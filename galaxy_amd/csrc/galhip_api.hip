@@ -27,7 +27,7 @@ template <typename R> hipError_t launch_ut(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_tmcent(const PushArgs<R> &, R, hipStream_t);
 template <typename R>
 hipError_t launch_integrals(const PushArgs<R> &, const R *, const double *, const double *,
-                            int, double, double *, double *, hipStream_t);
+                            int, double, int, double *, double *, hipStream_t);
 template <typename R>
 hipError_t launch_key(const typename Vec2<R>::T *, const uint8_t *, long long,
                       int, int, int, int, unsigned, unsigned *, unsigned *,
@@ -818,7 +818,7 @@ extern "C" int galhip_tmcent(galhip_ctx *c, int32_t start) {
     return c->prec == GALHIP_FP32 ? tmcent_impl<float>(c, start) : tmcent_impl<double>(c, start);
 }
 
-template <typename R> static int integrals_impl(galhip_ctx *c, double *out) {
+template <typename R> static int integrals_impl(galhip_ctx *c, int centred, double *out) {
     const long long nb = c->n > 0 ? (c->n + GH_BLOCK - 1) / GH_BLOCK : 0;
     double *part = nullptr, *dout = nullptr;
     int rc = dalloc(&part, sizeof(double) * 9 * (nb > 0 ? nb : 1)) |
@@ -826,8 +826,8 @@ template <typename R> static int integrals_impl(galhip_ctx *c, double *out) {
     if (!rc) {
         PushArgs<R> a = push_args<R>(c);
         hipError_t e = launch_integrals<R>(a, c->pot_valid ? (const R *)c->pot : nullptr,
-                                           c->fp_val, c->fp_der, c->nfp, c->fp_dr, part,
-                                           dout, c->s);
+                                           c->fp_val, c->fp_der, c->nfp, c->fp_dr,
+                                           centred, part, dout, c->s);
         if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 9,
                                                 hipMemcpyDeviceToHost, c->s);
         if (e == hipSuccess) e = hipStreamSynchronize(c->s);
@@ -842,9 +842,12 @@ template <typename R> static int integrals_impl(galhip_ctx *c, double *out) {
     return 0;
 }
 
-extern "C" int galhip_integrals(galhip_ctx *c, double *out) {
+extern "C" int galhip_integrals(galhip_ctx *c, int32_t centred, double *out) {
     if (!c || !out) return fail(GALHIP_EINVAL, "null argument");
-    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, out) : integrals_impl<double>(c, out);
+    if (centred && !c->have_field)
+        return fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out)
+                                  : integrals_impl<double>(c, centred != 0, out);
 }
 
 extern "C" int galhip_run(galhip_ctx *c, int32_t nsteps) {

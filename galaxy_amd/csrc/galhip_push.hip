@@ -621,22 +621,20 @@ __global__ __launch_bounds__(GH_BLOCK) void gather_kernel(GatherArgs<R> a) {
     }
 }
 
-// tmcent / tmcgrp for the retarded-velocity leap-frog (tmcent.f,
-// tmcgrp.f:26-36): every particle's velocity minus fac times its getacc
-// acceleration (grid interpolation from its cached (u, t), fixed radial
-// force, zone rescaling; none off the grid), in the push's operation order.
+// getacc for one particle of list z0 (getacc.f:35-163): grid acceleration
+// interpolated with the CIC weights of its cached (u, t) (grdacc.f:40-66),
+// fixed radial force, zone rescaling; none off the grid.  The push's
+// operation order.
 template <typename R>
-__global__ __launch_bounds__(GH_BLOCK) void tmcent_kernel(PushArgs<R> a, R fac) {
+__device__ __forceinline__ void getacc1(const PushArgs<R> &a, const typename Vec4<R>::T &st,
+                                        const typename Vec2<R>::T &cu, int z0, R &ax,
+                                        R &ay) {
     typedef typename Vec2<R>::T R2;
-    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
-    if (i >= a.n) return;
-    const int z0 = a.zone[i];
-    typename Vec4<R>::T st = a.xv[i];
-    R ax = 0, ay = 0;
+    ax = 0;
+    ay = 0;
     if (z0 != a.nz + 1) {
         R w[4];
         int ring, spoke, ic[4];
-        const R2 cu = a.ut[i];
         cic(cu.x, cu.y, a.nr, a.na, ring, spoke, w);
         corners_rs(ring, spoke, a.na, ic);
         for (int q = 0; q < 4; q++) {
@@ -661,6 +659,18 @@ __global__ __launch_bounds__(GH_BLOCK) void tmcent_kernel(PushArgs<R> a, R fac) 
         ax = t * ax;
         ay = t * ay;
     }
+}
+
+// tmcent / tmcgrp for the retarded-velocity leap-frog (tmcent.f,
+// tmcgrp.f:26-36): every particle's velocity minus fac times its getacc
+// acceleration.
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void tmcent_kernel(PushArgs<R> a, R fac) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    typename Vec4<R>::T st = a.xv[i];
+    R ax, ay;
+    getacc1(a, st, a.ut[i], (int)a.zone[i], ax, ay);
     st.z = st.z - fac * ax;
     st.w = st.w - fac * ay;
     a.xv[i] = st;
@@ -672,7 +682,9 @@ __global__ __launch_bounds__(GH_BLOCK) void tmcent_kernel(PushArgs<R> a, R fac) 
 // potential interpolated with the CIC weights of its cached (u, t) (zones
 // 1..nz, when a potential is given), the fixed potential (Hermite table,
 // when given).  Velocities are per zone-1 step: the stored ones carry the
-// step factor of their zone (the off-grid list that of the top zone).  Each
+// step factor of their zone (the off-grid list that of the top zone).
+// centred: the velocities time-centred as icheck takes them, v + a/2 in
+// the working precision (anlgrp's (v + a/2)/tfac) before the division.  Each
 // block writes its sums; a second pass adds the blocks in a fixed order, so
 // the result does not depend on scheduling.
 constexpr int NINT = 9;
@@ -687,7 +699,8 @@ template <typename R>
 __global__ __launch_bounds__(GH_BLOCK) void integrals_kernel(PushArgs<R> a, const R *pot,
                                                              const double *fp_val,
                                                              const double *fp_der, int nfp,
-                                                             double fp_dr, double *part) {
+                                                             double fp_dr, int centred,
+                                                             double *part) {
     __shared__ double red[GH_BLOCK / 64][NINT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const long long i = (long long)blockIdx.x * GH_BLOCK + tid;
@@ -697,7 +710,13 @@ __global__ __launch_bounds__(GH_BLOCK) void integrals_kernel(PushArgs<R> a, cons
     if (i < a.n) {
         const int z = a.zone[i];
         if (z >= 1) {
-            const typename Vec4<R>::T st = a.xv[i];
+            typename Vec4<R>::T st = a.xv[i];
+            if (centred) {
+                R ax, ay;
+                getacc1(a, st, a.ut[i], z, ax, ay);
+                st.z = st.z + (R)0.5 * ax;
+                st.w = st.w + (R)0.5 * ay;
+            }
             const int zs = z <= a.nz ? z : a.nz;
             const double sf = 1.0 / (double)a.nstep[zs - 1];
             const double x = st.x, y = st.y, vx = st.z * sf, vy = st.w * sf;
@@ -845,12 +864,12 @@ hipError_t launch_gather(const GatherArgs<R> &a, hipStream_t s) {
 
 template <typename R>
 hipError_t launch_integrals(const PushArgs<R> &a, const R *pot, const double *fp_val,
-                            const double *fp_der, int nfp, double fp_dr, double *part,
-                            double *out, hipStream_t s) {
+                            const double *fp_der, int nfp, double fp_dr, int centred,
+                            double *part, double *out, hipStream_t s) {
     const unsigned nb = a.n > 0 ? nblocks(a.n, GH_BLOCK) : 0;
     if (nb)
         hipLaunchKernelGGL(integrals_kernel<R>, dim3(nb), dim3(GH_BLOCK), 0, s, a, pot,
-                           fp_val, fp_der, nfp, fp_dr, part);
+                           fp_val, fp_der, nfp, fp_dr, centred, part);
     hipLaunchKernelGGL(integrals_sum_kernel, dim3(1), dim3(GH_BLOCK), 0, s,
                        (const double *)part, (int)nb, out);
     return hipGetLastError();
@@ -891,7 +910,8 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
     template hipError_t launch_tmcent<R>(const PushArgs<R> &, R, hipStream_t); \
     template hipError_t launch_integrals<R>(const PushArgs<R> &, const R *,     \
                                             const double *, const double *, int, \
-                                            double, double *, double *, hipStream_t); \
+                                            double, int, double *, double *,   \
+                                            hipStream_t);                        \
     template hipError_t launch_key<R>(const typename Vec2<R>::T *,           \
                                       const uint8_t *, long long, int, int,   \
                                       int, int, unsigned, unsigned *,         \

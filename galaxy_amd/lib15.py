@@ -174,12 +174,14 @@ class P2DGrid:
         self._c(self._lib.galhip_set_fixpot(self._h, len(val), float(dr), _ptr(val),
                                              _ptr(der)), "set_fixpot")
 
-    def integrals(self):
+    def integrals(self, centred=True):
         """icheck's global integrals (see galhip.h): a dict with mass, cm,
         momentum, ke, lz, w_grid (sum m Phi_grid; nan unless the last findf
-        formed the potential), w_fixed and the energy ke + w_grid/2 + w_fixed."""
+        formed the potential), w_fixed and the energy ke + w_grid/2 + w_fixed.
+        centred: velocities time-centred with the current field, as icheck."""
         out = np.zeros(9, np.float64)
-        self._c(self._lib.galhip_integrals(self._h, _ptr(out)), "integrals")
+        self._c(self._lib.galhip_integrals(self._h, int(bool(centred)), _ptr(out)),
+                "integrals")
         d = dict(mass=out[0], cm=out[1:3] / out[0] if out[0] else out[1:3],
                  momentum=out[3:5], ke=0.5 * out[5], lz=out[6], w_grid=out[7],
                  w_fixed=out[8])

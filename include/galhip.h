@@ -183,12 +183,13 @@ int galhip_set_fixpot(galhip_ctx *ctx, int32_t n, double dr, const double *val,
                       const double *der);
 /* Global integrals of icheck (SRC/lib15/icheck.f:24-70), summed in FP64 in a
  * fixed order over all particles, internal units with velocities per zone-1
- * step: out[0] mass, out[1..2] mass x position, out[3..4] momentum,
+ * step, time-centred as icheck takes them ((v + a/2)/tfac with the current
+ * field's getacc acceleration) when centred != 0: out[0] mass, out[1..2] mass x position, out[3..4] momentum,
  * out[5] sum m v^2 (twice the kinetic energy), out[6] angular momentum Lz,
  * out[7] sum m Phi_grid over the grid's particles (NaN unless the last
  * solve formed the potential: findf with lpot), out[8] sum m Phi_fixed.
  * Energy = out[5]/2 + out[7]/2 + out[8]. */
-int galhip_integrals(galhip_ctx *ctx, double *out);
+int galhip_integrals(galhip_ctx *ctx, int32_t centred, double *out);
 
 /* ---- performance controls ---- */
 

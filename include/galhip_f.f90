@@ -187,9 +187,11 @@ module galhip_f
        real(c_double), value :: dr
        real(c_double), intent(in) :: val(*), der(*)
      end function galhip_set_fixpot
-     integer(c_int) function galhip_integrals(ctx, out) bind(c, name='galhip_integrals')
-       import :: c_int, c_ptr, c_double
+     integer(c_int) function galhip_integrals(ctx, centred, out) &
+          bind(c, name='galhip_integrals')
+       import :: c_int, c_ptr, c_int32_t, c_double
        type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: centred
        real(c_double), intent(out) :: out(9)
      end function galhip_integrals
      integer(c_int) function galhip_set_sort_interval(ctx, k) &

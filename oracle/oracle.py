@@ -434,20 +434,29 @@ class P2D:
         self._zoncmb_scaled()
         return dn
 
-    def integrals(self, fixpot=None):
+    def integrals(self, fixpot=None, centred=False):
         """icheck's global integrals (SRC/lib15/icheck.f:24-70) in FP64 numpy,
         as galhip_integrals defines them: mass, cm (mass x position),
         momentum, sum m v^2, Lz with velocities per zone-1 step; sum m Phi
         over the grid's particles from the potential of the last
         findf(lpot=True), CIC-interpolated at the particle's own cell and
-        weights; sum m Phi_fixed from a (dr, val, der) Hermite table."""
+        weights; sum m Phi_fixed from a (dr, val, der) Hermite table.
+        centred: velocities v + a/2 with the current field (anlgrp's
+        time-centring before icheck) before the zone step factor."""
         x = self.x.astype(np.float64)
         y = self.y.astype(np.float64)
         iz = self.iz
         zs = np.minimum(iz, self.nzones)
         sf = 1.0 / np.asarray(self.nstep, np.float64)[np.maximum(zs, 1) - 1]
-        vx = self.vx.astype(np.float64) * sf
-        vy = self.vy.astype(np.float64) * sf
+        vx0, vy0 = self.vx, self.vy
+        if centred:
+            # v + a/2 in the working precision: tmcent's un-centring step
+            sv = (self.vx.copy(), self.vy.copy())
+            self.tmcent(start=False)
+            vx0, vy0 = self.vx, self.vy
+            self.vx, self.vy = sv
+        vx = vx0.astype(np.float64) * sf
+        vy = vy0.astype(np.float64) * sf
         on = iz >= 1
         pm = float(self.pmass)
         out = np.zeros(9)

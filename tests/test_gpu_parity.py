@@ -380,15 +380,15 @@ def test_integrals_match_oracle(orc, prec, op):
     m.masset()
     g.masset()
     g.findf(False)
-    assert np.isnan(g.integrals()["w_grid"])
+    assert np.isnan(g.integrals(centred=False)["w_grid"])
     for _ in range(3):
         m.findf()
         m.step()
     g.run(3)
     m.findf(lpot=True)
     g.findf(True)
-    ref = m.integrals(fixpot=fp)
-    got = g.integrals()
+    ref = m.integrals(fixpot=fp, centred=True)
+    got = g.integrals(centred=True)
     vec = np.concatenate([[got["mass"]], got["cm"] * got["mass"], got["momentum"],
                           [2 * got["ke"], got["lz"], got["w_grid"], got["w_fixed"]]])
     scale = np.abs(ref).max()
