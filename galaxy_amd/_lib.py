@@ -31,6 +31,20 @@ class P2DParams(C.Structure):
     ]
 
 
+class P3DParams(C.Structure):
+    """galhip_p3d_params"""
+    _fields_ = [
+        ("nr", C.c_int32), ("na", C.c_int32), ("ngz", C.c_int32),
+        ("hash", C.c_int32), ("nsect", C.c_int32), ("nskip", C.c_int32),
+        ("skip", C.c_int32 * MAXSKIP),
+        ("dzg", C.c_double), ("softl", C.c_double), ("tsoft", C.c_int32),
+        ("nzones", C.c_int32), ("nstep", C.c_int32 * MAXZONES),
+        ("zbr", C.c_double * MAXZONES),
+        ("offanal", C.c_int32), ("pmass", C.c_double),
+        ("precision", C.c_int32),
+    ]
+
+
 _vp = C.c_void_p
 _i32 = C.c_int32
 _i64 = C.c_int64
@@ -72,6 +86,11 @@ SIGNATURES = {
     "galhip_set_profile": [_vp, _i32],
     "galhip_get_profile": [_vp, _vp, _vp, _vp],
     "galhip_sync": [_vp],
+    "galhip_p3dset": [C.POINTER(P3DParams), C.POINTER(_vp)],
+    "galhip_restrt3": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "galhip_unload3": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "galhip_weight3": [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp],
+    "galhip_grdacc3": [_vp, _i64, _vp, _vp, _vp, _vp],
 }
 
 _lib = None

@@ -7,8 +7,7 @@
 // the host exactly as masscl/zoncmb advance them (masscl.f:31-98,
 // zoncmb.f:40-125); jrad/krad live on the device because the particle kernel
 // produces them, and the solve kernels read them there.
-#include "galhip_kern.h"
-#include "../../include/galhip.h"
+#include "galhip_ctx.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -61,7 +60,7 @@ using namespace gh;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...) {
+int gh_fail(int code, const char *fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -69,76 +68,6 @@ static int fail(int code, const char *fmt, ...) {
     va_end(ap);
     g_err = buf;
     return code;
-}
-
-#define HIPCHK(expr)                                                          \
-    do {                                                                      \
-        hipError_t e_ = (expr);                                               \
-        if (e_ != hipSuccess)                                                 \
-            return fail(GALHIP_EHIP, "%s: %s (%s:%d)", #expr,                 \
-                        hipGetErrorString(e_), __FILE__, __LINE__);           \
-    } while (0)
-
-struct galhip_ctx {
-    int prec;
-    int nr, na, ng, mmax, nsect, mesh, tsoft, nz, offanal;
-    int nstep[GH_MAXZ];
-    double zbr[GH_MAXZ];
-    double softl, pmass;
-    std::vector<int> lg;
-    double alpha, rgrid, thmax;   // values in the context's precision
-    // host time/zone bookkeeping
-    int istep, mzone;
-    int lstep[2][GH_MAXZ], mstep[GH_MAXZ];
-    double fnew[GH_MAXZ], fold[GH_MAXZ];
-    int use_new[GH_MAXZ], use_old[GH_MAXZ];
-    int masscl_done;              // masscl of this istep already on the device
-    int tcent;                    // velocities time-centred (tmcent)
-    int have_green, have_field;
-    // device storage
-    hipStream_t s;
-    long long n, cap;
-    void *xv, *ut;                     // (x, y, vx, vy) and (u, t) per particle
-    uint8_t *zone;
-    unsigned *id;
-    void *xv2, *ut2;
-    uint8_t *zone2;
-    unsigned *id2;
-    unsigned *key, *key2, *perm, *perm2;
-    void *sort_tmp;
-    size_t sort_tmp_bytes;
-    long long *mfx[GH_MAXZ], *moz[GH_MAXZ];
-    int *jrad, *krad, *ranges, *lg_d;
-    long long *counters;
-    void *rho, *coef, *fld2, *pot, *G, *tw;
-    double *Gd, *twd;
-    double *fr_val, *fr_der, fr_dr;
-    double *fp_val, *fp_der, fp_dr;   // fixed potential table (diagnostics)
-    int nfp;
-    int pot_valid;                    // the last solve formed the potential
-    int nfr, qbits, ncoef;
-    int sort_interval, steps_since_sort;
-    // profiling
-    int prof;
-    std::vector<hipEvent_t> ev_free, ev_used;
-    double prof_ms;
-    long long prof_launch, prof_push0;
-};
-
-static size_t esize(const galhip_ctx *c) { return c->prec == GALHIP_FP64 ? 8 : 4; }
-
-template <typename T> static int dalloc(T **p, size_t bytes) {
-    *p = nullptr;
-    if (bytes == 0) return 0;
-    hipError_t e = hipMalloc((void **)p, bytes);
-    if (e != hipSuccess)
-        return fail(GALHIP_EHIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-    return 0;
-}
-
-template <typename T> static void dfree(T *&p) {
-    if (p) (void)hipFree((void *)p);
-    p = nullptr;
 }
 
 static void free_particles(galhip_ctx *c) {
@@ -153,7 +82,7 @@ extern "C" const char *galhip_last_error(void) { return g_err.c_str(); }
 
 // ---- set-up ------------------------------------------------------------------
 
-static int qbits_for(long long n) {
+int qbits_for(long long n) {
     int b = 0;
     while (b < 62 && (1LL << b) < n + 1) b++;
     if (b < 1) b = 1;
@@ -161,19 +90,19 @@ static int qbits_for(long long n) {
 }
 
 extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
-    if (!p || !out) return fail(GALHIP_EINVAL, "galhip_p2dset: null argument");
+    if (!p || !out) return gh_fail(GALHIP_EINVAL, "galhip_p2dset: null argument");
     *out = nullptr;
     if (p->precision != GALHIP_FP32 && p->precision != GALHIP_FP64)
-        return fail(GALHIP_EINVAL, "precision must be GALHIP_FP32 or GALHIP_FP64");
+        return gh_fail(GALHIP_EINVAL, "precision must be GALHIP_FP32 or GALHIP_FP64");
     int na = 2 * (p->na / 2);                       // setgrd.f:58
     if (p->nr < 3 || na < 4 || na > 2048)
-        return fail(GALHIP_EINVAL, "grid %d x %d out of range", p->nr, p->na);
+        return gh_fail(GALHIP_EINVAL, "grid %d x %d out of range", p->nr, p->na);
     if (p->nzones < 1 || p->nzones > GALHIP_MAXZONES)
-        return fail(GALHIP_EINVAL, "nzones %d out of range", p->nzones);
-    if (p->nsect < 1) return fail(GALHIP_EINVAL, "nsect not set");
-    if (!(p->softl > 0.0)) return fail(GALHIP_EINVAL, "Softening length must be greater than 0");
-    if (p->tsoft != 1 && p->tsoft != 2) return fail(GALHIP_EINVAL, "Unrecognized softening rule");
-    if (p->nskip < 0 || p->nskip > GALHIP_MAXSKIP) return fail(GALHIP_EINVAL, "nskip out of range");
+        return gh_fail(GALHIP_EINVAL, "nzones %d out of range", p->nzones);
+    if (p->nsect < 1) return gh_fail(GALHIP_EINVAL, "nsect not set");
+    if (!(p->softl > 0.0)) return gh_fail(GALHIP_EINVAL, "Softening length must be greater than 0");
+    if (p->tsoft != 1 && p->tsoft != 2) return gh_fail(GALHIP_EINVAL, "Unrecognized softening rule");
+    if (p->nskip < 0 || p->nskip > GALHIP_MAXSKIP) return gh_fail(GALHIP_EINVAL, "nskip out of range");
     galhip_ctx *c = new galhip_ctx();   // value-initialised: scalars zeroed
     c->prec = p->precision;
     c->nr = p->nr;
@@ -181,7 +110,7 @@ extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
     c->mmax = na / 2 + 1;
     c->ng = p->hash + 1 < c->mmax ? p->hash + 1 : c->mmax;   // p2dset.f:40, setgrd.f:89
     if (c->ng < 1) c->ng = 1;
-    if (c->ng > 64) { delete c; return fail(GALHIP_EINVAL, "at most 64 harmonics supported"); }
+    if (c->ng > 64) { delete c; return gh_fail(GALHIP_EINVAL, "at most 64 harmonics supported"); }
     c->nsect = p->nsect;
     c->mesh = c->nr * na;
     c->tsoft = p->tsoft;
@@ -223,7 +152,7 @@ extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
         }
         if (aw / c->softl > 5.0) {
             delete c;
-            return fail(GALHIP_ERANGE, "SETGRD: Unsuitable choice of grid parameters "
+            return gh_fail(GALHIP_ERANGE, "SETGRD: Unsuitable choice of grid parameters "
                         "(outermost cell %.4g times the softening length)", aw / p->softl);
         }
     }
@@ -234,7 +163,7 @@ extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
     // a blocking stream: the synchronous hipMemcpy calls of the load/unload
     // and parity entry points (null stream) are ordered with its kernels
     hipError_t e = hipStreamCreate(&c->s);
-    if (e != hipSuccess) { delete c; return fail(GALHIP_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    if (e != hipSuccess) { delete c; return gh_fail(GALHIP_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     size_t es = esize(c);
     for (int z = 0; z < c->nz; z++) {
         rc |= dalloc(&c->mfx[z], sizeof(long long) * c->mesh);
@@ -292,6 +221,7 @@ extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
 extern "C" int galhip_free(galhip_ctx *c) {
     if (!c) return 0;
     (void)hipStreamSynchronize(c->s);
+    if (c->p3d) p3_free(c);
     free_particles(c);
     for (int z = 0; z < GH_MAXZ; z++) { dfree(c->mfx[z]); dfree(c->moz[z]); }
     dfree(c->jrad); dfree(c->krad); dfree(c->ranges); dfree(c->lg_d); dfree(c->counters);
@@ -308,7 +238,7 @@ extern "C" int galhip_free(galhip_ctx *c) {
 }
 
 extern "C" int galhip_geometry(const galhip_ctx *c, double *out) {
-    if (!c || !out) return fail(GALHIP_EINVAL, "null argument");
+    if (!c || !out) return gh_fail(GALHIP_EINVAL, "null argument");
     out[0] = c->alpha;
     out[1] = c->rgrid;
     out[2] = c->thmax;
@@ -330,7 +260,8 @@ static int green_to_work(galhip_ctx *c) {
 }
 
 extern "C" int galhip_greenm(galhip_ctx *c) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d) return p3_greenm(c);
     double alpha_d = 2.0 * 3.141592653589793238462643 / (double)c->na;
     HIPCHK(launch_green(c->Gd, c->twd, c->nr, c->na, c->ng, c->nsect, alpha_d,
                         c->softl, c->tsoft, c->s));
@@ -341,7 +272,8 @@ extern "C" int galhip_greenm(galhip_ctx *c) {
 }
 
 extern "C" int galhip_get_green(const galhip_ctx *c, double *G) {
-    if (!c || !G) return fail(GALHIP_EINVAL, "null argument");
+    if (c && G && c->p3d) return p3_get_green(c, G);
+    if (!c || !G) return gh_fail(GALHIP_EINVAL, "null argument");
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipMemcpy(G, c->Gd, sizeof(double) * 3 * (size_t)c->ng * c->nr * c->nr,
                      hipMemcpyDeviceToHost));
@@ -349,7 +281,8 @@ extern "C" int galhip_get_green(const galhip_ctx *c, double *G) {
 }
 
 extern "C" int galhip_set_green(galhip_ctx *c, const double *G) {
-    if (!c || !G) return fail(GALHIP_EINVAL, "null argument");
+    if (c && G && c->p3d) return p3_set_green(c, G);
+    if (!c || !G) return gh_fail(GALHIP_EINVAL, "null argument");
     HIPCHK(hipMemcpy(c->Gd, G, sizeof(double) * 3 * (size_t)c->ng * c->nr * c->nr,
                      hipMemcpyHostToDevice));
     int rc = green_to_work(c);
@@ -360,14 +293,14 @@ extern "C" int galhip_set_green(galhip_ctx *c, const double *G) {
 
 extern "C" int galhip_set_fixforce(galhip_ctx *c, int32_t n, double dr,
                                    const double *val, const double *der) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
     dfree(c->fr_val);
     dfree(c->fr_der);
     c->nfr = 0;
     if (n <= 0) return 0;
     if (n < 2 || !(dr > 0.0) || !val || !der)
-        return fail(GALHIP_EINVAL, "fixed force table needs n >= 2, dr > 0");
+        return gh_fail(GALHIP_EINVAL, "fixed force table needs n >= 2, dr > 0");
     int rc = dalloc(&c->fr_val, sizeof(double) * n) | dalloc(&c->fr_der, sizeof(double) * n);
     if (rc) return rc;
     HIPCHK(hipMemcpy(c->fr_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -380,14 +313,14 @@ extern "C" int galhip_set_fixforce(galhip_ctx *c, int32_t n, double dr,
 // Fixed potential table for the energy diagnostics, as galhip_set_fixforce.
 extern "C" int galhip_set_fixpot(galhip_ctx *c, int32_t n, double dr,
                                  const double *val, const double *der) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
     dfree(c->fp_val);
     dfree(c->fp_der);
     c->nfp = 0;
     if (n <= 0) return 0;
     if (n < 2 || !(dr > 0.0) || !val || !der)
-        return fail(GALHIP_EINVAL, "fixed potential table needs n >= 2, dr > 0");
+        return gh_fail(GALHIP_EINVAL, "fixed potential table needs n >= 2, dr > 0");
     int rc = dalloc(&c->fp_val, sizeof(double) * n) | dalloc(&c->fp_der, sizeof(double) * n);
     if (rc) return rc;
     HIPCHK(hipMemcpy(c->fp_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -414,7 +347,7 @@ static int ensure_particles(galhip_ctx *c, long long n) {
     size_t tmp = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortPairs(
         nullptr, tmp, c->key, c->key2, c->perm, c->perm2, (int)n, 0, 32, c->s);
-    if (e != hipSuccess) return fail(GALHIP_EHIP, "radix sort sizing: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return gh_fail(GALHIP_EHIP, "radix sort sizing: %s", hipGetErrorString(e));
     rc = dalloc(&c->sort_tmp, tmp);
     if (rc) { free_particles(c); return rc; }
     c->sort_tmp_bytes = tmp;
@@ -458,12 +391,13 @@ template <typename R> static int sort_impl(galhip_ctx *c) {
 }
 
 extern "C" int galhip_sort(galhip_ctx *c) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (c && c->p3d) return gh_fail(GALHIP_EINVAL, "galhip_sort: not built for p3d contexts");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     return c->prec == GALHIP_FP32 ? sort_impl<float>(c) : sort_impl<double>(c);
 }
 
 extern "C" int galhip_get_order(const galhip_ctx *c, int32_t *ids) {
-    if (!c || (!ids && c->n > 0)) return fail(GALHIP_EINVAL, "null argument");
+    if (!c || (!ids && c->n > 0)) return gh_fail(GALHIP_EINVAL, "null argument");
     HIPCHK(hipStreamSynchronize(c->s));
     if (c->n > 0)
         HIPCHK(hipMemcpy(ids, c->id, sizeof(unsigned) * c->n, hipMemcpyDeviceToHost));
@@ -471,7 +405,7 @@ extern "C" int galhip_get_order(const galhip_ctx *c, int32_t *ids) {
 }
 
 extern "C" int galhip_set_sort_interval(galhip_ctx *c, int32_t k) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     c->sort_interval = k < 0 ? 0 : k;
     return 0;
 }
@@ -482,8 +416,9 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
                              const void *y, const void *vx, const void *vy,
                              const int32_t *iz) {
     if (!c || n < 0 || (n > 0 && (!x || !y || !vx || !vy || !iz)))
-        return fail(GALHIP_EINVAL, "galhip_restrt: bad arguments");
-    if (n >= (1LL << 31)) return fail(GALHIP_EINVAL, "at most 2^31-1 particles");
+        return gh_fail(GALHIP_EINVAL, "galhip_restrt: bad arguments");
+    if (c->p3d) return gh_fail(GALHIP_EINVAL, "p3d context: use galhip_restrt3");
+    if (n >= (1LL << 31)) return gh_fail(GALHIP_EINVAL, "at most 2^31-1 particles");
     HIPCHK(hipStreamSynchronize(c->s));
     c->tcent = 0;   // loaded velocities: not (yet) time-centred
     int rc = ensure_particles(c, n);
@@ -495,7 +430,7 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
     for (long long i = 0; i < n; i++) {
         int v = iz[i];
         if (v < 1 || v > c->nz + 1)
-            return fail(GALHIP_EINVAL, "particle %lld has zone %d outside 1..%d", i, v, c->nz + 1);
+            return gh_fail(GALHIP_EINVAL, "particle %lld has zone %d outside 1..%d", i, v, c->nz + 1);
         z[i] = (uint8_t)v;
         ids[i] = (unsigned)i;
         if (v == c->nz + 1) cnt[0]++;    // /admin/ noff: the off-grid list
@@ -524,7 +459,8 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
 
 extern "C" int galhip_unload(galhip_ctx *c, void *x, void *y, void *vx,
                              void *vy, int32_t *iz) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d) return p3_unload(c, x, y, vx, vy, iz);
     HIPCHK(hipStreamSynchronize(c->s));
     long long n = c->n;
     std::vector<unsigned> ids(n);
@@ -549,7 +485,7 @@ extern "C" int galhip_unload(galhip_ctx *c, void *x, void *y, void *vx,
 
 // ---- zone bookkeeping (host) -----------------------------------------------------
 
-static void set_mzone(galhip_ctx *c) {
+void set_mzone(galhip_ctx *c) {
     int m = 1;
     if (c->nz > 1)
         for (int i = 1; i <= c->nz; i++)
@@ -558,7 +494,7 @@ static void set_mzone(galhip_ctx *c) {
 }
 
 // masscl's time stamps (masscl.f:31-40,92-96)
-static void masscl_host(galhip_ctx *c, bool normal) {
+void masscl_host(galhip_ctx *c, bool normal) {
     for (int z = 1; z <= c->mzone; z++) {
         int ns = c->nstep[z - 1];
         if (normal) c->mstep[z - 1] = c->istep + ns;
@@ -571,13 +507,13 @@ static void masscl_host(galhip_ctx *c, bool normal) {
 }
 
 // zoncmb's fractions for the current istep (zoncmb.f:40-125)
-static int zoncmb_host(galhip_ctx *c) {
+int zoncmb_host(galhip_ctx *c) {
     set_mzone(c);
     for (int z = 2; z <= c->nz; z++) {
         int ns = c->nstep[z - 1];
         int i = c->lstep[1][z - 1] - c->istep;
         if (i < 0 || i > ns)
-            return fail(GALHIP_ERANGE, "ZONCMB: New mass array at a useless time (istep %d, lstep %d)",
+            return gh_fail(GALHIP_ERANGE, "ZONCMB: New mass array at a useless time (istep %d, lstep %d)",
                         c->istep, c->lstep[1][z - 1]);
         c->use_new[z - 1] = i != ns;
         c->fnew[z - 1] = 1.0 - (double)i / (double)ns;
@@ -586,7 +522,7 @@ static int zoncmb_host(galhip_ctx *c) {
         if (i != 0) {
             int j = c->istep - c->lstep[0][z - 1];
             if (c->lstep[0][z - 1] < 0 || j < 0 || j > ns)
-                return fail(GALHIP_ERANGE, "ZONCMB: Old mass array at a useless time (istep %d, lstep %d)",
+                return gh_fail(GALHIP_ERANGE, "ZONCMB: Old mass array at a useless time (istep %d, lstep %d)",
                             c->istep, c->lstep[0][z - 1]);
             c->use_old[z - 1] = 1;
             c->fold[z - 1] = 1.0 - (double)j / (double)ns;
@@ -596,7 +532,7 @@ static int zoncmb_host(galhip_ctx *c) {
 }
 
 extern "C" int galhip_set_step(galhip_ctx *c, int32_t istep, const int32_t *lstep) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     c->istep = istep;
     if (lstep)
         for (int z = 0; z < c->nz; z++) {
@@ -605,12 +541,15 @@ extern "C" int galhip_set_step(galhip_ctx *c, int32_t istep, const int32_t *lste
         }
     set_mzone(c);
     c->masscl_done = 0;
+    // with time stamps given, zoncmb's fractions for the new time too, so a
+    // solve may follow mass arrays loaded by galhip_set_mass
+    if (lstep) return zoncmb_host(c);
     return 0;
 }
 
 extern "C" int galhip_get_step(const galhip_ctx *c, int32_t *istep, int32_t *mzone,
                                int32_t *lstep) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     if (istep) *istep = c->istep;
     if (mzone) *mzone = c->mzone;
     if (lstep)
@@ -732,14 +671,15 @@ template <typename R> static int masset_impl(galhip_ctx *c) {
 }
 
 extern "C" int galhip_masset(galhip_ctx *c) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d) return p3_masset(c);
     return c->prec == GALHIP_FP32 ? masset_impl<float>(c) : masset_impl<double>(c);
 }
 
 template <typename R>
 static int solve_impl(galhip_ctx *c, int lpot, int explicit_zones, int jzone,
                       int kzone, int masscl_z) {
-    if (!c->have_green) return fail(GALHIP_EINVAL, "P2CONV: Green function not set (call galhip_greenm)");
+    if (!c->have_green) return gh_fail(GALHIP_EINVAL, "P2CONV: Green function not set (call galhip_greenm)");
     AnaArgs<R> a = ana_args<R>(c, lpot, explicit_zones, jzone, kzone);
     HIPCHK(launch_analyse<R>(a, c->s));
     SynArgs<R> b = syn_args<R>(c, lpot, masscl_z);
@@ -750,19 +690,21 @@ static int solve_impl(galhip_ctx *c, int lpot, int explicit_zones, int jzone,
 }
 
 extern "C" int galhip_findf(galhip_ctx *c, int32_t lpot) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
-    if (c->mzone < 1 || c->mzone > c->nz) return fail(GALHIP_ERANGE, "FINDF: invalid mzone");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->mzone < 1 || c->mzone > c->nz) return gh_fail(GALHIP_ERANGE, "FINDF: invalid mzone");
+    if (c->p3d) return p3_solve(c, lpot != 0, 0, 0, 0);
     return c->prec == GALHIP_FP32 ? solve_impl<float>(c, lpot != 0, 0, 0, 0, 0)
                                   : solve_impl<double>(c, lpot != 0, 0, 0, 0, 0);
 }
 
 extern "C" int galhip_p2fndf(galhip_ctx *c, int32_t jzone, int32_t kzone, int32_t lpot) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d) return p3_solve(c, lpot != 0, 1, jzone, kzone);
     return c->prec == GALHIP_FP32 ? solve_impl<float>(c, lpot != 0, 1, jzone, kzone, 0)
                                   : solve_impl<double>(c, lpot != 0, 1, jzone, kzone, 0);
 }
 
-static hipEvent_t take_event(galhip_ctx *c) {
+hipEvent_t take_event(galhip_ctx *c) {
     hipEvent_t e;
     if (!c->ev_free.empty()) {
         e = c->ev_free.back();
@@ -796,8 +738,9 @@ template <typename R> static int step_impl(galhip_ctx *c) {
 }
 
 extern "C" int galhip_step(galhip_ctx *c) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
-    if (!c->have_field) return fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (!c->have_field) return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    if (c->p3d) return p3_step(c);
     return c->prec == GALHIP_FP32 ? step_impl<float>(c) : step_impl<double>(c);
 }
 
@@ -810,11 +753,12 @@ template <typename R> static int tmcent_impl(galhip_ctx *c, int start) {
 }
 
 extern "C" int galhip_tmcent(galhip_ctx *c, int32_t start) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     if ((start != 0) == (c->tcent != 0))
-        return fail(GALHIP_ERANGE, "TMCENT: Logic error in call (tcent flag already %s)",
+        return gh_fail(GALHIP_ERANGE, "TMCENT: Logic error in call (tcent flag already %s)",
                     c->tcent ? "up" : "down");
-    if (!c->have_field) return fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    if (!c->have_field) return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    if (c->p3d) return p3_tmcent(c, start);
     return c->prec == GALHIP_FP32 ? tmcent_impl<float>(c, start) : tmcent_impl<double>(c, start);
 }
 
@@ -831,7 +775,7 @@ template <typename R> static int integrals_impl(galhip_ctx *c, int centred, doub
         if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 9,
                                                 hipMemcpyDeviceToHost, c->s);
         if (e == hipSuccess) e = hipStreamSynchronize(c->s);
-        if (e != hipSuccess) rc = fail(GALHIP_EHIP, "integrals: %s", hipGetErrorString(e));
+        if (e != hipSuccess) rc = gh_fail(GALHIP_EHIP, "integrals: %s", hipGetErrorString(e));
     }
     dfree(part);
     dfree(dout);
@@ -843,15 +787,26 @@ template <typename R> static int integrals_impl(galhip_ctx *c, int centred, doub
 }
 
 extern "C" int galhip_integrals(galhip_ctx *c, int32_t centred, double *out) {
-    if (!c || !out) return fail(GALHIP_EINVAL, "null argument");
+    if (!c || !out) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->p3d) return gh_fail(GALHIP_EINVAL, "galhip_integrals: not built for p3d contexts");
     if (centred && !c->have_field)
-        return fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+        return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
     return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out)
                                   : integrals_impl<double>(c, centred != 0, out);
 }
 
 extern "C" int galhip_run(galhip_ctx *c, int32_t nsteps) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d) {
+        for (int k = 0; k < nsteps; k++) {
+            set_mzone(c);
+            int rc = p3_solve(c, 0, 0, 0, 0);
+            if (rc) return rc;
+            rc = galhip_step(c);
+            if (rc) return rc;
+        }
+        return 0;
+    }
     for (int k = 0; k < nsteps; k++) {
         set_mzone(c);
         // findf with the masscl of this step (zone sums saved and cleared,
@@ -895,7 +850,7 @@ static int gather_impl(galhip_ctx *c, int64_t n, const void *x, const void *y,
         a.thmax = (R)c->thmax;
         hipError_t e = launch_gather<R>(a, c->s);
         if (e == hipSuccess) e = hipStreamSynchronize(c->s);
-        if (e != hipSuccess) rc = fail(GALHIP_EHIP, "gather kernel: %s", hipGetErrorString(e));
+        if (e != hipSuccess) rc = gh_fail(GALHIP_EHIP, "gather kernel: %s", hipGetErrorString(e));
         if (!rc && cell) (void)hipMemcpy(cell, dcell, sizeof(int) * n, hipMemcpyDeviceToHost);
         if (!rc && wt) (void)hipMemcpy(wt, dwt, 4 * es * n, hipMemcpyDeviceToHost);
         if (!rc && acc) (void)hipMemcpy(acc, dacc, 2 * es * n, hipMemcpyDeviceToHost);
@@ -906,7 +861,8 @@ static int gather_impl(galhip_ctx *c, int64_t n, const void *x, const void *y,
 
 extern "C" int galhip_weight(galhip_ctx *c, int64_t n, const void *x, const void *y,
                              int32_t old, int32_t *cell, void *wt) {
-    if (!c || n < 0) return fail(GALHIP_EINVAL, "bad arguments");
+    if (c && c->p3d) return gh_fail(GALHIP_EINVAL, "p3d context: use the p3d entry point");
+    if (!c || n < 0) return gh_fail(GALHIP_EINVAL, "bad arguments");
     HIPCHK(hipStreamSynchronize(c->s));
     return c->prec == GALHIP_FP32 ? gather_impl<float>(c, n, x, y, !old, cell, wt, nullptr)
                                   : gather_impl<double>(c, n, x, y, !old, cell, wt, nullptr);
@@ -914,7 +870,8 @@ extern "C" int galhip_weight(galhip_ctx *c, int64_t n, const void *x, const void
 
 extern "C" int galhip_grdacc(galhip_ctx *c, int64_t n, const void *x, const void *y,
                              void *acc) {
-    if (!c || n < 0 || !acc) return fail(GALHIP_EINVAL, "bad arguments");
+    if (c && c->p3d) return gh_fail(GALHIP_EINVAL, "p3d context: use the p3d entry point");
+    if (!c || n < 0 || !acc) return gh_fail(GALHIP_EINVAL, "bad arguments");
     HIPCHK(hipStreamSynchronize(c->s));
     return c->prec == GALHIP_FP32 ? gather_impl<float>(c, n, x, y, 0, nullptr, nullptr, acc)
                                   : gather_impl<double>(c, n, x, y, 0, nullptr, nullptr, acc);
@@ -922,7 +879,7 @@ extern "C" int galhip_grdacc(galhip_ctx *c, int64_t n, const void *x, const void
 
 extern "C" int galhip_get_mass(const galhip_ctx *c, int64_t *mfx, int64_t *mozfx,
                                int32_t *jrad, int32_t *krad, int32_t *qbits) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
     for (int z = 0; z < c->nz; z++) {
         if (mfx) HIPCHK(hipMemcpy(mfx + (size_t)z * c->mesh, c->mfx[z], 8 * c->mesh, hipMemcpyDeviceToHost));
@@ -937,7 +894,7 @@ extern "C" int galhip_get_mass(const galhip_ctx *c, int64_t *mfx, int64_t *mozfx
 extern "C" int galhip_set_mass(galhip_ctx *c, const int64_t *mfx,
                                const int64_t *mozfx, const int32_t *jrad,
                                const int32_t *krad) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
     for (int z = 0; z < c->nz; z++) {
         if (mfx) HIPCHK(hipMemcpy(c->mfx[z], mfx + (size_t)z * c->mesh, 8 * c->mesh, hipMemcpyHostToDevice));
@@ -949,14 +906,15 @@ extern "C" int galhip_set_mass(galhip_ctx *c, const int64_t *mfx,
 }
 
 extern "C" int galhip_get_rho(const galhip_ctx *c, void *rho) {
-    if (!c || !rho) return fail(GALHIP_EINVAL, "null argument");
+    if (!c || !rho) return gh_fail(GALHIP_EINVAL, "null argument");
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipMemcpy(rho, c->rho, esize(c) * c->mesh, hipMemcpyDeviceToHost));
     return 0;
 }
 
 extern "C" int galhip_get_field(const galhip_ctx *c, void *fld) {
-    if (!c || !fld) return fail(GALHIP_EINVAL, "null argument");
+    if (!c || !fld) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->p3d) return p3_get_field(c, fld);
     HIPCHK(hipStreamSynchronize(c->s));
     size_t es = esize(c), m = c->mesh;
     std::vector<unsigned char> tmp(2 * es * m);
@@ -971,7 +929,8 @@ extern "C" int galhip_get_field(const galhip_ctx *c, void *fld) {
 }
 
 extern "C" int galhip_set_field(galhip_ctx *c, const void *fld) {
-    if (!c || !fld) return fail(GALHIP_EINVAL, "null argument");
+    if (!c || !fld) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->p3d) return p3_set_field(c, fld);
     HIPCHK(hipStreamSynchronize(c->s));
     size_t es = esize(c), m = c->mesh;
     std::vector<unsigned char> tmp(2 * es * m);
@@ -987,14 +946,14 @@ extern "C" int galhip_set_field(galhip_ctx *c, const void *fld) {
 }
 
 extern "C" int galhip_get_ranges(const galhip_ctx *c, int32_t *r4) {
-    if (!c || !r4) return fail(GALHIP_EINVAL, "null argument");
+    if (!c || !r4) return gh_fail(GALHIP_EINVAL, "null argument");
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipMemcpy(r4, c->ranges, sizeof(int) * 4, hipMemcpyDeviceToHost));
     return 0;
 }
 
 extern "C" int galhip_get_noff(const galhip_ctx *c, int64_t *noff) {
-    if (!c || !noff) return fail(GALHIP_EINVAL, "null argument");
+    if (!c || !noff) return gh_fail(GALHIP_EINVAL, "null argument");
     HIPCHK(hipStreamSynchronize(c->s));
     long long v[4];
     HIPCHK(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
@@ -1005,7 +964,7 @@ extern "C" int galhip_get_noff(const galhip_ctx *c, int64_t *noff) {
 // ---- profiling ----------------------------------------------------------------------
 
 extern "C" int galhip_set_profile(galhip_ctx *c, int32_t on) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
     for (auto e : c->ev_used) c->ev_free.push_back(e);
     c->ev_used.clear();
@@ -1020,7 +979,7 @@ extern "C" int galhip_set_profile(galhip_ctx *c, int32_t on) {
 
 extern "C" int galhip_get_profile(galhip_ctx *c, double *push_ms, int64_t *launches,
                                   int64_t *pushes) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
     double ms = 0.0;
     for (size_t i = 0; i + 1 < c->ev_used.size(); i += 2) {
@@ -1037,7 +996,7 @@ extern "C" int galhip_get_profile(galhip_ctx *c, double *push_ms, int64_t *launc
 }
 
 extern "C" int galhip_sync(galhip_ctx *c) {
-    if (!c) return fail(GALHIP_EINVAL, "null context");
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
     return 0;
 }

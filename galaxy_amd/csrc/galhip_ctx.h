@@ -1,0 +1,120 @@
+// galhip_ctx.h -- the library context (the opaque galhip_ctx of galhip.h)
+// and the host helpers shared by the translation units of libgalhip.
+#pragma once
+#include "galhip_kern.h"
+#include "../../include/galhip.h"
+
+#include <string>
+#include <vector>
+
+// Sets the message galhip_last_error returns; returns code.
+int gh_fail(int code, const char *fmt, ...);
+
+#define HIPCHK(expr)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return gh_fail(GALHIP_EHIP, "%s: %s (%s:%d)", #expr,                 \
+                        hipGetErrorString(e_), __FILE__, __LINE__);           \
+    } while (0)
+
+struct galhip_ctx {
+    int prec;
+    int nr, na, ng, mmax, nsect, mesh, tsoft, nz, offanal;
+    int nstep[GH_MAXZ];
+    double zbr[GH_MAXZ];
+    double softl, pmass;
+    std::vector<int> lg;
+    double alpha, rgrid, thmax;   // values in the context's precision
+    // host time/zone bookkeeping
+    int istep, mzone;
+    int lstep[2][GH_MAXZ], mstep[GH_MAXZ];
+    double fnew[GH_MAXZ], fold[GH_MAXZ];
+    int use_new[GH_MAXZ], use_old[GH_MAXZ];
+    int masscl_done;              // masscl of this istep already on the device
+    int tcent;                    // velocities time-centred (tmcent)
+    int have_green, have_field;
+    // device storage
+    hipStream_t s;
+    long long n, cap;
+    void *xv, *ut;                     // (x, y, vx, vy) and (u, t) per particle
+    uint8_t *zone;
+    unsigned *id;
+    void *xv2, *ut2;
+    uint8_t *zone2;
+    unsigned *id2;
+    unsigned *key, *key2, *perm, *perm2;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
+    long long *mfx[GH_MAXZ], *moz[GH_MAXZ];
+    int *jrad, *krad, *ranges, *lg_d;
+    long long *counters;
+    void *rho, *coef, *fld2, *pot, *G, *tw;
+    double *Gd, *twd;
+    double *fr_val, *fr_der, fr_dr;
+    double *fp_val, *fp_der, fp_dr;   // fixed potential table (diagnostics)
+    int nfp;
+    int pot_valid;                    // the last solve formed the potential
+    int nfr, qbits, ncoef;
+    int sort_interval, steps_since_sort;
+    // profiling
+    int prof;
+    std::vector<hipEvent_t> ev_free, ev_used;
+    double prof_ms;
+    long long prof_launch, prof_push0;
+    // 3-D polar grid (p3d); p3d = 0 for the 2-D grid
+    int p3d, ngz, nm3;
+    double dzg, zm;                   // in the context's precision
+    void *vw, *vw2;                   // (vy, vz) per particle
+    void *A3, *M3, *W3, *Z3;          // solve work arrays
+    void *fld4;                       // [mesh] (F_x, F_y, F_z, potential)
+    void *twz;                        // cos, sin of 2 pi k / (2 ngz)
+    double *twzd;                     // the same in double
+    int *slot_m_d, *slot_p_d, *slot_of_d;
+    size_t gsize3;                    // entries of the p3d Green's table
+};
+
+static size_t esize(const galhip_ctx *c) { return c->prec == GALHIP_FP64 ? 8 : 4; }
+
+template <typename T> static int dalloc(T **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return 0;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess)
+        return gh_fail(GALHIP_EHIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return 0;
+}
+
+template <typename T> static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+
+// zone/time bookkeeping of the reference (galhip_api.hip)
+int qbits_for(long long n);
+void set_mzone(galhip_ctx *c);
+void masscl_host(galhip_ctx *c, bool normal);
+int zoncmb_host(galhip_ctx *c);
+hipEvent_t take_event(galhip_ctx *c);
+
+namespace gh {
+hipError_t launch_d2f(const double *, float *, long long, hipStream_t);
+hipError_t launch_fill_i64(long long *, long long, long long, hipStream_t);
+hipError_t launch_fill_i32(int *, int, long long, hipStream_t);
+hipError_t launch_masscl(const ZonePtrs &, int, long long, int *, int *, int,
+                         hipStream_t);
+}  // namespace gh
+
+// p3d implementations of the generic entry points (galhip_p3d_api.hip)
+void p3_free(galhip_ctx *c);
+int p3_greenm(galhip_ctx *c);
+int p3_get_green(const galhip_ctx *c, double *G);
+int p3_set_green(galhip_ctx *c, const double *G);
+int p3_masset(galhip_ctx *c);
+int p3_solve(galhip_ctx *c, int lpot, int explicit_zones, int jzone, int kzone);
+int p3_step(galhip_ctx *c);
+int p3_tmcent(galhip_ctx *c, int start);
+int p3_get_field(const galhip_ctx *c, void *fld);
+int p3_set_field(galhip_ctx *c, const void *fld);
+int p3_unload(galhip_ctx *c, void *x, void *y, void *vx, void *vy, int32_t *iz);

@@ -167,6 +167,68 @@ __device__ __forceinline__ double fixforce(double r, int nfr, double dr,
     return fixforce_at(r, k, past, dr, inv_dr, val, der, 0);
 }
 
+// ---- shared by the solve and particle kernels ---------------------------------
+
+// Radial ranges of p2fndf (p2fndf.f:26-55, findf.f:42-54).  Returns false
+// if no mass is on the grid.  kr receives krad after p2fndf's cumulative
+// update.
+__device__ __forceinline__ bool solve_ranges(const int *jrad, const int *krad, int nz, int nr,
+                             int mzone, int lpot, int explicit_zones,
+                             int jzone_in, int kzone_in, int kr[GH_MAXZ],
+                             int out[4]) {
+    int jrs = nr, krs = 0;
+    for (int ipm = 1; ipm <= nz; ipm++) {
+        jrs = min(jrs, jrad[ipm - 1]);
+        krs = max(krs, krad[ipm - 1]);
+        kr[ipm - 1] = krad[ipm - 1];
+        if (ipm > 1) kr[ipm - 1] = max(kr[ipm - 1], krs);
+    }
+    if (jrs > krs || krs == 0) return false;
+    int jzone, kzone;
+    if (explicit_zones) {
+        jzone = jzone_in;
+        kzone = kzone_in;
+    } else if (lpot || mzone == nz) {
+        jzone = 0;
+        kzone = nz + 1;
+    } else {
+        jzone = 1;
+        kzone = mzone;
+    }
+    int jrf = 1;
+    if (jzone > 0 && jrad[jzone - 1] < nr) jrf = jrad[jzone - 1];
+    int krf = nr;
+    if (kzone <= nz) {
+        krf = kr[kzone - 1];
+        while (krf < jrf) {
+            kzone++;
+            if (kzone > nz) return false;
+            krf = kr[kzone - 1];
+        }
+    }
+    out[0] = jrs;
+    out[1] = krs;
+    out[2] = jrf;
+    out[3] = krf;
+    return true;
+}
+
+// Global deposit of one corner value with Fortran's flat indexing over the
+// [nz][mesh] zone arrays (grdmss(j, jz)); past the last array it is dropped.
+__device__ __forceinline__ void deposit_global(long long *const *mfx, int nz,
+                                               int mesh, int zone, int idx,
+                                               long long v) {
+    long long f = (long long)(zone - 1) * mesh + idx;
+    if (f < (long long)nz * mesh) {
+        int zf = zone - 1, cf = idx;
+        while (cf >= mesh) {
+            cf -= mesh;
+            zf++;
+        }
+        atomicAdd((unsigned long long *)&mfx[zf][cf], (unsigned long long)v);
+    }
+}
+
 // ---- kernel argument blocks -------------------------------------------------
 
 struct ZonePtrs {

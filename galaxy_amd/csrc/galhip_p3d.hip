@@ -1,0 +1,796 @@
+// galhip_p3d.hip -- kernels of the 3-D polar grid (p3d): cylindrical rings
+// x spokes x z-planes, linear (trilinear) interpolation (jmass = 2).
+//
+// Mesh cell (ring, plane, spoke) is ((ring)*ngz + plane)*na + spoke, 0-based
+// (SRC/lib15/weight.f:232).  The field is held interleaved as one 4-vector
+// per cell, (F_x, F_y, F_z, potential), so that an interpolation corner is
+// one 16-byte (FP32) load.
+//
+// Particle kernels (one thread per particle):
+//   p3_massgn_kernel  massgn of every on-grid particle (massgn.f:140-172)
+//   p3_push_kernel    getacc (grdacc.f:189-211, getacc.f:106-151) + stpgrp
+//                     + offgrd (|z| > zm or R > rgrid, offgrd.f:69-72) +
+//                     addoff + rezgrp with spherical zone radii
+//                     (zoneno.f:40-41) + massgn at the new position
+//   p3_tmcent_kernel  tmcent's half-step velocity shift
+//   p3_gather_kernel  weight / grdacc for given positions (parity checks)
+// Solve (p3fndf.f:37-104), FP32 or FP64:
+//   p3_analyse_kernel block (plane, ring): zoncmb + scaled, then the
+//                     azimuthal components of the active harmonics
+//                     (p3manl.f:60-89) as direct sums
+//   p3_zfwd_kernel    block per ring: the doubled vertical transform of each
+//                     component column, zero-padded to 2 ngz (p3manl.f:90-101)
+//   p3_conv_kernel    block per (type, k_z, m): the radial contraction with
+//                     that Green's matrix (p3conv.f:139-226); table streamed
+//                     once, field ring per thread
+//   p3_zinv_kernel    block per (type, ring): inverse vertical transform,
+//                     first ngz planes (p3fsyn.f:59-91)
+//   p3_synth_kernel   block per (plane, ring): inverse azimuthal transform,
+//                     1/(2 ngz na) (p3fsyn.f:92-129) and polcat
+// p3_green_kernel: p3grnm's tables in FP64 (p3grnm.f:93-263), p3gwrt's
+//   per-(k_z, m) matrices with the field ring fastest (p3gwrt.f:46-64).
+#include "galhip_p3d.h"
+
+#include <climits>
+
+namespace gh {
+
+constexpr int P3_BLOCK = 256;
+constexpr int P3_MAX_NA = 2048;
+constexpr int P3_MAX_NM = 128;   // compact components (2 ng)
+constexpr int P3_MAX_NR = 1024;
+
+template <typename R> __device__ __forceinline__ R p3_wave_sum(R v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// weight() for one position, p3d jmass = 2 (weight.f:195-250): returns the
+// 0-based cell (ju ngz + jz) na + jt and the 8 weights in the reference's
+// order (bit 0 = dt, bit 1 = dz, bit 2 = du).
+template <typename R>
+__device__ __forceinline__ int weight3(R x, R y, R z, const P3Args<R> &a,
+                                       bool fold, R w[8]) {
+    R zz = z + a.zm;
+    R rr = sqrt(x * x + y * y);
+    R u, t;
+    ut_of(x, y, rr, a.k, u, t);
+    if (!(rr > (R)0)) t = (R)0;
+    if (fold) t = (R)fmod(t, a.thmax);
+    int ju = (int)u;
+    if (ju > a.nr - 2) ju = a.nr - 2;
+    int jt = (int)t;
+    zz = zz / a.dzg;
+    int jz = (int)zz;
+    if (jz < 0) jz = 0;
+    if (jz > a.ngz - 2) jz = a.ngz - 2;
+    const R du = u - (R)ju, dt = t - (R)jt, dz = zz - (R)jz;
+    const R one = (R)1;
+    w[0] = (one - dt) * (one - dz) * (one - du);
+    w[1] = dt * (one - dz) * (one - du);
+    w[2] = (one - dt) * dz * (one - du);
+    w[3] = dt * dz * (one - du);
+    w[4] = (one - dt) * (one - dz) * du;
+    w[5] = dt * (one - dz) * du;
+    w[6] = (one - dt) * dz * du;
+    w[7] = dt * dz * du;
+    return (ju * a.ngz + jz) * a.na + jt;
+}
+
+// grdacc's corners (grdacc.f:193-203): c, c+1 (wrapped when c+1 is a
+// multiple of na), the next plane (+na), and the same one ring out.
+__device__ __forceinline__ void corners3(int c, int na, int ngz, int ic[8]) {
+    const bool wrap = (c + 1) % na == 0;
+    ic[0] = c;
+    ic[1] = wrap ? c + 1 - na : c + 1;
+    ic[2] = c + na;
+    ic[3] = ic[1] + na;
+    ic[4] = c + ngz * na;
+    ic[5] = wrap ? ic[4] + 1 - na : ic[4] + 1;
+    ic[6] = ic[4] + na;
+    ic[7] = ic[5] + na;
+}
+
+template <typename R>
+__device__ __forceinline__ R tsfac3(const P3Args<R> &a, int iz) {
+    return (R)a.nstep[(iz < a.nz ? iz : a.nz) - 1];
+}
+
+// zoneno with spherical boundaries (zoneno.f:40-55, setrun.f:810)
+template <typename R>
+__device__ __forceinline__ int zoneno3(const P3Args<R> &a, R x, R y, R z) {
+    const R rad = sqrt(x * x + y * y + z * z);
+    int k = 1;
+    if (a.nz > 1)
+        for (int i = 1; i <= a.nz; i++)
+            if (rad >= a.zbr[i - 1]) k = i;
+    return k;
+}
+
+// chstep, RVLF branch (chstep.f:22-31), one component
+template <typename R>
+__device__ __forceinline__ void chstep1(R tfac, R ox, R &ov, R &acc, R &nx, R &nv) {
+    const R h = (R)0.5;
+    ov = tfac * (ov + h * acc * ((R)1 - tfac));
+    acc = acc * tfac * tfac;
+    nv = ov + acc;
+    nx = ox + nv;
+}
+
+// getacc of one particle of list z0: the three Cartesian components
+// interpolated at its position (component sums in corner order), the fixed
+// force of the spherical radius, the zone rescaling.
+template <typename R>
+__device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz,
+                                        int z0, R acc[3]) {
+    typedef typename Vec4<R>::T R4;
+    acc[0] = acc[1] = acc[2] = (R)0;
+    if (z0 != a.nz + 1) {
+        R w[8];
+        int ic[8];
+        const int c = weight3<R>(ox, oy, oz, a, false, w);
+        corners3(c, a.na, a.ngz, ic);
+        R4 f[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (ic[q] < a.mesh) f[q] = a.fld[ic[q]];
+            else f[q] = mk4((R)0, (R)0, (R)0, (R)0);
+        }
+        R sx = (R)0, sy = (R)0, sz = (R)0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            sx = sx + f[q].x * w[q];
+            sy = sy + f[q].y * w[q];
+            sz = sz + f[q].z * w[q];
+        }
+        if (a.fixforce) {
+            R r2 = (R)0;
+            r2 = r2 + ox * ox;
+            r2 = r2 + oy * oy;
+            r2 = r2 + oz * oz;
+            const R rr = sqrt(r2);
+            R fr = (R)fixforce((double)rr, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val, a.fr_der);
+            if (fr != (R)0) {
+                fr = fr / rr;
+                sx = sx + fr * ox;
+                sy = sy + fr * oy;
+                sz = sz + fr * oz;
+            }
+        }
+        acc[0] = sx;
+        acc[1] = sy;
+        acc[2] = sz;
+    }
+    if (a.nz > 1 && z0 > 1) {
+        R t = tsfac3(a, z0);
+        t = t * t;
+        acc[0] = t * acc[0];
+        acc[1] = t * acc[1];
+        acc[2] = t * acc[2];
+    }
+}
+
+// Fixed-point deposit of one particle's 8 weights into zone z, with the
+// radial range folded into the block's per-zone minimum and maximum.
+template <typename R>
+__device__ __forceinline__ void deposit3(const P3Args<R> &a, int z, int c,
+                                         const R w[8], int *s_jr, int *s_kr) {
+    const int ring = c / (a.na * a.ngz);
+    atomicMin(&s_jr[z - 1], ring);
+    atomicMax(&s_kr[z - 1], ring);
+    int ic[8];
+    corners3(c, a.na, a.ngz, ic);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const long long v = quant<R>(w[q], a.qbits);
+        if (v != 0) deposit_global(a.mfx, a.nz, a.mesh, z, ic[q], v);
+    }
+}
+
+__device__ __forceinline__ void flush_ranges(int nz, int nr, const int *s_jr,
+                                             const int *s_kr, int *jrad, int *krad) {
+    const int tid = threadIdx.x;
+    if (tid < nz && s_kr[tid] >= 0) {
+        atomicMin(&jrad[tid], s_jr[tid] + 1);
+        const int k = s_kr[tid] + 2;
+        atomicMax(&krad[tid], k < nr ? k : nr);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_massgn_kernel(P3Args<R> a) {
+    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ];
+    const int tid = threadIdx.x;
+    if (tid < GH_MAXZ) {
+        s_jr[tid] = INT_MAX;
+        s_kr[tid] = -1;
+    }
+    __syncthreads();
+    const long long i = (long long)blockIdx.x * P3_BLOCK + tid;
+    if (i < a.n) {
+        const int z = a.zone[i];
+        if (z >= 1 && z <= a.nz) {
+            const typename Vec4<R>::T p = a.xv[i];
+            R w[8];
+            const int c = weight3<R>(p.x, p.y, p.z, a, a.nsect > 1, w);
+            deposit3<R>(a, z, c, w, s_jr, s_kr);
+        }
+    }
+    __syncthreads();
+    flush_ranges(a.nz, a.nr, s_jr, s_kr, a.jrad, a.krad);
+}
+
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
+    typedef typename Vec4<R>::T R4;
+    typedef typename Vec2<R>::T R2;
+    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ];
+    __shared__ int s_dn, s_np;
+    const int tid = threadIdx.x;
+    if (tid < GH_MAXZ) {
+        s_jr[tid] = INT_MAX;
+        s_kr[tid] = -1;
+    }
+    if (tid == 0) s_dn = s_np = 0;
+    __syncthreads();
+    const long long i = (long long)blockIdx.x * P3_BLOCK + tid;
+    int dn = 0, np = 0;
+    if (i < a.n) {
+        const int nz = a.nz, nlists = nz + 1;
+        const int z0 = a.zone[i];
+        const bool offlist = z0 == nlists;
+        const bool move = offlist ? (a.offanal && a.mzone == nz) : z0 <= a.mzone;
+        if (move) {
+            np = 1;
+            const R4 p = a.xv[i];
+            const R2 q = a.vw[i];
+            R o[3] = {p.x, p.y, p.z};
+            R ov[3] = {p.w, q.x, q.y};
+            R acc[3], nv[3], nx[3];
+            getacc3<R>(a, o[0], o[1], o[2], z0, acc);
+            // stpgrp
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                nv[d] = ov[d] + acc[d];
+                nx[d] = o[d] + nv[d];
+            }
+            int znew = z0;
+            const R rnew = sqrt(nx[0] * nx[0] + nx[1] * nx[1]);
+            const R az = nx[2] < (R)0 ? -nx[2] : nx[2];
+            const bool off = (az > a.zm) || (rnew > a.rgrid);
+            if (!offlist) {
+                if (off) {
+                    znew = nlists;
+                    dn = 1;
+                    if (a.offanal && z0 < nz) {
+                        const R tfac = tsfac3(a, nz) / tsfac3(a, z0);
+#pragma unroll
+                        for (int d = 0; d < 3; d++) chstep1(tfac, o[d], ov[d], acc[d], nx[d], nv[d]);
+                    }
+                }
+            } else if (!off) {
+                dn = -1;
+                znew = nz;
+            }
+            // rezgrp
+            if (a.mzone > 1 && !offlist && znew <= a.mzone) {
+                int kz = zoneno3(a, nx[0], nx[1], nx[2]);
+                if (kz > a.mzone) kz = a.mzone;
+                if (kz > znew) kz = zoneno3(a, o[0], o[1], o[2]);
+                if (kz != znew) {
+                    const R tfac = tsfac3(a, kz) / tsfac3(a, znew);
+#pragma unroll
+                    for (int d = 0; d < 3; d++) chstep1(tfac, o[d], ov[d], acc[d], nx[d], nv[d]);
+                    znew = kz;
+                }
+            }
+            a.xv[i] = mk4(nx[0], nx[1], nx[2], nv[0]);
+            a.vw[i] = mk2(nv[1], nv[2]);
+            a.zone[i] = (uint8_t)znew;
+            if (znew >= 1 && znew <= nz) {
+                R w[8];
+                const int c = weight3<R>(nx[0], nx[1], nx[2], a, a.nsect > 1, w);
+                deposit3<R>(a, znew, c, w, s_jr, s_kr);
+            }
+        }
+    }
+    // block sums of the off-grid change and the pushes
+    for (int o = 32; o > 0; o >>= 1) {
+        dn += __shfl_xor(dn, o, 64);
+        np += __shfl_xor(np, o, 64);
+    }
+    if ((tid & 63) == 0) {
+        if (dn) atomicAdd(&s_dn, dn);
+        if (np) atomicAdd(&s_np, np);
+    }
+    __syncthreads();
+    flush_ranges(a.nz, a.nr, s_jr, s_kr, a.jrad, a.krad);
+    if (tid == 0) {
+        if (s_dn) atomicAdd((unsigned long long *)&a.counters[0], (unsigned long long)(long long)s_dn);
+        if (s_np) atomicAdd((unsigned long long *)&a.counters[1], (unsigned long long)(long long)s_np);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_tmcent_kernel(P3Args<R> a) {
+    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    typename Vec4<R>::T p = a.xv[i];
+    typename Vec2<R>::T q = a.vw[i];
+    R acc[3];
+    getacc3<R>(a, p.x, p.y, p.z, a.zone[i], acc);
+    p.w = p.w - a.fac * acc[0];
+    q.x = q.x - a.fac * acc[1];
+    q.y = q.y - a.fac * acc[2];
+    a.xv[i] = p;
+    a.vw[i] = q;
+}
+
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_gather_kernel(P3GatherArgs<R> b) {
+    typedef typename Vec4<R>::T R4;
+    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+    if (i >= b.n) return;
+    R w[8];
+    const int c = weight3<R>(b.x[i], b.y[i], b.z[i], b.g, b.fold != 0, w);
+    if (b.cell) b.cell[i] = c;
+    if (b.wt)
+        for (int q = 0; q < 8; q++) b.wt[8 * i + q] = w[q];
+    if (b.acc) {
+        int ic[8];
+        corners3(c, b.g.na, b.g.ngz, ic);
+        R s[3] = {(R)0, (R)0, (R)0};
+        for (int q = 0; q < 8; q++) {
+            R4 f = ic[q] < b.g.mesh ? b.fld[ic[q]] : mk4((R)0, (R)0, (R)0, (R)0);
+            s[0] = s[0] + f.x * w[q];
+            s[1] = s[1] + f.y * w[q];
+            s[2] = s[2] + f.z * w[q];
+        }
+        for (int d = 0; d < 3; d++) b.acc[3 * i + d] = s[d];
+    }
+}
+
+// ---- field solve ------------------------------------------------------------------
+
+// grid (ngz, nr): block (plane, ring-1)
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_analyse_kernel(P3SolveArgs<R> a) {
+    __shared__ R rho_s[P3_MAX_NA];
+    __shared__ R tws[2 * P3_MAX_NA];
+    __shared__ int s_rng[4], s_ok;
+    const int tid = threadIdx.x, na = a.na;
+    const int plane = blockIdx.x, ir = blockIdx.y + 1;
+    if (tid == 0) {
+        int kr[GH_MAXZ], rg[4] = {0, 0, 0, 0};
+        const bool ok = solve_ranges(a.jrad, a.krad, a.nz, a.nr, a.mzone, a.lpot,
+                                     a.explicit_zones, a.jzone, a.kzone, kr, rg);
+        s_ok = ok;
+        for (int q = 0; q < 4; q++) s_rng[q] = rg[q];
+        if (plane == 0 && ir == 1) {
+            for (int q = 0; q < 4; q++) a.ranges[q] = ok ? rg[q] : -1;
+            if (ok)
+                for (int z = 1; z < a.nz; z++) a.krad[z] = kr[z];
+        }
+    }
+    __syncthreads();
+    // zoncmb + scaled for this ring and plane
+    const size_t row = ((size_t)(ir - 1) * a.ngz + plane) * na;
+    for (int j = tid; j < na; j += P3_BLOCK) {
+        const size_t idx = row + j;
+        double acc = (double)a.mfx[0][idx] * a.q;
+        for (int z = 2; z <= a.nz; z++) {
+            if (a.use_new[z - 1]) acc = acc + a.fnew[z - 1] * ((double)a.mfx[z - 1][idx] * a.q);
+            if (a.use_old[z - 1]) acc = acc + a.fold[z - 1] * ((double)a.moz[z - 1][idx] * a.q);
+        }
+        const R v = (R)(acc * a.pmass);
+        rho_s[j] = v;
+        a.rho[idx] = v;
+    }
+    for (int j = tid; j < 2 * na; j += P3_BLOCK) tws[j] = a.tw[j];
+    __syncthreads();
+    if (!s_ok || ir < s_rng[0] || ir > s_rng[1]) return;
+    const int wave = tid >> 6, lane = tid & 63, nw = P3_BLOCK / 64;
+    for (int s = wave; s < a.nm; s += nw) {
+        const int m = a.slot_m[s], part = a.slot_p[s];
+        R sa = 0, sb = 0;
+        int k = (int)(((long long)m * lane) % na);
+        const int step = (int)(((long long)m * 64) % na);
+        for (int j = lane; j < na; j += 64) {
+            const R rv = rho_s[j];
+            sa += rv * tws[2 * k];
+            sb += rv * tws[2 * k + 1];
+            k += step;
+            if (k >= na) k -= na;
+        }
+        sa = p3_wave_sum(sa);
+        sb = p3_wave_sum(sb);
+        if (lane == 0) a.A[((size_t)(ir - 1) * a.nm + s) * a.ngz + plane] = part ? -sb : sa;
+    }
+}
+
+// grid (nr): the doubled vertical transform of each component column
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_zfwd_kernel(P3SolveArgs<R> a) {
+    extern __shared__ unsigned char p3_lds[];
+    R *col = (R *)p3_lds;                     // [nm][ngz]
+    const int ngz = a.ngz, P = 2 * ngz, nm = a.nm;
+    R *tz = col + (size_t)nm * ngz;           // [P][2]
+    const int ir = blockIdx.x + 1, tid = threadIdx.x;
+    const int jrs = a.ranges[0], krs = a.ranges[1];
+    if (jrs < 1 || ir < jrs || ir > krs) return;
+    const R *src = a.A + (size_t)(ir - 1) * nm * ngz;
+    for (int e = tid; e < nm * ngz; e += P3_BLOCK) col[e] = src[e];
+    for (int e = tid; e < 2 * P; e += P3_BLOCK) tz[e] = a.twz[e];
+    __syncthreads();
+    R *dst = a.M + (size_t)(ir - 1) * nm * P;
+    for (int o = tid; o < nm * (ngz + 1); o += P3_BLOCK) {
+        const int s = o / (ngz + 1), k = o - s * (ngz + 1);
+        const R *c = col + (size_t)s * ngz;
+        R sa = 0, sb = 0;
+        int idx = 0;
+        for (int p = 0; p < ngz; p++) {
+            sa += c[p] * tz[2 * idx];
+            sb += c[p] * tz[2 * idx + 1];
+            idx += k;
+            if (idx >= P) idx -= P;
+        }
+        R *h = dst + (size_t)s * P;
+        if (k == 0) h[0] = sa;
+        else if (k == ngz) h[P - 1] = sa;
+        else {
+            h[2 * k - 1] = sa;
+            h[2 * k] = -sb;
+        }
+    }
+}
+
+// grid (ng, ngz + 1, ntypes): block (m, k_z, type) applies one Green's matrix
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_conv_kernel(P3SolveArgs<R> a) {
+    __shared__ R rhs[4][P3_MAX_NR];
+    const int im = blockIdx.x + 1, iz = blockIdx.y + 1, type = blockIdx.z;
+    const int ng = a.ng, ngz = a.ngz, mmax = a.mmax, nm = a.nm, nr = a.nr;
+    if (type == 1 && (im < 2 || im > min(ng, mmax - 1))) return;
+    if (type == 2 && (iz < 2 || iz > ngz)) return;
+    if (a.lg[im - 1]) return;
+    const int jrs = a.ranges[0], krs = a.ranges[1], jrf = a.ranges[2], krf = a.ranges[3];
+    if (jrs < 1) return;
+    const int P = 2 * ngz;
+    const bool nvskip = iz > 1 && iz <= ngz;
+    const bool naskip = im > 1 && im < mmax;
+    const int p = max(1, 2 * (iz - 1)) - 1;
+    const int sc = a.slot_of[im - 1], ss = sc + 1;
+    int src[4], dst[4];
+    R sgn[4];
+    bool val[4];
+    dst[0] = sc * P + p; dst[1] = sc * P + p + 1; dst[2] = ss * P + p; dst[3] = ss * P + p + 1;
+    if (type == 0 || type == 3) {
+        for (int q = 0; q < 4; q++) { src[q] = dst[q]; sgn[q] = (R)1; }
+        val[0] = true; val[1] = nvskip; val[2] = naskip; val[3] = naskip && nvskip;
+    } else if (type == 1) {
+        src[0] = ss * P + p; src[1] = ss * P + p + 1; src[2] = sc * P + p; src[3] = sc * P + p + 1;
+        sgn[0] = sgn[1] = (R)1; sgn[2] = sgn[3] = (R)-1;
+        val[0] = val[2] = true; val[1] = val[3] = nvskip;
+    } else {
+        src[0] = sc * P + p + 1; src[1] = sc * P + p; src[2] = ss * P + p + 1; src[3] = ss * P + p;
+        sgn[0] = sgn[2] = (R)1; sgn[1] = sgn[3] = (R)-1;
+        val[0] = val[1] = true; val[2] = val[3] = naskip;
+    }
+    const int nrs = krs - jrs + 1;
+    for (int e = threadIdx.x; e < 4 * nrs; e += P3_BLOCK) {
+        const int q = e / nrs, is = jrs + (e - q * nrs);
+        rhs[q][is - 1] = val[q] ? sgn[q] * a.M[(size_t)(is - 1) * nm * P + src[q]] : (R)0;
+    }
+    __syncthreads();
+    const R *Gm = a.G + ((((size_t)type * (ngz + 1) + (iz - 1)) * ng + (im - 1)) * nr) * nr;
+    for (int ir = jrf + threadIdx.x; ir <= krf; ir += P3_BLOCK) {
+        R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        const R *g = Gm + (ir - 1);
+#pragma unroll 8
+        for (int is = jrs; is <= krs; is++) {
+            const R gv = g[(size_t)(is - 1) * nr];
+            s0 += gv * rhs[0][is - 1];
+            s1 += gv * rhs[1][is - 1];
+            s2 += gv * rhs[2][is - 1];
+            s3 += gv * rhs[3][is - 1];
+        }
+        R *w = a.W + ((size_t)type * nr + (ir - 1)) * nm * P;
+        if (val[0]) w[dst[0]] = s0;
+        if (val[1]) w[dst[1]] = s1;
+        if (val[2]) w[dst[2]] = s2;
+        if (val[3]) w[dst[3]] = s3;
+    }
+}
+
+// grid (nr, ntypes): inverse vertical transform of one ring's columns
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_zinv_kernel(P3SolveArgs<R> a) {
+    extern __shared__ unsigned char p3_lds[];
+    const int ngz = a.ngz, P = 2 * ngz, nm = a.nm;
+    R *h = (R *)p3_lds;                       // [nm][P]
+    R *tz = h + (size_t)nm * P;               // [P][2]
+    const int ir = blockIdx.x + 1, type = blockIdx.y, tid = threadIdx.x;
+    const int jrs = a.ranges[0], jrf = a.ranges[2], krf = a.ranges[3];
+    if (jrs < 1 || ir < jrf || ir > krf) return;
+    const R *src = a.W + ((size_t)type * a.nr + (ir - 1)) * nm * P;
+    for (int e = tid; e < nm * P; e += P3_BLOCK) h[e] = src[e];
+    for (int e = tid; e < 2 * P; e += P3_BLOCK) tz[e] = a.twz[e];
+    __syncthreads();
+    R *dst = a.Z + ((size_t)type * a.nr + (ir - 1)) * ngz * nm;
+    for (int o = tid; o < nm * ngz; o += P3_BLOCK) {
+        const int s = o / ngz, q = o - s * ngz;
+        const R *c = h + (size_t)s * P;
+        R v = c[0] + ((q & 1) ? -c[P - 1] : c[P - 1]);
+        int idx = q;
+        for (int k = 1; k < ngz; k++) {
+            v += (R)2 * (c[2 * k - 1] * tz[2 * idx] - c[2 * k] * tz[2 * idx + 1]);
+            idx += q;
+            if (idx >= P) idx -= P;
+        }
+        dst[(size_t)q * nm + s] = v;
+    }
+}
+
+// grid (ngz, nr): block (plane, ring-1): inverse azimuthal transform of the
+// active components, scale, polcat; interleaved field store
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_synth_kernel(P3SolveArgs<R> a) {
+    typedef typename Vec4<R>::T R4;
+    __shared__ R zc[4][P3_MAX_NM];
+    __shared__ R tws[2 * P3_MAX_NA];
+    __shared__ int sm[P3_MAX_NM], sp[P3_MAX_NM];
+    const int plane = blockIdx.x, ir = blockIdx.y + 1, tid = threadIdx.x;
+    const int na = a.na, nm = a.nm, ngz = a.ngz, mmax = a.mmax;
+    const int jrs = a.ranges[0], jrf = a.ranges[2], krf = a.ranges[3];
+    if (jrs < 1 || ir < jrf || ir > krf) return;
+    for (int e = tid; e < 4 * nm; e += P3_BLOCK) {
+        const int t = e / nm, s = e - t * nm;
+        R v = (R)0;
+        if (t < a.ntypes && !(t == 1 && a.ng == 1))
+            v = a.Z[(((size_t)t * a.nr + (ir - 1)) * ngz + plane) * nm + s];
+        zc[t][s] = v;
+    }
+    for (int e = tid; e < nm; e += P3_BLOCK) {
+        sm[e] = a.slot_m[e];
+        sp[e] = a.slot_p[e];
+    }
+    for (int j = tid; j < 2 * na; j += P3_BLOCK) tws[j] = a.tw[j];
+    __syncthreads();
+    const R scale = (R)1 / (R)(2 * ngz * na);
+    const size_t row = ((size_t)(ir - 1) * ngz + plane) * na;
+    for (int j = tid; j < na; j += P3_BLOCK) {
+        R F0 = 0, F1 = 0, F2 = 0, F3 = 0;
+        for (int s = 0; s < nm; s++) {
+            const int m = sm[s];
+            R b;
+            if (m == 0) b = (R)1;
+            else if (m == mmax - 1) b = (j & 1) ? (R)-1 : (R)1;
+            else {
+                const int k = (int)(((long long)m * j) % na);
+                b = sp[s] ? (R)-2 * tws[2 * k + 1] : (R)2 * tws[2 * k];
+            }
+            F0 += zc[0][s] * b;
+            F1 += zc[1][s] * b;
+            F2 += zc[2][s] * b;
+            F3 += zc[3][s] * b;
+        }
+        F0 *= scale;
+        F1 *= scale;
+        F2 *= scale;
+        F3 *= scale;
+        const R cp = tws[2 * j], sn = tws[2 * j + 1];
+        R4 out;
+        out.x = F0 * cp - F1 * sn;
+        out.y = F0 * sn + F1 * cp;
+        out.z = F2;
+        out.w = a.lpot ? F3 : a.fld[row + j].w;
+        a.fld[row + j] = out;
+    }
+}
+
+// ---- Green's tables (FP64) --------------------------------------------------------
+
+__device__ double p3_softm(double ds) {
+    if (ds <= 0.0) return 0.0;
+    if (ds < 1.0)
+        return 4.0 * ds * ds * ds * (1.0 / 3.0 - 0.3 * ds * ds + 0.125 * ds * ds * ds);
+    if (ds < 2.0)
+        return -1.0 / 15.0 + ds * ds * ds *
+               (8.0 / 3.0 - 3.0 * ds + 1.2 * ds * ds - ds * ds * ds / 6.0);
+    return 1.0;
+}
+
+// radfor / azifor / vrtfor / grnfun with the field point at height zf
+// (radfor.f:25-38, azifor.f:25-40, vrtfor.f:25-40, sftpot.f:20-40)
+__device__ double p3_kernel(int type, double rs, double rf, double cth, double sth,
+                            double zf, double eps, int tsoft) {
+    const double d2 = rs * rs + rf * rf - 2.0 * rs * rf * cth + zf * zf;
+    if (type == 3) {
+        if (tsoft == 1) return -1.0 / sqrt(d2 + eps * eps);
+        const double x = sqrt(d2) / eps;
+        if (x <= 0.0) return -1.4 / eps;
+        if (x < 1.0) return (-1.4 + x * x * (2.0 / 3.0 - 0.3 * x * x + 0.1 * x * x * x)) / eps;
+        if (x < 2.0)
+            return (-1.6 + 1.0 / (15.0 * x) +
+                    x * x * (4.0 / 3.0 - x + 0.3 * x * x - x * x * x / 30.0)) / eps;
+        return -1.0 / sqrt(d2);
+    }
+    const double num = type == 0 ? rs * cth - rf : type == 1 ? -rs * sth : -zf;
+    if (tsoft == 1) {
+        const double e = d2 + eps * eps;
+        return num / (e * sqrt(e));
+    }
+    if (type == 2 ? zf != 0.0 : d2 > 0.0)
+        return num * p3_softm(sqrt(d2) / eps) / (d2 * sqrt(d2));
+    return 0.0;
+}
+
+constexpr int GREEN3_BLOCK = 128;
+constexpr int GREEN3_CHUNK = 16;
+
+// grid (nr, nr, 4): block (is, ir, type).  Phase 1: thread per height iz,
+// the angular DCT-I / DST-I of the kernel samples for the kept harmonics
+// (GREEN3_CHUNK at a time), filtered; phase 2: the vertical DCT-I (DST-I
+// for F_z) of each harmonic's column.  twd: cos, sin of 2 pi k / na;
+// twzd: cos, sin of 2 pi k / (2 ngz).
+__global__ __launch_bounds__(GREEN3_BLOCK) void p3_green_kernel(
+    double *__restrict__ G, const double *__restrict__ twd,
+    const double *__restrict__ twzd, const int *__restrict__ lg, int nr, int na,
+    int ngz, int ng, double alpha, double dzg, double eps, int tsoft) {
+    extern __shared__ unsigned char p3_lds[];
+    double *T = (double *)p3_lds;   // [nzp][nmq]
+    const int is = blockIdx.x + 1, ir = blockIdx.y + 1, type = blockIdx.z;
+    const int mmax = na / 2 + 1, P = 2 * ngz;
+    const bool azim = type == 1, vert = type == 2;
+    if (azim && ng == 1) return;
+    const int it1 = azim ? 2 : 1, it2 = azim ? mmax - 1 : mmax;
+    const int npt = it2 - it1 + 1;
+    const int iz1 = vert ? 2 : 1, iz2 = vert ? ngz : ngz + 1;
+    const int nzp = iz2 - iz1 + 1;
+    const int nmq = azim ? min(ng, mmax - 1) - 1 : min(ng, mmax);
+    if (nmq <= 0) return;
+    const double rs = exp(alpha * (double)(is - 1)) - 1.0;
+    const double rf = exp(alpha * (double)(ir - 1)) - 1.0;
+    for (int iz = iz1 + threadIdx.x; iz <= iz2; iz += GREEN3_BLOCK) {
+        const double zf = dzg * (double)(iz - 1);
+        for (int m0 = 0; m0 < nmq; m0 += GREEN3_CHUNK) {
+            double acc[GREEN3_CHUNK];
+            int idx[GREEN3_CHUNK], inc[GREEN3_CHUNK];
+#pragma unroll
+            for (int u = 0; u < GREEN3_CHUNK; u++) {
+                acc[u] = 0.0;
+                // output index mo = m0 + u: harmonic mo (DCT) or mo + 1 (DST);
+                // sample q has angle index it - 1 = q (DCT) or q + 1 (DST)
+                const int h = azim ? m0 + u + 1 : m0 + u;
+                inc[u] = h % na;
+                idx[u] = azim ? inc[u] : 0;
+            }
+            for (int q = 0; q < npt; q++) {
+                const int k = it1 - 1 + q;
+                const double f = p3_kernel(type, rs, rf, twd[2 * k], twd[2 * k + 1], zf, eps, tsoft);
+                const double wq = azim ? 2.0 : ((q == 0 || q == npt - 1) ? 1.0 : 2.0);
+                const double fw = wq * f;
+#pragma unroll
+                for (int u = 0; u < GREEN3_CHUNK; u++) {
+                    acc[u] += fw * (azim ? twd[2 * idx[u] + 1] : twd[2 * idx[u]]);
+                    idx[u] += inc[u];
+                    if (idx[u] >= na) idx[u] -= na;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < GREEN3_CHUNK; u++) {
+                const int mo = m0 + u;
+                if (mo >= nmq) break;
+                const int h = azim ? mo + 1 : mo;
+                T[(size_t)(iz - iz1) * nmq + mo] = (h >= ng || lg[h]) ? 0.0 : acc[u];
+            }
+        }
+    }
+    __syncthreads();
+    const size_t nn = (size_t)nr * nr;
+    for (int o = threadIdx.x; o < nzp * nmq; o += GREEN3_BLOCK) {
+        const int p = o / nmq, mo = o - p * nmq;
+        double v = 0.0;
+        if (!vert) {
+            // DCT-I over nzp = ngz + 1 points
+            v = T[mo] + ((p & 1) ? -T[(size_t)(nzp - 1) * nmq + mo] : T[(size_t)(nzp - 1) * nmq + mo]);
+            int id = p;
+            for (int k = 1; k < nzp - 1; k++) {
+                v += 2.0 * T[(size_t)k * nmq + mo] * twzd[2 * id];
+                id += p;
+                if (id >= P) id -= P;
+            }
+        } else {
+            // DST-I over nzp = ngz - 1 points: output i = p + 1
+            const int i = p + 1;
+            int id = i;
+            for (int k = 1; k <= nzp; k++) {
+                v += 2.0 * T[(size_t)(k - 1) * nmq + mo] * twzd[2 * id + 1];
+                id += i;
+                if (id >= P) id -= P;
+            }
+        }
+        const int kz = vert ? p + 1 : p;
+        const int m = azim ? mo + 1 : mo;
+        G[((((size_t)type * (ngz + 1) + kz) * ng + m) * nn) + (size_t)(is - 1) * nr + (ir - 1)] = v;
+    }
+}
+
+// ---- launchers ------------------------------------------------------------------------
+
+static unsigned p3_blocks(long long n) { return (unsigned)((n + P3_BLOCK - 1) / P3_BLOCK); }
+
+template <typename R> hipError_t launch_p3_massgn(const P3Args<R> &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_massgn_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename R> hipError_t launch_p3_push(const P3Args<R> &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_push_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_tmcent_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename R> hipError_t launch_p3_gather(const P3GatherArgs<R> &b, hipStream_t s) {
+    if (b.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_gather_kernel<R>, dim3(p3_blocks(b.n)), dim3(P3_BLOCK), 0, s, b);
+    return hipGetLastError();
+}
+
+// Limits the launches below rely on (checked again by the context set-up).
+template <typename R> bool p3_solve_fits(const P3SolveArgs<R> &a) {
+    const size_t P = 2 * (size_t)a.ngz;
+    return a.na <= P3_MAX_NA && a.nm <= P3_MAX_NM && a.nr <= P3_MAX_NR &&
+           ((size_t)a.nm * a.ngz + 2 * P) * sizeof(R) <= 48 * 1024 &&
+           ((size_t)a.nm * P + 2 * P) * sizeof(R) <= 48 * 1024;
+}
+
+template <typename R> hipError_t launch_p3_solve(const P3SolveArgs<R> &a, hipStream_t s) {
+    if (!p3_solve_fits(a)) return hipErrorInvalidValue;
+    const size_t P = 2 * (size_t)a.ngz;
+    hipLaunchKernelGGL(p3_analyse_kernel<R>, dim3(a.ngz, a.nr), dim3(P3_BLOCK), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(p3_zfwd_kernel<R>, dim3(a.nr), dim3(P3_BLOCK),
+                       ((size_t)a.nm * a.ngz + 2 * P) * sizeof(R), s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(p3_conv_kernel<R>, dim3(a.ng, a.ngz + 1, a.ntypes), dim3(P3_BLOCK),
+                       0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(p3_zinv_kernel<R>, dim3(a.nr, a.ntypes), dim3(P3_BLOCK),
+                       ((size_t)a.nm * P + 2 * P) * sizeof(R), s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(p3_synth_kernel<R>, dim3(a.ngz, a.nr), dim3(P3_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_p3_green(double *G, const double *twd, const double *twzd, const int *lg,
+                           int nr, int na, int ngz, int ng, double alpha, double dzg,
+                           double eps, int tsoft, hipStream_t s) {
+    const int mmax = na / 2 + 1;
+    const size_t lds = sizeof(double) * (size_t)(ngz + 1) * (ng < mmax ? ng : mmax);
+    if (lds > 48 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(p3_green_kernel, dim3(nr, nr, 4), dim3(GREEN3_BLOCK), lds, s, G, twd,
+                       twzd, lg, nr, na, ngz, ng, alpha, dzg, eps, tsoft);
+    return hipGetLastError();
+}
+
+#define P3_INST(R)                                                                     \
+    template hipError_t launch_p3_massgn<R>(const P3Args<R> &, hipStream_t);          \
+    template hipError_t launch_p3_push<R>(const P3Args<R> &, hipStream_t);            \
+    template hipError_t launch_p3_tmcent<R>(const P3Args<R> &, hipStream_t);          \
+    template hipError_t launch_p3_gather<R>(const P3GatherArgs<R> &, hipStream_t);    \
+    template hipError_t launch_p3_solve<R>(const P3SolveArgs<R> &, hipStream_t);      \
+    template bool p3_solve_fits<R>(const P3SolveArgs<R> &);
+P3_INST(float)
+P3_INST(double)
+
+}  // namespace gh

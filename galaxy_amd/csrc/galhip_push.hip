@@ -98,21 +98,6 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// Global deposit of one corner value with Fortran's flat indexing over the
-// [nz][mesh] zone arrays (grdmss(j, jz)); past the last array it is dropped.
-__device__ __forceinline__ void deposit_global(long long *const *mfx, int nz,
-                                               int mesh, int zone, int idx,
-                                               long long v) {
-    long long f = (long long)(zone - 1) * mesh + idx;
-    if (f < (long long)nz * mesh) {
-        int zf = zone - 1, cf = idx;
-        while (cf >= mesh) {
-            cf -= mesh;
-            zf++;
-        }
-        atomicAdd((unsigned long long *)&mfx[zf][cf], (unsigned long long)v);
-    }
-}
 
 template <typename R>
 // Wavefronts per SIMD, measured: FP32 at least 5 (96 VGPRs, no spill; 4

@@ -206,3 +206,43 @@ def to_internal(R, phi, vR, vphi, lscale, ts, zbr_model, nstep, rgrid):
     vy *= fac
     iz[r > rgrid] = nz + 1
     return x, y, vx, vy, iz
+
+
+def vertical_sech2(R, seed, z0=0.1, md=0.5, rtrunc=5.0):
+    """Heights and vertical velocities (model units) for a disc given the
+    vertical profile rho ~ sech^2(z / z0) at every radius, with sigma_z from
+    the isothermal sheet's equilibrium sigma_z^2 = pi G Sigma(R) z0 (Spitzer
+    1942); Gaussian vertical velocities.  Its own Philox stream (key
+    seed + 2^32), so (R, phi, vR, vphi) stay those of exponential_disc."""
+    rng = np.random.Generator(np.random.Philox(key=seed + (1 << 32)))
+    n = len(R)
+    u = rng.random(n)
+    # inverse of the sech^2 CDF (1 + tanh(z / z0)) / 2
+    z = z0 * np.arctanh(np.clip(2.0 * u - 1.0, -1.0 + 1e-15, 1.0 - 1e-15))
+    mt = 1.0 - (1.0 + rtrunc) * math.exp(-rtrunc)
+    sig = md / (2.0 * math.pi * mt) * np.exp(-R)
+    vz = np.sqrt(math.pi * sig * z0) * rng.standard_normal(n)
+    return z, vz
+
+
+def to_internal3(R, phi, z, vR, vphi, vz, lscale, ts, zbr_model, nstep, rgrid, zm):
+    """Internal units for the 3-D grid: as to_internal, zones by the
+    spherical radius (SRC/lib15/zoneno.f:40-41, setrun.f:810), off the grid
+    when R > rgrid or |z| > zm (SRC/lib15/offgrd.f:69-72)."""
+    x = R * np.cos(phi) * lscale
+    y = R * np.sin(phi) * lscale
+    zz = z * lscale
+    vx = (vR * np.cos(phi) - vphi * np.sin(phi)) * lscale * ts
+    vy = (vR * np.sin(phi) + vphi * np.cos(phi)) * lscale * ts
+    vzz = vz * lscale * ts
+    r = np.sqrt(R * R + z * z) * lscale
+    nz = len(nstep)
+    iz = np.ones(len(R), np.int32)
+    for i in range(nz):
+        iz[r >= zbr_model[i] * lscale] = i + 1
+    fac = np.asarray(nstep, np.float64)[iz - 1]
+    vx *= fac
+    vy *= fac
+    vzz *= fac
+    iz[(R * lscale > rgrid) | (np.abs(zz) > zm)] = nz + 1
+    return x, y, zz, vx, vy, vzz, iz

@@ -15,6 +15,9 @@ reference routines they replace and take the same meaning of arguments
   ``run(n)``        galaxy's loop of findf + step
   ``weight()``, ``grdacc()``  group-level routines for parity checks
 
+``P3DGrid`` is the same interface for the 3-D polar grid (p3dset; planes
+in z, three coordinates per particle; findf -> p3fndf).
+
 Errors raise ``GalhipError`` where the reference would call crash().
 Arrays are numpy; coordinates in grid units, velocities in displacement per
 step (SRC/lib15/setrun.f:858-873).
@@ -26,7 +29,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (GALHIP_FP32, GALHIP_FP64, MAXSKIP, MAXZONES, P2DParams,
-                   check, load)
+                   P3DParams, check, load)
 
 
 def _ptr(a):
@@ -268,3 +271,102 @@ class P2DGrid:
         self._c(self._lib.galhip_get_profile(self._h, C.byref(ms), C.byref(nl),
                                              C.byref(npush)), "get_profile")
         return ms.value, nl.value, npush.value
+
+
+class P3DGrid(P2DGrid):
+    """A 3-D polar grid (cylindrical rings x spokes x planes) with its
+    particles on one GPU: p3dset + setgrd + filset + setspc.  Zone
+    boundaries are spherical radii; the off-grid rule adds |z| > zm."""
+
+    def __init__(self, nr, na, ngz, dzg, hash_, softl, tsoft=1, nsect=1, skip=(),
+                 nzones=1, nstep=(1,), zbr=(0.0,), offanal=True, pmass=1.0,
+                 precision="fp32"):
+        lib = load()
+        p = P3DParams()
+        p.nr, p.na, p.ngz, p.hash, p.nsect = nr, na, ngz, hash_, nsect
+        skip = list(skip)
+        if len(skip) > MAXSKIP:
+            raise ValueError("at most %d skipped harmonics" % MAXSKIP)
+        p.nskip = len(skip)
+        for i, m in enumerate(skip):
+            p.skip[i] = m
+        p.dzg, p.softl, p.tsoft = float(dzg), float(softl), int(tsoft)
+        if nzones > MAXZONES:
+            raise ValueError("at most %d zones" % MAXZONES)
+        p.nzones = nzones
+        for i in range(nzones):
+            p.nstep[i] = int(nstep[i])
+            p.zbr[i] = float(zbr[i])
+        p.offanal = int(bool(offanal))
+        p.pmass = float(pmass)
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("precision must be 'fp32' or 'fp64'")
+        p.precision = GALHIP_FP32 if precision == "fp32" else GALHIP_FP64
+        self.dtype = np.float32 if precision == "fp32" else np.float64
+        self.precision = precision
+        h = C.c_void_p()
+        check(lib.galhip_p3dset(C.byref(p), C.byref(h)), "p3dset")
+        self._lib, self._h = lib, h
+        g = np.zeros(6)
+        check(lib.galhip_geometry(h, _ptr(g)), "geometry")
+        self.alpha, self.rgrid, self.thmax = g[0], g[1], g[2]
+        self.ng, self.mmax, self.mesh = int(g[3]), int(g[4]), int(g[5])
+        self.nr, self.na, self.ngz, self.nsect, self.nzones = nr, 2 * (na // 2), ngz, nsect, nzones
+        self.dzg = float(self.dtype(dzg))
+        self.zm = float(self.dtype(self.dtype(0.5) * self.dtype(dzg)) * self.dtype(ngz - 1))
+        self.n = 0
+
+    def get_green(self):
+        G = np.zeros((4, self.ngz + 1, self.ng, self.nr, self.nr), np.float64)
+        self._c(self._lib.galhip_get_green(self._h, _ptr(G)), "get_green")
+        return G
+
+    def set_green(self, G):
+        G = np.ascontiguousarray(G, np.float64)
+        assert G.shape == (4, self.ngz + 1, self.ng, self.nr, self.nr)
+        self._c(self._lib.galhip_set_green(self._h, _ptr(G)), "set_green")
+
+    def restrt(self, x, y, z, vx, vy, vz, iz=None):
+        dt = self.dtype
+        arrs = [np.ascontiguousarray(a, dt) for a in (x, y, z, vx, vy, vz)]
+        n = len(arrs[0])
+        iz = np.ones(n, np.int32) if iz is None else np.ascontiguousarray(iz, np.int32)
+        self._c(self._lib.galhip_restrt3(self._h, n, *[_ptr(a) for a in arrs], _ptr(iz)),
+                "restrt3")
+        self.n = n
+
+    def unload(self):
+        n = self.n
+        out = {k: np.empty(n, self.dtype) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        out["iz"] = np.empty(n, np.int32)
+        self._c(self._lib.galhip_unload3(self._h, *[_ptr(out[k]) for k in
+                                                   ("x", "y", "z", "vx", "vy", "vz", "iz")]),
+                "unload3")
+        return out
+
+    def weight(self, x, y, z, old=True):
+        x, y, z = (np.ascontiguousarray(a, self.dtype) for a in (x, y, z))
+        n = len(x)
+        cell = np.empty(n, np.int32)
+        wt = np.empty((n, 8), self.dtype)
+        self._c(self._lib.galhip_weight3(self._h, n, _ptr(x), _ptr(y), _ptr(z),
+                                         int(bool(old)), _ptr(cell), _ptr(wt)), "weight3")
+        return cell, wt
+
+    def grdacc(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, self.dtype) for a in (x, y, z))
+        acc = np.empty((len(x), 3), self.dtype)
+        self._c(self._lib.galhip_grdacc3(self._h, len(x), _ptr(x), _ptr(y), _ptr(z),
+                                         _ptr(acc)), "grdacc3")
+        return acc
+
+    def get_field(self):
+        f = np.zeros((4, self.mesh), self.dtype)
+        self._c(self._lib.galhip_get_field(self._h, _ptr(f)), "get_field")
+        return f
+
+    def set_field(self, fld):
+        f = np.ascontiguousarray(fld, self.dtype)
+        if f.shape[0] == 3:
+            f = np.concatenate([f, np.zeros((1, self.mesh), self.dtype)])
+        self._c(self._lib.galhip_set_field(self._h, _ptr(f)), "set_field")

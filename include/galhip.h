@@ -1,6 +1,7 @@
 /*
  * galhip.h -- C ABI of libgalhip.so, the MI355X (gfx950) implementation of
- * GALAXY's 2-D polar-grid particle-mesh step.
+ * GALAXY's polar-grid particle-mesh step: the 2-D grid (p2d) and the 3-D
+ * cylindrical grid (p3d) with vertical planes.
  *
  * Every entry point takes plain scalars and pointers and returns 0 on
  * success or a negative GALHIP_E* code; galhip_last_error() returns the
@@ -17,7 +18,8 @@
  * (the reference is single-threaded too).
  *
  * Array conventions (as the reference's): rings ir = 1..nr, spokes
- * it = 1..na, cell index (ir-1)*na + (it-1); zones 1..nzones, zone
+ * it = 1..na, cell index (ir-1)*na + (it-1), for p3d with planes
+ * iz = 1..ngz ((ir-1)*ngz + (iz-1))*na + (it-1); zones 1..nzones, zone
  * nzones+1 = the off-grid list.  Coordinates are grid units, velocities are
  * displacements per step and accelerations velocity changes per step
  * (internal units, SRC/lib15/setrun.f:858-873).  Floating point buffers are
@@ -106,7 +108,9 @@ int galhip_unload(galhip_ctx *ctx, void *x, void *y, void *vx, void *vy,
 
 /* Step counter and zone time stamps (/admin/ istep, /point/ lstep,
  * SRC/lib15/inc/admin.f:36-37).  lstep is [2][nzones] with zone 1 unused;
- * either pointer may be NULL on get. */
+ * either pointer may be NULL on get.  Setting lstep also forms zoncmb's
+ * zone weights for the new time (SRC/lib15/zoncmb.f:40-125), failing as
+ * zoncmb does when a zone's mass array is at a useless time. */
 int galhip_set_step(galhip_ctx *ctx, int32_t istep, const int32_t *lstep);
 int galhip_get_step(const galhip_ctx *ctx, int32_t *istep, int32_t *mzone,
                     int32_t *lstep);
@@ -208,6 +212,61 @@ int galhip_set_profile(galhip_ctx *ctx, int32_t on);
 int galhip_get_profile(galhip_ctx *ctx, double *push_ms, int64_t *launches,
                        int64_t *pushes);
 int galhip_sync(galhip_ctx *ctx);
+
+/* ---- 3-D polar grid (p3d) ----
+ * A p3d context takes the generic entry points above with these meanings:
+ * galhip_greenm / get_green / set_green (p3grnm, tables
+ * G[4][ngz+1][ng][nr][nr] doubles: type 0..3 = F_R, F_phi, F_z, potential,
+ * index [type][k_z][m][source ring][field ring] as p3gwrt writes them,
+ * SRC/lib15/p3gwrt.f:46-64; entries p3conv never reads are zero),
+ * set_fixforce (a spherical radial force, getacc.f:106-128), set_step /
+ * get_step, masset, findf and p2fndf (-> p3fndf, SRC/lib15/p3fndf.f:1, and
+ * polcat over every plane), step, tmcent, run, get_mass / set_mass /
+ * get_rho (mesh = nr ngz na), get_field / set_field (fld[4][mesh]: F_x,
+ * F_y, F_z, potential), get_ranges, get_noff, get_order, the profiling
+ * calls and sync.  Particles go in and out through galhip_restrt3 and
+ * galhip_unload3; galhip_restrt, galhip_unload, galhip_weight,
+ * galhip_grdacc, galhip_sort and galhip_integrals return GALHIP_EINVAL on a
+ * p3d context.  Zone boundaries are spherical radii (SRC/lib15/setrun.f:810,
+ * zoneno.f:40-41); a particle is off the grid when |z| > zm or R > rgrid
+ * (SRC/lib15/offgrd.f:69-72), zm = dzg (ngz - 1) / 2. */
+typedef struct {
+    int32_t nr, na, ngz;       /* GRID card: rings, spokes, planes */
+    int32_t hash;              /* HASH card */
+    int32_t nsect;             /* SECT card */
+    int32_t nskip;             /* SKIP card */
+    int32_t skip[GALHIP_MAXSKIP];
+    double dzg;                /* Z SPACING card: plane spacing, grid units */
+    double softl;              /* SOFT card (softl >= dzg) */
+    int32_t tsoft;             /* 1 Plummer, 2 cubic spline */
+    int32_t nzones;
+    int32_t nstep[GALHIP_MAXZONES];
+    double zbr[GALHIP_MAXZONES];     /* inner spherical radius of zone i */
+    int32_t offanal;
+    double pmass;
+    int32_t precision;         /* GALHIP_FP32 or GALHIP_FP64 */
+} galhip_p3d_params;
+
+/* p3dset + setgrd + filset + setspc for one p3d grid with linear
+ * interpolation (jmass = 2).  Replaces SRC/lib15/p3dset.f:1,
+ * SRC/lib15/setgrd.f:1, SRC/lib15/filset.f:1, SRC/lib15/setspc.f:1. */
+int galhip_p3dset(const galhip_p3d_params *p, galhip_ctx **out);
+/* restrt / unload (particle part) for p3d: positions and velocities in
+ * six arrays, zones as for galhip_restrt (SRC/lib15/restrt.f:1,
+ * SRC/lib15/unload.f:1).  Any output pointer may be NULL. */
+int galhip_restrt3(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
+                   const void *z, const void *vx, const void *vy, const void *vz,
+                   const int32_t *iz);
+int galhip_unload3(galhip_ctx *ctx, void *x, void *y, void *z, void *vx,
+                   void *vy, void *vz, int32_t *iz);
+/* weight(jst, old) for n p3d positions (SRC/lib15/weight.f:195-250):
+ * 0-based cells and wt[n][8] in the reference's corner order. */
+int galhip_weight3(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
+                   const void *z, int32_t old, int32_t *cell, void *wt);
+/* grdacc(jst) for n p3d positions (SRC/lib15/grdacc.f:189-211): acc[n][3]
+ * interpolated from the current Cartesian field. */
+int galhip_grdacc3(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
+                   const void *z, void *acc);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,18 @@ module galhip_f
      integer(c_int32_t) :: precision
   end type galhip_p2d_params
 
+  type, bind(c) :: galhip_p3d_params
+     integer(c_int32_t) :: nr, na, ngz, hash, nsect, nskip
+     integer(c_int32_t) :: skip(GALHIP_MAXSKIP)
+     real(c_double) :: dzg, softl
+     integer(c_int32_t) :: tsoft, nzones
+     integer(c_int32_t) :: nstep(GALHIP_MAXZONES)
+     real(c_double) :: zbr(GALHIP_MAXZONES)
+     integer(c_int32_t) :: offanal
+     real(c_double) :: pmass
+     integer(c_int32_t) :: precision
+  end type galhip_p3d_params
+
   interface
      integer(c_int) function galhip_p2dset(p, ctx) bind(c, name='galhip_p2dset')
        import :: c_int, c_ptr, galhip_p2d_params
@@ -225,5 +237,41 @@ module galhip_f
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function galhip_sync
+     integer(c_int) function galhip_p3dset(p, ctx) bind(c, name='galhip_p3dset')
+       import :: c_int, c_ptr, galhip_p3d_params
+       type(galhip_p3d_params), intent(in) :: p
+       type(c_ptr), intent(out) :: ctx
+     end function galhip_p3dset
+     integer(c_int) function galhip_restrt3(ctx, n, x, y, z, vx, vy, vz, iz) &
+          bind(c, name='galhip_restrt3')
+       import :: c_int, c_ptr, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y, z, vx, vy, vz
+       integer(c_int32_t), intent(in) :: iz(*)
+     end function galhip_restrt3
+     integer(c_int) function galhip_unload3(ctx, x, y, z, vx, vy, vz, iz) &
+          bind(c, name='galhip_unload3')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: x, y, z, vx, vy, vz, iz
+     end function galhip_unload3
+     integer(c_int) function galhip_weight3(ctx, n, x, y, z, old, cell, wt) &
+          bind(c, name='galhip_weight3')
+       import :: c_int, c_ptr, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y, z
+       integer(c_int32_t), value :: old
+       integer(c_int32_t), intent(out) :: cell(*)
+       type(c_ptr), value :: wt
+     end function galhip_weight3
+     integer(c_int) function galhip_grdacc3(ctx, n, x, y, z, acc) &
+          bind(c, name='galhip_grdacc3')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y, z, acc
+     end function galhip_grdacc3
   end interface
 end module galhip_f

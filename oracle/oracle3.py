@@ -223,7 +223,10 @@ class P3D(_o.P2D):
 
     def findf(self, lpot=False):
         """findf -> p3fndf -> polcat (findf.f:40-60, p3fndf.f:37-104,
-        polcat.f:22-40).  Rings outside the field range keep the last field."""
+        polcat.f:22-40).  The solve replaces rings jrf..krf; polcat then
+        turns every ring, as the reference's does, so on a partial step the
+        rings outside the field range are turned once more (no moving
+        particle reads them)."""
         jrs, krs = self.source_range()
         jrf, krf = self.field_range(lpot)
         ntypes = 4 if lpot else 3
