@@ -391,8 +391,8 @@ template <typename R> static int sort_impl(galhip_ctx *c) {
 }
 
 extern "C" int galhip_sort(galhip_ctx *c) {
-    if (c && c->p3d) return gh_fail(GALHIP_EINVAL, "galhip_sort: not built for p3d contexts");
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d) return p3_sort(c);
     return c->prec == GALHIP_FP32 ? sort_impl<float>(c) : sort_impl<double>(c);
 }
 
@@ -804,6 +804,10 @@ extern "C" int galhip_run(galhip_ctx *c, int32_t nsteps) {
             if (rc) return rc;
             rc = galhip_step(c);
             if (rc) return rc;
+            if (c->sort_interval > 0 && ++c->steps_since_sort >= c->sort_interval) {
+                rc = p3_sort(c);
+                if (rc) return rc;
+            }
         }
         return 0;
     }

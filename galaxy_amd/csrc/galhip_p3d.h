@@ -55,6 +55,23 @@ template <typename R> struct P3SolveArgs {
     int nr, na, ngz, ng, mmax, nm, nz, mzone, lpot, jzone, kzone, explicit_zones, ntypes;
 };
 
+template <typename R> struct P3PermArgs {
+    const typename Vec4<R>::T *xv_in;
+    typename Vec4<R>::T *xv_out;
+    const typename Vec2<R>::T *vw_in;
+    typename Vec2<R>::T *vw_out;
+    const uint8_t *zin;
+    uint8_t *zout;
+    const unsigned *iin;
+    unsigned *iout;
+    const unsigned *perm;
+    long long n;
+};
+
+template <typename R>
+hipError_t launch_p3_key(const P3Args<R> &, int kshift, unsigned kmax, unsigned *key,
+                         unsigned *perm, hipStream_t);
+template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_massgn(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_push(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &, hipStream_t);

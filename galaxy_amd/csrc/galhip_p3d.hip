@@ -171,20 +171,108 @@ __device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz,
     }
 }
 
-// Fixed-point deposit of one particle's 8 weights into zone z, with the
-// radial range folded into the block's per-zone minimum and maximum.
-template <typename R>
-__device__ __forceinline__ void deposit3(const P3Args<R> &a, int z, int c,
-                                         const R w[8], int *s_jr, int *s_kr) {
-    const int ring = c / (a.na * a.ngz);
-    atomicMin(&s_jr[z - 1], ring);
-    atomicMax(&s_kr[z - 1], ring);
-    int ic[8];
-    corners3(c, a.na, a.ngz, ic);
+__device__ __forceinline__ int p3_wave_min(int v) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const long long v = quant<R>(w[q], a.qbits);
-        if (v != 0) deposit_global(a.mfx, a.nz, a.mesh, z, ic[q], v);
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int p3_wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+constexpr int P3_TILE = 2048;   // cells of a block's LDS deposit window
+
+// Block-level fixed-point deposit.  The block's particles (sorted by cell,
+// so a block covers a few neighbouring cells) are accumulated into an LDS
+// window spanning the (ring, plane, spoke) box of their corners, then the
+// window's non-zero cells are added to the zone array with one global atomic
+// each.  Particles whose corners leave the box shape (the last spoke, whose
+// next corner wraps to spoke 0; the last plane, reachable only through
+// t == na) or that belong to another zone than the block's first, and every
+// particle when the box exceeds P3_TILE cells, add their 8 corners directly.
+// Integer sums: the result does not depend on the grouping.  Every thread of
+// the block must call it (on = false for threads without a deposit).
+// s_b: 7 ints, s_b[0] = INT_MAX on entry.
+template <typename R>
+__device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int z, int c,
+                                               const R w[8], long long *tile, int *s_b,
+                                               int *s_jr, int *s_kr) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int S = a.ngz * a.na;
+    int ring = 0, plane = 0, spoke = 0;
+    if (on) {
+        ring = c / S;
+        const int rem = c - ring * S;
+        plane = rem / a.na;
+        spoke = rem - plane * a.na;
+    }
+    // the block's zone: the lowest among its deposits
+    int zw = p3_wave_min(on ? z : INT_MAX);
+    if (lane == 0 && zw != INT_MAX) atomicMin(&s_b[0], zw);
+    if (tid == 0) {
+        s_b[1] = s_b[3] = s_b[5] = INT_MAX;
+        s_b[2] = s_b[4] = s_b[6] = -1;
+    }
+    __syncthreads();
+    const int tz = s_b[0];
+    const bool cand = on && z == tz && plane + 1 < a.ngz && spoke + 1 < a.na;
+    {
+        const int r0 = p3_wave_min(cand ? ring : INT_MAX), r1 = p3_wave_max(cand ? ring : -1);
+        const int p0 = p3_wave_min(cand ? plane : INT_MAX), p1 = p3_wave_max(cand ? plane : -1);
+        const int q0 = p3_wave_min(cand ? spoke : INT_MAX), q1 = p3_wave_max(cand ? spoke : -1);
+        if (lane == 0 && r1 >= 0) {
+            atomicMin(&s_b[1], r0); atomicMax(&s_b[2], r1);
+            atomicMin(&s_b[3], p0); atomicMax(&s_b[4], p1);
+            atomicMin(&s_b[5], q0); atomicMax(&s_b[6], q1);
+        }
+        // ring ranges: the block zone's by wave reduction, others directly
+        const int jr = p3_wave_min(on && z == tz ? ring : INT_MAX);
+        const int kr = p3_wave_max(on && z == tz ? ring : -1);
+        if (lane == 0 && kr >= 0) {
+            atomicMin(&s_jr[tz - 1], jr);
+            atomicMax(&s_kr[tz - 1], kr);
+        }
+        if (on && z != tz) {
+            atomicMin(&s_jr[z - 1], ring);
+            atomicMax(&s_kr[z - 1], ring);
+        }
+    }
+    __syncthreads();
+    const int r0 = s_b[1], p0 = s_b[3], q0 = s_b[5];
+    const int NR = s_b[2] - r0 + 2, NP = s_b[4] - p0 + 2, NQ = s_b[6] - q0 + 2;
+    const bool use = s_b[2] >= 0 && NR * NP * NQ <= P3_TILE;
+    const int ncell = use ? NR * NP * NQ : 0;
+    for (int e = tid; e < ncell; e += P3_BLOCK) tile[e] = 0;
+    __syncthreads();
+    if (on) {
+        long long v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = quant<R>(w[q], a.qbits);
+        if (use && cand) {
+            const int b = ((ring - r0) * NP + (plane - p0)) * NQ + (spoke - q0);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int o = b + ((q & 1) ? 1 : 0) + ((q & 2) ? NQ : 0) + ((q & 4) ? NP * NQ : 0);
+                if (v[q] != 0) atomicAdd((unsigned long long *)&tile[o], (unsigned long long)v[q]);
+            }
+        } else {
+            int ic[8];
+            corners3(c, a.na, a.ngz, ic);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (v[q] != 0) deposit_global(a.mfx, a.nz, a.mesh, z, ic[q], v[q]);
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < ncell; e += P3_BLOCK) {
+        const long long v = tile[e];
+        if (v != 0) {
+            const int dq = e % NQ, t2 = e / NQ, dp = t2 % NP, dr = t2 / NP;
+            const int cell = ((r0 + dr) * a.ngz + (p0 + dp)) * a.na + (q0 + dq);
+            atomicAdd((unsigned long long *)&a.mfx[tz - 1][cell], (unsigned long long)v);
+        }
     }
 }
 
@@ -200,23 +288,28 @@ __device__ __forceinline__ void flush_ranges(int nz, int nr, const int *s_jr,
 
 template <typename R>
 __global__ __launch_bounds__(P3_BLOCK) void p3_massgn_kernel(P3Args<R> a) {
-    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ];
+    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
+    __shared__ long long tile[P3_TILE];
     const int tid = threadIdx.x;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
         s_kr[tid] = -1;
     }
+    if (tid == 0) s_b[0] = INT_MAX;
     __syncthreads();
     const long long i = (long long)blockIdx.x * P3_BLOCK + tid;
+    bool on = false;
+    int z = 0, c = 0;
+    R w[8];
     if (i < a.n) {
-        const int z = a.zone[i];
+        z = a.zone[i];
         if (z >= 1 && z <= a.nz) {
             const typename Vec4<R>::T p = a.xv[i];
-            R w[8];
-            const int c = weight3<R>(p.x, p.y, p.z, a, a.nsect > 1, w);
-            deposit3<R>(a, z, c, w, s_jr, s_kr);
+            c = weight3<R>(p.x, p.y, p.z, a, a.nsect > 1, w);
+            on = true;
         }
     }
+    deposit3_block<R>(a, on, z, c, w, tile, s_b, s_jr, s_kr);
     __syncthreads();
     flush_ranges(a.nz, a.nr, s_jr, s_kr, a.jrad, a.krad);
 }
@@ -225,17 +318,24 @@ template <typename R>
 __global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
     typedef typename Vec4<R>::T R4;
     typedef typename Vec2<R>::T R2;
-    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ];
+    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
     __shared__ int s_dn, s_np;
+    __shared__ long long tile[P3_TILE];
     const int tid = threadIdx.x;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
         s_kr[tid] = -1;
     }
-    if (tid == 0) s_dn = s_np = 0;
+    if (tid == 0) {
+        s_dn = s_np = 0;
+        s_b[0] = INT_MAX;
+    }
     __syncthreads();
     const long long i = (long long)blockIdx.x * P3_BLOCK + tid;
     int dn = 0, np = 0;
+    bool don = false;
+    int dz = 0, dc = 0;
+    R dw[8];
     if (i < a.n) {
         const int nz = a.nz, nlists = nz + 1;
         const int z0 = a.zone[i];
@@ -289,12 +389,13 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
             a.vw[i] = mk2(nv[1], nv[2]);
             a.zone[i] = (uint8_t)znew;
             if (znew >= 1 && znew <= nz) {
-                R w[8];
-                const int c = weight3<R>(nx[0], nx[1], nx[2], a, a.nsect > 1, w);
-                deposit3<R>(a, znew, c, w, s_jr, s_kr);
+                dc = weight3<R>(nx[0], nx[1], nx[2], a, a.nsect > 1, dw);
+                dz = znew;
+                don = true;
             }
         }
     }
+    deposit3_block<R>(a, don, dz, dc, dw, tile, s_b, s_jr, s_kr);
     // block sums of the off-grid change and the pushes
     for (int o = 32; o > 0; o >>= 1) {
         dn += __shfl_xor(dn, o, 64);
@@ -349,6 +450,36 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_gather_kernel(P3GatherArgs<R> b) 
         }
         for (int d = 0; d < 3; d++) b.acc[3 * i + d] = s[d];
     }
+}
+
+// Sort keys: the particle's cell shifted right by kshift (kmax for the
+// off-grid list, which sorts last), with the identity permutation.
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_key_kernel(P3Args<R> a, int kshift,
+                                                          unsigned kmax, unsigned *key,
+                                                          unsigned *perm) {
+    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const int z = a.zone[i];
+    unsigned k = kmax;
+    if (z >= 1 && z <= a.nz) {
+        const typename Vec4<R>::T p = a.xv[i];
+        R w[8];
+        k = (unsigned)(weight3<R>(p.x, p.y, p.z, a, false, w) >> kshift);
+    }
+    key[i] = k;
+    perm[i] = (unsigned)i;
+}
+
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_permute_kernel(P3PermArgs<R> p) {
+    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    const unsigned j = p.perm[i];
+    p.xv_out[i] = p.xv_in[j];
+    p.vw_out[i] = p.vw_in[j];
+    p.zout[i] = p.zin[j];
+    p.iout[i] = p.iin[j];
 }
 
 // ---- field solve ------------------------------------------------------------------
@@ -745,6 +876,21 @@ template <typename R> hipError_t launch_p3_gather(const P3GatherArgs<R> &b, hipS
     return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_p3_key(const P3Args<R> &a, int kshift, unsigned kmax, unsigned *key,
+                         unsigned *perm, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_key_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a,
+                       kshift, kmax, key, perm);
+    return hipGetLastError();
+}
+
+template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &p, hipStream_t s) {
+    if (p.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_permute_kernel<R>, dim3(p3_blocks(p.n)), dim3(P3_BLOCK), 0, s, p);
+    return hipGetLastError();
+}
+
 // Limits the launches below rely on (checked again by the context set-up).
 template <typename R> bool p3_solve_fits(const P3SolveArgs<R> &a) {
     const size_t P = 2 * (size_t)a.ngz;
@@ -789,7 +935,10 @@ hipError_t launch_p3_green(double *G, const double *twd, const double *twzd, con
     template hipError_t launch_p3_tmcent<R>(const P3Args<R> &, hipStream_t);          \
     template hipError_t launch_p3_gather<R>(const P3GatherArgs<R> &, hipStream_t);    \
     template hipError_t launch_p3_solve<R>(const P3SolveArgs<R> &, hipStream_t);      \
-    template bool p3_solve_fits<R>(const P3SolveArgs<R> &);
+    template bool p3_solve_fits<R>(const P3SolveArgs<R> &);                            \
+    template hipError_t launch_p3_key<R>(const P3Args<R> &, int, unsigned, unsigned *,  \
+                                         unsigned *, hipStream_t);                     \
+    template hipError_t launch_p3_permute<R>(const P3PermArgs<R> &, hipStream_t);
 P3_INST(float)
 P3_INST(double)
 

@@ -7,6 +7,8 @@
 #include "galhip_ctx.h"
 #include "galhip_p3d.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -240,14 +242,38 @@ int p3_set_green(galhip_ctx *c, const double *G) {
 
 // ---- particles ----------------------------------------------------------------------
 
+template <typename R> static P3Args<R> p3_args(galhip_ctx *c);
+
+static void p3_free_particles(galhip_ctx *c) {
+    dfree(c->xv); dfree(c->vw); dfree(c->zone); dfree(c->id);
+    dfree(c->xv2); dfree(c->vw2); dfree(c->zone2); dfree(c->id2);
+    dfree(c->key); dfree(c->key2); dfree(c->perm); dfree(c->perm2); dfree(c->sort_tmp);
+    c->sort_tmp_bytes = 0;
+    c->n = c->cap = 0;
+}
+
+// Particle storage plus the sort's second copies, keys and scratch.
 static int p3_ensure(galhip_ctx *c, long long n) {
     if (n <= c->cap) { c->n = n; return 0; }
-    dfree(c->xv); dfree(c->vw); dfree(c->zone); dfree(c->id);
-    c->n = c->cap = 0;
-    const size_t es = esize(c);
-    int rc = dalloc(&c->xv, 4 * es * (n > 0 ? n : 1)) | dalloc(&c->vw, 2 * es * (n > 0 ? n : 1)) |
-             dalloc(&c->zone, n > 0 ? n : 1) | dalloc(&c->id, sizeof(unsigned) * (n > 0 ? n : 1));
-    if (rc) return rc;
+    p3_free_particles(c);
+    const size_t es = esize(c), m = n > 0 ? n : 1;
+    int rc = dalloc(&c->xv, 4 * es * m) | dalloc(&c->vw, 2 * es * m) | dalloc(&c->zone, m) |
+             dalloc(&c->id, sizeof(unsigned) * m) | dalloc(&c->xv2, 4 * es * m) |
+             dalloc(&c->vw2, 2 * es * m) | dalloc(&c->zone2, m) |
+             dalloc(&c->id2, sizeof(unsigned) * m) | dalloc(&c->key, sizeof(unsigned) * m) |
+             dalloc(&c->key2, sizeof(unsigned) * m) | dalloc(&c->perm, sizeof(unsigned) * m) |
+             dalloc(&c->perm2, sizeof(unsigned) * m);
+    if (rc) { p3_free_particles(c); return rc; }
+    size_t tmp = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->key, c->key2, c->perm,
+                                                      c->perm2, (int)m, 0, 32, c->s);
+    if (e != hipSuccess) {
+        p3_free_particles(c);
+        return gh_fail(GALHIP_EHIP, "radix sort sizing: %s", hipGetErrorString(e));
+    }
+    rc = dalloc(&c->sort_tmp, tmp);
+    if (rc) { p3_free_particles(c); return rc; }
+    c->sort_tmp_bytes = tmp;
     c->n = c->cap = n;
     return 0;
 }
@@ -291,6 +317,9 @@ extern "C" int galhip_restrt3(galhip_ctx *c, int64_t n, const void *x, const voi
         HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
     }
     c->qbits = qbits_for(n);
+    int rc2 = p3_sort(c);
+    if (rc2) return rc2;
+    HIPCHK(hipStreamSynchronize(c->s));
     return 0;
 }
 
@@ -325,6 +354,45 @@ extern "C" int galhip_unload3(galhip_ctx *c, void *x, void *y, void *z, void *vx
         for (long long i = 0; i < n; i++) iz[ids[i]] = zz[i];
     }
     return 0;
+}
+
+// Regroup the particles by cell (the storage order sets the locality of the
+// field gathers and lets the deposit combine the particles of a cell within
+// a wave; no result depends on it): the key is the cell itself, off-grid
+// particles last.
+template <typename R> static int p3_sort_impl(galhip_ctx *c) {
+    const long long n = c->n;
+    if (n <= 1) return 0;
+    const int kshift = 0;
+    const unsigned kmax = (unsigned)c->mesh;
+    HIPCHK(launch_p3_key<R>(p3_args<R>(c), kshift, kmax, c->key, c->perm, c->s));
+    int end_bit = 1;
+    while ((1LL << end_bit) <= (long long)kmax) end_bit++;
+    size_t tmp = c->sort_tmp_bytes;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->key, c->key2, c->perm,
+                                              c->perm2, (int)n, 0, end_bit, c->s));
+    P3PermArgs<R> p;
+    p.xv_in = (const typename Vec4<R>::T *)c->xv;
+    p.xv_out = (typename Vec4<R>::T *)c->xv2;
+    p.vw_in = (const typename Vec2<R>::T *)c->vw;
+    p.vw_out = (typename Vec2<R>::T *)c->vw2;
+    p.zin = c->zone;
+    p.zout = c->zone2;
+    p.iin = c->id;
+    p.iout = c->id2;
+    p.perm = c->perm2;
+    p.n = n;
+    HIPCHK(launch_p3_permute<R>(p, c->s));
+    std::swap(c->xv, c->xv2);
+    std::swap(c->vw, c->vw2);
+    std::swap(c->zone, c->zone2);
+    std::swap(c->id, c->id2);
+    c->steps_since_sort = 0;
+    return 0;
+}
+
+int p3_sort(galhip_ctx *c) {
+    return c->prec == GALHIP_FP32 ? p3_sort_impl<float>(c) : p3_sort_impl<double>(c);
 }
 
 int p3_unload(galhip_ctx *, void *, void *, void *, void *, int32_t *) {

@@ -223,11 +223,11 @@ int galhip_sync(galhip_ctx *ctx);
  * get_step, masset, findf and p2fndf (-> p3fndf, SRC/lib15/p3fndf.f:1, and
  * polcat over every plane), step, tmcent, run, get_mass / set_mass /
  * get_rho (mesh = nr ngz na), get_field / set_field (fld[4][mesh]: F_x,
- * F_y, F_z, potential), get_ranges, get_noff, get_order, the profiling
- * calls and sync.  Particles go in and out through galhip_restrt3 and
- * galhip_unload3; galhip_restrt, galhip_unload, galhip_weight,
- * galhip_grdacc, galhip_sort and galhip_integrals return GALHIP_EINVAL on a
- * p3d context.  Zone boundaries are spherical radii (SRC/lib15/setrun.f:810,
+ * F_y, F_z, potential), get_ranges, get_noff, set_sort_interval, sort,
+ * get_order, the profiling calls and sync.  Particles go in and out through
+ * galhip_restrt3 and galhip_unload3; galhip_restrt, galhip_unload,
+ * galhip_weight, galhip_grdacc and galhip_integrals return GALHIP_EINVAL on
+ * a p3d context.  Zone boundaries are spherical radii (SRC/lib15/setrun.f:810,
  * zoneno.f:40-41); a particle is off the grid when |z| > zm or R > rgrid
  * (SRC/lib15/offgrd.f:69-72), zm = dzg (ngz - 1) / 2. */
 typedef struct {
