@@ -21,6 +21,11 @@ read and written in FP32) over its summed HIP-event time on the library's
 stream, against 8 TB/s; traffic = measured HBM bytes per launch (per-push
 bytes from rocprofv3 counters times the pushes per launch).  cpu_baseline: the oracle's FP32 restatement on one
 host core over a bounded sample of the same workload.
+
+--workload C3 measures the 3-D polar grid instead (200 x 256 x 125 grid,
+N = 5e7, m <= 8, sech^2 vertical profile, 200 steps by default) and prints
+its own line: push-kernel roofline at 48 B per push ((x, y, z, vx) and
+(vy, vz) read and written), the p3d oracle as the CPU baseline.
 """
 import argparse
 import json
@@ -135,6 +140,106 @@ def cpu_baseline(w, n, steps):
                        % (w.name, n, steps, pushes, dt))
 
 
+def run_gpu3(w, precision, n, steps, warmup, sort_interval, want_green=False):
+    """The 3-D grid's step (findf -> p3fndf, step): wall-clock rate over
+    `steps` steps, then a profiled pass of as many steps."""
+    from galaxy_amd import configs
+    t0 = time.time()
+    g = configs.make_grid3(w, precision=precision, n=n)
+    g.set_sort_interval(sort_interval)
+    log("[bench] p3d %s set-up %.1f s (N=%d)" % (precision, time.time() - t0, n))
+    if warmup:
+        g.run(warmup)
+    g.sync()
+    g.set_profile(False)
+    t0 = time.perf_counter()
+    g.run(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    _, _, pushes = g.get_profile()
+    g.set_profile(True)
+    g.run(steps)
+    g.sync()
+    push_ms, launches, pushes_p = g.get_profile()
+    g.set_profile(False)
+    out = g.unload()
+    res = dict(seconds=t1 - t0, pushes=pushes, push_ms=push_ms, launches=launches,
+               pushes_prof=pushes_p, a2=a2(out["x"], out["y"], w.lscale()), noff=g.noff)
+    if want_green:
+        res["green"] = g.get_green()
+    log("[bench] p3d %s: %d steps in %.4f s, %d pushes; profiled pass: push kernel "
+        "%.3f ms over %d launches" % (precision, steps, t1 - t0, pushes, push_ms, launches))
+    g.close()
+    return res
+
+
+def cpu_baseline3(w, n, steps, G):
+    """The p3d oracle (FP32 particles, FP64 direct-sum solve, one core) on n
+    particles of workload w, with the GPU's Green's tables."""
+    from galaxy_amd import configs
+    from oracle import oracle3
+    kw = configs.grid_kwargs3(w, n)
+    m = oracle3.P3D(kw["nr"], kw["na"], kw["ngz"], kw["dzg"], kw["hash_"], kw["softl"],
+                    tsoft=1, nzones=kw["nzones"], nstep=kw["nstep"], zbr=kw["zbr"],
+                    pmass=kw["pmass"], offanal=True, prec="f", green_tables=G)
+    m.fixforce = configs.fixforce(w, math.hypot(float(m.rgrid), float(m.zm)))
+    m.load(*configs.particles3(w, n, None, float(m.rgrid)))
+    m.masset()
+    pushes = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.findf()
+        m.step()
+        pushes += int(m.moved.sum())
+    dt = time.perf_counter() - t0
+    return dict(value=pushes / dt, unit="particle-steps/s", cores=1, kind="port",
+                sample="oracle p3d (FP32 particles, FP64 direct-sum solve), %s geometry, "
+                       "N=%d, %d steps (%d pushes, %.1f s)" % (w.name, n, steps, pushes, dt))
+
+
+def main_p3d(args):
+    """--workload C3: the 3-D polar grid, 200 x 256 x 125, N = 5e7."""
+    from galaxy_amd import configs
+    w = configs.C3
+    n = args.n or w.n
+    mixed = run_gpu3(w, "fp32", n, args.steps, args.warmup, args.sort_interval,
+                     want_green=not args.no_cpu)
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_gpu3(w, "fp64", n, args.steps, args.warmup, args.sort_interval)
+    value = mixed["pushes"] / mixed["seconds"]
+    bytes_per_push = 48.0   # (x, y, z, vx) + (vy, vz) read and written, FP32
+    achieved = mixed["pushes_prof"] * bytes_per_push / (mixed["push_ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="p3_push_kernel<float>", bytes_per_push=bytes_per_push,
+                avg_launch_us=round(1e3 * mixed["push_ms"] / max(mixed["launches"], 1), 2))
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline3(w, args.cpu_n3, args.cpu_steps3, mixed.pop("green"))
+        log("[bench] p3d cpu baseline: %.3g pushes/s (%s)" % (cpu["value"], cpu["sample"]))
+    extra = dict(pushes_per_step=mixed["pushes"] / args.steps, sort_interval=args.sort_interval,
+                 push_share=round(mixed["push_ms"] * 1e-3 / mixed["seconds"], 3), noff=mixed["noff"])
+    if fp64:
+        v64 = fp64["pushes"] / fp64["seconds"]
+        extra.update(fp64_value=v64, mixed_over_fp64=value / v64,
+                     fp64_ms_per_step=1e3 * fp64["seconds"] / args.steps,
+                     a2_mixed=abs(mixed["a2"]), a2_fp64=abs(fp64["a2"]),
+                     m2_err=abs(mixed["a2"] - fp64["a2"]))
+    line = {
+        "metric": "particle-steps/s (3-D polar grid)", "value": value,
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["seconds"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded exponential disc, sech^2 vertical profile)",
+        "config": {"workload": w.name, "grid": [w.nr, w.na, w.ngz], "n": n,
+                   "harmonics": "m<=%d" % w.hash, "zones": list(w.nstep), "dt": w.dt,
+                   "parallelism": "none"},
+        "roofline": roof, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes):
     """A plain elementwise copy of known size (FP32 x 1.0 into a second
     buffer: nbytes read, nbytes written), run three times; its FETCH_SIZE and
@@ -163,6 +268,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain copy kernel of BYTES (counter calibration)")
+    ap.add_argument("--workload", default="C2", choices=("C2", "C3"),
+                    help="C2: 2-D grid, the headline (default); C3: 3-D grid")
+    ap.add_argument("--cpu-n3", type=int, default=4_000_000)
+    ap.add_argument("--cpu-steps3", type=int, default=2)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -172,6 +281,11 @@ def main():
               "multi-GPU form is a design, not measured); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "C3":
+        if args.steps == 500:
+            args.steps = 200
+        main_p3d(args)
+        return
     from galaxy_amd import configs
     w = configs.C2
     n = args.n or w.n

@@ -8,6 +8,7 @@ namespace gh {
 template <typename R> struct P3Args {
     typename Vec4<R>::T *xv;          // (x, y, z, vx)
     typename Vec2<R>::T *vw;          // (vy, vz)
+    typename Vec2<R>::T *ut;          // (u, t) of the current position
     uint8_t *zone;
     long long n;
     const typename Vec4<R>::T *fld;   // (F_x, F_y, F_z, potential) per cell
@@ -60,6 +61,8 @@ template <typename R> struct P3PermArgs {
     typename Vec4<R>::T *xv_out;
     const typename Vec2<R>::T *vw_in;
     typename Vec2<R>::T *vw_out;
+    const typename Vec2<R>::T *ut_in;
+    typename Vec2<R>::T *ut_out;
     const uint8_t *zin;
     uint8_t *zout;
     const unsigned *iin;
@@ -75,6 +78,7 @@ template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &, hipStr
 template <typename R> hipError_t launch_p3_massgn(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_push(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &, hipStream_t);
+template <typename R> hipError_t launch_p3_ut(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_gather(const P3GatherArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_solve(const P3SolveArgs<R> &, hipStream_t);
 template <typename R> bool p3_solve_fits(const P3SolveArgs<R> &);

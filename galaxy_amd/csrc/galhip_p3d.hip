@@ -46,17 +46,21 @@ template <typename R> __device__ __forceinline__ R p3_wave_sum(R v) {
     return v;
 }
 
-// weight() for one position, p3d jmass = 2 (weight.f:195-250): returns the
-// 0-based cell (ju ngz + jz) na + jt and the 8 weights in the reference's
-// order (bit 0 = dt, bit 1 = dz, bit 2 = du).
+// (u, t) of a position as weight() forms them (weight.f:210-218): the FP32
+// logarithm and angle of the p2d grid, t = 0 on the axis.
 template <typename R>
-__device__ __forceinline__ int weight3(R x, R y, R z, const P3Args<R> &a,
-                                       bool fold, R w[8]) {
-    R zz = z + a.zm;
-    R rr = sqrt(x * x + y * y);
-    R u, t;
+__device__ __forceinline__ void ut3(R x, R y, const P3Args<R> &a, R &u, R &t) {
+    const R rr = sqrt(x * x + y * y);
     ut_of(x, y, rr, a.k, u, t);
     if (!(rr > (R)0)) t = (R)0;
+}
+
+// weight() for one position given its (u, t), p3d jmass = 2
+// (weight.f:219-248): returns the 0-based cell (ju ngz + jz) na + jt and the
+// 8 weights in the reference's order (bit 0 = dt, bit 1 = dz, bit 2 = du).
+template <typename R>
+__device__ __forceinline__ int cell3(R u, R t, R z, const P3Args<R> &a, bool fold, R w[8]) {
+    R zz = z + a.zm;
     if (fold) t = (R)fmod(t, a.thmax);
     int ju = (int)u;
     if (ju > a.nr - 2) ju = a.nr - 2;
@@ -76,6 +80,13 @@ __device__ __forceinline__ int weight3(R x, R y, R z, const P3Args<R> &a,
     w[6] = (one - dt) * dz * du;
     w[7] = dt * dz * du;
     return (ju * a.ngz + jz) * a.na + jt;
+}
+
+template <typename R>
+__device__ __forceinline__ int weight3(R x, R y, R z, const P3Args<R> &a, bool fold, R w[8]) {
+    R u, t;
+    ut3(x, y, a, u, t);
+    return cell3(u, t, z, a, fold, w);
 }
 
 // grdacc's corners (grdacc.f:193-203): c, c+1 (wrapped when c+1 is a
@@ -122,14 +133,14 @@ __device__ __forceinline__ void chstep1(R tfac, R ox, R &ov, R &acc, R &nx, R &n
 // interpolated at its position (component sums in corner order), the fixed
 // force of the spherical radius, the zone rescaling.
 template <typename R>
-__device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz,
+__device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz, R u, R t,
                                         int z0, R acc[3]) {
     typedef typename Vec4<R>::T R4;
     acc[0] = acc[1] = acc[2] = (R)0;
     if (z0 != a.nz + 1) {
         R w[8];
         int ic[8];
-        const int c = weight3<R>(ox, oy, oz, a, false, w);
+        const int c = cell3<R>(u, t, oz, a, false, w);
         corners3(c, a.na, a.ngz, ic);
         R4 f[8];
 #pragma unroll
@@ -240,9 +251,18 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
         }
     }
     __syncthreads();
-    const int r0 = s_b[1], p0 = s_b[3], q0 = s_b[5];
-    const int NR = s_b[2] - r0 + 2, NP = s_b[4] - p0 + 2, NQ = s_b[6] - q0 + 2;
-    const bool use = s_b[2] >= 0 && NR * NP * NQ <= P3_TILE;
+    // the window: the box of the candidates' corners, clipped about its
+    // centre to at most P3_TILE cells (8 rings, 8 planes, the rest spokes);
+    // candidates whose corners fall outside go to the zone array directly
+    int r0 = s_b[1], p0 = s_b[3], q0 = s_b[5];
+    int NR = s_b[2] - r0 + 2, NP = s_b[4] - p0 + 2, NQ = s_b[6] - q0 + 2;
+    const bool use = s_b[2] >= 0;
+    if (use && NR * NP * NQ > P3_TILE) {
+        if (NR > 8) { r0 += (NR - 8) / 2; NR = 8; }
+        if (NP > 8) { p0 += (NP - 8) / 2; NP = 8; }
+        const int mq = P3_TILE / (NR * NP);
+        if (NQ > mq) { q0 += (NQ - mq) / 2; NQ = mq; }
+    }
     const int ncell = use ? NR * NP * NQ : 0;
     for (int e = tid; e < ncell; e += P3_BLOCK) tile[e] = 0;
     __syncthreads();
@@ -250,7 +270,10 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
         long long v[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) v[q] = quant<R>(w[q], a.qbits);
-        if (use && cand) {
+        const bool inside = use && cand && ring >= r0 && ring - r0 + 1 < NR &&
+                            plane >= p0 && plane - p0 + 1 < NP && spoke >= q0 &&
+                            spoke - q0 + 1 < NQ;
+        if (inside) {
             const int b = ((ring - r0) * NP + (plane - p0)) * NQ + (spoke - q0);
 #pragma unroll
             for (int q = 0; q < 8; q++) {
@@ -266,12 +289,16 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
         }
     }
     __syncthreads();
-    for (int e = tid; e < ncell; e += P3_BLOCK) {
-        const long long v = tile[e];
-        if (v != 0) {
-            const int dq = e % NQ, t2 = e / NQ, dp = t2 % NP, dr = t2 / NP;
-            const int cell = ((r0 + dr) * a.ngz + (p0 + dp)) * a.na + (q0 + dq);
-            atomicAdd((unsigned long long *)&a.mfx[tz - 1][cell], (unsigned long long)v);
+    // flush: one wave per (ring, plane) row of the window
+    const int wave = tid >> 6;
+    for (int row = wave; row < (use ? NR * NP : 0); row += P3_BLOCK / 64) {
+        const int dr = row / NP, dp = row - dr * NP;
+        const int base = ((r0 + dr) * a.ngz + (p0 + dp)) * a.na + q0;
+        const long long *t = tile + row * NQ;
+        for (int dq = lane; dq < NQ; dq += 64) {
+            const long long v = t[dq];
+            if (v != 0)
+                atomicAdd((unsigned long long *)&a.mfx[tz - 1][base + dq], (unsigned long long)v);
         }
     }
 }
@@ -303,9 +330,12 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_massgn_kernel(P3Args<R> a) {
     R w[8];
     if (i < a.n) {
         z = a.zone[i];
+        const typename Vec4<R>::T p = a.xv[i];
+        R u, t;
+        ut3(p.x, p.y, a, u, t);
+        a.ut[i] = mk2(u, t);
         if (z >= 1 && z <= a.nz) {
-            const typename Vec4<R>::T p = a.xv[i];
-            c = weight3<R>(p.x, p.y, p.z, a, a.nsect > 1, w);
+            c = cell3<R>(u, t, p.z, a, a.nsect > 1, w);
             on = true;
         }
     }
@@ -345,10 +375,11 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
             np = 1;
             const R4 p = a.xv[i];
             const R2 q = a.vw[i];
+            const R2 ut0 = a.ut[i];
             R o[3] = {p.x, p.y, p.z};
             R ov[3] = {p.w, q.x, q.y};
             R acc[3], nv[3], nx[3];
-            getacc3<R>(a, o[0], o[1], o[2], z0, acc);
+            getacc3<R>(a, o[0], o[1], o[2], ut0.x, ut0.y, z0, acc);
             // stpgrp
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -385,11 +416,14 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
                     znew = kz;
                 }
             }
+            R un, tn;
+            ut3(nx[0], nx[1], a, un, tn);
             a.xv[i] = mk4(nx[0], nx[1], nx[2], nv[0]);
             a.vw[i] = mk2(nv[1], nv[2]);
+            a.ut[i] = mk2(un, tn);
             a.zone[i] = (uint8_t)znew;
             if (znew >= 1 && znew <= nz) {
-                dc = weight3<R>(nx[0], nx[1], nx[2], a, a.nsect > 1, dw);
+                dc = cell3<R>(un, tn, nx[2], a, a.nsect > 1, dw);
                 dz = znew;
                 don = true;
             }
@@ -413,14 +447,26 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
     }
 }
 
+// (u, t) of every particle's position (after a load)
+template <typename R>
+__global__ __launch_bounds__(P3_BLOCK) void p3_ut_kernel(P3Args<R> a) {
+    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const typename Vec4<R>::T p = a.xv[i];
+    R u, t;
+    ut3(p.x, p.y, a, u, t);
+    a.ut[i] = mk2(u, t);
+}
+
 template <typename R>
 __global__ __launch_bounds__(P3_BLOCK) void p3_tmcent_kernel(P3Args<R> a) {
     const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
     if (i >= a.n) return;
     typename Vec4<R>::T p = a.xv[i];
     typename Vec2<R>::T q = a.vw[i];
+    const typename Vec2<R>::T ut = a.ut[i];
     R acc[3];
-    getacc3<R>(a, p.x, p.y, p.z, a.zone[i], acc);
+    getacc3<R>(a, p.x, p.y, p.z, ut.x, ut.y, a.zone[i], acc);
     p.w = p.w - a.fac * acc[0];
     q.x = q.x - a.fac * acc[1];
     q.y = q.y - a.fac * acc[2];
@@ -464,8 +510,9 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_key_kernel(P3Args<R> a, int kshif
     unsigned k = kmax;
     if (z >= 1 && z <= a.nz) {
         const typename Vec4<R>::T p = a.xv[i];
+        const typename Vec2<R>::T ut = a.ut[i];
         R w[8];
-        k = (unsigned)(weight3<R>(p.x, p.y, p.z, a, false, w) >> kshift);
+        k = (unsigned)(cell3<R>(ut.x, ut.y, p.z, a, false, w) >> kshift);
     }
     key[i] = k;
     perm[i] = (unsigned)i;
@@ -478,6 +525,7 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_permute_kernel(P3PermArgs<R> p) {
     const unsigned j = p.perm[i];
     p.xv_out[i] = p.xv_in[j];
     p.vw_out[i] = p.vw_in[j];
+    p.ut_out[i] = p.ut_in[j];
     p.zout[i] = p.zin[j];
     p.iout[i] = p.iin[j];
 }
@@ -864,6 +912,12 @@ template <typename R> hipError_t launch_p3_push(const P3Args<R> &a, hipStream_t 
     return hipGetLastError();
 }
 
+template <typename R> hipError_t launch_p3_ut(const P3Args<R> &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_ut_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
 template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(p3_tmcent_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
@@ -933,6 +987,7 @@ hipError_t launch_p3_green(double *G, const double *twd, const double *twzd, con
     template hipError_t launch_p3_massgn<R>(const P3Args<R> &, hipStream_t);          \
     template hipError_t launch_p3_push<R>(const P3Args<R> &, hipStream_t);            \
     template hipError_t launch_p3_tmcent<R>(const P3Args<R> &, hipStream_t);          \
+    template hipError_t launch_p3_ut<R>(const P3Args<R> &, hipStream_t);              \
     template hipError_t launch_p3_gather<R>(const P3GatherArgs<R> &, hipStream_t);    \
     template hipError_t launch_p3_solve<R>(const P3SolveArgs<R> &, hipStream_t);      \
     template bool p3_solve_fits<R>(const P3SolveArgs<R> &);                            \

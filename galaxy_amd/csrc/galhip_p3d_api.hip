@@ -245,8 +245,8 @@ int p3_set_green(galhip_ctx *c, const double *G) {
 template <typename R> static P3Args<R> p3_args(galhip_ctx *c);
 
 static void p3_free_particles(galhip_ctx *c) {
-    dfree(c->xv); dfree(c->vw); dfree(c->zone); dfree(c->id);
-    dfree(c->xv2); dfree(c->vw2); dfree(c->zone2); dfree(c->id2);
+    dfree(c->xv); dfree(c->vw); dfree(c->ut); dfree(c->zone); dfree(c->id);
+    dfree(c->xv2); dfree(c->vw2); dfree(c->ut2); dfree(c->zone2); dfree(c->id2);
     dfree(c->key); dfree(c->key2); dfree(c->perm); dfree(c->perm2); dfree(c->sort_tmp);
     c->sort_tmp_bytes = 0;
     c->n = c->cap = 0;
@@ -258,6 +258,7 @@ static int p3_ensure(galhip_ctx *c, long long n) {
     p3_free_particles(c);
     const size_t es = esize(c), m = n > 0 ? n : 1;
     int rc = dalloc(&c->xv, 4 * es * m) | dalloc(&c->vw, 2 * es * m) | dalloc(&c->zone, m) |
+             dalloc(&c->ut, 2 * es * m) | dalloc(&c->ut2, 2 * es * m) |
              dalloc(&c->id, sizeof(unsigned) * m) | dalloc(&c->xv2, 4 * es * m) |
              dalloc(&c->vw2, 2 * es * m) | dalloc(&c->zone2, m) |
              dalloc(&c->id2, sizeof(unsigned) * m) | dalloc(&c->key, sizeof(unsigned) * m) |
@@ -317,6 +318,8 @@ extern "C" int galhip_restrt3(galhip_ctx *c, int64_t n, const void *x, const voi
         HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
     }
     c->qbits = qbits_for(n);
+    if (c->prec == GALHIP_FP32) HIPCHK(launch_p3_ut<float>(p3_args<float>(c), c->s));
+    else HIPCHK(launch_p3_ut<double>(p3_args<double>(c), c->s));
     int rc2 = p3_sort(c);
     if (rc2) return rc2;
     HIPCHK(hipStreamSynchronize(c->s));
@@ -376,6 +379,8 @@ template <typename R> static int p3_sort_impl(galhip_ctx *c) {
     p.xv_out = (typename Vec4<R>::T *)c->xv2;
     p.vw_in = (const typename Vec2<R>::T *)c->vw;
     p.vw_out = (typename Vec2<R>::T *)c->vw2;
+    p.ut_in = (const typename Vec2<R>::T *)c->ut;
+    p.ut_out = (typename Vec2<R>::T *)c->ut2;
     p.zin = c->zone;
     p.zout = c->zone2;
     p.iin = c->id;
@@ -385,6 +390,7 @@ template <typename R> static int p3_sort_impl(galhip_ctx *c) {
     HIPCHK(launch_p3_permute<R>(p, c->s));
     std::swap(c->xv, c->xv2);
     std::swap(c->vw, c->vw2);
+    std::swap(c->ut, c->ut2);
     std::swap(c->zone, c->zone2);
     std::swap(c->id, c->id2);
     c->steps_since_sort = 0;
@@ -406,6 +412,7 @@ template <typename R> static P3Args<R> p3_args(galhip_ctx *c) {
     memset(&a, 0, sizeof a);
     a.xv = (typename Vec4<R>::T *)c->xv;
     a.vw = (typename Vec2<R>::T *)c->vw;
+    a.ut = (typename Vec2<R>::T *)c->ut;
     a.zone = c->zone;
     a.n = c->n;
     a.fld = (const typename Vec4<R>::T *)c->fld4;

@@ -231,6 +231,11 @@ void orc3_fndf(int nr, int na, int ngz, int ng, const int32_t *lg,
     }
     /* p3manl: azimuthal half-complex components, then the doubled vertical
      * transform of each component column: M[ring][slot][P] */
+    /* cos, sin of 2 pi k / na and of 2 pi k / P, k reduced exactly */
+    double *ca = malloc(sizeof(double) * na), *sa = malloc(sizeof(double) * na);
+    double *cz = malloc(sizeof(double) * P), *sz = malloc(sizeof(double) * P);
+    for (int k = 0; k < na; k++) { ca[k] = cos2pi(k, na); sa[k] = sin2pi(k, na); }
+    for (int k = 0; k < P; k++) { cz[k] = cos2pi(k, P); sz[k] = sin2pi(k, P); }
     double *M = calloc((size_t)nr * nm * P, sizeof(double));
     double *col = malloc(sizeof(double) * ngz);
     for (int ir = jrs; ir <= krs; ir++) {
@@ -240,8 +245,8 @@ void orc3_fndf(int nr, int na, int ngz, int ng, const int32_t *lg,
                 const double *row = rho + ((size_t)(ir - 1) * ngz + p) * na;
                 double acc = 0.0;
                 for (int j = 0; j < na; j++) {
-                    long k = (long)m * j;
-                    acc += row[j] * (part ? -sin2pi(k, na) : cos2pi(k, na));
+                    int k = (int)(((long)m * j) % na);
+                    acc += row[j] * (part ? -sa[k] : ca[k]);
                 }
                 col[p] = acc;
             }
@@ -249,8 +254,9 @@ void orc3_fndf(int nr, int na, int ngz, int ng, const int32_t *lg,
             for (int k = 0; k <= ngz; k++) {
                 double a = 0.0, b = 0.0;
                 for (int p = 0; p < ngz; p++) {
-                    a += col[p] * cos2pi((long)k * p, P);
-                    b += col[p] * sin2pi((long)k * p, P);
+                    int kk = (int)(((long)k * p) % P);
+                    a += col[p] * cz[kk];
+                    b += col[p] * sz[kk];
                 }
                 if (k == 0) H[0] = a;
                 else if (k == ngz) H[P - 1] = a;
@@ -318,9 +324,10 @@ void orc3_fndf(int nr, int na, int ngz, int ng, const int32_t *lg,
                 const double *H = W + ((size_t)(ir - 1) * nm + s) * P;
                 for (int q = 0; q < ngz; q++) {
                     double v = H[0] + (q % 2 ? -H[P - 1] : H[P - 1]);
-                    for (int k = 1; k < ngz; k++)
-                        v += 2.0 * (H[2 * k - 1] * cos2pi((long)k * q, P) -
-                                    H[2 * k] * sin2pi((long)k * q, P));
+                    for (int k = 1; k < ngz; k++) {
+                        int kk = (int)(((long)k * q) % P);
+                        v += 2.0 * (H[2 * k - 1] * cz[kk] - H[2 * k] * sz[kk]);
+                    }
                     zc[(size_t)s * ngz + q] = v;
                 }
             }
@@ -333,8 +340,8 @@ void orc3_fndf(int nr, int na, int ngz, int ng, const int32_t *lg,
                         const double c = zc[(size_t)s * ngz + q];
                         if (m == 0) v += c;
                         else if (m == mmax - 1) v += (j % 2 ? -c : c);
-                        else if (part == 0) v += 2.0 * c * cos2pi((long)m * j, na);
-                        else v -= 2.0 * c * sin2pi((long)m * j, na);
+                        else if (part == 0) v += 2.0 * c * ca[((long)m * j) % na];
+                        else v -= 2.0 * c * sa[((long)m * j) % na];
                     }
                     out[j] = v * scale;
                 }
@@ -342,6 +349,7 @@ void orc3_fndf(int nr, int na, int ngz, int ng, const int32_t *lg,
         }
     }
     free(slot); free(sm); free(M); free(col); free(W); free(zc);
+    free(ca); free(sa); free(cz); free(sz);
 }
 
 void orc3_direct_field(int nr, int na, int ngz, int type, const double *D,
