@@ -54,6 +54,11 @@ PUSH_HBM_WRITE_B = 25.3
 # VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz (one wave64 instruction per
 # CU per cycle, non-packed).
 PUSH_VALU_PER_PUSH = 521.0
+# HBM bytes per particle of p3_push_kernel<float> on the 3-D workload (mean
+# of the dispatches in profiles/p3_push_kernel_p3d-200x256x125-n5e7_traffic.txt,
+# scaled by the same copy calibration)
+P3_PUSH_HBM_READ_B = 35.9
+P3_PUSH_HBM_WRITE_B = 44.8
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 
@@ -210,8 +215,10 @@ def main_p3d(args):
     value = mixed["pushes"] / mixed["seconds"]
     bytes_per_push = 48.0   # (x, y, z, vx) + (vy, vz) read and written, FP32
     achieved = mixed["pushes_prof"] * bytes_per_push / (mixed["push_ms"] * 1e-3) / 1e9
+    per_launch = mixed["pushes_prof"] / max(mixed["launches"], 1)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=round((P3_PUSH_HBM_READ_B + P3_PUSH_HBM_WRITE_B) * per_launch),
                 kernel="p3_push_kernel<float>", bytes_per_push=bytes_per_push,
                 avg_launch_us=round(1e3 * mixed["push_ms"] / max(mixed["launches"], 1), 2))
     cpu = None
@@ -261,7 +268,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=None, help="particles (default 10M)")
-    ap.add_argument("--sort-interval", type=int, default=10)
+    ap.add_argument("--sort-interval", type=int, default=None,
+                    help="steps between re-sorts (default 10 for C2, 5 for C3)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--cpu-n", type=int, default=2_000_000)
     ap.add_argument("--cpu-steps", type=int, default=60)
@@ -284,8 +292,12 @@ def main():
     if args.workload == "C3":
         if args.steps == 500:
             args.steps = 200
+        if args.sort_interval is None:
+            args.sort_interval = 5
         main_p3d(args)
         return
+    if args.sort_interval is None:
+        args.sort_interval = 10
     from galaxy_amd import configs
     w = configs.C2
     n = args.n or w.n

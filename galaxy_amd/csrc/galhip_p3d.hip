@@ -35,7 +35,8 @@
 
 namespace gh {
 
-constexpr int P3_BLOCK = 256;
+constexpr int P3_BLOCK = 256;    // solve kernels
+constexpr int P3_PBLOCK = 512;   // particle kernels: a larger block shares one deposit window
 constexpr int P3_MAX_NA = 2048;
 constexpr int P3_MAX_NM = 128;   // compact components (2 ng)
 constexpr int P3_MAX_NR = 1024;
@@ -193,7 +194,7 @@ __device__ __forceinline__ int p3_wave_max(int v) {
     return v;
 }
 
-constexpr int P3_TILE = 2048;   // cells of a block's LDS deposit window
+constexpr int P3_TILE = 4096;   // cells of a block's LDS deposit window
 
 // Block-level fixed-point deposit.  The block's particles (sorted by cell,
 // so a block covers a few neighbouring cells) are accumulated into an LDS
@@ -264,7 +265,7 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
         if (NQ > mq) { q0 += (NQ - mq) / 2; NQ = mq; }
     }
     const int ncell = use ? NR * NP * NQ : 0;
-    for (int e = tid; e < ncell; e += P3_BLOCK) tile[e] = 0;
+    for (int e = tid; e < ncell; e += P3_PBLOCK) tile[e] = 0;
     __syncthreads();
     if (on) {
         long long v[8];
@@ -291,7 +292,7 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
     __syncthreads();
     // flush: one wave per (ring, plane) row of the window
     const int wave = tid >> 6;
-    for (int row = wave; row < (use ? NR * NP : 0); row += P3_BLOCK / 64) {
+    for (int row = wave; row < (use ? NR * NP : 0); row += P3_PBLOCK / 64) {
         const int dr = row / NP, dp = row - dr * NP;
         const int base = ((r0 + dr) * a.ngz + (p0 + dp)) * a.na + q0;
         const long long *t = tile + row * NQ;
@@ -314,7 +315,7 @@ __device__ __forceinline__ void flush_ranges(int nz, int nr, const int *s_jr,
 }
 
 template <typename R>
-__global__ __launch_bounds__(P3_BLOCK) void p3_massgn_kernel(P3Args<R> a) {
+__global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
     __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
     __shared__ long long tile[P3_TILE];
     const int tid = threadIdx.x;
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_massgn_kernel(P3Args<R> a) {
     }
     if (tid == 0) s_b[0] = INT_MAX;
     __syncthreads();
-    const long long i = (long long)blockIdx.x * P3_BLOCK + tid;
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + tid;
     bool on = false;
     int z = 0, c = 0;
     R w[8];
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_massgn_kernel(P3Args<R> a) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
+__global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
     typedef typename Vec4<R>::T R4;
     typedef typename Vec2<R>::T R2;
     __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
         s_b[0] = INT_MAX;
     }
     __syncthreads();
-    const long long i = (long long)blockIdx.x * P3_BLOCK + tid;
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + tid;
     int dn = 0, np = 0;
     bool don = false;
     int dz = 0, dc = 0;
@@ -449,8 +450,8 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_push_kernel(P3Args<R> a) {
 
 // (u, t) of every particle's position (after a load)
 template <typename R>
-__global__ __launch_bounds__(P3_BLOCK) void p3_ut_kernel(P3Args<R> a) {
-    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+__global__ __launch_bounds__(P3_PBLOCK) void p3_ut_kernel(P3Args<R> a) {
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= a.n) return;
     const typename Vec4<R>::T p = a.xv[i];
     R u, t;
@@ -459,8 +460,8 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_ut_kernel(P3Args<R> a) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(P3_BLOCK) void p3_tmcent_kernel(P3Args<R> a) {
-    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+__global__ __launch_bounds__(P3_PBLOCK) void p3_tmcent_kernel(P3Args<R> a) {
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= a.n) return;
     typename Vec4<R>::T p = a.xv[i];
     typename Vec2<R>::T q = a.vw[i];
@@ -475,9 +476,9 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_tmcent_kernel(P3Args<R> a) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(P3_BLOCK) void p3_gather_kernel(P3GatherArgs<R> b) {
+__global__ __launch_bounds__(P3_PBLOCK) void p3_gather_kernel(P3GatherArgs<R> b) {
     typedef typename Vec4<R>::T R4;
-    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= b.n) return;
     R w[8];
     const int c = weight3<R>(b.x[i], b.y[i], b.z[i], b.g, b.fold != 0, w);
@@ -501,10 +502,10 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_gather_kernel(P3GatherArgs<R> b) 
 // Sort keys: the particle's cell shifted right by kshift (kmax for the
 // off-grid list, which sorts last), with the identity permutation.
 template <typename R>
-__global__ __launch_bounds__(P3_BLOCK) void p3_key_kernel(P3Args<R> a, int kshift,
+__global__ __launch_bounds__(P3_PBLOCK) void p3_key_kernel(P3Args<R> a, int kshift,
                                                           unsigned kmax, unsigned *key,
                                                           unsigned *perm) {
-    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= a.n) return;
     const int z = a.zone[i];
     unsigned k = kmax;
@@ -519,8 +520,8 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_key_kernel(P3Args<R> a, int kshif
 }
 
 template <typename R>
-__global__ __launch_bounds__(P3_BLOCK) void p3_permute_kernel(P3PermArgs<R> p) {
-    const long long i = (long long)blockIdx.x * P3_BLOCK + threadIdx.x;
+__global__ __launch_bounds__(P3_PBLOCK) void p3_permute_kernel(P3PermArgs<R> p) {
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= p.n) return;
     const unsigned j = p.perm[i];
     p.xv_out[i] = p.xv_in[j];
@@ -898,35 +899,35 @@ __global__ __launch_bounds__(GREEN3_BLOCK) void p3_green_kernel(
 
 // ---- launchers ------------------------------------------------------------------------
 
-static unsigned p3_blocks(long long n) { return (unsigned)((n + P3_BLOCK - 1) / P3_BLOCK); }
+static unsigned p3_blocks(long long n) { return (unsigned)((n + P3_PBLOCK - 1) / P3_PBLOCK); }
 
 template <typename R> hipError_t launch_p3_massgn(const P3Args<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_massgn_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(p3_massgn_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename R> hipError_t launch_p3_push(const P3Args<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_push_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(p3_push_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename R> hipError_t launch_p3_ut(const P3Args<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_ut_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(p3_ut_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_tmcent_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(p3_tmcent_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename R> hipError_t launch_p3_gather(const P3GatherArgs<R> &b, hipStream_t s) {
     if (b.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_gather_kernel<R>, dim3(p3_blocks(b.n)), dim3(P3_BLOCK), 0, s, b);
+    hipLaunchKernelGGL(p3_gather_kernel<R>, dim3(p3_blocks(b.n)), dim3(P3_PBLOCK), 0, s, b);
     return hipGetLastError();
 }
 
@@ -934,14 +935,14 @@ template <typename R>
 hipError_t launch_p3_key(const P3Args<R> &a, int kshift, unsigned kmax, unsigned *key,
                          unsigned *perm, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_key_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_BLOCK), 0, s, a,
+    hipLaunchKernelGGL(p3_key_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a,
                        kshift, kmax, key, perm);
     return hipGetLastError();
 }
 
 template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &p, hipStream_t s) {
     if (p.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_permute_kernel<R>, dim3(p3_blocks(p.n)), dim3(P3_BLOCK), 0, s, p);
+    hipLaunchKernelGGL(p3_permute_kernel<R>, dim3(p3_blocks(p.n)), dim3(P3_PBLOCK), 0, s, p);
     return hipGetLastError();
 }
 
