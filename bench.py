@@ -153,8 +153,11 @@ def run_gpu3(w, precision, n, steps, warmup, sort_interval, want_green=False):
     g = configs.make_grid3(w, precision=precision, n=n)
     g.set_sort_interval(sort_interval)
     log("[bench] p3d %s set-up %.1f s (N=%d)" % (precision, time.time() - t0, n))
+    g.set_fixpot(*configs.fixpot(w, math.hypot(g.rgrid, g.zm)))
     if warmup:
         g.run(warmup)
+    g.findf(True)                      # energy at the start (outside the timing)
+    e0 = g.integrals()
     g.sync()
     g.set_profile(False)
     t0 = time.perf_counter()
@@ -167,9 +170,12 @@ def run_gpu3(w, precision, n, steps, warmup, sort_interval, want_green=False):
     g.sync()
     push_ms, launches, pushes_p = g.get_profile()
     g.set_profile(False)
+    g.findf(True)
+    e1 = g.integrals()
     out = g.unload()
     res = dict(seconds=t1 - t0, pushes=pushes, push_ms=push_ms, launches=launches,
-               pushes_prof=pushes_p, a2=a2(out["x"], out["y"], w.lscale()), noff=g.noff)
+               pushes_prof=pushes_p, a2=a2(out["x"], out["y"], w.lscale()), noff=g.noff,
+               e0=e0["energy"], e1=e1["energy"], lz0=e0["lz"], lz1=e1["lz"])
     if want_green:
         res["green"] = g.get_green()
     log("[bench] p3d %s: %d steps in %.4f s, %d pushes; profiled pass: push kernel "
@@ -232,7 +238,11 @@ def main_p3d(args):
         extra.update(fp64_value=v64, mixed_over_fp64=value / v64,
                      fp64_ms_per_step=1e3 * fp64["seconds"] / args.steps,
                      a2_mixed=abs(mixed["a2"]), a2_fp64=abs(fp64["a2"]),
-                     m2_err=abs(mixed["a2"] - fp64["a2"]))
+                     m2_err=abs(mixed["a2"] - fp64["a2"]),
+                     energy_drift_mixed=drift(mixed), energy_drift_fp64=drift(fp64),
+                     energy_drift_err=abs(drift(mixed) - drift(fp64)),
+                     lz_drift_mixed=(mixed["lz1"] - mixed["lz0"]) / abs(mixed["lz0"]),
+                     lz_drift_fp64=(fp64["lz1"] - fp64["lz0"]) / abs(fp64["lz0"]))
     line = {
         "metric": "particle-steps/s (3-D polar grid)", "value": value,
         "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -788,9 +788,9 @@ template <typename R> static int integrals_impl(galhip_ctx *c, int centred, doub
 
 extern "C" int galhip_integrals(galhip_ctx *c, int32_t centred, double *out) {
     if (!c || !out) return gh_fail(GALHIP_EINVAL, "null argument");
-    if (c->p3d) return gh_fail(GALHIP_EINVAL, "galhip_integrals: not built for p3d contexts");
     if (centred && !c->have_field)
         return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    if (c->p3d) return p3_integrals(c, centred != 0, out);
     return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out)
                                   : integrals_impl<double>(c, centred != 0, out);
 }

@@ -79,6 +79,11 @@ template <typename R> hipError_t launch_p3_massgn(const P3Args<R> &, hipStream_t
 template <typename R> hipError_t launch_p3_push(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_ut(const P3Args<R> &, hipStream_t);
+constexpr int P3_NINT = 11;   // entries of the p3d integrals
+template <typename R>
+hipError_t launch_p3_integrals(const P3Args<R> &, int with_pot, const double *fp_val,
+                               const double *fp_der, int nfp, double fp_dr, int centred,
+                               double *part, double *out, hipStream_t);
 template <typename R> hipError_t launch_p3_gather(const P3GatherArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_solve(const P3SolveArgs<R> &, hipStream_t);
 template <typename R> bool p3_solve_fits(const P3SolveArgs<R> &);

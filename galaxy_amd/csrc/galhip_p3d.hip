@@ -499,6 +499,107 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_gather_kernel(P3GatherArgs<R> b)
     }
 }
 
+// icheck's global integrals for the 3-D grid (SRC/lib15/icheck.f:24-70):
+// per-block FP64 sums of 11 quantities in a fixed order, then one block adds
+// the block sums in order.  [0] count, [1..3] position, [4..6] velocity per
+// zone-1 step ((v + a/2)/tfac when centred), [7] v^2, [8] Lz, [9] the grid
+// potential interpolated with the particle's own weights (grid particles),
+// [10] the fixed potential of the spherical radius.
+constexpr int NINT3 = 11;
+
+__device__ __forceinline__ double p3_wave_sumd(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename R>
+__global__ __launch_bounds__(P3_PBLOCK) void p3_integrals_kernel(P3Args<R> a, int with_pot,
+                                                                 const double *fp_val,
+                                                                 const double *fp_der, int nfp,
+                                                                 double fp_dr, int centred,
+                                                                 double *part) {
+    typedef typename Vec4<R>::T R4;
+    __shared__ double red[P3_PBLOCK / 64][NINT3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + tid;
+    double v[NINT3];
+#pragma unroll
+    for (int q = 0; q < NINT3; q++) v[q] = 0.0;
+    if (i < a.n) {
+        const int z = a.zone[i];
+        if (z >= 1) {
+            const R4 p = a.xv[i];
+            const typename Vec2<R>::T q2 = a.vw[i];
+            const typename Vec2<R>::T ut = a.ut[i];
+            R vx = p.w, vy = q2.x, vz = q2.y;
+            if (centred) {
+                R acc[3];
+                getacc3<R>(a, p.x, p.y, p.z, ut.x, ut.y, z, acc);
+                vx = vx + (R)0.5 * acc[0];
+                vy = vy + (R)0.5 * acc[1];
+                vz = vz + (R)0.5 * acc[2];
+            }
+            const int zs = z <= a.nz ? z : a.nz;
+            const double sf = 1.0 / (double)a.nstep[zs - 1];
+            const double x = p.x, y = p.y, zz = p.z;
+            const double ux = vx * sf, uy = vy * sf, uz = vz * sf;
+            v[0] = 1.0;
+            v[1] = x;
+            v[2] = y;
+            v[3] = zz;
+            v[4] = ux;
+            v[5] = uy;
+            v[6] = uz;
+            v[7] = ux * ux + uy * uy + uz * uz;
+            v[8] = x * uy - y * ux;
+            if (with_pot && z <= a.nz) {
+                R w[8];
+                int ic[8];
+                const int c = cell3<R>(ut.x, ut.y, p.z, a, false, w);
+                corners3(c, a.na, a.ngz, ic);
+                double ph = 0.0;
+                for (int k = 0; k < 8; k++)
+                    if (ic[k] < a.mesh) ph += (double)w[k] * (double)a.fld[ic[k]].w;
+                v[9] = ph;
+            }
+            if (nfp > 0)
+                v[10] = fixforce(sqrt(x * x + y * y + zz * zz), nfp, fp_dr, 1.0 / fp_dr, fp_val,
+                                 fp_der);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NINT3; q++) {
+        const double t = p3_wave_sumd(v[q]);
+        if (lane == 0) red[wave][q] = t;
+    }
+    __syncthreads();
+    if (tid < NINT3) {
+        double t = 0.0;
+        for (int w = 0; w < P3_PBLOCK / 64; w++) t += red[w][tid];
+        part[(long long)blockIdx.x * NINT3 + tid] = t;
+    }
+}
+
+__global__ __launch_bounds__(P3_BLOCK) void p3_integrals_sum_kernel(const double *part, int nb,
+                                                                    double *out) {
+    __shared__ double red[P3_BLOCK / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int q = 0; q < NINT3; q++) {
+        double t = 0.0;
+        for (int b = tid; b < nb; b += P3_BLOCK) t += part[(long long)b * NINT3 + q];
+        t = p3_wave_sumd(t);
+        if (lane == 0) red[wave] = t;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < P3_BLOCK / 64; w++) s += red[w];
+            out[q] = s;
+        }
+        __syncthreads();
+    }
+}
+
 // Sort keys: the particle's cell shifted right by kshift (kmax for the
 // off-grid list, which sorts last), with the identity permutation.
 template <typename R>
@@ -946,6 +1047,21 @@ template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &p, hipSt
     return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_p3_integrals(const P3Args<R> &a, int with_pot, const double *fp_val,
+                               const double *fp_der, int nfp, double fp_dr, int centred,
+                               double *part, double *out, hipStream_t s) {
+    const unsigned nb = a.n > 0 ? p3_blocks(a.n) : 0;
+    if (nb)
+        hipLaunchKernelGGL(p3_integrals_kernel<R>, dim3(nb), dim3(P3_PBLOCK), 0, s, a, with_pot,
+                           fp_val, fp_der, nfp, fp_dr, centred, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(p3_integrals_sum_kernel, dim3(1), dim3(P3_BLOCK), 0, s, part, (int)nb,
+                       out);
+    return hipGetLastError();
+}
+
 // Limits the launches below rely on (checked again by the context set-up).
 template <typename R> bool p3_solve_fits(const P3SolveArgs<R> &a) {
     const size_t P = 2 * (size_t)a.ngz;
@@ -989,6 +1105,9 @@ hipError_t launch_p3_green(double *G, const double *twd, const double *twzd, con
     template hipError_t launch_p3_push<R>(const P3Args<R> &, hipStream_t);            \
     template hipError_t launch_p3_tmcent<R>(const P3Args<R> &, hipStream_t);          \
     template hipError_t launch_p3_ut<R>(const P3Args<R> &, hipStream_t);              \
+    template hipError_t launch_p3_integrals<R>(const P3Args<R> &, int, const double *,  \
+                                               const double *, int, double, int,       \
+                                               double *, double *, hipStream_t);       \
     template hipError_t launch_p3_gather<R>(const P3GatherArgs<R> &, hipStream_t);    \
     template hipError_t launch_p3_solve<R>(const P3SolveArgs<R> &, hipStream_t);      \
     template bool p3_solve_fits<R>(const P3SolveArgs<R> &);                            \

@@ -538,6 +538,34 @@ int p3_tmcent(galhip_ctx *c, int start) {
     return c->prec == GALHIP_FP32 ? p3_tmcent_impl<float>(c, start) : p3_tmcent_impl<double>(c, start);
 }
 
+// ---- global integrals ------------------------------------------------------------------------
+
+template <typename R> static int p3_integrals_impl(galhip_ctx *c, int centred, double *out) {
+    const long long nb = c->n > 0 ? (c->n + 511) / 512 : 0;
+    double *part = nullptr, *dout = nullptr;
+    int rc = dalloc(&part, sizeof(double) * P3_NINT * (nb > 0 ? nb : 1)) |
+             dalloc(&dout, sizeof(double) * P3_NINT);
+    if (!rc) {
+        hipError_t e = launch_p3_integrals<R>(p3_args<R>(c), c->pot_valid, c->fp_val, c->fp_der,
+                                              c->nfp, c->fp_dr, centred, part, dout, c->s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, dout, sizeof(double) * P3_NINT, hipMemcpyDeviceToHost, c->s);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+        if (e != hipSuccess) rc = gh_fail(GALHIP_EHIP, "integrals: %s", hipGetErrorString(e));
+    }
+    dfree(part);
+    dfree(dout);
+    if (rc) return rc;
+    for (int q = 0; q < P3_NINT; q++) out[q] *= c->pmass;
+    if (!c->pot_valid) out[9] = NAN;
+    return 0;
+}
+
+int p3_integrals(galhip_ctx *c, int centred, double *out) {
+    return c->prec == GALHIP_FP32 ? p3_integrals_impl<float>(c, centred, out)
+                                  : p3_integrals_impl<double>(c, centred, out);
+}
+
 // ---- fields ------------------------------------------------------------------------------------
 
 int p3_get_field(const galhip_ctx *c, void *fld) {

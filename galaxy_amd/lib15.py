@@ -360,6 +360,18 @@ class P3DGrid(P2DGrid):
                                          _ptr(acc)), "grdacc3")
         return acc
 
+    def integrals(self, centred=True):
+        """icheck's global integrals on the 3-D grid: mass, cm (3), momentum
+        (3), ke, lz, w_grid (nan without a potential), w_fixed, energy."""
+        out = np.zeros(11, np.float64)
+        self._c(self._lib.galhip_integrals(self._h, int(bool(centred)), _ptr(out)),
+                "integrals")
+        d = dict(mass=out[0], cm=out[1:4] / out[0] if out[0] else out[1:4],
+                 momentum=out[4:7], ke=0.5 * out[7], lz=out[8], w_grid=out[9],
+                 w_fixed=out[10])
+        d["energy"] = d["ke"] + 0.5 * d["w_grid"] + d["w_fixed"]
+        return d
+
     def get_field(self):
         f = np.zeros((4, self.mesh), self.dtype)
         self._c(self._lib.galhip_get_field(self._h, _ptr(f)), "get_field")

@@ -192,7 +192,10 @@ int galhip_set_fixpot(galhip_ctx *ctx, int32_t n, double dr, const double *val,
  * out[5] sum m v^2 (twice the kinetic energy), out[6] angular momentum Lz,
  * out[7] sum m Phi_grid over the grid's particles (NaN unless the last
  * solve formed the potential: findf with lpot), out[8] sum m Phi_fixed.
- * Energy = out[5]/2 + out[7]/2 + out[8]. */
+ * Energy = out[5]/2 + out[7]/2 + out[8].  On a p3d context out has 11
+ * entries: [0] mass, [1..3] m x position, [4..6] momentum, [7] sum m v^2,
+ * [8] Lz, [9] sum m Phi_grid (NaN as above), [10] sum m Phi_fixed of the
+ * spherical radius; energy = out[7]/2 + out[9]/2 + out[10]. */
 int galhip_integrals(galhip_ctx *ctx, int32_t centred, double *out);
 
 /* ---- performance controls ---- */
@@ -224,10 +227,10 @@ int galhip_sync(galhip_ctx *ctx);
  * polcat over every plane), step, tmcent, run, get_mass / set_mass /
  * get_rho (mesh = nr ngz na), get_field / set_field (fld[4][mesh]: F_x,
  * F_y, F_z, potential), get_ranges, get_noff, set_sort_interval, sort,
- * get_order, the profiling calls and sync.  Particles go in and out through
- * galhip_restrt3 and galhip_unload3; galhip_restrt, galhip_unload,
- * galhip_weight, galhip_grdacc and galhip_integrals return GALHIP_EINVAL on
- * a p3d context.  Zone boundaries are spherical radii (SRC/lib15/setrun.f:810,
+ * get_order, set_fixpot, integrals (11 entries, see there), the profiling
+ * calls and sync.  Particles go in and out through galhip_restrt3 and
+ * galhip_unload3; galhip_restrt, galhip_unload, galhip_weight and
+ * galhip_grdacc return GALHIP_EINVAL on a p3d context.  Zone boundaries are spherical radii (SRC/lib15/setrun.f:810,
  * zoneno.f:40-41); a particle is off the grid when |z| > zm or R > rgrid
  * (SRC/lib15/offgrd.f:69-72), zm = dzg (ngz - 1) / 2. */
 typedef struct {

@@ -260,3 +260,43 @@ class P3D(_o.P2D):
         c = self.cfg()
         self._fn("orc3_tmcent")(C.byref(c), len(self.x), self.x, self.y, self.z,
                                 self.vx, self.vy, self.vz, self.iz, f, int(bool(start)))
+
+    def integrals(self, fixpot=None, centred=False):
+        """icheck's global integrals (SRC/lib15/icheck.f:24-70) in FP64 numpy,
+        as galhip_integrals defines them for the 3-D grid: mass, m x position
+        (3), momentum (3), sum m v^2, Lz, sum m Phi_grid (8-corner
+        interpolation of the last potential with the particle's own weights),
+        sum m Phi_fixed of the spherical radius.  centred: v + a/2 from the
+        current field before the zone step factor."""
+        x, y, z = (a.astype(np.float64) for a in (self.x, self.y, self.z))
+        iz = self.iz
+        zs = np.minimum(iz, self.nzones)
+        sf = 1.0 / np.asarray(self.nstep, np.float64)[np.maximum(zs, 1) - 1]
+        v = [self.vx, self.vy, self.vz]
+        if centred:
+            acc = self.getacc()
+            v = [v[d] + self.dt(0.5) * acc[:, d] for d in range(3)]
+        vx, vy, vz = (a.astype(np.float64) * sf for a in v)
+        on = iz >= 1
+        pm = float(self.pmass)
+        out = np.zeros(11)
+        out[0] = pm * on.sum()
+        out[1], out[2], out[3] = pm * x[on].sum(), pm * y[on].sum(), pm * z[on].sum()
+        out[4], out[5], out[6] = pm * vx[on].sum(), pm * vy[on].sum(), pm * vz[on].sum()
+        out[7] = pm * (vx[on] ** 2 + vy[on] ** 2 + vz[on] ** 2).sum()
+        out[8] = pm * (x[on] * vy[on] - y[on] * vx[on]).sum()
+        grid = (iz >= 1) & (iz <= self.nzones)
+        cell, w, _ = self.weight(self.x[grid], self.y[grid], self.z[grid])
+        na, S = self.na, self.ngz * self.na
+        wrap = (cell + 1) % na == 0
+        c1 = np.where(wrap, cell + 1 - na, cell + 1)
+        ic = np.stack([cell, c1, cell + na, c1 + na, cell + S, c1 + S, cell + S + na,
+                       c1 + S + na], 1)
+        pot = np.append(self.fld[3].astype(np.float64), 0.0)
+        ic = np.where(ic < self.mesh, ic, self.mesh)
+        out[9] = pm * (w.astype(np.float64) * pot[ic]).sum()
+        if fixpot is not None:
+            dr, val, der = fixpot
+            out[10] = pm * _o.fixforce(np.sqrt(x[on] ** 2 + y[on] ** 2 + z[on] ** 2),
+                                       dr, val, der).sum()
+        return out

@@ -328,3 +328,36 @@ def test_run_equals_step_loop():
         out.append(g.unload())
     for key in out[0]:
         _same(out[0][key], out[1][key], key)
+
+
+@pytest.mark.parametrize("prec,op", PRECS)
+def test_integrals3_match_oracle(o3, prec, op):
+    """galhip_integrals on the 3-D grid against the oracle's numpy sums:
+    mass, centre of mass, momentum, kinetic energy and Lz; the grid
+    potential energy to the solve's accuracy; the fixed-potential energy of
+    the spherical radius from the same table; NaN without a potential."""
+    from galaxy_amd import disc
+    _say("p3d integrals", prec)
+    m, g = models(o3, prec, op)
+    rmax = float(np.hypot(g.rgrid, g.zm))
+    fp = disc.halo_potential_table(rmax, P3SMALL.lscale(), P3SMALL.dt)
+    g.set_fixpot(*fp)
+    m.masset()
+    g.masset()
+    g.findf(False)
+    assert np.isnan(g.integrals(centred=False)["w_grid"])
+    for _ in range(3):
+        m.findf()
+        m.step()
+    g.run(3)
+    m.findf(lpot=True)
+    g.findf(True)
+    ref = m.integrals(fixpot=fp, centred=True)
+    got = g.integrals(centred=True)
+    vec = np.concatenate([[got["mass"]], got["cm"] * got["mass"], got["momentum"],
+                          [2 * got["ke"], got["lz"], got["w_grid"], got["w_fixed"]]])
+    scale = np.abs(ref).max()
+    tol_kin = 1e-10 if prec == "fp64" else 1e-6
+    assert np.abs(vec[:9] - ref[:9]).max() <= tol_kin * scale, (vec[:9], ref[:9])
+    assert abs(vec[9] - ref[9]) <= (1e-9 if prec == "fp64" else 2e-5) * abs(ref[9])
+    assert abs(vec[10] - ref[10]) <= (1e-10 if prec == "fp64" else 1e-6) * abs(ref[10])
