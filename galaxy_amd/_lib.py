@@ -41,7 +41,7 @@ class P3DParams(C.Structure):
         ("nzones", C.c_int32), ("nstep", C.c_int32 * MAXZONES),
         ("zbr", C.c_double * MAXZONES),
         ("offanal", C.c_int32), ("pmass", C.c_double),
-        ("precision", C.c_int32),
+        ("precision", C.c_int32), ("jmass", C.c_int32),
     ]
 
 

@@ -63,7 +63,7 @@ struct galhip_ctx {
     double prof_ms;
     long long prof_launch, prof_push0;
     // 3-D polar grid (p3d); p3d = 0 for the 2-D grid
-    int p3d, ngz, nm3;
+    int p3d, ngz, nm3, jmass;
     double dzg, zm;                   // in the context's precision
     void *vw, *vw2;                   // (vy, vz) per particle
     void *A3, *M3, *W3, *Z3;          // solve work arrays

@@ -214,16 +214,21 @@ __device__ __forceinline__ bool solve_ranges(const int *jrad, const int *krad, i
 }
 
 // Global deposit of one corner value with Fortran's flat indexing over the
-// [nz][mesh] zone arrays (grdmss(j, jz)); past the last array it is dropped.
+// [nz][mesh] zone arrays (grdmss(j, jz)); before the first array or past the
+// last it is dropped.
 __device__ __forceinline__ void deposit_global(long long *const *mfx, int nz,
                                                int mesh, int zone, int idx,
                                                long long v) {
     long long f = (long long)(zone - 1) * mesh + idx;
-    if (f < (long long)nz * mesh) {
+    if (f >= 0 && f < (long long)nz * mesh) {
         int zf = zone - 1, cf = idx;
         while (cf >= mesh) {
             cf -= mesh;
             zf++;
+        }
+        while (cf < 0) {
+            cf += mesh;
+            zf--;
         }
         atomicAdd((unsigned long long *)&mfx[zf][cf], (unsigned long long)v);
     }

@@ -19,6 +19,7 @@ template <typename R> struct P3Args {
     double fr_dr, fr_inv_dr;
     int nfr, fixforce;
     int nr, na, ngz, mesh, nz, mzone, offanal, nsect, qbits;
+    int jmass;                        // 2 linear (8 corners), 3 quadratic (27)
     UtConst k;
     R rgrid, thmax, zm, dzg, fac;
     R zbr[GH_MAXZ];
@@ -31,7 +32,7 @@ template <typename R> struct P3GatherArgs {
     const typename Vec4<R>::T *fld;
     R *acc;        // [n][3] or NULL
     int *cell;     // [n] or NULL
-    R *wt;         // [n][8] or NULL
+    R *wt;         // [n][8] (jmass 3: [n][27]) or NULL
     int fold;
     P3Args<R> g;   // geometry only
 };

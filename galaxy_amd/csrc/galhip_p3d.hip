@@ -90,6 +90,51 @@ __device__ __forceinline__ int weight3(R x, R y, R z, const P3Args<R> &a, bool f
     return cell3(u, t, z, a, fold, w);
 }
 
+// weight() for jmass = 3 given (u, t) (weight.f:281-300): the central cell
+// of the 27 (nearest node, ju >= 1, jt == na folded to spoke 0) and their
+// weights, spoke offset -1, 0, +1 fastest, then plane, then ring.
+template <typename R>
+__device__ __forceinline__ int cell3q(R u, R t, R z, const P3Args<R> &a, bool fold, R w[27]) {
+    R zz = z + a.zm;
+    if (fold) t = (R)fmod(t, a.thmax);
+    const R h = (R)0.5;
+    int ju = (int)(u + h);
+    if (ju > a.nr - 2) ju = a.nr - 2;
+    if (ju < 1) ju = 1;
+    const int jt = (int)(t + h);
+    zz = zz / a.dzg;
+    int jz = (int)(zz + h);
+    if (jz < 0) jz = 0;
+    if (jz > a.ngz - 2) jz = a.ngz - 2;
+    int c = (ju * a.ngz + jz) * a.na + jt;
+    if (jt == a.na) c -= a.na;
+    R dp[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const R dd = i == 0 ? t + h - (R)jt : i == 1 ? zz + h - (R)jz : u + h - (R)ju;
+        dp[0][i] = (R)1 - (R)2 * dd + dd * dd;
+        dp[1][i] = (R)1 + (R)2 * dd - (R)2 * (dd * dd);
+        dp[2][i] = dd * dd;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                w[9 * k + 3 * j + i] = (R)0.125 * dp[i][0] * dp[j][1] * dp[k][2];
+    return c;
+}
+
+// weights per particle of interpolation scheme J (jmass 2 or 3)
+template <int J> struct P3NW { static constexpr int v = J == 3 ? 27 : 8; };
+
+template <typename R, int J>
+__device__ __forceinline__ int cellJ(R u, R t, R z, const P3Args<R> &a, bool fold, R *w) {
+    if constexpr (J == 3) return cell3q<R>(u, t, z, a, fold, w);
+    else return cell3<R>(u, t, z, a, fold, w);
+}
+
 // grdacc's corners (grdacc.f:193-203): c, c+1 (wrapped when c+1 is a
 // multiple of na), the next plane (+na), and the same one ring out.
 __device__ __forceinline__ void corners3(int c, int na, int ngz, int ic[8]) {
@@ -102,6 +147,27 @@ __device__ __forceinline__ void corners3(int c, int na, int ngz, int ic[8]) {
     ic[5] = wrap ? ic[4] + 1 - na : ic[4] + 1;
     ic[6] = ic[4] + na;
     ic[7] = ic[5] + na;
+}
+
+// the 27 cells about a central cell c (grdacc.f:262-275, massgn.f:182-195):
+// spoke -1, 0, +1 (with the azimuthal wrap) fastest, then plane -1, 0, +1
+// (-+ na), then ring -1, 0, +1 (-+ ngz na), as flat indices
+__device__ __forceinline__ void corners27(int c, int na, int ngz, int ic[27]) {
+    int m[3];
+    m[1] = c;
+    m[0] = c % na == 0 ? c - 1 + na : c - 1;
+    m[2] = (c + 1) % na == 0 ? c + 1 - na : c + 1;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) ic[9 * k + 3 * j + i] = m[i] + (j - 1) * na + (k - 1) * ngz * na;
+}
+
+template <int J> __device__ __forceinline__ void cornersJ(int c, int na, int ngz, int *ic) {
+    if constexpr (J == 3) corners27(c, na, ngz, ic);
+    else corners3(c, na, ngz, ic);
 }
 
 template <typename R>
@@ -133,28 +199,27 @@ __device__ __forceinline__ void chstep1(R tfac, R ox, R &ov, R &acc, R &nx, R &n
 // getacc of one particle of list z0: the three Cartesian components
 // interpolated at its position (component sums in corner order), the fixed
 // force of the spherical radius, the zone rescaling.
-template <typename R>
+template <typename R, int J>
 __device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz, R u, R t,
                                         int z0, R acc[3]) {
     typedef typename Vec4<R>::T R4;
+    constexpr int NW = P3NW<J>::v;
     acc[0] = acc[1] = acc[2] = (R)0;
     if (z0 != a.nz + 1) {
-        R w[8];
-        int ic[8];
-        const int c = cell3<R>(u, t, oz, a, false, w);
-        corners3(c, a.na, a.ngz, ic);
-        R4 f[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            if (ic[q] < a.mesh) f[q] = a.fld[ic[q]];
-            else f[q] = mk4((R)0, (R)0, (R)0, (R)0);
-        }
+        R w[NW];
+        int ic[NW];
+        const int c = cellJ<R, J>(u, t, oz, a, false, w);
+        cornersJ<J>(c, a.na, a.ngz, ic);
         R sx = (R)0, sy = (R)0, sz = (R)0;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            sx = sx + f[q].x * w[q];
-            sy = sy + f[q].y * w[q];
-            sz = sz + f[q].z * w[q];
+        for (int q = 0; q < NW; q++) {
+            // a corner outside the field array (the last ring with t == na;
+            // for jmass 3 also the first ring's bottom plane) reads 0
+            const R4 f = (ic[q] >= 0 && ic[q] < a.mesh) ? a.fld[ic[q]]
+                                                        : mk4((R)0, (R)0, (R)0, (R)0);
+            sx = sx + f.x * w[q];
+            sy = sy + f.y * w[q];
+            sz = sz + f.z * w[q];
         }
         if (a.fixforce) {
             R r2 = (R)0;
@@ -200,17 +265,22 @@ constexpr int P3_TILE = 4096;   // cells of a block's LDS deposit window
 // so a block covers a few neighbouring cells) are accumulated into an LDS
 // window spanning the (ring, plane, spoke) box of their corners, then the
 // window's non-zero cells are added to the zone array with one global atomic
-// each.  Particles whose corners leave the box shape (the last spoke, whose
-// next corner wraps to spoke 0; the last plane, reachable only through
-// t == na) or that belong to another zone than the block's first, and every
-// particle when the box exceeds P3_TILE cells, add their 8 corners directly.
-// Integer sums: the result does not depend on the grouping.  Every thread of
-// the block must call it (on = false for threads without a deposit).
-// s_b: 7 ints, s_b[0] = INT_MAX on entry.
-template <typename R>
+// each.  The corners of a central cell are the next ring, plane and spoke
+// (jmass 2) or the cells one either side (jmass 3).  Particles whose
+// corners leave the box shape (the azimuthal wrap; the plane next to the
+// last or, for jmass 3, the first), or that belong to another zone than the
+// block's first, and those outside the window when the box is clipped to
+// P3_TILE cells, add their corners directly.  Integer sums: the result does
+// not depend on the grouping.  Every thread of the block must call it
+// (on = false for threads without a deposit).  s_b: 7 ints, s_b[0] =
+// INT_MAX on entry; s_jr/s_kr receive the block's central-ring minimum and
+// maximum per zone.
+template <typename R, int J>
 __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int z, int c,
-                                               const R w[8], long long *tile, int *s_b,
+                                               const R *w, long long *tile, int *s_b,
                                                int *s_jr, int *s_kr) {
+    constexpr int NW = P3NW<J>::v;
+    constexpr int LO = J == 3 ? 1 : 0;   // corner cells below the central one
     const int tid = threadIdx.x, lane = tid & 63;
     const int S = a.ngz * a.na;
     int ring = 0, plane = 0, spoke = 0;
@@ -229,7 +299,8 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
     }
     __syncthreads();
     const int tz = s_b[0];
-    const bool cand = on && z == tz && plane + 1 < a.ngz && spoke + 1 < a.na;
+    const bool cand = on && z == tz && plane - LO >= 0 && plane + 1 < a.ngz &&
+                      spoke - LO >= 0 && spoke + 1 < a.na && ring - LO >= 0;
     {
         const int r0 = p3_wave_min(cand ? ring : INT_MAX), r1 = p3_wave_max(cand ? ring : -1);
         const int p0 = p3_wave_min(cand ? plane : INT_MAX), p1 = p3_wave_max(cand ? plane : -1);
@@ -255,8 +326,9 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
     // the window: the box of the candidates' corners, clipped about its
     // centre to at most P3_TILE cells (8 rings, 8 planes, the rest spokes);
     // candidates whose corners fall outside go to the zone array directly
-    int r0 = s_b[1], p0 = s_b[3], q0 = s_b[5];
-    int NR = s_b[2] - r0 + 2, NP = s_b[4] - p0 + 2, NQ = s_b[6] - q0 + 2;
+    int r0 = s_b[1] - LO, p0 = s_b[3] - LO, q0 = s_b[5] - LO;
+    int NR = s_b[2] - s_b[1] + 2 + LO, NP = s_b[4] - s_b[3] + 2 + LO;
+    int NQ = s_b[6] - s_b[5] + 2 + LO;
     const bool use = s_b[2] >= 0;
     if (use && NR * NP * NQ > P3_TILE) {
         if (NR > 8) { r0 += (NR - 8) / 2; NR = 8; }
@@ -268,24 +340,28 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
     for (int e = tid; e < ncell; e += P3_PBLOCK) tile[e] = 0;
     __syncthreads();
     if (on) {
-        long long v[8];
+        long long v[NW];
 #pragma unroll
-        for (int q = 0; q < 8; q++) v[q] = quant<R>(w[q], a.qbits);
-        const bool inside = use && cand && ring >= r0 && ring - r0 + 1 < NR &&
-                            plane >= p0 && plane - p0 + 1 < NP && spoke >= q0 &&
+        for (int q = 0; q < NW; q++) v[q] = quant<R>(w[q], a.qbits);
+        const bool inside = use && cand && ring - LO >= r0 && ring - r0 + 1 < NR &&
+                            plane - LO >= p0 && plane - p0 + 1 < NP && spoke - LO >= q0 &&
                             spoke - q0 + 1 < NQ;
         if (inside) {
             const int b = ((ring - r0) * NP + (plane - p0)) * NQ + (spoke - q0);
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int o = b + ((q & 1) ? 1 : 0) + ((q & 2) ? NQ : 0) + ((q & 4) ? NP * NQ : 0);
+            for (int q = 0; q < NW; q++) {
+                int o;
+                if constexpr (J == 3)
+                    o = b + (q % 3 - 1) + ((q / 3) % 3 - 1) * NQ + (q / 9 - 1) * NP * NQ;
+                else
+                    o = b + ((q & 1) ? 1 : 0) + ((q & 2) ? NQ : 0) + ((q & 4) ? NP * NQ : 0);
                 if (v[q] != 0) atomicAdd((unsigned long long *)&tile[o], (unsigned long long)v[q]);
             }
         } else {
-            int ic[8];
-            corners3(c, a.na, a.ngz, ic);
+            int ic[NW];
+            cornersJ<J>(c, a.na, a.ngz, ic);
 #pragma unroll
-            for (int q = 0; q < 8; q++)
+            for (int q = 0; q < NW; q++)
                 if (v[q] != 0) deposit_global(a.mfx, a.nz, a.mesh, z, ic[q], v[q]);
         }
     }
@@ -304,17 +380,19 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
     }
 }
 
-__device__ __forceinline__ void flush_ranges(int nz, int nr, const int *s_jr,
+// jrad/krad from the block's central rings: 1-based rings j+1..j+2 about a
+// central ring j (massgn.f:146-149), j..j+2 for jmass 3 (massgn.f:176-179)
+__device__ __forceinline__ void flush_ranges(int nz, int nr, int lo, const int *s_jr,
                                              const int *s_kr, int *jrad, int *krad) {
     const int tid = threadIdx.x;
     if (tid < nz && s_kr[tid] >= 0) {
-        atomicMin(&jrad[tid], s_jr[tid] + 1);
+        atomicMin(&jrad[tid], s_jr[tid] + 1 - lo);
         const int k = s_kr[tid] + 2;
         atomicMax(&krad[tid], k < nr ? k : nr);
     }
 }
 
-template <typename R>
+template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
     __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
     __shared__ long long tile[P3_TILE];
@@ -328,7 +406,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
     const long long i = (long long)blockIdx.x * P3_PBLOCK + tid;
     bool on = false;
     int z = 0, c = 0;
-    R w[8];
+    R w[P3NW<J>::v];
     if (i < a.n) {
         z = a.zone[i];
         const typename Vec4<R>::T p = a.xv[i];
@@ -336,16 +414,16 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
         ut3(p.x, p.y, a, u, t);
         a.ut[i] = mk2(u, t);
         if (z >= 1 && z <= a.nz) {
-            c = cell3<R>(u, t, p.z, a, a.nsect > 1, w);
+            c = cellJ<R, J>(u, t, p.z, a, a.nsect > 1, w);
             on = true;
         }
     }
-    deposit3_block<R>(a, on, z, c, w, tile, s_b, s_jr, s_kr);
+    deposit3_block<R, J>(a, on, z, c, w, tile, s_b, s_jr, s_kr);
     __syncthreads();
-    flush_ranges(a.nz, a.nr, s_jr, s_kr, a.jrad, a.krad);
+    flush_ranges(a.nz, a.nr, J == 3 ? 1 : 0, s_jr, s_kr, a.jrad, a.krad);
 }
 
-template <typename R>
+template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
     typedef typename Vec4<R>::T R4;
     typedef typename Vec2<R>::T R2;
@@ -366,7 +444,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
     int dn = 0, np = 0;
     bool don = false;
     int dz = 0, dc = 0;
-    R dw[8];
+    R dw[P3NW<J>::v];
     if (i < a.n) {
         const int nz = a.nz, nlists = nz + 1;
         const int z0 = a.zone[i];
@@ -380,7 +458,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
             R o[3] = {p.x, p.y, p.z};
             R ov[3] = {p.w, q.x, q.y};
             R acc[3], nv[3], nx[3];
-            getacc3<R>(a, o[0], o[1], o[2], ut0.x, ut0.y, z0, acc);
+            getacc3<R, J>(a, o[0], o[1], o[2], ut0.x, ut0.y, z0, acc);
             // stpgrp
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -424,13 +502,13 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
             a.ut[i] = mk2(un, tn);
             a.zone[i] = (uint8_t)znew;
             if (znew >= 1 && znew <= nz) {
-                dc = cell3<R>(un, tn, nx[2], a, a.nsect > 1, dw);
+                dc = cellJ<R, J>(un, tn, nx[2], a, a.nsect > 1, dw);
                 dz = znew;
                 don = true;
             }
         }
     }
-    deposit3_block<R>(a, don, dz, dc, dw, tile, s_b, s_jr, s_kr);
+    deposit3_block<R, J>(a, don, dz, dc, dw, tile, s_b, s_jr, s_kr);
     // block sums of the off-grid change and the pushes
     for (int o = 32; o > 0; o >>= 1) {
         dn += __shfl_xor(dn, o, 64);
@@ -441,7 +519,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
         if (np) atomicAdd(&s_np, np);
     }
     __syncthreads();
-    flush_ranges(a.nz, a.nr, s_jr, s_kr, a.jrad, a.krad);
+    flush_ranges(a.nz, a.nr, J == 3 ? 1 : 0, s_jr, s_kr, a.jrad, a.krad);
     if (tid == 0) {
         if (s_dn) atomicAdd((unsigned long long *)&a.counters[0], (unsigned long long)(long long)s_dn);
         if (s_np) atomicAdd((unsigned long long *)&a.counters[1], (unsigned long long)(long long)s_np);
@@ -459,7 +537,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_ut_kernel(P3Args<R> a) {
     a.ut[i] = mk2(u, t);
 }
 
-template <typename R>
+template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_tmcent_kernel(P3Args<R> a) {
     const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= a.n) return;
@@ -467,7 +545,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_tmcent_kernel(P3Args<R> a) {
     typename Vec2<R>::T q = a.vw[i];
     const typename Vec2<R>::T ut = a.ut[i];
     R acc[3];
-    getacc3<R>(a, p.x, p.y, p.z, ut.x, ut.y, a.zone[i], acc);
+    getacc3<R, J>(a, p.x, p.y, p.z, ut.x, ut.y, a.zone[i], acc);
     p.w = p.w - a.fac * acc[0];
     q.x = q.x - a.fac * acc[1];
     q.y = q.y - a.fac * acc[2];
@@ -475,22 +553,25 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_tmcent_kernel(P3Args<R> a) {
     a.vw[i] = q;
 }
 
-template <typename R>
+template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_gather_kernel(P3GatherArgs<R> b) {
     typedef typename Vec4<R>::T R4;
+    constexpr int NW = P3NW<J>::v;
     const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= b.n) return;
-    R w[8];
-    const int c = weight3<R>(b.x[i], b.y[i], b.z[i], b.g, b.fold != 0, w);
+    R w[NW];
+    R u, t;
+    ut3(b.x[i], b.y[i], b.g, u, t);
+    const int c = cellJ<R, J>(u, t, b.z[i], b.g, b.fold != 0, w);
     if (b.cell) b.cell[i] = c;
     if (b.wt)
-        for (int q = 0; q < 8; q++) b.wt[8 * i + q] = w[q];
+        for (int q = 0; q < NW; q++) b.wt[NW * i + q] = w[q];
     if (b.acc) {
-        int ic[8];
-        corners3(c, b.g.na, b.g.ngz, ic);
+        int ic[NW];
+        cornersJ<J>(c, b.g.na, b.g.ngz, ic);
         R s[3] = {(R)0, (R)0, (R)0};
-        for (int q = 0; q < 8; q++) {
-            R4 f = ic[q] < b.g.mesh ? b.fld[ic[q]] : mk4((R)0, (R)0, (R)0, (R)0);
+        for (int q = 0; q < NW; q++) {
+            R4 f = (ic[q] >= 0 && ic[q] < b.g.mesh) ? b.fld[ic[q]] : mk4((R)0, (R)0, (R)0, (R)0);
             s[0] = s[0] + f.x * w[q];
             s[1] = s[1] + f.y * w[q];
             s[2] = s[2] + f.z * w[q];
@@ -513,7 +594,7 @@ __device__ __forceinline__ double p3_wave_sumd(double v) {
     return v;
 }
 
-template <typename R>
+template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_integrals_kernel(P3Args<R> a, int with_pot,
                                                                  const double *fp_val,
                                                                  const double *fp_der, int nfp,
@@ -535,7 +616,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_integrals_kernel(P3Args<R> a, in
             R vx = p.w, vy = q2.x, vz = q2.y;
             if (centred) {
                 R acc[3];
-                getacc3<R>(a, p.x, p.y, p.z, ut.x, ut.y, z, acc);
+                getacc3<R, J>(a, p.x, p.y, p.z, ut.x, ut.y, z, acc);
                 vx = vx + (R)0.5 * acc[0];
                 vy = vy + (R)0.5 * acc[1];
                 vz = vz + (R)0.5 * acc[2];
@@ -554,13 +635,14 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_integrals_kernel(P3Args<R> a, in
             v[7] = ux * ux + uy * uy + uz * uz;
             v[8] = x * uy - y * ux;
             if (with_pot && z <= a.nz) {
-                R w[8];
-                int ic[8];
-                const int c = cell3<R>(ut.x, ut.y, p.z, a, false, w);
-                corners3(c, a.na, a.ngz, ic);
+                constexpr int NW = P3NW<J>::v;
+                R w[NW];
+                int ic[NW];
+                const int c = cellJ<R, J>(ut.x, ut.y, p.z, a, false, w);
+                cornersJ<J>(c, a.na, a.ngz, ic);
                 double ph = 0.0;
-                for (int k = 0; k < 8; k++)
-                    if (ic[k] < a.mesh) ph += (double)w[k] * (double)a.fld[ic[k]].w;
+                for (int k = 0; k < NW; k++)
+                    if (ic[k] >= 0 && ic[k] < a.mesh) ph += (double)w[k] * (double)a.fld[ic[k]].w;
                 v[9] = ph;
             }
             if (nfp > 0)
@@ -602,7 +684,7 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_integrals_sum_kernel(const double
 
 // Sort keys: the particle's cell shifted right by kshift (kmax for the
 // off-grid list, which sorts last), with the identity permutation.
-template <typename R>
+template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_key_kernel(P3Args<R> a, int kshift,
                                                           unsigned kmax, unsigned *key,
                                                           unsigned *perm) {
@@ -613,8 +695,8 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_key_kernel(P3Args<R> a, int kshi
     if (z >= 1 && z <= a.nz) {
         const typename Vec4<R>::T p = a.xv[i];
         const typename Vec2<R>::T ut = a.ut[i];
-        R w[8];
-        k = (unsigned)(cell3<R>(ut.x, ut.y, p.z, a, false, w) >> kshift);
+        R w[P3NW<J>::v];
+        k = (unsigned)(cellJ<R, J>(ut.x, ut.y, p.z, a, false, w) >> kshift);
     }
     key[i] = k;
     perm[i] = (unsigned)i;
@@ -1004,13 +1086,19 @@ static unsigned p3_blocks(long long n) { return (unsigned)((n + P3_PBLOCK - 1) /
 
 template <typename R> hipError_t launch_p3_massgn(const P3Args<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_massgn_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
+    if (a.jmass == 3)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_massgn_kernel<R, 3>), dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_massgn_kernel<R, 2>), dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename R> hipError_t launch_p3_push(const P3Args<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_push_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
+    if (a.jmass == 3)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_push_kernel<R, 3>), dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_push_kernel<R, 2>), dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1022,13 +1110,19 @@ template <typename R> hipError_t launch_p3_ut(const P3Args<R> &a, hipStream_t s)
 
 template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_tmcent_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
+    if (a.jmass == 3)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_tmcent_kernel<R, 3>), dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_tmcent_kernel<R, 2>), dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename R> hipError_t launch_p3_gather(const P3GatherArgs<R> &b, hipStream_t s) {
     if (b.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_gather_kernel<R>, dim3(p3_blocks(b.n)), dim3(P3_PBLOCK), 0, s, b);
+    if (b.g.jmass == 3)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_gather_kernel<R, 3>), dim3(p3_blocks(b.n)), dim3(P3_PBLOCK), 0, s, b);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_gather_kernel<R, 2>), dim3(p3_blocks(b.n)), dim3(P3_PBLOCK), 0, s, b);
     return hipGetLastError();
 }
 
@@ -1036,8 +1130,12 @@ template <typename R>
 hipError_t launch_p3_key(const P3Args<R> &a, int kshift, unsigned kmax, unsigned *key,
                          unsigned *perm, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_key_kernel<R>, dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a,
-                       kshift, kmax, key, perm);
+    if (a.jmass == 3)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_key_kernel<R, 3>), dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a,
+                           kshift, kmax, key, perm);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_key_kernel<R, 2>), dim3(p3_blocks(a.n)), dim3(P3_PBLOCK), 0, s, a,
+                           kshift, kmax, key, perm);
     return hipGetLastError();
 }
 
@@ -1052,9 +1150,12 @@ hipError_t launch_p3_integrals(const P3Args<R> &a, int with_pot, const double *f
                                const double *fp_der, int nfp, double fp_dr, int centred,
                                double *part, double *out, hipStream_t s) {
     const unsigned nb = a.n > 0 ? p3_blocks(a.n) : 0;
-    if (nb)
-        hipLaunchKernelGGL(p3_integrals_kernel<R>, dim3(nb), dim3(P3_PBLOCK), 0, s, a, with_pot,
-                           fp_val, fp_der, nfp, fp_dr, centred, part);
+    if (nb && a.jmass == 3)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_integrals_kernel<R, 3>), dim3(nb), dim3(P3_PBLOCK), 0, s, a,
+                           with_pot, fp_val, fp_der, nfp, fp_dr, centred, part);
+    else if (nb)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_integrals_kernel<R, 2>), dim3(nb), dim3(P3_PBLOCK), 0, s, a,
+                           with_pot, fp_val, fp_der, nfp, fp_dr, centred, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(p3_integrals_sum_kernel, dim3(1), dim3(P3_BLOCK), 0, s, part, (int)nb,

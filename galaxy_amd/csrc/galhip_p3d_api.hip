@@ -36,8 +36,12 @@ extern "C" int galhip_p3dset(const galhip_p3d_params *p, galhip_ctx **out) {
                        "P3DSET: Softening length cannot be less than z-spacing of grid planes");
     if (p->tsoft != 1 && p->tsoft != 2) return gh_fail(GALHIP_EINVAL, "Unrecognized softening rule");
     if (p->nskip < 0 || p->nskip > GALHIP_MAXSKIP) return gh_fail(GALHIP_EINVAL, "nskip out of range");
+    const int jmass = p->jmass == 0 ? 2 : p->jmass;
+    if (jmass != 2 && jmass != 3)   // p3dset.f:80-83
+        return gh_fail(GALHIP_EINVAL, "P3DSET: Interpolation scheme must be 2 or 3");
     galhip_ctx *c = new galhip_ctx();
     c->p3d = 1;
+    c->jmass = jmass;
     c->prec = p->precision;
     c->nr = p->nr;
     c->na = na;
@@ -82,18 +86,20 @@ extern "C" int galhip_p3dset(const galhip_p3d_params *p, galhip_ctx **out) {
     if (c->prec == GALHIP_FP32) {
         const float a = (float)(two_pi / (double)na);
         c->alpha = a;
-        c->rgrid = expf(a * (float)(c->nr - 1)) - 1.0f;
+        // rgrid = grofu(nr - jmass/2), zm = dzg (ngz + 1 - jmass)/2
+        // (setgrd.f:135-136,146-150)
+        c->rgrid = expf(a * ((float)c->nr - 0.5f * (float)jmass)) - 1.0f;
         c->thmax = (float)na / (float)c->nsect;
         const float dz = (float)p->dzg;
         c->dzg = dz;
-        c->zm = (0.5f * dz) * (float)(c->ngz - 1);      // setgrd.f:146-150
+        c->zm = (0.5f * dz) * (float)(c->ngz + 1 - jmass);
         aw = (double)((float)c->rgrid - (expf(a * (float)(c->nr - 2)) - 1.0f));
     } else {
         c->alpha = two_pi / (double)na;
-        c->rgrid = exp(c->alpha * (double)(c->nr - 1)) - 1.0;
+        c->rgrid = exp(c->alpha * ((double)c->nr - 0.5 * (double)jmass)) - 1.0;
         c->thmax = (double)na / (double)c->nsect;
         c->dzg = p->dzg;
-        c->zm = 0.5 * p->dzg * (double)(c->ngz - 1);
+        c->zm = 0.5 * p->dzg * (double)(c->ngz + 1 - jmass);
         aw = c->rgrid - (exp(c->alpha * (double)(c->nr - 2)) - 1.0);
     }
     if (aw / c->softl > 5.0) {   // setgrd.f:181-199
@@ -428,6 +434,7 @@ template <typename R> static P3Args<R> p3_args(galhip_ctx *c) {
     a.fixforce = c->nfr > 0;
     a.nr = c->nr; a.na = c->na; a.ngz = c->ngz; a.mesh = c->mesh; a.nz = c->nz;
     a.mzone = c->mzone; a.offanal = c->offanal; a.nsect = c->nsect; a.qbits = c->qbits;
+    a.jmass = c->jmass;
     a.k.alpha_d = c->alpha;
     a.k.inv_alpha = 1.0 / c->alpha;
     a.k.tscale = (double)c->na / (2.0 * 3.141592653589793238462643);
@@ -602,7 +609,8 @@ static int p3_gather_impl(galhip_ctx *c, int64_t n, const void *x, const void *y
     int *dcell = nullptr;
     int rc = dalloc(&dx, es * n) | dalloc(&dy, es * n) | dalloc(&dz, es * n);
     if (cell) rc |= dalloc(&dcell, sizeof(int) * n);
-    if (wt) rc |= dalloc(&dwt, 8 * es * n);
+    const int nw = c->jmass == 3 ? 27 : 8;
+    if (wt) rc |= dalloc(&dwt, nw * es * n);
     if (acc) rc |= dalloc(&dacc, 3 * es * n);
     hipError_t e = hipSuccess;
     if (!rc) {
@@ -618,7 +626,7 @@ static int p3_gather_impl(galhip_ctx *c, int64_t n, const void *x, const void *y
         if (e == hipSuccess) e = launch_p3_gather<R>(b, c->s);
         if (e == hipSuccess) e = hipStreamSynchronize(c->s);
         if (e == hipSuccess && cell) e = hipMemcpy(cell, dcell, sizeof(int) * n, hipMemcpyDeviceToHost);
-        if (e == hipSuccess && wt) e = hipMemcpy(wt, dwt, 8 * es * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && wt) e = hipMemcpy(wt, dwt, nw * es * n, hipMemcpyDeviceToHost);
         if (e == hipSuccess && acc) e = hipMemcpy(acc, dacc, 3 * es * n, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = gh_fail(GALHIP_EHIP, "p3d gather: %s", hipGetErrorString(e));
     }

@@ -280,7 +280,7 @@ class P3DGrid(P2DGrid):
 
     def __init__(self, nr, na, ngz, dzg, hash_, softl, tsoft=1, nsect=1, skip=(),
                  nzones=1, nstep=(1,), zbr=(0.0,), offanal=True, pmass=1.0,
-                 precision="fp32"):
+                 precision="fp32", jmass=2):
         lib = load()
         p = P3DParams()
         p.nr, p.na, p.ngz, p.hash, p.nsect = nr, na, ngz, hash_, nsect
@@ -299,11 +299,14 @@ class P3DGrid(P2DGrid):
             p.zbr[i] = float(zbr[i])
         p.offanal = int(bool(offanal))
         p.pmass = float(pmass)
+        p.jmass = int(jmass)
         if precision not in ("fp32", "fp64"):
             raise ValueError("precision must be 'fp32' or 'fp64'")
         p.precision = GALHIP_FP32 if precision == "fp32" else GALHIP_FP64
         self.dtype = np.float32 if precision == "fp32" else np.float64
         self.precision = precision
+        self.jmass = int(jmass)
+        self.nw = 27 if jmass == 3 else 8
         h = C.c_void_p()
         check(lib.galhip_p3dset(C.byref(p), C.byref(h)), "p3dset")
         self._lib, self._h = lib, h
@@ -313,7 +316,8 @@ class P3DGrid(P2DGrid):
         self.ng, self.mmax, self.mesh = int(g[3]), int(g[4]), int(g[5])
         self.nr, self.na, self.ngz, self.nsect, self.nzones = nr, 2 * (na // 2), ngz, nsect, nzones
         self.dzg = float(self.dtype(dzg))
-        self.zm = float(self.dtype(self.dtype(0.5) * self.dtype(dzg)) * self.dtype(ngz - 1))
+        self.zm = float(self.dtype(self.dtype(0.5) * self.dtype(dzg)) *
+                        self.dtype(ngz + 1 - self.jmass))
         self.n = 0
 
     def get_green(self):
@@ -348,7 +352,7 @@ class P3DGrid(P2DGrid):
         x, y, z = (np.ascontiguousarray(a, self.dtype) for a in (x, y, z))
         n = len(x)
         cell = np.empty(n, np.int32)
-        wt = np.empty((n, 8), self.dtype)
+        wt = np.empty((n, self.nw), self.dtype)
         self._c(self._lib.galhip_weight3(self._h, n, _ptr(x), _ptr(y), _ptr(z),
                                          int(bool(old)), _ptr(cell), _ptr(wt)), "weight3")
         return cell, wt

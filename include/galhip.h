@@ -232,7 +232,7 @@ int galhip_sync(galhip_ctx *ctx);
  * galhip_unload3; galhip_restrt, galhip_unload, galhip_weight and
  * galhip_grdacc return GALHIP_EINVAL on a p3d context.  Zone boundaries are spherical radii (SRC/lib15/setrun.f:810,
  * zoneno.f:40-41); a particle is off the grid when |z| > zm or R > rgrid
- * (SRC/lib15/offgrd.f:69-72), zm = dzg (ngz - 1) / 2. */
+ * (SRC/lib15/offgrd.f:69-72). */
 typedef struct {
     int32_t nr, na, ngz;       /* GRID card: rings, spokes, planes */
     int32_t hash;              /* HASH card */
@@ -248,10 +248,13 @@ typedef struct {
     int32_t offanal;
     double pmass;
     int32_t precision;         /* GALHIP_FP32 or GALHIP_FP64 */
+    int32_t jmass;             /* INTE card: 2 linear (default; 0 reads as 2),
+                                  3 quadratic, 27 cells (weight.f:251-300) */
 } galhip_p3d_params;
 
-/* p3dset + setgrd + filset + setspc for one p3d grid with linear
- * interpolation (jmass = 2).  Replaces SRC/lib15/p3dset.f:1,
+/* p3dset + setgrd + filset + setspc for one p3d grid, linear or quadratic
+ * interpolation (jmass 2 or 3: rgrid = grofu(nr - jmass/2), zm = dzg (ngz +
+ * 1 - jmass)/2, setgrd.f:135-150).  Replaces SRC/lib15/p3dset.f:1,
  * SRC/lib15/setgrd.f:1, SRC/lib15/filset.f:1, SRC/lib15/setspc.f:1. */
 int galhip_p3dset(const galhip_p3d_params *p, galhip_ctx **out);
 /* restrt / unload (particle part) for p3d: positions and velocities in
@@ -262,8 +265,9 @@ int galhip_restrt3(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
                    const int32_t *iz);
 int galhip_unload3(galhip_ctx *ctx, void *x, void *y, void *z, void *vx,
                    void *vy, void *vz, int32_t *iz);
-/* weight(jst, old) for n p3d positions (SRC/lib15/weight.f:195-250):
- * 0-based cells and wt[n][8] in the reference's corner order. */
+/* weight(jst, old) for n p3d positions (SRC/lib15/weight.f:195-300):
+ * 0-based cells (the central cell for jmass 3) and wt[n][8] (jmass 3:
+ * wt[n][27]) in the reference's order. */
 int galhip_weight3(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
                    const void *z, int32_t old, int32_t *cell, void *wt);
 /* grdacc(jst) for n p3d positions (SRC/lib15/grdacc.f:189-211): acc[n][3]

@@ -38,7 +38,7 @@ module galhip_f
      real(c_double) :: zbr(GALHIP_MAXZONES)
      integer(c_int32_t) :: offanal
      real(c_double) :: pmass
-     integer(c_int32_t) :: precision
+     integer(c_int32_t) :: precision, jmass
   end type galhip_p3d_params
 
   interface

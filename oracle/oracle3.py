@@ -29,7 +29,7 @@ class Cfg3(C.Structure):
     _fields_ = [
         ("nr", C.c_int), ("na", C.c_int), ("ngz", C.c_int), ("nsect", C.c_int),
         ("nzones", C.c_int), ("mzone", C.c_int), ("offanal", C.c_int),
-        ("fixforce", C.c_int),
+        ("fixforce", C.c_int), ("jmass", C.c_int),
         ("nstep", C.c_int * MAXZ),
         ("zbr", C.c_double * MAXZ),
         ("alpha", C.c_double), ("rgrid", C.c_double), ("thmax", C.c_double),
@@ -101,11 +101,27 @@ def direct_field3(nr, na, ngz, type_, D, rho):
     return out
 
 
-def zmax(dzg, ngz, prec="d"):
-    """zm = dzg (ngz + 1 - jmass) / 2 for jmass = 2 (setgrd.f:146-150)."""
+def zmax(dzg, ngz, prec="d", jmass=2):
+    """zm = dzg (ngz + 1 - jmass) / 2 (setgrd.f:146-150)."""
     if prec == "f":
-        return float(np.float32(np.float32(0.5) * np.float32(dzg)) * np.float32(ngz - 1))
-    return 0.5 * dzg * float(ngz - 1)
+        return float(np.float32(np.float32(0.5) * np.float32(dzg)) *
+                     np.float32(ngz + 1 - jmass))
+    return 0.5 * dzg * float(ngz + 1 - jmass)
+
+
+def corners27(cell, na, ngz):
+    """The 27 cells about each central cell (massgn.f:182-195), spoke offset
+    fastest, then plane, then ring, as flat indices."""
+    c = np.asarray(cell, np.int64)
+    m0 = np.where(c % na == 0, c - 1 + na, c - 1)
+    m2 = np.where((c + 1) % na == 0, c + 1 - na, c + 1)
+    m = np.stack([m0, c, m2], 1)
+    out = np.empty((len(c), 27), np.int64)
+    for k in range(3):
+        for j in range(3):
+            for i in range(3):
+                out[:, 9 * k + 3 * j + i] = m[:, i] + (j - 1) * na + (k - 1) * ngz * na
+    return out
 
 
 class P3D(_o.P2D):
@@ -115,7 +131,8 @@ class P3D(_o.P2D):
 
     def __init__(self, nr, na, ngz, dzg, hash_, softl, tsoft=1, nsect=1, skip=(),
                  nzones=1, nstep=(1,), zbr=(0.0,), pmass=1.0, offanal=True,
-                 fixforce=None, prec="d", use_fx=True, qbits=None, green_tables=None):
+                 fixforce=None, prec="d", use_fx=True, qbits=None, green_tables=None,
+                 jmass=2):
         self.nr, self.na, self.ngz, self.nsect = nr, na, ngz, nsect
         self.mesh = nr * ngz * na
         self.mmax = na // 2 + 1
@@ -123,10 +140,14 @@ class P3D(_o.P2D):
         self.prec = prec
         self.dt = np.float32 if prec == "f" else np.float64
         self.alpha = _o.alpha(na, prec)
-        self.rgrid = _o.grofu(self.dt(nr - 1), self.alpha, prec)
+        self.jmass = jmass
+        self.nw = 27 if jmass == 3 else 8
+        # rgrid = grofu(nr - jmass/2) (setgrd.f:135-136)
+        self.rgrid = _o.grofu(self.dt(self.dt(nr) - self.dt(0.5) * self.dt(jmass)),
+                              self.alpha, prec)
         self.thmax = self.dt(na) / self.dt(nsect)
         self.dzg = float(self.dt(dzg))
-        self.zm = zmax(dzg, ngz, prec)
+        self.zm = zmax(dzg, ngz, prec, jmass)
         self.softl, self.tsoft = softl, tsoft
         self.lg = filter_flags(na, self.ng, nsect, skip)
         self.nzones = nzones
@@ -161,6 +182,7 @@ class P3D(_o.P2D):
         c.nr, c.na, c.ngz, c.nsect = self.nr, self.na, self.ngz, self.nsect
         c.nzones, c.mzone = self.nzones, self.mzone
         c.offanal = int(bool(self.offanal))
+        c.jmass = self.jmass
         for i in range(self.nzones):
             c.nstep[i] = self.nstep[i]
             c.zbr[i] = self.zbr[i]
@@ -194,7 +216,7 @@ class P3D(_o.P2D):
         x, y, z = (np.ascontiguousarray(a, dt) for a in (x, y, z))
         n = len(x)
         cell = np.empty(n, np.int32)
-        wt = np.empty((n, 8), dt)
+        wt = np.empty((n, self.nw), dt)
         rr = np.empty(n, dt)
         c = self.cfg()
         self._fn("orc3_weight")(C.byref(c), n, x, y, z, int(fold), cell, wt, rr)
@@ -288,12 +310,15 @@ class P3D(_o.P2D):
         grid = (iz >= 1) & (iz <= self.nzones)
         cell, w, _ = self.weight(self.x[grid], self.y[grid], self.z[grid])
         na, S = self.na, self.ngz * self.na
-        wrap = (cell + 1) % na == 0
-        c1 = np.where(wrap, cell + 1 - na, cell + 1)
-        ic = np.stack([cell, c1, cell + na, c1 + na, cell + S, c1 + S, cell + S + na,
-                       c1 + S + na], 1)
+        if self.jmass == 3:
+            ic = corners27(cell, na, self.ngz)
+        else:
+            wrap = (cell + 1) % na == 0
+            c1 = np.where(wrap, cell + 1 - na, cell + 1)
+            ic = np.stack([cell, c1, cell + na, c1 + na, cell + S, c1 + S, cell + S + na,
+                           c1 + S + na], 1)
         pot = np.append(self.fld[3].astype(np.float64), 0.0)
-        ic = np.where(ic < self.mesh, ic, self.mesh)
+        ic = np.where((ic >= 0) & (ic < self.mesh), ic, self.mesh)
         out[9] = pm * (w.astype(np.float64) * pot[ic]).sum()
         if fixpot is not None:
             dr, val, der = fixpot

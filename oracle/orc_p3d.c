@@ -31,6 +31,24 @@ static inline void corners3(int32_t c, int na, int ngz, int32_t ic[8])
     ic[7] = ic[5] + na;
 }
 
+/* The 27 cells of a quadratic-interpolation particle about its central
+ * cell c (massgn.f:182-195, grdacc.f:262-275): spoke offsets -1, 0, +1
+ * fastest (with the azimuthal wrap), then plane -1, 0, +1 (+-na), then ring
+ * -1, 0, +1 (+-ngz na), as flat indices. */
+static inline void corners27(int32_t c, int na, int ngz, int32_t ic[27])
+{
+    int32_t m[3];
+    m[1] = c;
+    m[2] = c + 1;
+    m[0] = c - 1;
+    if ((c + 1) % na == 0) m[2] -= na;
+    if (c % na == 0) m[0] += na;
+    for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++)
+                ic[9 * k + 3 * j + i] = m[i] + (j - 1) * na + (k - 1) * ngz * na;
+}
+
 #define REAL float
 #define SFX(name) name##_f
 #define RSQRT sqrtf

@@ -23,8 +23,12 @@
 
 #define ORC3_MAXZ 10
 
+/* jmass: 2 = linear (8 corners, weight.f:195-250), 3 = quadratic (27
+ * corners, weight.f:251-300); ORC3_NW(j) weights per particle. */
+#define ORC3_NW(j) ((j) == 3 ? 27 : 8)
+
 typedef struct {
-    int nr, na, ngz, nsect, nzones, mzone, offanal, fixforce;
+    int nr, na, ngz, nsect, nzones, mzone, offanal, fixforce, jmass;
     int nstep[ORC3_MAXZ];
     double zbr[ORC3_MAXZ];   /* zone inner radii (spherical, grid units) */
     double alpha, rgrid, thmax, zm, dzg;
@@ -32,8 +36,9 @@ typedef struct {
 } orc3_cfg;
 
 /* ---- weight, p3d linear interpolation (SRC/lib15/weight.f:195-250) ----
- * Returns 0-based cells (ncl - 1), 8 weights per particle in the
- * reference's order (dt, dz, du bits), cylindrical radii. */
+ * Returns 0-based cells (ncl - 1; for jmass 3 the central cell of the 27),
+ * ORC3_NW(jmass) weights per particle in the reference's order (spoke
+ * fastest, then plane, then ring), cylindrical radii. */
 void orc3_weight_f(const orc3_cfg *c, int n, const float *x, const float *y,
                    const float *z, int fold, int32_t *cell, float *wt, float *rr);
 void orc3_weight_d(const orc3_cfg *c, int n, const double *x, const double *y,

@@ -71,6 +71,9 @@ contains
     call reset3(q)
     q%precision = 3
     call try3(q, 'p3precision')
+    call reset3(q)
+    q%jmass = 4
+    call try3(q, 'p3jmass')
   end subroutine p3checks
   subroutine reset3(q)
     type(galhip_p3d_params), intent(out) :: q
@@ -90,6 +93,7 @@ contains
     q%offanal = 1
     q%pmass = 1.0d0
     q%precision = GALHIP_FP64
+    q%jmass = 2
   end subroutine reset3
   subroutine try3(q, what)
     type(galhip_p3d_params), intent(in) :: q
@@ -182,6 +186,7 @@ def test_fortran_interfaces_match_c_abi(tmp_path):
         "p3nzones": (-1, "nzones 0 out of range"),
         "p3tsoft": (-1, "Unrecognized softening rule"),
         "p3precision": (-1, "precision must be"),
+        "p3jmass": (-1, "P3DSET: Interpolation scheme must be 2 or 3"),
     }
     for k, (code, msg) in expect3.items():
         rc, text = got[k]

@@ -44,7 +44,8 @@ def models(o3, prec, op, w=P3SMALL, n=None, scale=1.0, fixed=True, **over):
     m = o3.P3D(kw["nr"], kw["na"], kw["ngz"], kw["dzg"], kw["hash_"], kw["softl"],
                tsoft=kw["tsoft"], nsect=kw["nsect"], skip=kw.get("skip", ()),
                nzones=kw["nzones"], nstep=kw["nstep"], zbr=kw["zbr"], pmass=kw["pmass"],
-               offanal=kw["offanal"], prec=op, green_tables=g.get_green())
+               offanal=kw["offanal"], prec=op, green_tables=g.get_green(),
+               jmass=kw.get("jmass", 2))
     if fixed:
         rmax = float(np.hypot(g.rgrid, g.zm))
         ff = configs.fixforce(w, rmax)
@@ -96,9 +97,10 @@ def edge_positions3(g, rng, n=20000):
 
 @pytest.mark.parametrize("prec,op", PRECS)
 @pytest.mark.parametrize("old", [True, False])
-def test_weight3(o3, prec, op, old):
-    _say("p3d weight", prec, "old" if old else "new")
-    m, g = models(o3, prec, op, n=100, nsect=2)
+@pytest.mark.parametrize("jmass", [2, 3])
+def test_weight3(o3, prec, op, old, jmass):
+    _say("p3d weight", prec, "old" if old else "new", "jmass", jmass)
+    m, g = models(o3, prec, op, n=100, nsect=2, jmass=jmass)
     dt = g.dtype
     x, y, z = (a.astype(dt) for a in edge_positions3(g, np.random.default_rng(7)))
     cell, wt = g.weight(x, y, z, old=old)
@@ -116,6 +118,7 @@ def test_weight3(o3, prec, op, old):
         cell, oc = cell[~tie], oc[~tie]
     _same(np.bincount(cell, minlength=g.mesh + 2 * g.na),
           np.bincount(oc, minlength=g.mesh + 2 * g.na), "counts")
+    assert wt.shape[1] == (27 if jmass == 3 else 8)
 
 
 @pytest.mark.parametrize("tsoft", [1, 2])
@@ -361,3 +364,38 @@ def test_integrals3_match_oracle(o3, prec, op):
     assert np.abs(vec[:9] - ref[:9]).max() <= tol_kin * scale, (vec[:9], ref[:9])
     assert abs(vec[9] - ref[9]) <= (1e-9 if prec == "fp64" else 2e-5) * abs(ref[9])
     assert abs(vec[10] - ref[10]) <= (1e-10 if prec == "fp64" else 1e-6) * abs(ref[10])
+
+
+@pytest.mark.parametrize("prec,op", PRECS)
+def test_step_chain3_quadratic(o3, prec, op):
+    """Quadratic interpolation (jmass 3, 27 cells: weight.f:251-300,
+    massgn.f:172-205, grdacc.f:259-289): the six-step chain through the
+    two-zone cycle, FP32 bit-exact, FP64 to 1e-13."""
+    _say("p3d step chain jmass 3", prec)
+    m, g = models(o3, prec, op, scale=1.3, jmass=3)
+    chain3(m, g, 6, prec == "fp32")
+
+
+def test_tmcent_integrals_quadratic(o3):
+    """tmcent and the integrals with jmass 3 (FP32 tmcent bit-exact)."""
+    from galaxy_amd import disc
+    _say("p3d tmcent + integrals jmass 3")
+    m, g = models(o3, "fp32", "f", jmass=3)
+    fp = disc.halo_potential_table(float(np.hypot(g.rgrid, g.zm)), P3SMALL.lscale(), P3SMALL.dt)
+    g.set_fixpot(*fp)
+    m.masset()
+    g.masset()
+    m.findf(lpot=True)
+    fld = m.fld.astype(np.float32)
+    g.set_field(fld)
+    g.tmcent(True)
+    m.tmcent(True, field=fld[:3])
+    out = g.unload()
+    for key in ("vx", "vy", "vz"):
+        _same(out[key], getattr(m, key), key)
+    g.findf(True)
+    m.findf(lpot=True)
+    ref = m.integrals(fixpot=fp, centred=True)
+    got = g.integrals(centred=True)
+    assert abs(got["w_grid"] - ref[9]) <= 2e-5 * abs(ref[9])
+    assert abs(2 * got["ke"] - ref[7]) <= 1e-6 * abs(ref[7])

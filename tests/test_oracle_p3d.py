@@ -265,3 +265,56 @@ def test_push_fp32_tracks_fp64():
             m.step()
     assert np.max(np.abs(mf.x - md.x)) < 1e-4
     assert np.max(np.abs(mf.z - md.z)) < 1e-4
+
+
+def test_quadratic_weights():
+    """jmass 3 (weight.f:251-300): the 27 weights are a partition of unity,
+    non-negative, and interpolate linear functions of (u, t, zg) exactly
+    about the central cell (the TSC property)."""
+    m = _model(jmass=3)
+    rng = np.random.default_rng(17)
+    n = 3000
+    R = rng.uniform(0.3, 0.9 * m.rgrid, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    z = rng.uniform(-0.9 * m.zm, 0.9 * m.zm, n)
+    x, y = R * np.cos(ph), R * np.sin(ph)
+    cell, w, rr = m.weight(x, y, z)
+    assert w.shape == (n, 27)
+    assert np.allclose(w.sum(1), 1.0, atol=1e-14)
+    assert np.all(w >= -1e-15)
+    na, ngz = m.na, m.ngz
+    u = np.log1p(rr) / m.alpha
+    t = np.mod(np.arctan2(y, x) * na / (2 * np.pi), na)
+    zg = (z + m.zm) / m.dzg
+    ju, rem = cell // (ngz * na), cell % (ngz * na)
+    jz, jt = rem // na, rem % na
+    off = np.array([-1, 0, 1])
+    ct = jt[:, None] + np.tile(off, 9)[None, :]
+    cz = jz[:, None] + np.tile(np.repeat(off, 3), 3)[None, :]
+    cu = ju[:, None] + np.repeat(off, 9)[None, :]
+    # unwrap the angle of cells near spoke 0 against t
+    ct = np.where(ct - t[:, None] > na / 2, ct - na, ct)
+    ct = np.where(t[:, None] - ct > na / 2, ct + na, ct)
+    f = lambda a, b, c: 0.5 - 1.25 * a + 0.5 * b + 2.0 * c
+    assert np.allclose((w * f(cu, ct, cz)).sum(1), f(u, t, zg), atol=1e-9)
+
+
+def test_quadratic_massgn_and_geometry():
+    """jmass 3: rgrid = grofu(nr - 1.5), zm = dzg (ngz - 2)/2
+    (setgrd.f:135-150); ring ranges j..j+2 about the central ring
+    (massgn.f:176-179); mass conserved."""
+    m = _model(jmass=3, prec="d")
+    assert abs(m.rgrid - (np.exp(m.alpha * (m.nr - 1.5)) - 1.0)) < 1e-12
+    assert abs(m.zm - 0.5 * m.dzg * (m.ngz - 2)) < 1e-15
+    rng = np.random.default_rng(21)
+    n = 2000
+    R = rng.uniform(0.5, 0.9 * m.rgrid, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    z = rng.uniform(-0.8 * m.zm, 0.8 * m.zm, n)
+    x, y = R * np.cos(ph), R * np.sin(ph)
+    m.load(x, y, z, 0 * x, 0 * x, 0 * x)
+    m.masset()
+    assert abs(m.mfx[0].sum() * 2.0 ** -m.qbits - n) < 1e-9
+    cell, _, _ = m.weight(x, y, z)
+    ring = cell // (m.ngz * m.na)
+    assert m.jrad[0] == ring.min() and m.krad[0] == ring.max() + 2
