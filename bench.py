@@ -22,6 +22,12 @@ stream, against 8 TB/s; traffic = measured HBM bytes per launch (per-push
 bytes from rocprofv3 counters times the pushes per launch).  cpu_baseline: the oracle's FP32 restatement on one
 host core over a bounded sample of the same workload.
 
+--workload C4 measures the ensemble of 64 independent discs on 128 x 256
+grids (N_i log-uniform in [1e5, 4e6], Q_i in [1, 2], 300 steps) advanced by
+one launch per kernel, the timed steps replayed as HIP graphs: pushes over
+all members per second, push-kernel roofline at 32 B per push, the oracle
+on the largest member's disc as the CPU baseline.
+
 --workload C3 measures the 3-D polar grid instead (200 x 256 x 125 grid,
 N = 5e7, m <= 8, sech^2 vertical profile, 200 steps by default) and prints
 its own line: push-kernel roofline at 48 B per push ((x, y, z, vx) and
@@ -59,6 +65,10 @@ PUSH_VALU_PER_PUSH = 521.0
 # scaled by the same copy calibration)
 P3_PUSH_HBM_READ_B = 35.9
 P3_PUSH_HBM_WRITE_B = 44.8
+# the same for push_kernel<float, true> on the 64-disc ensemble (None until
+# measured)
+ENS_PUSH_HBM_READ_B = None
+ENS_PUSH_HBM_WRITE_B = None
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 
@@ -77,7 +87,7 @@ def a2(x, y, lscale, r1=1.0, r2=3.0):
     return complex(c)
 
 
-def run_gpu(w, precision, n, steps, warmup, sort_interval):
+def run_gpu(w, precision, n, steps, warmup, sort_interval, graph=0):
     """Wall-clock rate over `steps` steps without events on the stream, then
     the same number of further steps with HIP events around every push
     launch (the push kernel's own time for the roofline)."""
@@ -85,6 +95,7 @@ def run_gpu(w, precision, n, steps, warmup, sort_interval):
     t0 = time.time()
     g = configs.make_grid(w, precision=precision, n=n)
     g.set_sort_interval(sort_interval)
+    g.set_graph(graph)
     log("[bench] %s set-up %.1f s (N=%d)" % (precision, time.time() - t0, n))
     if warmup:
         g.run(warmup)
@@ -257,6 +268,105 @@ def main_p3d(args):
     print(json.dumps(line), flush=True)
 
 
+def run_gpu_ens(w, precision, scale, steps, warmup, sort_interval, graph, parts):
+    """The ensemble workload: timed run (HIP graphs when graph is set), then
+    a profiled pass of the same length with events around every push."""
+    from galaxy_amd import configs
+    t0 = time.time()
+    g = configs.make_ensemble(w, precision=precision, scale=scale, parts=parts)
+    g.set_sort_interval(sort_interval)
+    g.set_graph(graph)
+    log("[bench] ensemble %s set-up %.1f s (%d members, N=%d)"
+        % (precision, time.time() - t0, g.nmem, g.n))
+    if warmup:
+        g.run(warmup)
+    g.findf(True)
+    e0 = [d["energy"] for d in g.integrals()]
+    g.sync()
+    g.set_profile(False)
+    t0 = time.perf_counter()
+    g.run(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    _, _, pushes = g.get_profile()
+    g.set_profile(True)
+    g.run(steps)
+    g.sync()
+    push_ms, launches, pushes_p = g.get_profile()
+    g.set_profile(False)
+    g.findf(True)
+    e1 = [d["energy"] for d in g.integrals()]
+    members = g.unload()
+    a2s = [a2(m["x"], m["y"], w.lscale()) for m in members]
+    res = dict(seconds=t1 - t0, pushes=pushes, push_ms=push_ms, launches=launches,
+               pushes_prof=pushes_p, a2=a2s, e0=e0, e1=e1, n=g.n, nmem=g.nmem)
+    log("[bench] ensemble %s: %d steps in %.4f s, %d pushes; profiled pass: push kernel "
+        "%.3f ms over %d launches" % (precision, steps, t1 - t0, pushes, push_ms, launches))
+    g.close()
+    return res
+
+
+def main_ens(args):
+    """--workload C4: 64 discs on 128 x 256 grids in one ensemble."""
+    from galaxy_amd import configs, lib15
+    w = configs.C4
+    scale = 1.0 if not args.n else args.n / w.n
+    counts, _ = w.members(scale)
+    t0 = time.time()
+    rgrid = lib15.P2DGrid(precision="fp32", **configs.grid_kwargs(w, 1)).rgrid
+    parts = configs.ensemble_particles(w, counts, rgrid)
+    log("[bench] ensemble particles drawn in %.1f s" % (time.time() - t0))
+    mixed = run_gpu_ens(w, "fp32", scale, args.steps, args.warmup, args.sort_interval,
+                        args.graph, parts)
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_gpu_ens(w, "fp64", scale, args.steps, args.warmup, args.sort_interval,
+                           args.graph, parts)
+    del parts
+    value = mixed["pushes"] / mixed["seconds"]
+    bytes_per_push = 32.0
+    achieved = mixed["pushes_prof"] * bytes_per_push / (mixed["push_ms"] * 1e-3) / 1e9
+    per_launch = mixed["pushes_prof"] / max(mixed["launches"], 1)
+    traffic = None
+    if ENS_PUSH_HBM_READ_B is not None:
+        traffic = round((ENS_PUSH_HBM_READ_B + ENS_PUSH_HBM_WRITE_B) * per_launch)
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel="push_kernel<float, true>", bytes_per_push=bytes_per_push,
+                avg_launch_us=round(1e3 * mixed["push_ms"] / max(mixed["launches"], 1), 2))
+    cpu = None
+    if not args.no_cpu:
+        i = int(np.argmax(counts))
+        cpu = cpu_baseline(w.member(i, counts[i]), args.cpu_n, args.cpu_steps)
+        cpu["sample"] = "member %d's disc: " % i + cpu["sample"]
+        log("[bench] ensemble cpu baseline: %.3g pushes/s (%s)" % (cpu["value"], cpu["sample"]))
+    extra = dict(members=mixed["nmem"], n_total=mixed["n"], graph=bool(args.graph),
+                 sort_interval=args.sort_interval,
+                 push_share=round(mixed["push_ms"] * 1e-3 / mixed["seconds"], 3))
+    if fp64:
+        v64 = fp64["pushes"] / fp64["seconds"]
+        d_m = [(b - a) / abs(a) for a, b in zip(mixed["e0"], mixed["e1"])]
+        d_f = [(b - a) / abs(a) for a, b in zip(fp64["e0"], fp64["e1"])]
+        extra.update(fp64_value=v64, mixed_over_fp64=value / v64,
+                     fp64_ms_per_step=1e3 * fp64["seconds"] / args.steps,
+                     m2_err_max=max(abs(a - b) for a, b in zip(mixed["a2"], fp64["a2"])),
+                     a2_mixed_max=max(abs(a) for a in mixed["a2"]),
+                     energy_drift_mixed_max=max(abs(v) for v in d_m),
+                     energy_drift_err_max=max(abs(a - b) for a, b in zip(d_m, d_f)))
+    line = {
+        "metric": "particle-steps/s (ensemble of p2d discs)", "value": value,
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["seconds"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (64 seeded exponential discs, log-uniform counts, Q in [1, 2])",
+        "config": {"workload": w.name, "members": w.nmem, "grid": [w.nr, w.na],
+                   "n": mixed["n"], "harmonics": "m<=%d" % w.hash, "zones": list(w.nstep),
+                   "dt": w.dt, "parallelism": "none"},
+        "roofline": roof, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes):
     """A plain elementwise copy of known size (FP32 x 1.0 into a second
     buffer: nbytes read, nbytes written), run three times; its FETCH_SIZE and
@@ -286,8 +396,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain copy kernel of BYTES (counter calibration)")
-    ap.add_argument("--workload", default="C2", choices=("C2", "C3"),
-                    help="C2: 2-D grid, the headline (default); C3: 3-D grid")
+    ap.add_argument("--workload", default="C2", choices=("C2", "C3", "C4"),
+                    help="C2: 2-D grid, the headline (default); C3: 3-D grid; "
+                         "C4: ensemble of 64 2-D discs")
+    ap.add_argument("--graph", type=int, default=None, choices=(0, 1),
+                    help="run the timed steps as HIP graphs (default: 1 for C4, 0 otherwise)")
     ap.add_argument("--cpu-n3", type=int, default=4_000_000)
     ap.add_argument("--cpu-steps3", type=int, default=2)
     args = ap.parse_args()
@@ -299,6 +412,15 @@ def main():
               "multi-GPU form is a design, not measured); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "C4":
+        if args.steps == 500:
+            args.steps = 300
+        if args.sort_interval is None:
+            args.sort_interval = 10
+        if args.graph is None:
+            args.graph = 1
+        main_ens(args)
+        return
     if args.workload == "C3":
         if args.steps == 500:
             args.steps = 200
@@ -311,10 +433,11 @@ def main():
     from galaxy_amd import configs
     w = configs.C2
     n = args.n or w.n
-    mixed = run_gpu(w, "fp32", n, args.steps, args.warmup, args.sort_interval)
+    graph = args.graph or 0
+    mixed = run_gpu(w, "fp32", n, args.steps, args.warmup, args.sort_interval, graph)
     fp64 = None
     if not args.no_fp64:
-        fp64 = run_gpu(w, "fp64", n, args.steps, args.warmup, args.sort_interval)
+        fp64 = run_gpu(w, "fp64", n, args.steps, args.warmup, args.sort_interval, graph)
     value = mixed["pushes"] / mixed["seconds"]
     ms_per_step = 1e3 * mixed["seconds"] / args.steps
     bytes_per_push = 32.0
@@ -337,6 +460,7 @@ def main():
         push_valu_frac=round(in_kernel * PUSH_VALU_PER_PUSH / VALU_PEAK_LANE_OPS, 3),
         pushes_per_step=mixed["pushes"] / args.steps,
         sort_interval=args.sort_interval,
+        graph=bool(graph),
     )
     if fp64:
         v64 = fp64["pushes"] / fp64["seconds"]

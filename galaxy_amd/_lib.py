@@ -91,6 +91,16 @@ SIGNATURES = {
     "galhip_unload3": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "galhip_weight3": [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp],
     "galhip_grdacc3": [_vp, _i64, _vp, _vp, _vp, _vp],
+    "galhip_ensset": [C.POINTER(P2DParams), _i32, _vp, C.POINTER(_vp)],
+    "galhip_ens_restrt": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "galhip_ens_unload": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "galhip_ens_get_mass": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
+    "galhip_ens_get_field": [_vp, _i32, _vp],
+    "galhip_ens_set_field": [_vp, _i32, _vp],
+    "galhip_ens_get_ranges": [_vp, _i32, _vp],
+    "galhip_ens_get_noff": [_vp, _vp],
+    "galhip_ens_integrals": [_vp, _i32, _vp],
+    "galhip_set_graph": [_vp, _i32],
 }
 
 _lib = None

@@ -13,7 +13,9 @@ workload descriptions leave open are fixed here and marked "assumed":
     EXAMPLES/p2d/run100.dat 'offgrid t');
   - 3-D grid: plane spacing 0.025 R_d (a quarter of the softening, which
     p3dset requires to be at least the spacing), dt = 0.05, one zone
-    (assumed; the workload names neither).
+    (assumed; the workload names neither);
+  - ensemble: member counts and Q drawn with numpy's default_rng(4), member
+    i seeded i, m <= 8, one zone, dt = 0.05 (assumed).
 """
 from __future__ import annotations
 
@@ -79,7 +81,47 @@ class Workload3(Workload):
 C3 = Workload3("p3d-200x256x125-n5e7", nr=200, na=256, hash=8, n=50_000_000,
                steps=200, dt=0.05, ngz=125, seed=3)
 
-WORKLOADS = {"C1": C1, "C2": C2, "C3": C3}
+
+
+@dataclass
+class EnsembleWorkload(Workload):
+    """nmem independent discs on one grid: member i has count_i particles
+    with log10(count_i) uniform in [log10 nmin, log10 nmax], Toomre Q_i
+    uniform in [qmin, qmax] (both drawn with draw_seed) and its own seed
+    seed + i.  n is the total particle count."""
+    nmem: int = 64
+    nmin: float = 1e5
+    nmax: float = 4e6
+    qmin: float = 1.0
+    qmax: float = 2.0
+    draw_seed: int = 4
+
+    def members(self, scale=1.0):
+        """(counts, Q) of the members; scale multiplies the counts (smaller
+        ensembles of the same shape for tests)."""
+        rng = np.random.default_rng(self.draw_seed)
+        logn = rng.uniform(math.log10(self.nmin), math.log10(self.nmax), self.nmem)
+        Q = rng.uniform(self.qmin, self.qmax, self.nmem)
+        counts = np.maximum(np.rint(scale * 10.0 ** logn), 0).astype(np.int64)
+        return counts, Q
+
+    def member(self, i, n):
+        """Workload of member i as a lone disc of n particles."""
+        counts, Q = self.members()
+        return Workload(self.name + "-m%d" % i, nr=self.nr, na=self.na, hash=self.hash,
+                        n=int(n), steps=self.steps, dt=self.dt, nstep=self.nstep,
+                        zbr_model=self.zbr_model, eps_model=self.eps_model, Q=float(Q[i]),
+                        seed=self.seed + i, md=self.md, rtrunc=self.rtrunc, rcore=self.rcore,
+                        redge=self.redge)
+
+
+# 64 discs on 128 x 256 grids, N_i log-uniform in [1e5, 4e6], Q_i uniform in
+# [1, 2], m <= 8 (assumed), one zone and dt = 0.05 (assumed), 300 steps
+C4 = EnsembleWorkload("ens-64x(128x256)", nr=128, na=256, hash=8, n=0, steps=300,
+                      dt=0.05, seed=0)
+C4.n = int(C4.members()[0].sum())
+
+WORKLOADS = {"C1": C1, "C2": C2, "C3": C3, "C4": C4}
 
 
 def grid_kwargs(w: Workload, n=None):
@@ -168,5 +210,48 @@ def make_grid3(w: Workload3, precision="fp32", n=None, seed=None, green=True):
     dr, val, der = fixforce(w, rmax)
     g.set_fixforce(dr, val, der)
     g.restrt(*particles3(w, n, seed, g.rgrid))
+    g.masset()
+    return g
+
+
+def _member_particles(args):
+    w, i, n, rgrid = args
+    if n <= 0:
+        e = np.zeros(0)
+        return (e, e, e, e, np.zeros(0, np.int32))
+    return particles(w.member(i, n), n, None, rgrid)
+
+
+def ensemble_particles(w: EnsembleWorkload, counts, rgrid, workers=None,
+                       min_parallel=2_000_000):
+    """The members' particles (x, y, vx, vy, iz per member), drawn in
+    parallel worker processes (each member from its own seed, so the result
+    does not depend on the worker count)."""
+    import concurrent.futures as cf
+    import os
+    jobs = [(w, i, int(n), rgrid) for i, n in enumerate(counts)]
+    workers = workers or min(8, os.cpu_count() or 1)
+    if workers <= 1 or sum(counts) < min_parallel:
+        return [_member_particles(j) for j in jobs]
+    with cf.ProcessPoolExecutor(workers) as ex:
+        return list(ex.map(_member_particles, jobs))
+
+
+def make_ensemble(w: EnsembleWorkload, precision="fp32", scale=1.0, green=True, parts=None):
+    """A ready P2DEnsemble for workload w (member counts times scale): one
+    Green's table, the halo tables, every member's particles (parts, or
+    drawn here) loaded and assigned."""
+    from .lib15 import P2DEnsemble
+    counts, _ = w.members(scale)
+    kw = grid_kwargs(w, 1)
+    kw.pop("pmass")
+    ls = w.lscale()
+    pmass = [w.md * ls ** 3 * w.dt * w.dt / max(int(n), 1) for n in counts]
+    g = P2DEnsemble(w.nmem, pmass=pmass, precision=precision, **kw)
+    if green:
+        g.greenm()
+    g.set_fixforce(*fixforce(w, g.rgrid))
+    g.set_fixpot(*fixpot(w, g.rgrid))
+    g.restrt(parts if parts is not None else ensemble_particles(w, counts, g.rgrid))
     g.masset()
     return g

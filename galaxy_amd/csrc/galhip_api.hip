@@ -11,6 +11,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -31,6 +32,10 @@ template <typename R>
 hipError_t launch_key(const typename Vec2<R>::T *, const uint8_t *, long long,
                       int, int, int, int, unsigned, unsigned *, unsigned *,
                       hipStream_t);
+template <typename R>
+hipError_t launch_key_ens(const typename Vec2<R>::T *, const uint8_t *, long long, int, int,
+                          int, int, unsigned, const long long *, int, unsigned *, unsigned *,
+                          hipStream_t);
 template <typename R> struct PermArgs {
     const typename Vec4<R>::T *xv_in;
     typename Vec4<R>::T *xv_out;
@@ -52,7 +57,7 @@ hipError_t launch_green(double *, const double *, int, int, int, int, double,
 hipError_t launch_d2f(const double *, float *, long long, hipStream_t);
 hipError_t launch_fill_i64(long long *, long long, long long, hipStream_t);
 hipError_t launch_fill_i32(int *, int, long long, hipStream_t);
-hipError_t launch_masscl(const ZonePtrs &, int, long long, int *, int *, int,
+hipError_t launch_masscl(const ZonePtrs &, int, long long, int *, int *, int, int,
                          hipStream_t);
 }  // namespace gh
 
@@ -89,7 +94,10 @@ int qbits_for(long long n) {
     return 62 - b;
 }
 
-extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
+// p2dset for one grid (nmem = 1, pmass from the parameters) or an ensemble
+// of nmem grids with per-member particle masses.
+static int p2d_create(const galhip_p2d_params *p, int nmem, const double *pmass,
+                      galhip_ctx **out) {
     if (!p || !out) return gh_fail(GALHIP_EINVAL, "galhip_p2dset: null argument");
     *out = nullptr;
     if (p->precision != GALHIP_FP32 && p->precision != GALHIP_FP64)
@@ -104,6 +112,12 @@ extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
     if (p->tsoft != 1 && p->tsoft != 2) return gh_fail(GALHIP_EINVAL, "Unrecognized softening rule");
     if (p->nskip < 0 || p->nskip > GALHIP_MAXSKIP) return gh_fail(GALHIP_EINVAL, "nskip out of range");
     galhip_ctx *c = new galhip_ctx();   // value-initialised: scalars zeroed
+    c->nmem = nmem;
+    c->m_pmass.assign(nmem, p->pmass);
+    if (pmass)
+        for (int m = 0; m < nmem; m++) c->m_pmass[m] = pmass[m];
+    c->m_qbits.assign(nmem, qbits_for(1));
+    c->m_off.assign(nmem + 1, 0);
     c->prec = p->precision;
     c->nr = p->nr;
     c->na = na;
@@ -165,19 +179,22 @@ extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
     hipError_t e = hipStreamCreate(&c->s);
     if (e != hipSuccess) { delete c; return gh_fail(GALHIP_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     size_t es = esize(c);
+    const size_t nm = (size_t)nmem, cells = nm * c->mesh;
     for (int z = 0; z < c->nz; z++) {
-        rc |= dalloc(&c->mfx[z], sizeof(long long) * c->mesh);
-        rc |= dalloc(&c->moz[z], sizeof(long long) * c->mesh);
+        rc |= dalloc(&c->mfx[z], sizeof(long long) * cells);
+        rc |= dalloc(&c->moz[z], sizeof(long long) * cells);
     }
-    rc |= dalloc(&c->jrad, sizeof(int) * GH_MAXZ);
-    rc |= dalloc(&c->krad, sizeof(int) * GH_MAXZ);
-    rc |= dalloc(&c->ranges, sizeof(int) * 4);
+    rc |= dalloc(&c->jrad, sizeof(int) * GH_MAXZ * nm);
+    rc |= dalloc(&c->krad, sizeof(int) * GH_MAXZ * nm);
+    rc |= dalloc(&c->ranges, sizeof(int) * 4 * nm);
     rc |= dalloc(&c->lg_d, sizeof(int) * c->mmax);
-    rc |= dalloc(&c->counters, sizeof(long long) * 4);
-    rc |= dalloc(&c->rho, es * c->mesh);
-    rc |= dalloc(&c->coef, es * c->ncoef * c->nr);
-    rc |= dalloc(&c->fld2, 2 * es * c->mesh);
-    rc |= dalloc(&c->pot, es * c->mesh);
+    rc |= dalloc(&c->counters, sizeof(long long) * 4 * nm);
+    rc |= dalloc(&c->rho, es * cells);
+    rc |= dalloc(&c->coef, es * c->ncoef * c->nr * nm);
+    rc |= dalloc(&c->fld2, 2 * es * cells);
+    rc |= dalloc(&c->pot, es * cells);
+    rc |= dalloc(&c->m_qbits_d, sizeof(int) * nm);
+    rc |= dalloc(&c->m_pmass_d, sizeof(double) * nm);
     rc |= dalloc(&c->tw, 2 * es * na);
     rc |= dalloc(&c->twd, 2 * sizeof(double) * na);
     rc |= dalloc(&c->Gd, sizeof(double) * 3 * (size_t)c->ng * c->nr * c->nr);
@@ -201,21 +218,46 @@ extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
         HIPCHK(hipMemcpy(c->tw, twd.data(), sizeof(double) * 2 * na, hipMemcpyHostToDevice));
     }
     HIPCHK(hipMemcpy(c->lg_d, c->lg.data(), sizeof(int) * c->mmax, hipMemcpyHostToDevice));
-    std::vector<int> jr(GH_MAXZ, c->nr), kr(GH_MAXZ, 0);
-    HIPCHK(hipMemcpy(c->jrad, jr.data(), sizeof(int) * GH_MAXZ, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->krad, kr.data(), sizeof(int) * GH_MAXZ, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(c->counters, 0, sizeof(long long) * 4));
-    HIPCHK(hipMemset(c->fld2, 0, 2 * es * c->mesh));
-    HIPCHK(hipMemset(c->pot, 0, es * c->mesh));
-    HIPCHK(hipMemset(c->rho, 0, es * c->mesh));
+    std::vector<int> jr(GH_MAXZ * nm, c->nr), kr(GH_MAXZ * nm, 0);
+    HIPCHK(hipMemcpy(c->jrad, jr.data(), sizeof(int) * GH_MAXZ * nm, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->krad, kr.data(), sizeof(int) * GH_MAXZ * nm, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->counters, 0, sizeof(long long) * 4 * nm));
+    HIPCHK(hipMemset(c->fld2, 0, 2 * es * cells));
+    HIPCHK(hipMemset(c->pot, 0, es * cells));
+    HIPCHK(hipMemset(c->rho, 0, es * cells));
     for (int z = 0; z < c->nz; z++) {
-        HIPCHK(hipMemset(c->mfx[z], 0, sizeof(long long) * c->mesh));
-        HIPCHK(hipMemset(c->moz[z], 0, sizeof(long long) * c->mesh));
+        HIPCHK(hipMemset(c->mfx[z], 0, sizeof(long long) * cells));
+        HIPCHK(hipMemset(c->moz[z], 0, sizeof(long long) * cells));
     }
+    HIPCHK(hipMemcpy(c->m_qbits_d, c->m_qbits.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->m_pmass_d, c->m_pmass.data(), sizeof(double) * nm, hipMemcpyHostToDevice));
     c->sort_interval = 0;
     c->qbits = qbits_for(1);
     *out = c;
     return 0;
+}
+
+extern "C" int galhip_p2dset(const galhip_p2d_params *p, galhip_ctx **out) {
+    return p2d_create(p, 1, nullptr, out);
+}
+
+extern "C" int galhip_ensset(const galhip_p2d_params *p, int32_t nmem, const double *pmass,
+                             galhip_ctx **out) {
+    if (!p || !out || !pmass) return gh_fail(GALHIP_EINVAL, "galhip_ensset: null argument");
+    *out = nullptr;
+    if (nmem < 1 || nmem > GALHIP_MAXMEMBERS)
+        return gh_fail(GALHIP_EINVAL, "nmem %d outside 1..%d", nmem, GALHIP_MAXMEMBERS);
+    for (int m = 0; m < nmem; m++)
+        if (!(pmass[m] > 0.0)) return gh_fail(GALHIP_EINVAL, "member %d: particle mass not positive", m);
+    int rc = p2d_create(p, nmem, pmass, out);
+    if (rc) return rc;
+    (*out)->ens = 1;
+    return 0;
+}
+
+static void drop_graphs(galhip_ctx *c) {
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+    c->graphs.clear();
 }
 
 extern "C" int galhip_free(galhip_ctx *c) {
@@ -230,6 +272,8 @@ extern "C" int galhip_free(galhip_ctx *c) {
     if (c->G != (void *)c->Gd) dfree(c->G);
     dfree(c->Gd);
     dfree(c->fr_val); dfree(c->fr_der);
+    dfree(c->m_qbits_d); dfree(c->m_pmass_d); dfree(c->m_off_d); dfree(c->blk_d);
+    drop_graphs(c);
     for (auto e : c->ev_free) (void)hipEventDestroy(e);
     for (auto e : c->ev_used) (void)hipEventDestroy(e);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -362,10 +406,25 @@ template <typename R> static int sort_impl(galhip_ctx *c) {
     int kshift = 0;
     while (((long long)c->mesh >> kshift) > 65534) kshift++;
     unsigned kmax = (unsigned)((c->mesh - 1) >> kshift) + 1;
-    HIPCHK(launch_key<R>((const typename Vec2<R>::T *)c->ut, c->zone, n, c->nr,
-                         c->na, c->nz, kshift, kmax, c->key, c->perm, c->s));
     int end_bit = 1;
     while ((1LL << end_bit) <= (long long)kmax) end_bit++;
+    if (c->ens) {
+        // member-major keys: the cell (unshifted, off-grid = mesh) below the
+        // member index
+        kmax = (unsigned)c->mesh;
+        int kbits = 1;
+        while ((1LL << kbits) <= (long long)kmax) kbits++;
+        int mbits = 0;
+        while ((1LL << mbits) < (long long)c->nmem) mbits++;
+        if (kbits + mbits > 32) return gh_fail(GALHIP_EINVAL, "ensemble sort keys exceed 32 bits");
+        HIPCHK(launch_key_ens<R>((const typename Vec2<R>::T *)c->ut, c->zone, n, c->nr, c->na,
+                                 c->nz, kbits, kmax, c->m_off_d, c->nmem, c->key, c->perm,
+                                 c->s));
+        end_bit = kbits + mbits;
+    } else {
+        HIPCHK(launch_key<R>((const typename Vec2<R>::T *)c->ut, c->zone, n, c->nr,
+                             c->na, c->nz, kshift, kmax, c->key, c->perm, c->s));
+    }
     size_t tmp = c->sort_tmp_bytes;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->key, c->key2,
                                               c->perm, c->perm2, (int)n, 0,
@@ -418,6 +477,7 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
     if (!c || n < 0 || (n > 0 && (!x || !y || !vx || !vy || !iz)))
         return gh_fail(GALHIP_EINVAL, "galhip_restrt: bad arguments");
     if (c->p3d) return gh_fail(GALHIP_EINVAL, "p3d context: use galhip_restrt3");
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use galhip_ens_restrt");
     if (n >= (1LL << 31)) return gh_fail(GALHIP_EINVAL, "at most 2^31-1 particles");
     HIPCHK(hipStreamSynchronize(c->s));
     c->tcent = 0;   // loaded velocities: not (yet) time-centred
@@ -436,6 +496,8 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
         if (v == c->nz + 1) cnt[0]++;    // /admin/ noff: the off-grid list
     }
     HIPCHK(hipMemcpy(c->counters, cnt, sizeof cnt, hipMemcpyHostToDevice));
+    c->qbits = qbits_for(n);
+    if (n == 0) return 0;
     {
         // interleave (x, y, vx, vy) per particle
         std::vector<unsigned char> buf(4 * es * n);
@@ -457,12 +519,22 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
     return 0;
 }
 
+static int unload_all(galhip_ctx *c, void *x, void *y, void *vx, void *vy, int32_t *iz);
+
 extern "C" int galhip_unload(galhip_ctx *c, void *x, void *y, void *vx,
                              void *vy, int32_t *iz) {
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use galhip_ens_unload");
     if (c->p3d) return p3_unload(c, x, y, vx, vy, iz);
+    return unload_all(c, x, y, vx, vy, iz);
+}
+
+// every particle back to host arrays in load order (ensembles: the members
+// one after another)
+static int unload_all(galhip_ctx *c, void *x, void *y, void *vx, void *vy, int32_t *iz) {
     HIPCHK(hipStreamSynchronize(c->s));
     long long n = c->n;
+    if (n <= 0) return 0;
     std::vector<unsigned> ids(n);
     HIPCHK(hipMemcpy(ids.data(), c->id, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
     void *dst[4] = {x, y, vx, vy};
@@ -597,6 +669,34 @@ template <typename R> static PushArgs<R> push_args(galhip_ctx *c) {
         a.zbr[z] = (R)c->zbr[z];
         a.nstep[z] = c->nstep[z];
     }
+    if (c->ens) {
+        a.blk = c->blk_d;
+        a.qbits_m = c->m_qbits_d;
+        a.nblocks = c->nblk;
+    }
+    return a;
+}
+
+// The argument block of one ensemble member as a single grid: its particle
+// range, grids and fixed-point scale (one-grid kernels on member m).
+template <typename R> static PushArgs<R> member_args(galhip_ctx *c, int m) {
+    PushArgs<R> a = push_args<R>(c);
+    if (!c->ens) return a;
+    a.blk = nullptr;
+    a.qbits_m = nullptr;
+    a.nblocks = 0;
+    const long long o = c->m_off[m];
+    const size_t mo = (size_t)m * c->mesh;
+    a.xv = (typename Vec4<R>::T *)c->xv + o;
+    a.ut = (typename Vec2<R>::T *)c->ut + o;
+    a.zone = c->zone + o;
+    a.n = c->m_off[m + 1] - o;
+    a.fld = (const typename Vec2<R>::T *)c->fld2 + mo;
+    for (int z = 0; z < c->nz; z++) a.mfx[z] = c->mfx[z] + mo;
+    a.jrad = c->jrad + m * GH_MAXZ;
+    a.krad = c->krad + m * GH_MAXZ;
+    a.counters = c->counters + m * 4;
+    a.qbits = c->m_qbits[m];
     return a;
 }
 
@@ -626,6 +726,11 @@ static AnaArgs<R> ana_args(galhip_ctx *c, int lpot, int explicit_zones,
     a.nr = c->nr; a.na = c->na; a.ng = c->ng; a.ncoef = c->ncoef; a.nz = c->nz;
     a.mzone = c->mzone; a.lpot = lpot; a.jzone = jzone; a.kzone = kzone;
     a.explicit_zones = explicit_zones;
+    a.nmem = c->nmem;
+    if (c->ens) {
+        a.qbits_m = c->m_qbits_d;
+        a.pmass_m = c->m_pmass_d;
+    }
     return a;
 }
 
@@ -649,6 +754,7 @@ template <typename R> static SynArgs<R> syn_args(galhip_ctx *c, int lpot,
     a.pot = (R *)c->pot;
     a.nr = c->nr; a.na = c->na; a.ng = c->ng; a.ncoef = c->ncoef; a.mmax = c->mmax;
     a.lpot = lpot;
+    a.nmem = c->nmem;
     return a;
 }
 
@@ -660,9 +766,8 @@ template <typename R> static int masset_impl(galhip_ctx *c) {
     ZonePtrs zp;
     memset(&zp, 0, sizeof zp);
     for (int z = 0; z < c->nz; z++) { zp.mfx[z] = c->mfx[z]; zp.moz[z] = c->moz[z]; }
-    HIPCHK(launch_masscl(zp, c->nz, c->mesh, c->jrad, c->krad, c->nr, c->s));
-    PushArgs<R> a = push_args<R>(c);
-    HIPCHK(launch_massgn<R>(a, c->s));
+    HIPCHK(launch_masscl(zp, c->nz, c->mesh, c->jrad, c->krad, c->nr, c->nmem, c->s));
+    for (int m = 0; m < c->nmem; m++) HIPCHK(launch_massgn<R>(member_args<R>(c, m), c->s));
     int rc = zoncmb_host(c);
     if (rc) return rc;
     c->masscl_done = 0;
@@ -723,7 +828,7 @@ template <typename R> static int step_impl(galhip_ctx *c) {
         ZonePtrs zp;
         memset(&zp, 0, sizeof zp);
         for (int z = 0; z < c->nz; z++) { zp.mfx[z] = c->mfx[z]; zp.moz[z] = c->moz[z]; }
-        HIPCHK(launch_masscl(zp, c->mzone, c->mesh, c->jrad, c->krad, c->nr, c->s));
+        HIPCHK(launch_masscl(zp, c->mzone, c->mesh, c->jrad, c->krad, c->nr, c->nmem, c->s));
     }
     c->masscl_done = 0;
     PushArgs<R> a = push_args<R>(c);
@@ -745,8 +850,8 @@ extern "C" int galhip_step(galhip_ctx *c) {
 }
 
 template <typename R> static int tmcent_impl(galhip_ctx *c, int start) {
-    PushArgs<R> a = push_args<R>(c);
-    HIPCHK(launch_tmcent<R>(a, start ? (R)0.5 : (R)-0.5, c->s));
+    for (int m = 0; m < c->nmem; m++)
+        HIPCHK(launch_tmcent<R>(member_args<R>(c, m), start ? (R)0.5 : (R)-0.5, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     c->tcent = start != 0;
     return 0;
@@ -762,17 +867,27 @@ extern "C" int galhip_tmcent(galhip_ctx *c, int32_t start) {
     return c->prec == GALHIP_FP32 ? tmcent_impl<float>(c, start) : tmcent_impl<double>(c, start);
 }
 
+// out[nmem][9]: each grid's integrals (one grid: nmem = 1), member by
+// member with the one-grid kernels, so that a member's sums are those of
+// the same disc in its own context.
 template <typename R> static int integrals_impl(galhip_ctx *c, int centred, double *out) {
-    const long long nb = c->n > 0 ? (c->n + GH_BLOCK - 1) / GH_BLOCK : 0;
+    long long nbmax = 1;
+    for (int m = 0; m < c->nmem; m++) {
+        const long long nm = c->ens ? c->m_off[m + 1] - c->m_off[m] : c->n;
+        nbmax = std::max(nbmax, (nm + GH_BLOCK - 1) / GH_BLOCK);
+    }
     double *part = nullptr, *dout = nullptr;
-    int rc = dalloc(&part, sizeof(double) * 9 * (nb > 0 ? nb : 1)) |
-             dalloc(&dout, sizeof(double) * 9);
+    int rc = dalloc(&part, sizeof(double) * 9 * nbmax) |
+             dalloc(&dout, sizeof(double) * 9 * c->nmem);
     if (!rc) {
-        PushArgs<R> a = push_args<R>(c);
-        hipError_t e = launch_integrals<R>(a, c->pot_valid ? (const R *)c->pot : nullptr,
-                                           c->fp_val, c->fp_der, c->nfp, c->fp_dr,
-                                           centred, part, dout, c->s);
-        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 9,
+        hipError_t e = hipSuccess;
+        for (int m = 0; m < c->nmem && e == hipSuccess; m++) {
+            PushArgs<R> a = member_args<R>(c, m);
+            const R *pot = c->pot_valid ? (const R *)c->pot + (size_t)m * c->mesh : nullptr;
+            e = launch_integrals<R>(a, pot, c->fp_val, c->fp_der, c->nfp, c->fp_dr, centred,
+                                    part, dout + 9 * m, c->s);
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 9 * c->nmem,
                                                 hipMemcpyDeviceToHost, c->s);
         if (e == hipSuccess) e = hipStreamSynchronize(c->s);
         if (e != hipSuccess) rc = gh_fail(GALHIP_EHIP, "integrals: %s", hipGetErrorString(e));
@@ -781,8 +896,11 @@ template <typename R> static int integrals_impl(galhip_ctx *c, int centred, doub
     dfree(dout);
     if (rc) return rc;
     // particle sums times the particle mass; no potential: NaN
-    for (int q = 0; q < 9; q++) out[q] *= c->pmass;
-    if (!c->pot_valid) out[7] = NAN;
+    for (int m = 0; m < c->nmem; m++) {
+        const double pm = c->ens ? c->m_pmass[m] : c->pmass;
+        for (int q = 0; q < 9; q++) out[9 * m + q] *= pm;
+        if (!c->pot_valid) out[9 * m + 7] = NAN;
+    }
     return 0;
 }
 
@@ -791,8 +909,65 @@ extern "C" int galhip_integrals(galhip_ctx *c, int32_t centred, double *out) {
     if (centred && !c->have_field)
         return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
     if (c->p3d) return p3_integrals(c, centred != 0, out);
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use galhip_ens_integrals");
     return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out)
                                   : integrals_impl<double>(c, centred != 0, out);
+}
+
+// One step of galhip_run -- findf with the fused masscl, then step -- as a
+// HIP graph of its three launches, with the host bookkeeping of
+// solve_impl and step_impl in the same order.  A graph is fully determined
+// by its launch arguments, so graphs are cached under the bytes of the
+// argument blocks and replayed whenever a step repeats them (the zone
+// schedule is periodic; a sort swaps the particle arrays).
+template <typename R> static int graph_step(galhip_ctx *c) {
+    if (!c->have_green) return gh_fail(GALHIP_EINVAL, "P2CONV: Green function not set (call galhip_greenm)");
+    const AnaArgs<R> a = ana_args<R>(c, 0, 0, 0, 0);
+    const SynArgs<R> b = syn_args<R>(c, 0, c->mzone);
+    c->have_field = 1;
+    c->pot_valid = 0;
+    set_mzone(c);
+    masscl_host(c, true);
+    c->masscl_done = 0;
+    const PushArgs<R> p = push_args<R>(c);
+    std::string key((const char *)&a, sizeof a);
+    key.append((const char *)&b, sizeof b);
+    key.append((const char *)&p, sizeof p);
+    hipGraphExec_t ex = nullptr;
+    for (auto &g : c->graphs)
+        if (g.first == key) {
+            ex = g.second;
+            break;
+        }
+    if (!ex) {
+        if (c->graphs.size() >= 64) {
+            HIPCHK(hipStreamSynchronize(c->s));
+            drop_graphs(c);
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = launch_analyse<R>(a, c->s);
+        if (e == hipSuccess) e = launch_synth<R>(b, c->s);
+        if (e == hipSuccess) e = launch_push<R>(p, c->s);
+        hipError_t e2 = hipStreamEndCapture(c->s, &g);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return gh_fail(GALHIP_EHIP, "step graph: %s", hipGetErrorString(e));
+        c->graphs.emplace_back(key, ex);
+    }
+    HIPCHK(hipGraphLaunch(ex, c->s));
+    c->istep += 1;
+    return zoncmb_host(c);
+}
+
+extern "C" int galhip_set_graph(galhip_ctx *c, int32_t on) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d) return gh_fail(GALHIP_EINVAL, "graphs are built for p2d grids and ensembles");
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->graph = on != 0;
+    if (!c->graph) drop_graphs(c);
+    return 0;
 }
 
 extern "C" int galhip_run(galhip_ctx *c, int32_t nsteps) {
@@ -813,14 +988,20 @@ extern "C" int galhip_run(galhip_ctx *c, int32_t nsteps) {
     }
     for (int k = 0; k < nsteps; k++) {
         set_mzone(c);
-        // findf with the masscl of this step (zone sums saved and cleared,
-        // jrad/krad reset) fused into the synthesis kernel
-        int rc = c->prec == GALHIP_FP32 ? solve_impl<float>(c, 0, 0, 0, 0, c->mzone)
+        int rc;
+        if (c->graph && !c->prof) {
+            rc = c->prec == GALHIP_FP32 ? graph_step<float>(c) : graph_step<double>(c);
+            if (rc) return rc;
+        } else {
+            // findf with the masscl of this step (zone sums saved and cleared,
+            // jrad/krad reset) fused into the synthesis kernel
+            rc = c->prec == GALHIP_FP32 ? solve_impl<float>(c, 0, 0, 0, 0, c->mzone)
                                         : solve_impl<double>(c, 0, 0, 0, 0, c->mzone);
-        if (rc) return rc;
-        c->masscl_done = 1;
-        rc = galhip_step(c);
-        if (rc) return rc;
+            if (rc) return rc;
+            c->masscl_done = 1;
+            rc = galhip_step(c);
+            if (rc) return rc;
+        }
         if (c->sort_interval > 0 && ++c->steps_since_sort >= c->sort_interval) {
             rc = galhip_sort(c);
             if (rc) return rc;
@@ -875,30 +1056,40 @@ extern "C" int galhip_weight(galhip_ctx *c, int64_t n, const void *x, const void
 extern "C" int galhip_grdacc(galhip_ctx *c, int64_t n, const void *x, const void *y,
                              void *acc) {
     if (c && c->p3d) return gh_fail(GALHIP_EINVAL, "p3d context: use the p3d entry point");
+    if (c && c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: no single field");
     if (!c || n < 0 || !acc) return gh_fail(GALHIP_EINVAL, "bad arguments");
     HIPCHK(hipStreamSynchronize(c->s));
     return c->prec == GALHIP_FP32 ? gather_impl<float>(c, n, x, y, 0, nullptr, nullptr, acc)
                                   : gather_impl<double>(c, n, x, y, 0, nullptr, nullptr, acc);
 }
 
+// grid m's zone masses, ring ranges and fixed-point scale
+static int get_mass_m(const galhip_ctx *c, int m, int64_t *mfx, int64_t *mozfx, int32_t *jrad,
+                      int32_t *krad, int32_t *qbits) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    const size_t mo = (size_t)m * c->mesh;
+    for (int z = 0; z < c->nz; z++) {
+        if (mfx) HIPCHK(hipMemcpy(mfx + (size_t)z * c->mesh, c->mfx[z] + mo, 8 * c->mesh, hipMemcpyDeviceToHost));
+        if (mozfx) HIPCHK(hipMemcpy(mozfx + (size_t)z * c->mesh, c->moz[z] + mo, 8 * c->mesh, hipMemcpyDeviceToHost));
+    }
+    if (jrad) HIPCHK(hipMemcpy(jrad, c->jrad + m * GH_MAXZ, sizeof(int) * c->nz, hipMemcpyDeviceToHost));
+    if (krad) HIPCHK(hipMemcpy(krad, c->krad + m * GH_MAXZ, sizeof(int) * c->nz, hipMemcpyDeviceToHost));
+    if (qbits) *qbits = c->ens ? c->m_qbits[m] : c->qbits;
+    return 0;
+}
+
 extern "C" int galhip_get_mass(const galhip_ctx *c, int64_t *mfx, int64_t *mozfx,
                                int32_t *jrad, int32_t *krad, int32_t *qbits) {
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
-    HIPCHK(hipStreamSynchronize(c->s));
-    for (int z = 0; z < c->nz; z++) {
-        if (mfx) HIPCHK(hipMemcpy(mfx + (size_t)z * c->mesh, c->mfx[z], 8 * c->mesh, hipMemcpyDeviceToHost));
-        if (mozfx) HIPCHK(hipMemcpy(mozfx + (size_t)z * c->mesh, c->moz[z], 8 * c->mesh, hipMemcpyDeviceToHost));
-    }
-    if (jrad) HIPCHK(hipMemcpy(jrad, c->jrad, sizeof(int) * c->nz, hipMemcpyDeviceToHost));
-    if (krad) HIPCHK(hipMemcpy(krad, c->krad, sizeof(int) * c->nz, hipMemcpyDeviceToHost));
-    if (qbits) *qbits = c->qbits;
-    return 0;
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use the galhip_ens_* calls");
+    return get_mass_m(c, 0, mfx, mozfx, jrad, krad, qbits);
 }
 
 extern "C" int galhip_set_mass(galhip_ctx *c, const int64_t *mfx,
                                const int64_t *mozfx, const int32_t *jrad,
                                const int32_t *krad) {
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use the galhip_ens_* calls");
     HIPCHK(hipStreamSynchronize(c->s));
     for (int z = 0; z < c->nz; z++) {
         if (mfx) HIPCHK(hipMemcpy(c->mfx[z], mfx + (size_t)z * c->mesh, 8 * c->mesh, hipMemcpyHostToDevice));
@@ -911,30 +1102,30 @@ extern "C" int galhip_set_mass(galhip_ctx *c, const int64_t *mfx,
 
 extern "C" int galhip_get_rho(const galhip_ctx *c, void *rho) {
     if (!c || !rho) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use the galhip_ens_* calls");
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipMemcpy(rho, c->rho, esize(c) * c->mesh, hipMemcpyDeviceToHost));
     return 0;
 }
 
-extern "C" int galhip_get_field(const galhip_ctx *c, void *fld) {
-    if (!c || !fld) return gh_fail(GALHIP_EINVAL, "null argument");
-    if (c->p3d) return p3_get_field(c, fld);
+// grid g's field as [3][mesh] (F_x, F_y, potential)
+static int get_field_m(const galhip_ctx *c, int g, void *fld) {
     HIPCHK(hipStreamSynchronize(c->s));
     size_t es = esize(c), m = c->mesh;
     std::vector<unsigned char> tmp(2 * es * m);
-    HIPCHK(hipMemcpy(tmp.data(), c->fld2, 2 * es * m, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tmp.data(), (const unsigned char *)c->fld2 + 2 * es * m * g, 2 * es * m,
+                     hipMemcpyDeviceToHost));
     unsigned char *o = (unsigned char *)fld;
     for (size_t i = 0; i < m; i++) {
         memcpy(o + i * es, tmp.data() + 2 * i * es, es);
         memcpy(o + (m + i) * es, tmp.data() + (2 * i + 1) * es, es);
     }
-    HIPCHK(hipMemcpy(o + 2 * m * es, c->pot, es * m, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(o + 2 * m * es, (const unsigned char *)c->pot + es * m * g, es * m,
+                     hipMemcpyDeviceToHost));
     return 0;
 }
 
-extern "C" int galhip_set_field(galhip_ctx *c, const void *fld) {
-    if (!c || !fld) return gh_fail(GALHIP_EINVAL, "null argument");
-    if (c->p3d) return p3_set_field(c, fld);
+static int set_field_m(galhip_ctx *c, int g, const void *fld) {
     HIPCHK(hipStreamSynchronize(c->s));
     size_t es = esize(c), m = c->mesh;
     std::vector<unsigned char> tmp(2 * es * m);
@@ -943,14 +1134,33 @@ extern "C" int galhip_set_field(galhip_ctx *c, const void *fld) {
         memcpy(tmp.data() + 2 * i * es, in + i * es, es);
         memcpy(tmp.data() + (2 * i + 1) * es, in + (m + i) * es, es);
     }
-    HIPCHK(hipMemcpy(c->fld2, tmp.data(), 2 * es * m, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->pot, in + 2 * m * es, es * m, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((unsigned char *)c->fld2 + 2 * es * m * g, tmp.data(), 2 * es * m,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((unsigned char *)c->pot + es * m * g, in + 2 * m * es, es * m,
+                     hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int galhip_get_field(const galhip_ctx *c, void *fld) {
+    if (!c || !fld) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->p3d) return p3_get_field(c, fld);
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use the galhip_ens_* calls");
+    return get_field_m(c, 0, fld);
+}
+
+extern "C" int galhip_set_field(galhip_ctx *c, const void *fld) {
+    if (!c || !fld) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->p3d) return p3_set_field(c, fld);
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use the galhip_ens_* calls");
+    int rc = set_field_m(c, 0, fld);
+    if (rc) return rc;
     c->have_field = 1;
     return 0;
 }
 
 extern "C" int galhip_get_ranges(const galhip_ctx *c, int32_t *r4) {
     if (!c || !r4) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use the galhip_ens_* calls");
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipMemcpy(r4, c->ranges, sizeof(int) * 4, hipMemcpyDeviceToHost));
     return 0;
@@ -958,6 +1168,7 @@ extern "C" int galhip_get_ranges(const galhip_ctx *c, int32_t *r4) {
 
 extern "C" int galhip_get_noff(const galhip_ctx *c, int64_t *noff) {
     if (!c || !noff) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use the galhip_ens_* calls");
     HIPCHK(hipStreamSynchronize(c->s));
     long long v[4];
     HIPCHK(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
@@ -967,6 +1178,17 @@ extern "C" int galhip_get_noff(const galhip_ctx *c, int64_t *noff) {
 
 // ---- profiling ----------------------------------------------------------------------
 
+// pushes counted so far over every grid of the context
+static int pushes_total(const galhip_ctx *c, long long *out) {
+    const int nm = c->nmem > 0 ? c->nmem : 1;
+    std::vector<long long> v(4 * (size_t)nm);
+    HIPCHK(hipMemcpy(v.data(), c->counters, sizeof(long long) * v.size(), hipMemcpyDeviceToHost));
+    long long t = 0;
+    for (int m = 0; m < nm; m++) t += v[4 * m + 1];
+    *out = t;
+    return 0;
+}
+
 extern "C" int galhip_set_profile(galhip_ctx *c, int32_t on) {
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
@@ -975,10 +1197,7 @@ extern "C" int galhip_set_profile(galhip_ctx *c, int32_t on) {
     c->prof = on != 0;
     c->prof_ms = 0.0;
     c->prof_launch = 0;
-    long long v[4];
-    HIPCHK(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
-    c->prof_push0 = v[1];
-    return 0;
+    return pushes_total(c, &c->prof_push0);
 }
 
 extern "C" int galhip_get_profile(galhip_ctx *c, double *push_ms, int64_t *launches,
@@ -991,11 +1210,12 @@ extern "C" int galhip_get_profile(galhip_ctx *c, double *push_ms, int64_t *launc
         HIPCHK(hipEventElapsedTime(&t, c->ev_used[i], c->ev_used[i + 1]));
         ms += t;
     }
-    long long v[4];
-    HIPCHK(hipMemcpy(v, c->counters, sizeof v, hipMemcpyDeviceToHost));
+    long long tot = 0;
+    int rc = pushes_total(c, &tot);
+    if (rc) return rc;
     if (push_ms) *push_ms = ms;
     if (launches) *launches = c->prof_launch;
-    if (pushes) *pushes = v[1] - c->prof_push0;
+    if (pushes) *pushes = tot - c->prof_push0;
     return 0;
 }
 
@@ -1003,4 +1223,147 @@ extern "C" int galhip_sync(galhip_ctx *c) {
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
     return 0;
+}
+
+// ---- ensembles of p2d discs -------------------------------------------------------
+
+extern "C" int galhip_ens_restrt(galhip_ctx *c, const int64_t *count, const void *x,
+                                 const void *y, const void *vx, const void *vy,
+                                 const int32_t *iz) {
+    if (!c || !count) return gh_fail(GALHIP_EINVAL, "galhip_ens_restrt: bad arguments");
+    if (!c->ens) return gh_fail(GALHIP_EINVAL, "not an ensemble context");
+    long long n = 0;
+    for (int m = 0; m < c->nmem; m++) {
+        if (count[m] < 0) return gh_fail(GALHIP_EINVAL, "member %d: negative count", m);
+        n += count[m];
+    }
+    if (n > 0 && (!x || !y || !vx || !vy || !iz))
+        return gh_fail(GALHIP_EINVAL, "galhip_ens_restrt: null particle array");
+    if (n >= (1LL << 31)) return gh_fail(GALHIP_EINVAL, "at most 2^31-1 particles in an ensemble");
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->tcent = 0;
+    int rc = ensure_particles(c, n);
+    if (rc) return rc;
+    // member ranges, push blocks of at most GH_BLOCK x 4 particles that never
+    // straddle two members, fixed-point scales as a lone disc of that size
+    // would have them
+    const int per = GH_BLOCK * 4;
+    std::vector<int4> blk;
+    c->m_off.assign(c->nmem + 1, 0);
+    for (int m = 0; m < c->nmem; m++) {
+        c->m_off[m + 1] = c->m_off[m] + count[m];
+        for (long long s0 = c->m_off[m]; s0 < c->m_off[m + 1]; s0 += per)
+            blk.push_back(make_int4(m, (int)s0, (int)std::min<long long>(per, c->m_off[m + 1] - s0), 0));
+        c->m_qbits[m] = qbits_for(count[m]);
+    }
+    dfree(c->blk_d);
+    dfree(c->m_off_d);
+    rc = dalloc(&c->blk_d, sizeof(int4) * std::max<size_t>(blk.size(), 1)) |
+         dalloc(&c->m_off_d, sizeof(long long) * (c->nmem + 1));
+    if (rc) return rc;
+    c->nblk = (int)blk.size();
+    if (!blk.empty())
+        HIPCHK(hipMemcpy(c->blk_d, blk.data(), sizeof(int4) * blk.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->m_off_d, c->m_off.data(), sizeof(long long) * (c->nmem + 1),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->m_qbits_d, c->m_qbits.data(), sizeof(int) * c->nmem, hipMemcpyHostToDevice));
+    size_t es = esize(c);
+    std::vector<uint8_t> z(n);
+    std::vector<unsigned> ids(n);
+    std::vector<long long> cnt(4 * (size_t)c->nmem, 0);   // per member: [0] noff
+    for (int m = 0; m < c->nmem; m++)
+        for (long long i = c->m_off[m]; i < c->m_off[m + 1]; i++) {
+            int v = iz[i];
+            if (v < 1 || v > c->nz + 1)
+                return gh_fail(GALHIP_EINVAL, "particle %lld has zone %d outside 1..%d", i, v, c->nz + 1);
+            z[i] = (uint8_t)v;
+            ids[i] = (unsigned)i;
+            if (v == c->nz + 1) cnt[4 * m]++;
+        }
+    HIPCHK(hipMemcpy(c->counters, cnt.data(), sizeof(long long) * cnt.size(), hipMemcpyHostToDevice));
+    drop_graphs(c);
+    if (n == 0) return 0;
+    {
+        std::vector<unsigned char> buf(4 * es * n);
+        const unsigned char *src[4] = {(const unsigned char *)x, (const unsigned char *)y,
+                                       (const unsigned char *)vx, (const unsigned char *)vy};
+        for (long long i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++)
+                memcpy(buf.data() + (4 * i + k) * es, src[k] + i * es, es);
+        HIPCHK(hipMemcpy(c->xv, buf.data(), 4 * es * n, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(c->zone, z.data(), n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+    rc = compute_ut(c);
+    if (rc) return rc;
+    rc = galhip_sort(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int galhip_ens_unload(galhip_ctx *c, void *x, void *y, void *vx, void *vy,
+                                 int32_t *iz) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (!c->ens) return gh_fail(GALHIP_EINVAL, "not an ensemble context");
+    return unload_all(c, x, y, vx, vy, iz);
+}
+
+static int ens_member_ok(const galhip_ctx *c, int32_t m) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (!c->ens) return gh_fail(GALHIP_EINVAL, "not an ensemble context");
+    if (m < 0 || m >= c->nmem) return gh_fail(GALHIP_EINVAL, "member %d outside 0..%d", m, c->nmem - 1);
+    return 0;
+}
+
+extern "C" int galhip_ens_get_mass(const galhip_ctx *c, int32_t member, int64_t *mfx,
+                                   int64_t *mozfx, int32_t *jrad, int32_t *krad,
+                                   int32_t *qbits) {
+    int rc = ens_member_ok(c, member);
+    return rc ? rc : get_mass_m(c, member, mfx, mozfx, jrad, krad, qbits);
+}
+
+extern "C" int galhip_ens_get_field(const galhip_ctx *c, int32_t member, void *fld) {
+    int rc = ens_member_ok(c, member);
+    if (rc) return rc;
+    if (!fld) return gh_fail(GALHIP_EINVAL, "null argument");
+    return get_field_m(c, member, fld);
+}
+
+extern "C" int galhip_ens_set_field(galhip_ctx *c, int32_t member, const void *fld) {
+    int rc = ens_member_ok(c, member);
+    if (rc) return rc;
+    if (!fld) return gh_fail(GALHIP_EINVAL, "null argument");
+    rc = set_field_m(c, member, fld);
+    if (rc) return rc;
+    c->have_field = 1;
+    return 0;
+}
+
+extern "C" int galhip_ens_get_ranges(const galhip_ctx *c, int32_t member, int32_t *r4) {
+    int rc = ens_member_ok(c, member);
+    if (rc) return rc;
+    if (!r4) return gh_fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(r4, c->ranges + 4 * member, sizeof(int) * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int galhip_ens_get_noff(const galhip_ctx *c, int64_t *noff) {
+    if (!c || !noff) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (!c->ens) return gh_fail(GALHIP_EINVAL, "not an ensemble context");
+    HIPCHK(hipStreamSynchronize(c->s));
+    std::vector<long long> v(4 * (size_t)c->nmem);
+    HIPCHK(hipMemcpy(v.data(), c->counters, sizeof(long long) * v.size(), hipMemcpyDeviceToHost));
+    for (int m = 0; m < c->nmem; m++) noff[m] = v[4 * m];
+    return 0;
+}
+
+extern "C" int galhip_ens_integrals(galhip_ctx *c, int32_t centred, double *out) {
+    if (!c || !out) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (!c->ens) return gh_fail(GALHIP_EINVAL, "not an ensemble context");
+    if (centred && !c->have_field)
+        return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out)
+                                  : integrals_impl<double>(c, centred != 0, out);
 }

@@ -72,6 +72,23 @@ struct galhip_ctx {
     double *twzd;                     // the same in double
     int *slot_m_d, *slot_p_d, *slot_of_d;
     size_t gsize3;                    // entries of the p3d Green's table
+    // Ensemble of nmem independent p2d discs (ens = 1) sharing the grid,
+    // filter, zones and Green's table; nmem = 1 for a single grid.  Member
+    // m's part of the per-grid arrays above sits at offset m*mesh (mass
+    // sums, rho, field, potential), m*ncoef*nr (coefficients), m*GH_MAXZ
+    // (jrad, krad) and m*4 (ranges, counters).
+    int ens, nmem;
+    std::vector<long long> m_off;     // particle offsets [nmem + 1]
+    std::vector<double> m_pmass;      // particle mass per member
+    std::vector<int> m_qbits;         // fixed-point scale per member
+    int *m_qbits_d;
+    double *m_pmass_d;
+    long long *m_off_d;
+    int4 *blk_d;                      // push blocks: (member, first, count, 0)
+    int nblk;
+    // HIP graphs of galhip_run's steps, keyed by their launch arguments
+    int graph;
+    std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
 };
 
 static size_t esize(const galhip_ctx *c) { return c->prec == GALHIP_FP64 ? 8 : 4; }
@@ -102,7 +119,7 @@ namespace gh {
 hipError_t launch_d2f(const double *, float *, long long, hipStream_t);
 hipError_t launch_fill_i64(long long *, long long, long long, hipStream_t);
 hipError_t launch_fill_i32(int *, int, long long, hipStream_t);
-hipError_t launch_masscl(const ZonePtrs &, int, long long, int *, int *, int,
+hipError_t launch_masscl(const ZonePtrs &, int, long long, int *, int *, int, int,
                          hipStream_t);
 }  // namespace gh
 

@@ -215,10 +215,11 @@ __device__ __forceinline__ bool solve_ranges(const int *jrad, const int *krad, i
 
 // Global deposit of one corner value with Fortran's flat indexing over the
 // [nz][mesh] zone arrays (grdmss(j, jz)); before the first array or past the
-// last it is dropped.
+// last it is dropped.  mo: the grid's offset in each zone array (ensemble
+// member x mesh).
 __device__ __forceinline__ void deposit_global(long long *const *mfx, int nz,
                                                int mesh, int zone, int idx,
-                                               long long v) {
+                                               long long v, long long mo = 0) {
     long long f = (long long)(zone - 1) * mesh + idx;
     if (f >= 0 && f < (long long)nz * mesh) {
         int zf = zone - 1, cf = idx;
@@ -230,7 +231,7 @@ __device__ __forceinline__ void deposit_global(long long *const *mfx, int nz,
             cf += mesh;
             zf--;
         }
-        atomicAdd((unsigned long long *)&mfx[zf][cf], (unsigned long long)v);
+        atomicAdd((unsigned long long *)&mfx[zf][mo + cf], (unsigned long long)v);
     }
 }
 
@@ -262,6 +263,14 @@ template <typename R> struct PushArgs {
     R rgrid, thmax;
     R zbr[GH_MAXZ];
     int nstep[GH_MAXZ];
+    // ensembles (push kernel): nblocks blocks, block b covering particles
+    // [blk[b].y, blk[b].y + blk[b].z) of member blk[b].x, whose grids sit
+    // at member x mesh in fld and the zone arrays, member x GH_MAXZ in
+    // jrad/krad and member x 4 in counters, with fixed-point scale
+    // qbits_m[member].  blk = NULL: one grid.
+    const int4 *blk;
+    const int *qbits_m;
+    int nblocks;
 };
 
 // grdacc-only / weight-only argument block (parity entry points)
@@ -290,6 +299,11 @@ template <typename R> struct AnaArgs {
     const int *lg;                    // [mmax] skip flags
     double q, pmass;
     int nr, na, ng, ncoef, nz, mzone, lpot, jzone, kzone, explicit_zones;
+    // ensembles: nmem grids (grid index blockIdx.z) with per-member qbits
+    // and particle mass (NULL: q and pmass above); nmem = 1 for one grid
+    int nmem;
+    const int *qbits_m;
+    const double *pmass_m;
 };
 
 template <typename R> struct SynArgs {
@@ -308,6 +322,7 @@ template <typename R> struct SynArgs {
     long long *moz[GH_MAXZ];
     int *jrad, *krad;
     int masscl_z;
+    int nmem;                         // grids (blockIdx.z); 1 for one grid
 };
 
 }  // namespace gh

@@ -41,6 +41,7 @@ extern "C" int galhip_p3dset(const galhip_p3d_params *p, galhip_ctx **out) {
         return gh_fail(GALHIP_EINVAL, "P3DSET: Interpolation scheme must be 2 or 3");
     galhip_ctx *c = new galhip_ctx();
     c->p3d = 1;
+    c->nmem = 1;
     c->jmass = jmass;
     c->prec = p->precision;
     c->nr = p->nr;
@@ -482,7 +483,7 @@ template <typename R> static int p3_masset_impl(galhip_ctx *c) {
     ZonePtrs zp;
     memset(&zp, 0, sizeof zp);
     for (int z = 0; z < c->nz; z++) { zp.mfx[z] = c->mfx[z]; zp.moz[z] = c->moz[z]; }
-    HIPCHK(launch_masscl(zp, c->nz, c->mesh, c->jrad, c->krad, c->nr, c->s));
+    HIPCHK(launch_masscl(zp, c->nz, c->mesh, c->jrad, c->krad, c->nr, 1, c->s));
     HIPCHK(launch_p3_massgn<R>(p3_args<R>(c), c->s));
     int rc = zoncmb_host(c);
     if (rc) return rc;
@@ -515,7 +516,7 @@ template <typename R> static int p3_step_impl(galhip_ctx *c) {
     ZonePtrs zp;
     memset(&zp, 0, sizeof zp);
     for (int z = 0; z < c->nz; z++) { zp.mfx[z] = c->mfx[z]; zp.moz[z] = c->moz[z]; }
-    HIPCHK(launch_masscl(zp, c->mzone, c->mesh, c->jrad, c->krad, c->nr, c->s));
+    HIPCHK(launch_masscl(zp, c->mzone, c->mesh, c->jrad, c->krad, c->nr, 1, c->s));
     c->masscl_done = 0;
     const P3Args<R> a = p3_args<R>(c);
     if (c->prof) HIPCHK(hipEventRecord(take_event(c), c->s));

@@ -99,7 +99,8 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 
-template <typename R>
+// ENS: an ensemble launch, the block's particles and member from a.blk.
+template <typename R, bool ENS>
 // Wavefronts per SIMD, measured: FP32 at least 5 (96 VGPRs, no spill; 4
 // waves at the unconstrained 102 VGPRs is 8 % slower, 6 waves spill); FP64
 // at least 4 (128 VGPRs, a few spills, 4 % faster than 3 waves without).
@@ -115,7 +116,15 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     __shared__ double s_ffv[FF_CAP], s_ffd[FF_CAP];
 
     const int tid = threadIdx.x, lane = tid & 63;
-    const long long base = (long long)blockIdx.x * (GH_BLOCK * PPT);
+    // the block's particles and its grid (ensembles: member eb.x)
+    const int4 eb = ENS ? a.blk[blockIdx.x] : make_int4(0, 0, 0, 0);
+    const long long base = ENS ? (long long)eb.y : (long long)blockIdx.x * (GH_BLOCK * PPT);
+    const long long mo = ENS ? (long long)eb.x * a.mesh : 0;
+    const R2 *const fld = a.fld + mo;
+    int *const jrad = a.jrad + (ENS ? eb.x * GH_MAXZ : 0);
+    int *const krad = a.krad + (ENS ? eb.x * GH_MAXZ : 0);
+    long long *const counters = a.counters + (ENS ? eb.x * 4 : 0);
+    const int qbits = ENS ? a.qbits_m[eb.x] : a.qbits;
     const int nz = a.nz, nlists = a.nz + 1, na = a.na, nr = a.nr;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     R4 *const xvb = a.xv + base;
     R2 *const utb = a.ut + base;
     uint8_t *const zb = a.zone + base;
-    const int nblk = (int)min((long long)(GH_BLOCK * PPT), a.n - base);
+    const int nblk = ENS ? eb.z : (int)min((long long)(GH_BLOCK * PPT), a.n - base);
     int zin[PPT];
     bool mv[PPT];
     int any = 0;
@@ -266,7 +275,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
             if (spoke < 0) spoke += na;
             else if (spoke >= na) spoke -= na;
             if (spoke >= na) spoke -= na;
-            ftile[j] = ring < nr ? a.fld[ring * na + spoke] : zero2<R>();
+            ftile[j] = ring < nr ? fld[ring * na + spoke] : zero2<R>();
         }
     }
     // the table entries k0 .. k1 + 1 that the block's grid particles read
@@ -300,7 +309,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
             for (int q = 0; q < 4; q++) {
                 // the last ring with t == na points past the field: 0
                 const bool in = ic[q] < a.mesh;
-                R2 f = a.fld[in ? ic[q] : 0];
+                R2 f = fld[in ? ic[q] : 0];
                 R wq = (grid && in) ? gw[k][q] : (R)0;
                 sx = sx + f.x * wq;
                 sy = sy + f.y * wq;
@@ -462,7 +471,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
         if (dzone[k] == 0) continue;
         long long qv[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) qv[q] = quant<R>(dw[k][q], a.qbits);
+        for (int q = 0; q < 4; q++) qv[q] = quant<R>(dw[k][q], qbits);
         if (use_tile && dzone[k] == zt) {
             const int b = (dring[k] - r0) * tw + (dd[k] - d0);
             const int ti[4] = {b, b + 1, b + tw, b + tw + 1};
@@ -475,7 +484,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
             corners_rs(dring[k], dspoke[k], na, ic);
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                if (qv[q] != 0) deposit_global(a.mfx, nz, a.mesh, dzone[k], ic[q], qv[q]);
+                if (qv[q] != 0) deposit_global(a.mfx, nz, a.mesh, dzone[k], ic[q], qv[q], mo);
         }
     }
     __syncthreads();
@@ -492,20 +501,20 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
                 if (spoke < 0) spoke += na;
                 else if (spoke >= na) spoke -= na;
                 if (spoke >= na) spoke -= na;
-                atomicAdd((unsigned long long *)&a.mfx[zt - 1][ring * na + spoke],
+                atomicAdd((unsigned long long *)&a.mfx[zt - 1][mo + ring * na + spoke],
                           (unsigned long long)v);
             }
         }
     }
     if (tid < nz && s_kr[tid] >= 0) {
-        atomicMin(&a.jrad[tid], s_jr[tid] + 1);
+        atomicMin(&jrad[tid], s_jr[tid] + 1);
         int k = s_kr[tid] + 2;
-        atomicMax(&a.krad[tid], k < nr ? k : nr);
+        atomicMax(&krad[tid], k < nr ? k : nr);
     }
     if (tid == 0) {
-        if (s_dnoff) atomicAdd((unsigned long long *)&a.counters[0],
+        if (s_dnoff) atomicAdd((unsigned long long *)&counters[0],
                                (unsigned long long)(long long)s_dnoff);
-        if (s_pushes) atomicAdd((unsigned long long *)&a.counters[1],
+        if (s_pushes) atomicAdd((unsigned long long *)&counters[1],
                                 (unsigned long long)s_pushes);
     }
 }
@@ -784,6 +793,39 @@ __global__ __launch_bounds__(GH_BLOCK) void key_kernel(
     idx[i] = (unsigned)i;
 }
 
+// Ensemble sort key: (member << kbits) | the member's cell key (off-grid
+// particles last within their member), so that one stable sort groups the
+// members in order and cells within each.  Member m holds particles
+// [off[m], off[m + 1]).
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void key_ens_kernel(
+    const typename Vec2<R>::T *__restrict__ ut, const uint8_t *__restrict__ zone,
+    long long n, int nr, int na, int nz, int kbits, unsigned kmax,
+    const long long *__restrict__ off, int nmem, unsigned *__restrict__ key,
+    unsigned *__restrict__ idx) {
+    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    // the last member whose first particle is at or before i
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    unsigned k = kmax;
+    int z = zone[i];
+    if (z >= 1 && z <= nz) {
+        R w[4];
+        int ring, spoke;
+        typename Vec2<R>::T c = ut[i];
+        cic(c.x, c.y, nr, na, ring, spoke, w);
+        k = (unsigned)(ring * na + spoke);
+        if (k >= kmax) k = kmax - 1;
+    }
+    key[i] = ((unsigned)lo << kbits) | k;
+    idx[i] = (unsigned)i;
+}
+
 // one pass moving every particle array to sorted order
 template <typename R> struct PermArgs {
     const typename Vec4<R>::T *xv_in;
@@ -818,8 +860,14 @@ static inline unsigned nblocks(long long n, long long per) {
 template <typename R>
 hipError_t launch_push(const PushArgs<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(push_kernel<R>, dim3(nblocks(a.n, GH_BLOCK * PPT)),
-                       dim3(GH_BLOCK), 0, s, a);
+    if (a.blk) {
+        if (a.nblocks > 0)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(push_kernel<R, true>), dim3(a.nblocks),
+                               dim3(GH_BLOCK), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(push_kernel<R, false>),
+                           dim3(nblocks(a.n, GH_BLOCK * PPT)), dim3(GH_BLOCK), 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -880,6 +928,17 @@ hipError_t launch_key(const typename Vec2<R>::T *ut, const uint8_t *zone,
 }
 
 template <typename R>
+hipError_t launch_key_ens(const typename Vec2<R>::T *ut, const uint8_t *zone, long long n,
+                          int nr, int na, int nz, int kbits, unsigned kmax,
+                          const long long *off, int nmem, unsigned *key, unsigned *idx,
+                          hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(key_ens_kernel<R>, dim3(nblocks(n, GH_BLOCK)), dim3(GH_BLOCK), 0, s,
+                       ut, zone, n, nr, na, nz, kbits, kmax, off, nmem, key, idx);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
     if (p.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(permute_all_kernel<R>, dim3(nblocks(p.n, GH_BLOCK)),
@@ -901,6 +960,10 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
                                       const uint8_t *, long long, int, int,   \
                                       int, int, unsigned, unsigned *,         \
                                       unsigned *, hipStream_t);               \
+    template hipError_t launch_key_ens<R>(const typename Vec2<R>::T *, const uint8_t *, \
+                                          long long, int, int, int, int, unsigned,    \
+                                          const long long *, int, unsigned *,         \
+                                          unsigned *, hipStream_t);                   \
     template hipError_t launch_permute_all<R>(const PermArgs<R> &, hipStream_t);
 GH_INST(float)
 GH_INST(double)

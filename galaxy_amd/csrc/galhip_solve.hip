@@ -35,42 +35,52 @@ constexpr int SOLVE_BLOCK = 1024;
 constexpr int ANA_SPLIT = 1;   // blocks per ring in the analysis (harmonics)
 constexpr int SYN_SPLIT = 1;   // blocks per ring in the synthesis (spokes)
 
-// grid (nr, ANA_SPLIT): block (ir, h) forms the ring's combined mass and
-// the harmonics m = h, h + ANA_SPLIT, ...
+// grid (nr, ANA_SPLIT, nmem): block (ir, h, mem) forms the ring's combined
+// mass and the harmonics m = h, h + ANA_SPLIT, ... of grid mem (ensemble
+// member; 0 for a single grid).
 template <typename R>
 __global__ __launch_bounds__(SOLVE_BLOCK) void analyse_kernel(AnaArgs<R> a) {
     __shared__ R rho_s[MAX_NA];
     __shared__ R tws[2 * MAX_NA];
     __shared__ int s_rng[4], s_ok;
     const int tid = threadIdx.x, na = a.na, ir = blockIdx.x + 1;
-    const int h = blockIdx.y;
+    const int h = blockIdx.y, mem = blockIdx.z;
+    // member mem's grids
+    const size_t mo = (size_t)mem * a.nr * na;
+    int *const jrad = a.jrad + mem * GH_MAXZ;
+    int *const krad = a.krad + mem * GH_MAXZ;
+    int *const ranges = a.ranges + mem * 4;
+    R *const rho = a.rho + mo;
+    R *const coef = a.coef + (size_t)mem * a.ncoef * a.nr;
+    const double q = a.qbits_m ? ldexp(1.0, -a.qbits_m[mem]) : a.q;
+    const double pmass = a.pmass_m ? a.pmass_m[mem] : a.pmass;
     if (tid == 0) {
         int kr[GH_MAXZ], rg[4] = {0, 0, 0, 0};
-        bool ok = solve_ranges(a.jrad, a.krad, a.nz, a.nr, a.mzone, a.lpot,
+        bool ok = solve_ranges(jrad, krad, a.nz, a.nr, a.mzone, a.lpot,
                                a.explicit_zones, a.jzone, a.kzone, kr, rg);
         s_ok = ok;
         for (int q = 0; q < 4; q++) s_rng[q] = rg[q];
         if (blockIdx.x == 0 && h == 0) {
-            for (int q = 0; q < 4; q++) a.ranges[q] = ok ? rg[q] : -1;
+            for (int q = 0; q < 4; q++) ranges[q] = ok ? rg[q] : -1;
             if (ok)
-                for (int z = 1; z < a.nz; z++) a.krad[z] = kr[z];
+                for (int z = 1; z < a.nz; z++) krad[z] = kr[z];
         }
     }
     __syncthreads();
     // zoncmb + scaled for this ring
-    const size_t row = (size_t)(ir - 1) * na;
+    const size_t row = mo + (size_t)(ir - 1) * na;
     for (int j = tid; j < na; j += SOLVE_BLOCK) {
         const size_t idx = row + j;
-        double acc = (double)a.mfx[0][idx] * a.q;
+        double acc = (double)a.mfx[0][idx] * q;
         for (int z = 2; z <= a.nz; z++) {
             if (a.use_new[z - 1])
-                acc = acc + a.fnew[z - 1] * ((double)a.mfx[z - 1][idx] * a.q);
+                acc = acc + a.fnew[z - 1] * ((double)a.mfx[z - 1][idx] * q);
             if (a.use_old[z - 1])
-                acc = acc + a.fold[z - 1] * ((double)a.moz[z - 1][idx] * a.q);
+                acc = acc + a.fold[z - 1] * ((double)a.moz[z - 1][idx] * q);
         }
-        R v = (R)(acc * a.pmass);
+        R v = (R)(acc * pmass);
         rho_s[j] = v;
-        if (h == 0) a.rho[idx] = v;
+        if (h == 0) rho[idx - mo] = v;
     }
     for (int j = tid; j < 2 * na; j += SOLVE_BLOCK) tws[j] = a.tw[j];
     __syncthreads();
@@ -94,30 +104,34 @@ __global__ __launch_bounds__(SOLVE_BLOCK) void analyse_kernel(AnaArgs<R> a) {
         }
         if (lane == 0) {
             if (m == 0) {
-                a.coef[ir - 1] = sa;
+                coef[ir - 1] = sa;
             } else if (2 * m == na) {
-                a.coef[(size_t)(2 * m - 1) * a.nr + ir - 1] = sa;
+                coef[(size_t)(2 * m - 1) * a.nr + ir - 1] = sa;
             } else {
-                a.coef[(size_t)(2 * m - 1) * a.nr + ir - 1] = sa;
-                a.coef[(size_t)(2 * m) * a.nr + ir - 1] = -sb;
+                coef[(size_t)(2 * m - 1) * a.nr + ir - 1] = sa;
+                coef[(size_t)(2 * m) * a.nr + ir - 1] = -sb;
             }
         }
     }
 }
 
-// grid (nr, SYN_SPLIT): block (ir, q) forms the ring's contraction (small)
-// and synthesises spokes [q na/SYN_SPLIT, (q+1) na/SYN_SPLIT).
+// grid (nr, SYN_SPLIT, nmem): block (ir, q, mem) forms the ring's
+// contraction (small) and synthesises spokes [q na/SYN_SPLIT,
+// (q+1) na/SYN_SPLIT) of grid mem.
 template <typename R>
 __global__ __launch_bounds__(SOLVE_BLOCK) void synth_kernel(SynArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     __shared__ R W[3][2][MAX_NG];
     __shared__ R tws[2 * MAX_NA];
     const int tid = threadIdx.x, na = a.na, nr = a.nr, ir = blockIdx.x + 1;
-    const int q = blockIdx.y;
+    const int q = blockIdx.y, mem = blockIdx.z;
     const int jlo = (q * na) / SYN_SPLIT, jhi = ((q + 1) * na) / SYN_SPLIT;
+    const size_t mo = (size_t)mem * nr * na;
+    const int *const ranges = a.ranges + mem * 4;
+    const R *const coef = a.coef + (size_t)mem * a.ncoef * nr;
     // masscl of the coming push, on this block's cells
     if (a.masscl_z > 0) {
-        const size_t row = (size_t)(ir - 1) * na;
+        const size_t row = mo + (size_t)(ir - 1) * na;
         for (int j = jlo + tid; j < jhi; j += SOLVE_BLOCK) {
             const size_t idx = row + j;
             for (int z = 1; z <= a.masscl_z; z++) {
@@ -126,12 +140,12 @@ __global__ __launch_bounds__(SOLVE_BLOCK) void synth_kernel(SynArgs<R> a) {
             }
         }
         if (blockIdx.x == 0 && q == 0 && tid < a.masscl_z) {
-            a.jrad[tid] = nr;
-            a.krad[tid] = 0;
+            a.jrad[mem * GH_MAXZ + tid] = nr;
+            a.krad[mem * GH_MAXZ + tid] = 0;
         }
     }
-    const int jrs = a.ranges[0], krs = a.ranges[1];
-    const int jrf = a.ranges[2], krf = a.ranges[3];
+    const int jrs = ranges[0], krs = ranges[1];
+    const int jrf = ranges[2], krf = ranges[3];
     if (jrs < 1 || ir < jrf || ir > krf) return;
     for (int j = tid; j < 2 * na; j += SOLVE_BLOCK) tws[j] = a.tw[j];
     const int ntypes = a.lpot ? 3 : 2;
@@ -147,8 +161,8 @@ __global__ __launch_bounds__(SOLVE_BLOCK) void synth_kernel(SynArgs<R> a) {
             const bool pair = m >= 1 && m < mmax - 1;
             const int off = m == 0 ? 0 : 2 * m - 1;
             const R *g = a.G + (((size_t)type * a.ng + m) * nr + (ir - 1)) * nr;
-            const R *mc = a.coef + (size_t)off * nr;
-            const R *ms = a.coef + (size_t)(off + 1) * nr;
+            const R *mc = coef + (size_t)off * nr;
+            const R *ms = coef + (size_t)(off + 1) * nr;
             for (int is = jrs + lane; is <= krs; is += 64) {
                 R gv = g[is - 1];
                 R c = mc[is - 1];
@@ -191,7 +205,7 @@ __global__ __launch_bounds__(SOLVE_BLOCK) void synth_kernel(SynArgs<R> a) {
         R2 out;
         out.x = F[0] * cp - F[1] * sp;
         out.y = F[0] * sp + F[1] * cp;
-        const size_t idx = (size_t)(ir - 1) * na + j;
+        const size_t idx = mo + (size_t)(ir - 1) * na + j;
         a.fld[idx] = out;
         if (a.lpot) a.pot[idx] = F[2];
     }
@@ -286,19 +300,21 @@ __global__ void fill_kernel(T *__restrict__ p, T v, long long n) {
 }
 
 // masscl (masscl.f:67-146) as a kernel: save zones 2..mz, clear zones 1..mz,
-// reset jrad/krad of those zones.
-__global__ void masscl_kernel(ZonePtrs zp, int mz, long long mesh, int *jrad,
-                              int *krad, int nr) {
+// reset jrad/krad of those zones, for the ncell = nmem x mesh cells of nmem
+// grids (ring ranges GH_MAXZ apart).
+__global__ void masscl_kernel(ZonePtrs zp, int mz, long long ncell, int *jrad,
+                              int *krad, int nr, int nmem) {
     long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
-    if (i < mesh) {
+    if (i < ncell) {
         for (int z = 1; z <= mz; z++) {
             if (z >= 2) zp.moz[z - 1][i] = zp.mfx[z - 1][i];
             zp.mfx[z - 1][i] = 0;
         }
     }
-    if (i < mz) {
-        jrad[i] = nr;
-        krad[i] = 0;
+    if (i < (long long)mz * nmem) {
+        const int m = (int)(i / mz), z = (int)(i - (long long)m * mz);
+        jrad[m * GH_MAXZ + z] = nr;
+        krad[m * GH_MAXZ + z] = 0;
     }
 }
 
@@ -307,14 +323,16 @@ __global__ void masscl_kernel(ZonePtrs zp, int mz, long long mesh, int *jrad,
 template <typename R>
 hipError_t launch_analyse(const AnaArgs<R> &a, hipStream_t s) {
     if (a.na > MAX_NA) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(analyse_kernel<R>, dim3(a.nr, ANA_SPLIT), dim3(SOLVE_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(analyse_kernel<R>, dim3(a.nr, ANA_SPLIT, a.nmem), dim3(SOLVE_BLOCK), 0,
+                       s, a);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_synth(const SynArgs<R> &a, hipStream_t s) {
     if (a.ng > MAX_NG) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(synth_kernel<R>, dim3(a.nr, SYN_SPLIT), dim3(SOLVE_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(synth_kernel<R>, dim3(a.nr, SYN_SPLIT, a.nmem), dim3(SOLVE_BLOCK), 0, s,
+                       a);
     return hipGetLastError();
 }
 
@@ -351,10 +369,11 @@ hipError_t launch_fill_i32(int *p, int v, long long n, hipStream_t s) {
 }
 
 hipError_t launch_masscl(const ZonePtrs &zp, int mz, long long mesh, int *jrad,
-                         int *krad, int nr, hipStream_t s) {
-    long long n = mesh > mz ? mesh : mz;
+                         int *krad, int nr, int nmem, hipStream_t s) {
+    const long long ncell = mesh * nmem, nz = (long long)mz * nmem;
+    long long n = ncell > nz ? ncell : nz;
     hipLaunchKernelGGL(masscl_kernel, dim3((unsigned)((n + GH_BLOCK - 1) / GH_BLOCK)),
-                       dim3(GH_BLOCK), 0, s, zp, mz, mesh, jrad, krad, nr);
+                       dim3(GH_BLOCK), 0, s, zp, mz, ncell, jrad, krad, nr, nmem);
     return hipGetLastError();
 }
 

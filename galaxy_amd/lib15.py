@@ -42,7 +42,14 @@ class P2DGrid:
     def __init__(self, nr, na, hash_, softl, tsoft=1, nsect=1, skip=(),
                  nzones=1, nstep=(1,), zbr=(0.0,), offanal=True, pmass=1.0,
                  precision="fp32"):
-        lib = load()
+        p = self._params(nr, na, hash_, softl, tsoft, nsect, skip, nzones, nstep, zbr,
+                         offanal, pmass, precision)
+        h = C.c_void_p()
+        check(load().galhip_p2dset(C.byref(p), C.byref(h)), "p2dset")
+        self._init_handle(h, nr, na, nsect, nzones)
+
+    def _params(self, nr, na, hash_, softl, tsoft, nsect, skip, nzones, nstep, zbr,
+                offanal, pmass, precision):
         p = P2DParams()
         p.nr, p.na, p.hash, p.nsect = nr, na, hash_, nsect
         skip = list(skip)
@@ -65,8 +72,10 @@ class P2DGrid:
         p.precision = GALHIP_FP32 if precision == "fp32" else GALHIP_FP64
         self.dtype = np.float32 if precision == "fp32" else np.float64
         self.precision = precision
-        h = C.c_void_p()
-        check(lib.galhip_p2dset(C.byref(p), C.byref(h)), "p2dset")
+        return p
+
+    def _init_handle(self, h, nr, na, nsect, nzones):
+        lib = load()
         self._lib, self._h = lib, h
         g = np.zeros(6)
         check(lib.galhip_geometry(h, _ptr(g)), "geometry")
@@ -199,6 +208,10 @@ class P2DGrid:
 
     def set_sort_interval(self, k):
         self._c(self._lib.galhip_set_sort_interval(self._h, int(k)), "set_sort_interval")
+
+    def set_graph(self, on=True):
+        """Run the steps of run() as replayed HIP graphs."""
+        self._c(self._lib.galhip_set_graph(self._h, int(bool(on))), "set_graph")
 
     # -- group level ---------------------------------------------------------------
     def weight(self, x, y, old=True):
@@ -386,3 +399,116 @@ class P3DGrid(P2DGrid):
         if f.shape[0] == 3:
             f = np.concatenate([f, np.zeros((1, self.mesh), self.dtype)])
         self._c(self._lib.galhip_set_field(self._h, _ptr(f)), "set_field")
+
+
+class P2DEnsemble(P2DGrid):
+    """``nmem`` independent p2d discs on one grid geometry (galhip_ensset).
+
+    Members share the grid, filter, zones, softening, Green's table, fixed
+    force and potential tables and the step counter; each has its own
+    particles and particle mass (``pmass``, one per member).  greenm, masset,
+    findf, step, tmcent, run, sort and the profiling calls act on every
+    member; particles, grids and integrals are per member.  No counterpart in
+    the reference, which runs one model per process.
+    """
+
+    def __init__(self, nmem, nr, na, hash_, softl, pmass, tsoft=1, nsect=1, skip=(),
+                 nzones=1, nstep=(1,), zbr=(0.0,), offanal=True, precision="fp32"):
+        pm = np.ascontiguousarray(pmass, np.float64)
+        if pm.shape != (nmem,):
+            raise ValueError("one particle mass per member")
+        p = self._params(nr, na, hash_, softl, tsoft, nsect, skip, nzones, nstep, zbr,
+                         offanal, 1.0, precision)
+        h = C.c_void_p()
+        check(load().galhip_ensset(C.byref(p), int(nmem), _ptr(pm), C.byref(h)), "ensset")
+        self._init_handle(h, nr, na, nsect, nzones)
+        self.nmem = int(nmem)
+        self.counts = np.zeros(self.nmem, np.int64)
+
+    def restrt(self, members):
+        """members: one (x, y, vx, vy[, iz]) tuple per member."""
+        if len(members) != self.nmem:
+            raise ValueError("one particle set per member")
+        dt = self.dtype
+        cols = [[], [], [], [], []]
+        counts = []
+        for mem in members:
+            x = np.ascontiguousarray(mem[0], dt)
+            n = len(x)
+            counts.append(n)
+            cols[0].append(x)
+            for k in (1, 2, 3):
+                cols[k].append(np.ascontiguousarray(mem[k], dt))
+            iz = mem[4] if len(mem) > 4 and mem[4] is not None else np.ones(n, np.int32)
+            cols[4].append(np.ascontiguousarray(iz, np.int32))
+        arrs = [np.concatenate(c) if c else np.zeros(0) for c in cols]
+        arrs = [a.astype(dt) for a in arrs[:4]] + [arrs[4].astype(np.int32)]
+        self.counts = np.asarray(counts, np.int64)
+        self._c(self._lib.galhip_ens_restrt(self._h, _ptr(self.counts),
+                                            *[_ptr(a) for a in arrs]), "ens_restrt")
+        self.n = int(self.counts.sum())
+
+    def unload(self):
+        """A list of per-member dicts x, y, vx, vy, iz."""
+        n = self.n
+        out = {k: np.empty(n, self.dtype) for k in ("x", "y", "vx", "vy")}
+        out["iz"] = np.empty(n, np.int32)
+        self._c(self._lib.galhip_ens_unload(self._h, _ptr(out["x"]), _ptr(out["y"]),
+                                            _ptr(out["vx"]), _ptr(out["vy"]),
+                                            _ptr(out["iz"])), "ens_unload")
+        off = np.concatenate([[0], np.cumsum(self.counts)])
+        return [{k: v[off[m]:off[m + 1]] for k, v in out.items()} for m in range(self.nmem)]
+
+    def get_mass(self, member):
+        mfx = np.zeros((self.nzones, self.mesh), np.int64)
+        moz = np.zeros((self.nzones, self.mesh), np.int64)
+        jr = np.zeros(self.nzones, np.int32)
+        kr = np.zeros(self.nzones, np.int32)
+        qb = C.c_int32()
+        self._c(self._lib.galhip_ens_get_mass(self._h, int(member), _ptr(mfx), _ptr(moz),
+                                              _ptr(jr), _ptr(kr), C.byref(qb)), "ens_get_mass")
+        return mfx, moz, jr, kr, qb.value
+
+    def get_field(self, member):
+        f = np.zeros((3, self.mesh), self.dtype)
+        self._c(self._lib.galhip_ens_get_field(self._h, int(member), _ptr(f)), "ens_get_field")
+        return f
+
+    def set_field(self, member, fld):
+        f = np.ascontiguousarray(fld, self.dtype)
+        if f.shape[0] == 2:
+            f = np.concatenate([f, np.zeros((1, self.mesh), self.dtype)])
+        self._c(self._lib.galhip_ens_set_field(self._h, int(member), _ptr(f)), "ens_set_field")
+
+    def ranges(self, member):
+        r = np.zeros(4, np.int32)
+        self._c(self._lib.galhip_ens_get_ranges(self._h, int(member), _ptr(r)), "ens_get_ranges")
+        return tuple(int(v) for v in r)
+
+    @property
+    def noff(self):
+        v = np.zeros(self.nmem, np.int64)
+        self._c(self._lib.galhip_ens_get_noff(self._h, _ptr(v)), "ens_get_noff")
+        return v
+
+    def integrals(self, centred=True):
+        """P2DGrid.integrals for every member: a list of dicts."""
+        out = np.zeros((self.nmem, 9), np.float64)
+        self._c(self._lib.galhip_ens_integrals(self._h, int(bool(centred)), _ptr(out)),
+                "ens_integrals")
+        res = []
+        for o in out:
+            d = dict(mass=o[0], cm=o[1:3] / o[0] if o[0] else o[1:3], momentum=o[3:5],
+                     ke=0.5 * o[5], lz=o[6], w_grid=o[7], w_fixed=o[8])
+            d["energy"] = d["ke"] + 0.5 * d["w_grid"] + d["w_fixed"]
+            res.append(d)
+        return res
+
+    def get_rho(self):
+        raise NotImplementedError("per-member rho is not exported; use get_field")
+
+    def set_mass(self, *a, **k):
+        raise NotImplementedError("set_mass is per grid; not available on an ensemble")
+
+    def grdacc(self, x, y):
+        raise NotImplementedError("an ensemble has one field per member")

@@ -275,6 +275,54 @@ int galhip_weight3(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
 int galhip_grdacc3(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
                    const void *z, void *acc);
 
+/* ---- ensembles of independent p2d discs ----
+ * No counterpart in the reference, which runs one model per process (its
+ * up to four grids, SRC/lib15/inc/params.f:17-18, are nested grids of one
+ * model).  nmem discs share one grid geometry, harmonic filter, zone set-up,
+ * softening and Green's table (the p2d parameters), the fixed-force and
+ * fixed-potential tables and the step counter; each member has its own
+ * particles, particle mass, zone mass arrays, ring ranges and field.  One
+ * launch per kernel advances every member.  A member's results are
+ * bit-identical to the same disc run alone in a galhip_p2dset context with
+ * the same parameters and particle mass.
+ * On an ensemble context the generic entry points galhip_free,
+ * galhip_geometry, greenm / get_green / set_green, set_fixforce,
+ * set_fixpot, set_step / get_step, masset, findf, p2fndf, step, tmcent,
+ * run, weight, set_sort_interval, sort, get_order (load-order index of the
+ * concatenated members), set_graph, the profiling calls (pushes summed over
+ * members) and sync act on every member; galhip_restrt, unload, grdacc,
+ * get_mass, set_mass, get_rho, get_field, set_field, get_ranges, get_noff
+ * and integrals return GALHIP_EINVAL: use the galhip_ens_* calls below. */
+#define GALHIP_MAXMEMBERS 4096
+
+/* An ensemble of nmem discs on the grid of p (p->pmass is ignored);
+ * pmass[nmem] the particle mass of each member. */
+int galhip_ensset(const galhip_p2d_params *p, int32_t nmem, const double *pmass,
+                  galhip_ctx **out);
+/* Load the members' particles: count[nmem] particles each, the arrays
+ * holding member 0's particles, then member 1's, ... (zones as for
+ * galhip_restrt; at most 2^31 - 1 particles in all). */
+int galhip_ens_restrt(galhip_ctx *ctx, const int64_t *count, const void *x, const void *y,
+                      const void *vx, const void *vy, const int32_t *iz);
+/* Copy every member's particles back in the layout galhip_ens_restrt took. */
+int galhip_ens_unload(galhip_ctx *ctx, void *x, void *y, void *vx, void *vy, int32_t *iz);
+/* galhip_get_mass, get_field, set_field and get_ranges of one member. */
+int galhip_ens_get_mass(const galhip_ctx *ctx, int32_t member, int64_t *mfx, int64_t *mozfx,
+                        int32_t *jrad, int32_t *krad, int32_t *qbits);
+int galhip_ens_get_field(const galhip_ctx *ctx, int32_t member, void *fld);
+int galhip_ens_set_field(galhip_ctx *ctx, int32_t member, const void *fld);
+int galhip_ens_get_ranges(const galhip_ctx *ctx, int32_t member, int32_t *r4);
+/* Off-grid counts noff[nmem]. */
+int galhip_ens_get_noff(const galhip_ctx *ctx, int64_t *noff);
+/* galhip_integrals of every member: out[nmem][9]. */
+int galhip_ens_integrals(galhip_ctx *ctx, int32_t centred, double *out);
+
+/* Run galhip_run's steps as HIP graphs (p2d grids and ensembles): each
+ * step's three launches are captured once per distinct set of launch
+ * arguments and replayed.  Results are identical to on = 0.  While
+ * profiling is on, galhip_run launches directly. */
+int galhip_set_graph(galhip_ctx *ctx, int32_t on);
+
 #ifdef __cplusplus
 }
 #endif
