@@ -65,10 +65,10 @@ PUSH_VALU_PER_PUSH = 521.0
 # scaled by the same copy calibration)
 P3_PUSH_HBM_READ_B = 35.9
 P3_PUSH_HBM_WRITE_B = 44.8
-# the same for push_kernel<float, true> on the 64-disc ensemble (None until
-# measured)
-ENS_PUSH_HBM_READ_B = None
-ENS_PUSH_HBM_WRITE_B = None
+# the same for push_kernel<float, true> on the 64-disc ensemble (mean of the
+# dispatches in profiles/push_kernel_ens-64x128x256-n8.6e7_traffic.txt)
+ENS_PUSH_HBM_READ_B = 26.7
+ENS_PUSH_HBM_WRITE_B = 27.5
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 
