@@ -367,6 +367,69 @@ def main_ens(args):
     print(json.dumps(line), flush=True)
 
 
+def main_adj(args):
+    """--workload C5: the differentiated run (128 x 256 grid, N = 1e6, Q = 1.2,
+    200 steps): |A_2|^2 at the end and its gradient in every initial
+    velocity, Q and the disc mass by the adjoint kernels with checkpoints
+    every 20 steps; the FP64 forward's gradient and a central difference
+    along one random velocity direction as the checks."""
+    from galaxy_amd import adjoint, configs
+    w = configs.C5
+    n = args.n or w.n
+    t0 = time.time()
+    m32 = adjoint.DiscModel(w, precision="fp32", n=n)
+    log("[bench] adjoint model set-up %.1f s (N=%d)" % (time.time() - t0, n))
+    if args.warmup:
+        m32.gradient(w.Q, w.md, min(args.warmup, 20), every=20)
+    t0 = time.perf_counter()
+    g32 = m32.gradient(w.Q, w.md, args.steps, every=20)
+    t32 = time.perf_counter() - t0
+    log("[bench] adjoint fp32: %.3f s (%s)" % (t32, g32["seconds"]))
+    extra = dict(loss=g32["loss"], dQ=g32["dQ"], dmd=g32["dmd"], every=20,
+                 seconds_fp32={k: round(v, 4) for k, v in g32["seconds"].items()})
+    if not args.no_fp64:
+        m64 = adjoint.DiscModel(w, precision="fp64", n=n)
+        t0 = time.perf_counter()
+        g64 = m64.gradient(w.Q, w.md, args.steps, every=20)
+        t64 = time.perf_counter() - t0
+        v32 = np.concatenate([g32["dvx"], g32["dvy"]])
+        v64 = np.concatenate([g64["dvx"], g64["dvy"]])
+        rng = np.random.default_rng(17)
+        dx, dy = rng.standard_normal(n), rng.standard_normal(n)
+        vx, vy = m64.velocities(w.Q, w.md)
+        h = 1e-6 * float(np.sqrt(np.mean(vx * vx + vy * vy)))
+        # the annulus membership of the unperturbed end state held fixed, as
+        # the adjoint holds it (at this N single particles crossing R = 1 or
+        # 3 would otherwise dominate the difference)
+        lp = m64.loss_v(vx + h * dx, vy + h * dy, w.md, args.steps, sel=g64["sel"])
+        lm = m64.loss_v(vx - h * dx, vy - h * dy, w.md, args.steps, sel=g64["sel"])
+        fd = (lp - lm) / (2 * h)
+        ad = float(np.dot(g64["dvx"], dx) + np.dot(g64["dvy"], dy))
+        extra.update(fp64_seconds=round(t64, 3), loss_fp64=g64["loss"],
+                     grad_v_rel_err_fp32_vs_fp64=float(np.linalg.norm(v32 - v64)
+                                                       / np.linalg.norm(v64)),
+                     dQ_fp64=g64["dQ"], dmd_fp64=g64["dmd"],
+                     fd_direction=dict(adjoint=ad, central_difference=fd, h=h,
+                                       rel_err=abs(ad - fd) / abs(fd)))
+    value = n * args.steps / t32
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, args.cpu_n // 2, args.cpu_steps)
+        cpu["sample"] = "forward only (no CPU adjoint exists): " + cpu["sample"]
+    line = {
+        "metric": "particle-steps/s of the differentiated run (forward + adjoint)",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t32 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded exponential disc, Q = 1.2)",
+        "config": {"workload": w.name, "grid": [w.nr, w.na], "n": n,
+                   "harmonics": "m<=%d" % w.hash, "dt": w.dt, "loss": "|A2|^2, R in [1, 3]",
+                   "checkpoint_every": 20, "parallelism": "none"},
+        "roofline": None, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes):
     """A plain elementwise copy of known size (FP32 x 1.0 into a second
     buffer: nbytes read, nbytes written), run three times; its FETCH_SIZE and
@@ -396,7 +459,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain copy kernel of BYTES (counter calibration)")
-    ap.add_argument("--workload", default="C2", choices=("C2", "C3", "C4"),
+    ap.add_argument("--workload", default="C2", choices=("C2", "C3", "C4", "C5"),
                     help="C2: 2-D grid, the headline (default); C3: 3-D grid; "
                          "C4: ensemble of 64 2-D discs")
     ap.add_argument("--graph", type=int, default=None, choices=(0, 1),
@@ -412,6 +475,11 @@ def main():
               "multi-GPU form is a design, not measured); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "C5":
+        if args.steps == 500:
+            args.steps = 200
+        main_adj(args)
+        return
     if args.workload == "C4":
         if args.steps == 500:
             args.steps = 300

@@ -1,0 +1,269 @@
+"""Gradients of a p2d run by the adjoint kernels (galhip_adj_*), with
+checkpoint/recompute, and a torch.autograd.Function around them.
+
+No counterpart in the reference.  The differentiated run is galaxy's loop
+(findf, step) for one zone and one sector: from initial internal-unit
+positions and leap-frog velocities (x0, v0) to the final positions.  The
+loss used here is |A_2|^2, the squared m = 2 Fourier coefficient of the
+particles between R = 1 and 3 disc scale lengths at the final step
+(A_2 = sum e^{-2 i phi} / count over that annulus, equal masses).
+
+Backward pass, for checkpoints every `every` steps: each segment is
+recomputed from its checkpoint, recording the state before each step in a
+device slot; then, from the segment's last step down, the state is
+restored, masset + findf rebuild that step's mass and field (bit for bit:
+the mass sums are integers), and galhip_adj_step takes the adjoints one
+step back.  Parameter terms come out of the same pass: gpar[0] (the
+derivative for a factor on the particle mass, hence the grid force) and
+gpar[1] (the fixed force's parameter derivative table).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import configs, disc
+
+
+def a2_loss(x, y, lscale, r1=1.0, r2=3.0, sel=None):
+    """L = |A_2|^2 over r1 <= R <= r2 (R = r / lscale) and dL/dx, dL/dy
+    (membership of the annulus held fixed), FP64.  sel: a given membership
+    mask instead (a finite-difference check of that derivative at large N,
+    where single particles crossing the annulus edges make L jump)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    r2_ = x * x + y * y
+    R = np.sqrt(r2_) / lscale
+    if sel is None:
+        sel = (R >= r1) & (R <= r2)
+    cnt = max(int(sel.sum()), 1)
+    ph = np.arctan2(y, x)
+    c2, s2 = np.cos(2 * ph), np.sin(2 * ph)
+    A = (c2[sel].sum() - 1j * s2[sel].sum()) / cnt
+    L = abs(A) ** 2
+    # dL/dphi = 2 Re(conj(A) dA/dphi), dA/dphi_j = -2i e^{-2i phi_j} / cnt
+    dphi = np.where(sel, 2.0 * np.real(np.conj(A) * (-2j) * (c2 - 1j * s2)) / cnt, 0.0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        gx = np.where(r2_ > 0, -dphi * y / r2_, 0.0)
+        gy = np.where(r2_ > 0, dphi * x / r2_, 0.0)
+    return L, gx, gy
+
+
+class Trajectory:
+    """A p2d grid run for `nsteps` steps with checkpoints every `every`
+    steps, and its adjoint.  grid: a P2DGrid with one zone and one sector,
+    Green's tables and fixed force set."""
+
+    def __init__(self, grid, nsteps, every=20):
+        self.g, self.nsteps, self.every = grid, int(nsteps), int(every)
+        self.nck = (self.nsteps + self.every - 1) // self.every
+        self.seconds = dict(forward=0.0, recompute=0.0, backward=0.0)
+
+    def forward(self, x, y, vx, vy, iz):
+        """Load, run, return the final state (dict in load order)."""
+        g = self.g
+        g.set_sort_interval(10)
+        g.restrt(x, y, vx, vy, iz)
+        g.masset()
+        g.sync()
+        t0 = time.perf_counter()
+        for s in range(self.nsteps):
+            if s % self.every == 0:
+                g.adj_record(s // self.every)
+            g.run(1)
+        g.sync()
+        self.seconds["forward"] += time.perf_counter() - t0
+        return g.unload()
+
+    def backward(self, lam):
+        """lam [4][n]: adjoints of the final (x, y, vx, vy).  Returns the
+        adjoints of the initial state and gpar [2]."""
+        g, E = self.g, self.every
+        g.adj_set(lam)
+        gpar = np.zeros(2, np.float64)
+        base = self.nck
+        g.sync()
+        for k in range(self.nck - 1, -1, -1):
+            m = min(E, self.nsteps - k * E)
+            t0 = time.perf_counter()
+            g.adj_restore(k)
+            g.masset()
+            for j in range(m):
+                g.adj_record(base + j)
+                g.run(1)
+            g.sync()
+            t1 = time.perf_counter()
+            for j in range(m - 1, -1, -1):
+                g.adj_restore(base + j)
+                g.masset()
+                g.findf(False)
+                g.adj_step(gpar)
+            g.sync()
+            t2 = time.perf_counter()
+            self.seconds["recompute"] += t1 - t0
+            self.seconds["backward"] += t2 - t1
+        return g.adj_get(), gpar
+
+
+class DiscModel:
+    """The differentiated workload: a seeded exponential disc whose initial
+    velocities depend on Toomre Q and the disc mass md (sigma_R = Q md
+    k_sig(R)), in a rigid halo that keeps the rotation curve fixed, so the
+    halo force depends on md too (F_halo = -v_c^2/R - md F_disc1(R))."""
+
+    def __init__(self, w: configs.Workload, precision="fp32", n=None):
+        from .lib15 import P2DGrid
+        self.w = w
+        self.n = w.n if n is None else int(n)
+        self.precision = precision
+        self.ls = w.lscale()
+        kw = configs.grid_kwargs(w, self.n)
+        self._kw = kw
+        g = P2DGrid(precision=precision, **kw)
+        g.greenm()
+        self.G = g.get_green()
+        self.rgrid = float(g.rgrid)
+        g.close()
+        R, phi, g1, g2 = disc.disc_draws(self.n, w.seed, w.rtrunc)
+        self.R, self.phi, self.g1, self.g2 = R, phi, g1, g2
+        self.k_sig, self.k_phi, self.vc2, self.c = disc.velocity_terms(R, w.rtrunc, w.rcore)
+        self.x0 = R * np.cos(phi) * self.ls
+        self.y0 = R * np.sin(phi) * self.ls
+        self.iz0 = np.where(R * self.ls > self.rgrid, 2, 1).astype(np.int32)
+
+    def velocities(self, Q, md):
+        """Internal-unit initial velocities (numpy FP64) for Q, md."""
+        sR = Q * md * self.k_sig
+        vm = np.sqrt(np.maximum(self.vc2 + sR * sR * self.c, 0.0))
+        vR = sR * self.g1
+        vphi = vm + sR * self.k_phi * self.g2
+        f = self.ls * self.w.dt
+        cp, sp = np.cos(self.phi), np.sin(self.phi)
+        return (vR * cp - vphi * sp) * f, (vR * sp + vphi * cp) * f
+
+    def grid(self, md):
+        """A fresh grid for disc mass md: particle mass and halo force scale
+        with it; the parameter derivative table is dF_halo/dmd."""
+        from .lib15 import P2DGrid
+        w, ls, ts = self.w, self.ls, self.w.dt
+        kw = dict(self._kw)
+        kw["pmass"] = md * ls ** 3 * ts * ts / self.n
+        g = P2DGrid(precision=self.precision, **kw)
+        g.set_green(self.G)
+        g.set_fixforce(*disc.halo_table(self.rgrid, ls, ts, md=md, rcore=w.rcore))
+        dr, v1, d1 = disc.halo_table(self.rgrid, ls, ts, md=1.0, rcore=w.rcore)
+        _, v0, d0 = disc.halo_table(self.rgrid, ls, ts, md=0.0, rcore=w.rcore)
+        g.set_fixforce_dpar(dr, v1 - v0, d1 - d0)
+        return g
+
+    def loss(self, Q, md, nsteps):
+        """Forward only: L for (Q, md) (no checkpoints)."""
+        vx, vy = self.velocities(Q, md)
+        return self.loss_v(vx, vy, md, nsteps)
+
+    def loss_v(self, vx, vy, md, nsteps, sel=None):
+        g = self.grid(md)
+        g.restrt(self.x0, self.y0, vx, vy, self.iz0)
+        g.set_sort_interval(10)
+        g.masset()
+        g.run(int(nsteps))
+        out = g.unload()
+        g.close()
+        return a2_loss(out["x"], out["y"], self.ls, sel=sel)[0]
+
+    def gradient(self, Q, md, nsteps, every=20):
+        """L and its gradient: dL/dvx0, dL/dvy0 (internal units), dL/dQ,
+        dL/dmd, through the adjoint kernels."""
+        vx, vy = self.velocities(Q, md)
+        g = self.grid(md)
+        tr = Trajectory(g, nsteps, every)
+        out = tr.forward(self.x0, self.y0, vx, vy, self.iz0)
+        L, gx, gy = a2_loss(out["x"], out["y"], self.ls)
+        R = np.hypot(out["x"].astype(np.float64), out["y"].astype(np.float64)) / self.ls
+        sel = (R >= 1.0) & (R <= 3.0)
+        lam = np.zeros((4, self.n))
+        lam[0], lam[1] = gx, gy
+        lam0, gpar = tr.backward(lam)
+        g.close()
+        lvx, lvy = lam0[2], lam0[3]
+        dQ, dmd_ic = self._ic_chain(Q, md, lvx, lvy)
+        dmd = dmd_ic + gpar[0] / md + gpar[1]
+        return dict(loss=L, dvx=lvx, dvy=lvy, dQ=dQ, dmd=dmd, gpar=gpar,
+                    seconds=dict(tr.seconds), sel=sel)
+
+    def _ic_chain(self, Q, md, lvx, lvy):
+        """dL/dQ and the initial-velocity part of dL/dmd from the velocity
+        adjoints: both parameters enter only through sigma_R = Q md k_sig."""
+        sR = Q * md * self.k_sig
+        vphi2 = self.vc2 + sR * sR * self.c
+        vm = np.sqrt(np.maximum(vphi2, 0.0))
+        # d/d sR of (vR, vphi)
+        dvR = self.g1
+        with np.errstate(divide="ignore", invalid="ignore"):
+            dvm = np.where(vphi2 > 0, sR * self.c / np.where(vm > 0, vm, 1.0), 0.0)
+        dvphi = dvm + self.k_phi * self.g2
+        f = self.ls * self.w.dt
+        cp, sp = np.cos(self.phi), np.sin(self.phi)
+        dvx = (dvR * cp - dvphi * sp) * f
+        dvy = (dvR * sp + dvphi * cp) * f
+        # sR_i = Q md k_sig_i: dsR_i/dQ = sR_i / Q, dsR_i/dmd = sR_i / md
+        dQ = float(np.dot(lvx, dvx * (sR / Q)) + np.dot(lvy, dvy * (sR / Q)))
+        return dQ, dQ * Q / md
+
+
+def torch_function():
+    """A torch.autograd.Function mapping (vx0, vy0, md) to the final
+    (x, y) of a DiscModel run, its backward the adjoint pass."""
+    import torch
+
+    class PMRun(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, vx0, vy0, md, model, nsteps, every):
+            g = model.grid(float(md))
+            tr = Trajectory(g, nsteps, every)
+            out = tr.forward(model.x0, model.y0, vx0.detach().cpu().numpy(),
+                             vy0.detach().cpu().numpy(), model.iz0)
+            ctx.tr, ctx.g, ctx.md = tr, g, float(md)
+            return (torch.from_numpy(out["x"].astype(np.float64)),
+                    torch.from_numpy(out["y"].astype(np.float64)))
+
+        @staticmethod
+        def backward(ctx, gx, gy):
+            n = gx.shape[0]
+            lam = np.zeros((4, n))
+            lam[0] = gx.detach().cpu().numpy()
+            lam[1] = gy.detach().cpu().numpy()
+            lam0, gpar = ctx.tr.backward(lam)
+            ctx.g.close()
+            dmd = gpar[0] / ctx.md + gpar[1]
+            return (torch.from_numpy(lam0[2]), torch.from_numpy(lam0[3]),
+                    torch.tensor(dmd, dtype=torch.float64), None, None, None)
+
+    return PMRun
+
+
+def torch_loss(model, Q, md, nsteps, every=20):
+    """|A_2|^2 as a torch graph: initial velocities from (Q, md) in torch,
+    the run through the adjoint-backed Function, A_2 from the final
+    positions.  Q and md are 0-d FP64 tensors (requires_grad for their
+    gradients)."""
+    import torch
+    t = lambda a: torch.from_numpy(np.asarray(a, np.float64))
+    k_sig, k_phi, vc2, c = t(model.k_sig), t(model.k_phi), t(model.vc2), t(model.c)
+    g1, g2, phi = t(model.g1), t(model.g2), t(model.phi)
+    sR = Q * md * k_sig
+    vm = torch.sqrt(torch.clamp(vc2 + sR * sR * c, min=1e-300))
+    vR = sR * g1
+    vphi = vm + sR * k_phi * g2
+    f = model.ls * model.w.dt
+    vx = (vR * torch.cos(phi) - vphi * torch.sin(phi)) * f
+    vy = (vR * torch.sin(phi) + vphi * torch.cos(phi)) * f
+    x, y = torch_function().apply(vx, vy, md, model, nsteps, every)
+    R = torch.sqrt(x * x + y * y).detach() / model.ls
+    sel = (R >= 1.0) & (R <= 3.0)
+    ph = torch.atan2(y[sel], x[sel])
+    cnt = max(int(sel.sum()), 1)
+    ar = torch.cos(2 * ph).sum() / cnt
+    ai = torch.sin(2 * ph).sum() / cnt
+    return ar * ar + ai * ai

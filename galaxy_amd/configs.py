@@ -121,7 +121,12 @@ C4 = EnsembleWorkload("ens-64x(128x256)", nr=128, na=256, hash=8, n=0, steps=300
                       dt=0.05, seed=0)
 C4.n = int(C4.members()[0].sum())
 
-WORKLOADS = {"C1": C1, "C2": C2, "C3": C3, "C4": C4}
+# differentiated run: 128 x 256 grid, N = 1e6, Q = 1.2, 200 steps of dt =
+# 0.05, m <= 8 (assumed), one zone; loss |A_2|^2 over R = 1..3 at the end
+C5 = Workload("p2d-128x256-n1e6-adjoint", nr=128, na=256, hash=8, n=1_000_000,
+              steps=200, dt=0.05, Q=1.2, seed=5)
+
+WORKLOADS = {"C1": C1, "C2": C2, "C3": C3, "C4": C4, "C5": C5}
 
 
 def grid_kwargs(w: Workload, n=None):

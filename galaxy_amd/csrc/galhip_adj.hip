@@ -1,0 +1,589 @@
+// galhip_adj.hip -- reverse-mode derivative (adjoint) of the p2d leap-frog
+// step, for one zone and the whole grid (full steps).  The reference has no
+// adjoint; this differentiates the map the forward kernels compute:
+//
+//   rho   = pmass sum_i w(x_i)                 mass assignment (massgn)
+//   F     = K rho                              findf: transform, Green's
+//                                              contraction, synthesis, polcat
+//   a_i   = sum_q w_q(x_i) F(c_q) + h(r_i) x_i / r_i     grdacc + fixed force
+//   v'    = v + a,  x' = x + v'                stpgrp (off-grid: a = 0)
+//
+// Given the adjoints (lx, lv) of (x', v'), one backward step forms
+//   la = lv + lx                               (adjoint of v' and of a)
+//   g  = sum_i w(x_i) la_i                     adj_scatter: transpose of grdacc
+//   rb = K^T g                                 adj_wbar + adj_rbar
+//   lx += J_direct^T la + pmass sum_q rb(c_q) grad w_q      adj_gather
+//   lv  = la
+// with J_direct the derivative of a_i in x_i through the interpolation
+// weights and the fixed force.  The weights are CIC in the grid coordinates
+// u = log(1 + r) / alpha, t = atan2(y, x) na / 2 pi (weight.f:61-97), so
+// grad w_q = dw_q/du grad u + dw_q/dt grad t; the cell and range choices are
+// held fixed (the map is piecewise smooth).  Adjoints and grids are FP64;
+// the forward state (positions, (u, t), field) is the context's.
+#include "galhip_ctx.h"
+
+#include <cstring>
+#include <vector>
+
+namespace gh {
+
+template <typename R> struct AdjArgs {
+    const typename Vec4<R>::T *xv;
+    const typename Vec2<R>::T *ut;
+    const uint8_t *zone;
+    const unsigned *id;
+    long long n;
+    double *lam;                      // [4][n] in load order: x, y, vx, vy
+    double *gx, *gy;                  // [mesh]
+    const double *rbar;               // [mesh]
+    const typename Vec2<R>::T *fld;   // forward (F_x, F_y)
+    const double *fr_val, *fr_der;    // fixed force table
+    double fr_dr, fr_inv_dr;
+    int nfr;
+    const double *fd_val, *fd_der;    // its derivative in the model parameter
+    double fd_dr, fd_inv_dr;
+    int nfd;
+    double pmass, inv_alpha, tscale;
+    int nr, na, mesh;
+    double *part;                     // [blocks][2] parameter partial sums
+};
+
+struct AdjSolveArgs {
+    const double *gx, *gy;
+    double *wbar;                     // [2 types][2 (cos, sin)][ng][nr]
+    double *rbar;                     // [mesh]
+    const double *G;                  // FP64 tables [3][ng][nr][nr]
+    const double *twd;                // cos, sin of 2 pi k / na
+    const int *lg, *ranges;
+    int nr, na, ng, mmax;
+};
+
+// CIC weights and their derivatives in (u, t), with the cell choice of cic()
+__device__ __forceinline__ void cic_grad(double u, double t, int nr, int na, int &ring,
+                                         int &spoke, double w[4], double wu[4], double wt[4]) {
+    int ju = (int)u;
+    if (ju > nr - 2) ju = nr - 2;
+    const int jt = (int)t;
+    const double du = u - (double)ju, dt = t - (double)jt;
+    w[0] = (1.0 - du) * (1.0 - dt);
+    w[1] = dt * (1.0 - du);
+    w[2] = du * (1.0 - dt);
+    w[3] = du * dt;
+    wu[0] = -(1.0 - dt);
+    wu[1] = -dt;
+    wu[2] = 1.0 - dt;
+    wu[3] = dt;
+    wt[0] = -(1.0 - du);
+    wt[1] = 1.0 - du;
+    wt[2] = -du;
+    wt[3] = du;
+    ring = ju;
+    spoke = jt;
+    if (jt >= na) {
+        ring = ju + 1;
+        spoke = jt - na;
+    }
+}
+
+// Derivative in r of fixforce's cubic Hermite interpolant (0 past the end,
+// where the force is held at the last entry).
+__device__ __forceinline__ double fixforce_deriv(double r, int nfr, double dr, double inv_dr,
+                                                 const double *__restrict__ val,
+                                                 const double *__restrict__ der) {
+    bool past;
+    const int k = fixforce_index(r, nfr, inv_dr, past);
+    if (past) return 0.0;
+    const double t = r * inv_dr - (double)k;
+    const double d00 = 6.0 * t * t - 6.0 * t, d10 = 3.0 * t * t - 4.0 * t + 1.0;
+    const double d01 = -6.0 * t * t + 6.0 * t, d11 = 3.0 * t * t - 2.0 * t;
+    return (d00 * val[k] + d10 * dr * der[k] + d01 * val[k + 1] + d11 * dr * der[k + 1]) *
+           inv_dr;
+}
+
+// la = lv + lx stored as the new lv; on-grid particles scatter la with their
+// weights (the transpose of grdacc's interpolation)
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void adj_scatter_kernel(AdjArgs<R> a) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const long long n = a.n, p = a.id[i];
+    const double lax = a.lam[2 * n + p] + a.lam[p];
+    const double lay = a.lam[3 * n + p] + a.lam[n + p];
+    a.lam[2 * n + p] = lax;
+    a.lam[3 * n + p] = lay;
+    if (a.zone[i] != 1) return;
+    const typename Vec2<R>::T cu = a.ut[i];
+    double w[4], wu[4], wt[4];
+    int ring, spoke, ic[4];
+    cic_grad((double)cu.x, (double)cu.y, a.nr, a.na, ring, spoke, w, wu, wt);
+    corners_rs(ring, spoke, a.na, ic);
+    for (int q = 0; q < 4; q++)
+        if (ic[q] < a.mesh) {
+            atomicAdd(&a.gx[ic[q]], w[q] * lax);
+            atomicAdd(&a.gy[ic[q]], w[q] * lay);
+        }
+}
+
+// lx += J_direct^T la + pmass sum_q rb(c_q) grad w_q; block sums of the
+// parameter terms la . a_grid and la . (dF/dp)(r) x / r
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void adj_gather_kernel(AdjArgs<R> a) {
+    __shared__ double red[GH_BLOCK / 64][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long i = (long long)blockIdx.x * GH_BLOCK + tid;
+    double p0 = 0.0, p1 = 0.0;
+    if (i < a.n && a.zone[i] == 1) {
+        const long long n = a.n, p = a.id[i];
+        const double lax = a.lam[2 * n + p], lay = a.lam[3 * n + p];
+        const typename Vec4<R>::T st = a.xv[i];
+        const typename Vec2<R>::T cu = a.ut[i];
+        const double x = st.x, y = st.y;
+        double w[4], wu[4], wt[4];
+        int ring, spoke, ic[4];
+        cic_grad((double)cu.x, (double)cu.y, a.nr, a.na, ring, spoke, w, wu, wt);
+        corners_rs(ring, spoke, a.na, ic);
+        double su = 0.0, sv = 0.0, agx = 0.0, agy = 0.0;
+        for (int q = 0; q < 4; q++)
+            if (ic[q] < a.mesh) {
+                const typename Vec2<R>::T f = a.fld[ic[q]];
+                const double fx = f.x, fy = f.y;
+                const double s = fx * lax + fy * lay + a.pmass * a.rbar[ic[q]];
+                su += s * wu[q];
+                sv += s * wt[q];
+                agx += w[q] * fx;
+                agy += w[q] * fy;
+            }
+        const double r2 = x * x + y * y, r = sqrt(r2);
+        double gx = 0.0, gy = 0.0;
+        if (r > 0.0) {
+            // u = log(1 + r) / alpha, t = atan2(y, x) na / (2 pi)
+            const double cu_ = a.inv_alpha / (r * (1.0 + r));
+            const double ct = a.tscale / r2;
+            gx = su * cu_ * x - sv * ct * y;
+            gy = su * cu_ * y + sv * ct * x;
+            if (a.nfr > 0) {
+                const double f = fixforce(r, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val, a.fr_der);
+                const double fp = fixforce_deriv(r, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val,
+                                                 a.fr_der);
+                const double xl = x * lax + y * lay;
+                // d/dx of f(r) x / r, transposed, applied to la
+                gx += fp * xl * x / r2 + f * (lax / r - xl * x / (r2 * r));
+                gy += fp * xl * y / r2 + f * (lay / r - xl * y / (r2 * r));
+            }
+            if (a.nfd > 0)
+                p1 = fixforce(r, a.nfd, a.fd_dr, a.fd_inv_dr, a.fd_val, a.fd_der) *
+                     (x * lax + y * lay) / r;
+        }
+        a.lam[p] += gx;
+        a.lam[n + p] += gy;
+        p0 = lax * agx + lay * agy;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        p0 += __shfl_xor(p0, o, 64);
+        p1 += __shfl_xor(p1, o, 64);
+    }
+    if (lane == 0) {
+        red[wave][0] = p0;
+        red[wave][1] = p1;
+    }
+    __syncthreads();
+    if (tid < 2) {
+        double s = 0.0;
+        for (int k = 0; k < GH_BLOCK / 64; k++) s += red[k][tid];
+        a.part[2 * (long long)blockIdx.x + tid] = s;
+    }
+}
+
+// block per field ring: polcat transposed, then the transpose of p2fsyn's
+// synthesis for F_R and F_phi -> wbar (zero outside the field range)
+constexpr int ADJ_MAX_NA = 2048;
+
+__global__ __launch_bounds__(GH_BLOCK) void adj_wbar_kernel(AdjSolveArgs a) {
+    __shared__ double gt[2][ADJ_MAX_NA];
+    const int ir = blockIdx.x + 1, tid = threadIdx.x, na = a.na, nr = a.nr;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int jrs = a.ranges[0], jrf = a.ranges[2], krf = a.ranges[3];
+    const bool live = jrs >= 1 && ir >= jrf && ir <= krf;
+    const size_t row = (size_t)(ir - 1) * na;
+    if (live)
+        for (int j = tid; j < na; j += GH_BLOCK) {
+            const double c = a.twd[2 * j], s = a.twd[2 * j + 1];
+            const double gx = a.gx[row + j], gy = a.gy[row + j];
+            gt[0][j] = gx * c + gy * s;
+            gt[1][j] = -gx * s + gy * c;
+        }
+    __syncthreads();
+    const int maz = min(a.mmax - 2, a.ng - 1);
+    for (int task = wave; task < 2 * a.ng; task += GH_BLOCK / 64) {
+        const int type = task / a.ng, m = task - type * a.ng;
+        const bool valid = live && !a.lg[m] && (type == 0 || (m >= 1 && m <= maz));
+        double sc = 0.0, ss = 0.0;
+        if (valid) {
+            for (int j = lane; j < na; j += 64) {
+                const int k = (int)(((long long)m * j) % na);
+                const double g = gt[type][j];
+                sc += g * a.twd[2 * k];
+                ss += g * a.twd[2 * k + 1];
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                sc += __shfl_xor(sc, o, 64);
+                ss += __shfl_xor(ss, o, 64);
+            }
+        }
+        if (lane == 0) {
+            double wc = 0.0, ws = 0.0;
+            if (valid) {
+                if (m == 0) {
+                    wc = sc / na;
+                } else if (2 * m < na) {
+                    wc = 2.0 * sc / na;
+                    ws = -2.0 * ss / na;
+                } else {
+                    wc = sc / na;   // cos(pi j) = (-1)^j
+                }
+            }
+            a.wbar[(((size_t)type * 2 + 0) * a.ng + m) * nr + ir - 1] = wc;
+            a.wbar[(((size_t)type * 2 + 1) * a.ng + m) * nr + ir - 1] = ws;
+        }
+    }
+}
+
+// block per source ring: the transposed Green's contraction (p2conv) and
+// the transposed forward transform (p2manl) -> rbar (zero outside the
+// source range)
+__global__ __launch_bounds__(GH_BLOCK) void adj_rbar_kernel(AdjSolveArgs a) {
+    __shared__ double mc[64], ms[64];
+    const int is = blockIdx.x + 1, tid = threadIdx.x, na = a.na, nr = a.nr, ng = a.ng;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int jrs = a.ranges[0], krs = a.ranges[1], jrf = a.ranges[2], krf = a.ranges[3];
+    const bool live = jrs >= 1 && is >= jrs && is <= krs;
+    const size_t row = (size_t)(is - 1) * na;
+    const int maz = min(a.mmax - 2, ng - 1);
+    for (int m = wave; m < ng; m += GH_BLOCK / 64) {
+        double c = 0.0, s = 0.0;
+        if (live && !a.lg[m]) {
+            const bool azim = m >= 1 && m <= maz;
+            const double *g0 = a.G + ((size_t)0 * ng + m) * nr * nr;
+            const double *g1 = a.G + ((size_t)1 * ng + m) * nr * nr;
+            const double *wc0 = a.wbar + (((size_t)0 * 2 + 0) * ng + m) * nr;
+            const double *ws0 = a.wbar + (((size_t)0 * 2 + 1) * ng + m) * nr;
+            const double *wc1 = a.wbar + (((size_t)1 * 2 + 0) * ng + m) * nr;
+            const double *ws1 = a.wbar + (((size_t)1 * 2 + 1) * ng + m) * nr;
+            for (int ir = jrf + lane; ir <= krf; ir += 64) {
+                const double G0 = g0[(size_t)(ir - 1) * nr + is - 1];
+                c += G0 * wc0[ir - 1];
+                s += G0 * ws0[ir - 1];
+                if (azim) {
+                    const double G1 = g1[(size_t)(ir - 1) * nr + is - 1];
+                    c -= G1 * ws1[ir - 1];
+                    s += G1 * wc1[ir - 1];
+                }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                c += __shfl_xor(c, o, 64);
+                s += __shfl_xor(s, o, 64);
+            }
+        }
+        if (lane == 0) {
+            mc[m] = c;
+            // only the cos/sin pairs 1 <= m < mmax - 1 carry a sine part
+            ms[m] = (m >= 1 && m < a.mmax - 1) ? s : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < na; j += GH_BLOCK) {
+        double v = 0.0;
+        if (live) {
+            v = mc[0];
+            for (int m = 1; m < ng; m++) {
+                const int k = (int)(((long long)m * j) % na);
+                if (2 * m < na) v += mc[m] * a.twd[2 * k] - ms[m] * a.twd[2 * k + 1];
+                else v += (j & 1) ? -mc[m] : mc[m];
+            }
+        }
+        a.rbar[row + j] = v;
+    }
+}
+
+// out[q] = sum over blocks of part[b][q] (q = 0, 1), blocks in order
+__global__ __launch_bounds__(GH_BLOCK) void adj_sum_kernel(const double *part, int nb,
+                                                           double *out) {
+    __shared__ double red[GH_BLOCK / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int q = 0; q < 2; q++) {
+        double t = 0.0;
+        for (int b = tid; b < nb; b += GH_BLOCK) t += part[2 * (long long)b + q];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane == 0) red[wave] = t;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < GH_BLOCK / 64; w++) s += red[w];
+            out[q] = s;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace gh
+
+using namespace gh;
+
+// ---- host side ----------------------------------------------------------------------
+
+void adj_free(galhip_ctx *c) {
+    dfree(c->adj_lam);
+    dfree(c->adj_g);
+    dfree(c->adj_rbar);
+    dfree(c->adj_wbar);
+    dfree(c->adj_part);
+    dfree(c->fd_val);
+    dfree(c->fd_der);
+    for (auto &s : c->adj_slots) {
+        dfree(s.xv);
+        dfree(s.ut);
+        dfree(s.zone);
+        dfree(s.id);
+        dfree(s.counters);
+    }
+    c->adj_slots.clear();
+}
+
+static int adj_check(galhip_ctx *c) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d || c->ens) return gh_fail(GALHIP_EINVAL, "the adjoint is built for one p2d grid");
+    if (c->nz != 1 || c->nsect != 1)
+        return gh_fail(GALHIP_EINVAL, "the adjoint needs one zone and one sector");
+    return 0;
+}
+
+// per-particle adjoints and grid work arrays, sized for the loaded particles
+static int adj_ensure(galhip_ctx *c) {
+    if (c->adj_lam && c->adj_n == c->n) return 0;
+    dfree(c->adj_lam);
+    dfree(c->adj_part);
+    int rc = dalloc(&c->adj_lam, sizeof(double) * 4 * (size_t)(c->n > 0 ? c->n : 1));
+    const long long nb = (c->n + GH_BLOCK - 1) / GH_BLOCK;
+    rc |= dalloc(&c->adj_part, sizeof(double) * 2 * (size_t)(nb > 0 ? nb : 1) + 16);
+    if (!c->adj_g) {
+        rc |= dalloc(&c->adj_g, sizeof(double) * 2 * (size_t)c->mesh);
+        rc |= dalloc(&c->adj_rbar, sizeof(double) * (size_t)c->mesh);
+        rc |= dalloc(&c->adj_wbar, sizeof(double) * 4 * (size_t)c->ng * c->nr);
+    }
+    if (rc) return rc;
+    c->adj_n = c->n;
+    HIPCHK(hipMemset(c->adj_lam, 0, sizeof(double) * 4 * (size_t)(c->n > 0 ? c->n : 1)));
+    return 0;
+}
+
+extern "C" int galhip_adj_record(galhip_ctx *c, int32_t slot) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (slot < 0 || slot > 4096) return gh_fail(GALHIP_EINVAL, "slot %d out of range", slot);
+    if ((size_t)slot >= c->adj_slots.size()) c->adj_slots.resize(slot + 1);
+    galhip_ctx::AdjSlot &s = c->adj_slots[slot];
+    const size_t es = esize(c), n = (size_t)c->n;
+    if (s.n != c->n || !s.xv) {
+        dfree(s.xv);
+        dfree(s.ut);
+        dfree(s.zone);
+        dfree(s.id);
+        dfree(s.counters);
+        rc = dalloc(&s.xv, 4 * es * (n ? n : 1)) | dalloc(&s.ut, 2 * es * (n ? n : 1)) |
+             dalloc(&s.zone, n ? n : 1) | dalloc(&s.id, sizeof(unsigned) * (n ? n : 1)) |
+             dalloc(&s.counters, sizeof(long long) * 4);
+        if (rc) return rc;
+        s.n = c->n;
+    }
+    if (n) {
+        HIPCHK(hipMemcpyAsync(s.xv, c->xv, 4 * es * n, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(s.ut, c->ut, 2 * es * n, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(s.zone, c->zone, n, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(s.id, c->id, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, c->s));
+    }
+    HIPCHK(hipMemcpyAsync(s.counters, c->counters, sizeof(long long) * 4,
+                          hipMemcpyDeviceToDevice, c->s));
+    s.istep = c->istep;
+    return 0;
+}
+
+extern "C" int galhip_adj_restore(galhip_ctx *c, int32_t slot) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (slot < 0 || (size_t)slot >= c->adj_slots.size() || !c->adj_slots[slot].xv)
+        return gh_fail(GALHIP_EINVAL, "slot %d was not recorded", slot);
+    galhip_ctx::AdjSlot &s = c->adj_slots[slot];
+    if (s.n != c->n) return gh_fail(GALHIP_EINVAL, "slot %d holds %lld particles, not %lld",
+                                    slot, s.n, c->n);
+    const size_t es = esize(c), n = (size_t)c->n;
+    if (n) {
+        HIPCHK(hipMemcpyAsync(c->xv, s.xv, 4 * es * n, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(c->ut, s.ut, 2 * es * n, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(c->zone, s.zone, n, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(c->id, s.id, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, c->s));
+    }
+    HIPCHK(hipMemcpyAsync(c->counters, s.counters, sizeof(long long) * 4,
+                          hipMemcpyDeviceToDevice, c->s));
+    c->istep = s.istep;
+    c->have_field = 0;
+    c->steps_since_sort = 0;
+    return 0;
+}
+
+extern "C" int galhip_adj_set(galhip_ctx *c, const double *lam) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!lam) return gh_fail(GALHIP_EINVAL, "null argument");
+    rc = adj_ensure(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->n)
+        HIPCHK(hipMemcpy(c->adj_lam, lam, sizeof(double) * 4 * (size_t)c->n,
+                         hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int galhip_adj_get(galhip_ctx *c, double *lam) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!lam) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (!c->adj_lam || c->adj_n != c->n) return gh_fail(GALHIP_EINVAL, "no adjoint set");
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->n)
+        HIPCHK(hipMemcpy(lam, c->adj_lam, sizeof(double) * 4 * (size_t)c->n,
+                         hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int galhip_set_fixforce_dpar(galhip_ctx *c, int32_t n, double dr, const double *val,
+                                        const double *der) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (n != 0 && (n < 2 || !(dr > 0.0) || !val || !der))
+        return gh_fail(GALHIP_EINVAL, "galhip_set_fixforce_dpar: bad table");
+    HIPCHK(hipStreamSynchronize(c->s));
+    dfree(c->fd_val);
+    dfree(c->fd_der);
+    c->nfd = 0;
+    if (n == 0) return 0;
+    int rc = dalloc(&c->fd_val, sizeof(double) * n) | dalloc(&c->fd_der, sizeof(double) * n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(c->fd_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fd_der, der, sizeof(double) * n, hipMemcpyHostToDevice));
+    c->nfd = n;
+    c->fd_dr = dr;
+    return 0;
+}
+
+static AdjSolveArgs adj_solve_args(galhip_ctx *c) {
+    AdjSolveArgs a;
+    a.gx = c->adj_g;
+    a.gy = c->adj_g + c->mesh;
+    a.wbar = c->adj_wbar;
+    a.rbar = c->adj_rbar;
+    a.G = c->Gd;
+    a.twd = c->twd;
+    a.lg = c->lg_d;
+    a.ranges = c->ranges;
+    a.nr = c->nr;
+    a.na = c->na;
+    a.ng = c->ng;
+    a.mmax = c->mmax;
+    return a;
+}
+
+// rbar = K^T (gx, gy), with the ranges of the last solve
+static int adj_solve(galhip_ctx *c) {
+    if (c->na > ADJ_MAX_NA || c->ng > 64) return gh_fail(GALHIP_EINVAL, "grid too large for the adjoint");
+    AdjSolveArgs a = adj_solve_args(c);
+    hipLaunchKernelGGL(adj_wbar_kernel, dim3(c->nr), dim3(GH_BLOCK), 0, c->s, a);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(adj_rbar_kernel, dim3(c->nr), dim3(GH_BLOCK), 0, c->s, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <typename R> static AdjArgs<R> adj_args(galhip_ctx *c) {
+    AdjArgs<R> a;
+    memset(&a, 0, sizeof a);
+    a.xv = (const typename Vec4<R>::T *)c->xv;
+    a.ut = (const typename Vec2<R>::T *)c->ut;
+    a.zone = c->zone;
+    a.id = c->id;
+    a.n = c->n;
+    a.lam = c->adj_lam;
+    a.gx = c->adj_g;
+    a.gy = c->adj_g + c->mesh;
+    a.rbar = c->adj_rbar;
+    a.fld = (const typename Vec2<R>::T *)c->fld2;
+    a.fr_val = c->fr_val;
+    a.fr_der = c->fr_der;
+    a.fr_dr = c->fr_dr;
+    a.fr_inv_dr = c->nfr > 0 ? 1.0 / c->fr_dr : 0.0;
+    a.nfr = c->nfr;
+    a.fd_val = c->fd_val;
+    a.fd_der = c->fd_der;
+    a.fd_dr = c->fd_dr;
+    a.fd_inv_dr = c->nfd > 0 ? 1.0 / c->fd_dr : 0.0;
+    a.nfd = c->nfd;
+    a.pmass = c->pmass;
+    a.inv_alpha = 1.0 / c->alpha;
+    a.tscale = (double)c->na / (2.0 * 3.141592653589793238462643);
+    a.nr = c->nr;
+    a.na = c->na;
+    a.mesh = c->mesh;
+    a.part = c->adj_part;
+    return a;
+}
+
+template <typename R> static int adj_step_impl(galhip_ctx *c, double *gpar) {
+    const long long nb = (c->n + GH_BLOCK - 1) / GH_BLOCK;
+    HIPCHK(hipMemsetAsync(c->adj_g, 0, sizeof(double) * 2 * (size_t)c->mesh, c->s));
+    AdjArgs<R> a = adj_args<R>(c);
+    if (nb > 0) {
+        hipLaunchKernelGGL(adj_scatter_kernel<R>, dim3((unsigned)nb), dim3(GH_BLOCK), 0, c->s, a);
+        HIPCHK(hipGetLastError());
+    }
+    int rc = adj_solve(c);
+    if (rc) return rc;
+    if (nb > 0) {
+        hipLaunchKernelGGL(adj_gather_kernel<R>, dim3((unsigned)nb), dim3(GH_BLOCK), 0, c->s, a);
+        HIPCHK(hipGetLastError());
+    }
+    double *dout = c->adj_part + 2 * (nb > 0 ? nb : 1);
+    hipLaunchKernelGGL(adj_sum_kernel, dim3(1), dim3(GH_BLOCK), 0, c->s,
+                       (const double *)c->adj_part, (int)nb, dout);
+    HIPCHK(hipGetLastError());
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, dout, sizeof h, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (gpar) {
+        gpar[0] += h[0];
+        gpar[1] += h[1];
+    }
+    return 0;
+}
+
+extern "C" int galhip_adj_step(galhip_ctx *c, double *gpar) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!c->have_field) return gh_fail(GALHIP_ERANGE, "adjoint step needs the step's field (findf)");
+    if (!c->have_green) return gh_fail(GALHIP_EINVAL, "Green function not set");
+    if (!c->adj_lam || c->adj_n != c->n) return gh_fail(GALHIP_EINVAL, "no adjoint set");
+    return c->prec == GALHIP_FP32 ? adj_step_impl<float>(c, gpar) : adj_step_impl<double>(c, gpar);
+}
+
+extern "C" int galhip_adj_field(galhip_ctx *c, const double *g, double *rbar) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!g || !rbar) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (!c->have_green) return gh_fail(GALHIP_EINVAL, "Green function not set");
+    rc = adj_ensure(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(c->adj_g, g, sizeof(double) * 2 * (size_t)c->mesh, hipMemcpyHostToDevice));
+    rc = adj_solve(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(rbar, c->adj_rbar, sizeof(double) * (size_t)c->mesh, hipMemcpyDeviceToHost));
+    return 0;
+}

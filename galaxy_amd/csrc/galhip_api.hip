@@ -274,6 +274,7 @@ extern "C" int galhip_free(galhip_ctx *c) {
     dfree(c->fr_val); dfree(c->fr_der);
     dfree(c->m_qbits_d); dfree(c->m_pmass_d); dfree(c->m_off_d); dfree(c->blk_d);
     drop_graphs(c);
+    adj_free(c);
     for (auto e : c->ev_free) (void)hipEventDestroy(e);
     for (auto e : c->ev_used) (void)hipEventDestroy(e);
     if (c->s) (void)hipStreamDestroy(c->s);

@@ -89,6 +89,22 @@ struct galhip_ctx {
     // HIP graphs of galhip_run's steps, keyed by their launch arguments
     int graph;
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+    // adjoint (galhip_adj.hip): per-particle adjoints [4][n] in load order,
+    // FP64 grid work arrays, the fixed force's parameter derivative table
+    // and recorded particle states
+    double *adj_lam, *adj_g, *adj_rbar, *adj_wbar, *adj_part;
+    long long adj_n;
+    double *fd_val, *fd_der, fd_dr;
+    int nfd;
+    struct AdjSlot {
+        void *xv, *ut;
+        uint8_t *zone;
+        unsigned *id;
+        long long *counters;
+        long long n;
+        int istep;
+    };
+    std::vector<AdjSlot> adj_slots;
 };
 
 static size_t esize(const galhip_ctx *c) { return c->prec == GALHIP_FP64 ? 8 : 4; }
@@ -122,6 +138,9 @@ hipError_t launch_fill_i32(int *, int, long long, hipStream_t);
 hipError_t launch_masscl(const ZonePtrs &, int, long long, int *, int *, int, int,
                          hipStream_t);
 }  // namespace gh
+
+// adjoint storage (galhip_adj.hip)
+void adj_free(galhip_ctx *c);
 
 // p3d implementations of the generic entry points (galhip_p3d_api.hip)
 void p3_free(galhip_ctx *c);

@@ -157,6 +157,42 @@ def _enclosed_fraction_inverse(u, rtrunc):
     return R
 
 
+def disc_draws(n, seed, rtrunc=5.0):
+    """The random part of exponential_disc: radii, angles and the two
+    standard normal deviates (vR, vphi) of each particle."""
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    u = rng.random(n)
+    R = _enclosed_fraction_inverse(u, rtrunc)
+    phi = rng.random(n) * (2.0 * math.pi)
+    g1 = rng.standard_normal(n)
+    g2 = rng.standard_normal(n)
+    return R, phi, g1, g2
+
+
+def velocity_terms(R, rtrunc=5.0, rcore=0.1):
+    """Per-particle terms of exponential_disc's velocities that do not depend
+    on Q or md: sigma_R = Q md k_sig, sigma_phi = sigma_R k_phi,
+    mean v_phi^2 = vc2 + sigma_R^2 c (clamped at 0).  Returns
+    (k_sig, k_phi, vc2, c)."""
+    mt = 1.0 - (1.0 + rtrunc) * math.exp(-rtrunc)
+    s2 = R * R + rcore * rcore
+    vc2 = R * R / s2
+    om2 = 1.0 / s2
+    kap2 = 2.0 * (R * R + 2.0 * rcore * rcore) / (s2 * s2)
+    k_sig = 3.36 * np.exp(-R) / (2.0 * math.pi * mt) / np.sqrt(kap2)
+    k_phi = np.sqrt(kap2 / (4.0 * om2))
+    h = 1e-6
+
+    def lnss(Rv):
+        s2v = Rv * Rv + rcore * rcore
+        k2 = 2.0 * (Rv * Rv + 2.0 * rcore * rcore) / (s2v * s2v)
+        return -3.0 * Rv - np.log(k2)
+    Rs = np.maximum(R, 1e-8)
+    dlog = (lnss(Rs * (1 + h)) - lnss(Rs * (1 - h))) / (2 * h)
+    c = dlog + 1.0 - kap2 / (4.0 * om2)
+    return k_sig, k_phi, vc2, c
+
+
 def exponential_disc(n, seed, Q=1.5, md=0.5, rtrunc=5.0, rcore=0.1):
     """Model-unit particles: (R, phi, vR, vphi) arrays, FP64."""
     rng = np.random.Generator(np.random.Philox(key=seed))

@@ -213,6 +213,41 @@ class P2DGrid:
         """Run the steps of run() as replayed HIP graphs."""
         self._c(self._lib.galhip_set_graph(self._h, int(bool(on))), "set_graph")
 
+    # -- adjoint (one zone, one sector) ------------------------------------------
+    def adj_record(self, slot):
+        self._c(self._lib.galhip_adj_record(self._h, int(slot)), "adj_record")
+
+    def adj_restore(self, slot):
+        self._c(self._lib.galhip_adj_restore(self._h, int(slot)), "adj_restore")
+
+    def adj_set(self, lam):
+        """lam: [4][n] FP64 adjoints of x, y, vx, vy in load order."""
+        lam = np.ascontiguousarray(lam, np.float64)
+        assert lam.shape == (4, self.n)
+        self._c(self._lib.galhip_adj_set(self._h, _ptr(lam)), "adj_set")
+
+    def adj_get(self):
+        lam = np.zeros((4, self.n), np.float64)
+        self._c(self._lib.galhip_adj_get(self._h, _ptr(lam)), "adj_get")
+        return lam
+
+    def set_fixforce_dpar(self, dr, val, der):
+        val = np.ascontiguousarray(val, np.float64)
+        der = np.ascontiguousarray(der, np.float64)
+        self._c(self._lib.galhip_set_fixforce_dpar(self._h, len(val), float(dr), _ptr(val),
+                                                    _ptr(der)), "set_fixforce_dpar")
+
+    def adj_step(self, gpar):
+        """One backward step; gpar (2 FP64) accumulates the parameter terms."""
+        self._c(self._lib.galhip_adj_step(self._h, _ptr(gpar)), "adj_step")
+
+    def adj_field(self, g):
+        """K^T: g[2][mesh] -> rbar[mesh] (FP64)."""
+        g = np.ascontiguousarray(g, np.float64)
+        rb = np.zeros(self.mesh, np.float64)
+        self._c(self._lib.galhip_adj_field(self._h, _ptr(g), _ptr(rb)), "adj_field")
+        return rb
+
     # -- group level ---------------------------------------------------------------
     def weight(self, x, y, old=True):
         x = np.ascontiguousarray(x, self.dtype)

@@ -323,6 +323,40 @@ int galhip_ens_integrals(galhip_ctx *ctx, int32_t centred, double *out);
  * profiling is on, galhip_run launches directly. */
 int galhip_set_graph(galhip_ctx *ctx, int32_t on);
 
+/* ---- adjoint of the p2d step ----
+ * No counterpart in the reference.  Reverse-mode derivative of the map one
+ * step of galhip_run computes (findf, then step: interpolation, fixed
+ * force, leap-frog, mass assignment), for one grid with one zone and one
+ * sector, full steps.  The cell, ring-range and off-grid choices are held
+ * fixed (the map is piecewise smooth).  Adjoints are FP64 and kept on the
+ * device per particle in load order: lam[4][n] = adjoints of x, y, vx, vy.
+ *
+ * galhip_adj_record / galhip_adj_restore save and restore the particle
+ * state (positions, velocities, grid coordinates, zones, order) in device
+ * memory slot `slot` (checkpoints, and the positions of each step of a
+ * recomputed segment).  After a restore, masset + findf rebuild the step's
+ * mass and field exactly (integer mass sums do not depend on order). */
+int galhip_adj_record(galhip_ctx *ctx, int32_t slot);
+int galhip_adj_restore(galhip_ctx *ctx, int32_t slot);
+int galhip_adj_set(galhip_ctx *ctx, const double *lam);
+int galhip_adj_get(galhip_ctx *ctx, double *lam);
+/* Derivative of the fixed radial force with respect to a model parameter
+ * p, tabulated like galhip_set_fixforce's table (n = 0 removes it). */
+int galhip_set_fixforce_dpar(galhip_ctx *ctx, int32_t n, double dr, const double *val,
+                             const double *der);
+/* One backward step at the context's current positions x_n, whose field
+ * the last findf formed: lam holds the adjoints of (x_{n+1}, v_{n+1}) on
+ * entry and of (x_n, v_n) on return.  gpar[0] += sum over particles of
+ * la . a_grid (the derivative for a factor on the particle mass, and so on
+ * the grid force), gpar[1] += sum of la . (dF/dp)(r) x / r (the fixed
+ * force's parameter derivative table), la being the adjoint of the step's
+ * acceleration. */
+int galhip_adj_step(galhip_ctx *ctx, double *gpar);
+/* The transposed field solve alone: g[2][mesh] (adjoints of F_x, F_y) ->
+ * rbar[mesh] (adjoint of the combined, scaled mass), with the ring ranges
+ * of the last findf (FP64 tables). */
+int galhip_adj_field(galhip_ctx *ctx, const double *g, double *rbar);
+
 #ifdef __cplusplus
 }
 #endif
