@@ -70,6 +70,13 @@ P3_PUSH_HBM_WRITE_B = 44.8
 ENS_PUSH_HBM_READ_B = 26.7
 ENS_PUSH_HBM_WRITE_B = 27.5
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# adj_scatter_kernel<float>: algorithmic bytes per particle -- id (4), zone
+# (1), (u, t) (8), the four FP64 adjoints read (32) and the two velocity
+# adjoints written (16); the grid sums stay in LDS and L2.  Measured HBM
+# bytes per particle: 41.1 read + 19.8 written
+# (profiles/adj_scatter_kernel_p2d-128x256-n1e6-adjoint_traffic.txt).
+ADJ_SCATTER_BYTES = 61.0
+ADJ_SCATTER_HBM_B = 60.9
 
 
 def log(*a):
@@ -385,6 +392,22 @@ def main_adj(args):
     g32 = m32.gradient(w.Q, w.md, args.steps, every=20)
     t32 = time.perf_counter() - t0
     log("[bench] adjoint fp32: %.3f s (%s)" % (t32, g32["seconds"]))
+    # the same gradient again with HIP events around the adjoint particle
+    # kernels (their own time for the roofline)
+    gp = m32.gradient(w.Q, w.md, args.steps, every=20, profile=True)
+    sc_ms, ga_ms, nprof = gp["kernel_ms"]
+    log("[bench] adjoint profiled pass: scatter %.3f ms, gather %.3f ms over %d steps"
+        % (sc_ms, ga_ms, nprof))
+    bpp = ADJ_SCATTER_BYTES
+    achieved = n * nprof * bpp / (sc_ms * 1e-3) / 1e9
+    traffic = None
+    if ADJ_SCATTER_HBM_B is not None:
+        traffic = round(ADJ_SCATTER_HBM_B * n)
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel="adj_scatter_kernel<float>", bytes_per_particle=bpp,
+                avg_launch_us=round(1e3 * sc_ms / max(nprof, 1), 2),
+                gather_avg_launch_us=round(1e3 * ga_ms / max(nprof, 1), 2))
     extra = dict(loss=g32["loss"], dQ=g32["dQ"], dmd=g32["dmd"], every=20,
                  seconds_fp32={k: round(v, 4) for k, v in g32["seconds"].items()})
     if not args.no_fp64:
@@ -397,20 +420,23 @@ def main_adj(args):
         rng = np.random.default_rng(17)
         dx, dy = rng.standard_normal(n), rng.standard_normal(n)
         vx, vy = m64.velocities(w.Q, w.md)
-        h = 1e-6 * float(np.sqrt(np.mean(vx * vx + vy * vy)))
-        # the annulus membership of the unperturbed end state held fixed, as
-        # the adjoint holds it (at this N single particles crossing R = 1 or
-        # 3 would otherwise dominate the difference)
-        lp = m64.loss_v(vx + h * dx, vy + h * dy, w.md, args.steps, sel=g64["sel"])
-        lm = m64.loss_v(vx - h * dx, vy - h * dy, w.md, args.steps, sel=g64["sel"])
-        fd = (lp - lm) / (2 * h)
+        vs = float(np.sqrt(np.mean(vx * vx + vy * vy)))
         ad = float(np.dot(g64["dvx"], dx) + np.dot(g64["dvy"], dy))
+        fds = []
+        for hr in (1e-6, 1e-7):
+            h = hr * vs
+            # the annulus membership of the unperturbed end state held fixed,
+            # as the adjoint holds it (at this N single particles crossing
+            # R = 1 or 3 would otherwise dominate the difference)
+            lp = m64.loss_v(vx + h * dx, vy + h * dy, w.md, args.steps, sel=g64["sel"])
+            lm = m64.loss_v(vx - h * dx, vy - h * dy, w.md, args.steps, sel=g64["sel"])
+            fd = (lp - lm) / (2 * h)
+            fds.append(dict(h_rel=hr, central_difference=fd, rel_err=abs(ad - fd) / abs(fd)))
         extra.update(fp64_seconds=round(t64, 3), loss_fp64=g64["loss"],
                      grad_v_rel_err_fp32_vs_fp64=float(np.linalg.norm(v32 - v64)
                                                        / np.linalg.norm(v64)),
                      dQ_fp64=g64["dQ"], dmd_fp64=g64["dmd"],
-                     fd_direction=dict(adjoint=ad, central_difference=fd, h=h,
-                                       rel_err=abs(ad - fd) / abs(fd)))
+                     fd_direction=dict(adjoint=ad, checks=fds))
     value = n * args.steps / t32
     cpu = None
     if not args.no_cpu:
@@ -425,7 +451,7 @@ def main_adj(args):
         "config": {"workload": w.name, "grid": [w.nr, w.na], "n": n,
                    "harmonics": "m<=%d" % w.hash, "dt": w.dt, "loss": "|A2|^2, R in [1, 3]",
                    "checkpoint_every": 20, "parallelism": "none"},
-        "roofline": None, "cpu_baseline": cpu, "extra": extra,
+        "roofline": roof, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(line), flush=True)
 

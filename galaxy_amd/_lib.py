@@ -108,6 +108,7 @@ SIGNATURES = {
     "galhip_set_fixforce_dpar": [_vp, _i32, _dbl, _vp, _vp],
     "galhip_adj_step": [_vp, _vp],
     "galhip_adj_field": [_vp, _vp, _vp],
+    "galhip_adj_get_profile": [_vp, _vp, _vp],
 }
 
 _lib = None

@@ -10,10 +10,10 @@ particles between R = 1 and 3 disc scale lengths at the final step
 
 Backward pass, for checkpoints every `every` steps: each segment is
 recomputed from its checkpoint, recording the state before each step in a
-device slot; then, from the segment's last step down, the state is
-restored, masset + findf rebuild that step's mass and field (bit for bit:
-the mass sums are integers), and galhip_adj_step takes the adjoints one
-step back.  Parameter terms come out of the same pass: gpar[0] (the
+device slot (particles, zone mass sums, ring ranges); then, from the
+segment's last step down, the state is restored, findf rebuilds that
+step's field bit for bit, and galhip_adj_step takes the adjoints one step
+back.  Parameter terms come out of the same pass: gpar[0] (the
 derivative for a factor on the particle mass, hence the grid force) and
 gpar[1] (the fixed force's parameter derivative table).
 """
@@ -38,8 +38,10 @@ def a2_loss(x, y, lscale, r1=1.0, r2=3.0, sel=None):
     if sel is None:
         sel = (R >= r1) & (R <= r2)
     cnt = max(int(sel.sum()), 1)
-    ph = np.arctan2(y, x)
-    c2, s2 = np.cos(2 * ph), np.sin(2 * ph)
+    # cos 2 phi, sin 2 phi from the coordinates
+    with np.errstate(invalid="ignore", divide="ignore"):
+        inv = np.where(r2_ > 0, 1.0 / r2_, 0.0)
+    c2, s2 = (x * x - y * y) * inv, 2.0 * x * y * inv
     A = (c2[sel].sum() - 1j * s2[sel].sum()) / cnt
     L = abs(A) ** 2
     # dL/dphi = 2 Re(conj(A) dA/dphi), dA/dphi_j = -2i e^{-2i phi_j} / cnt
@@ -88,7 +90,6 @@ class Trajectory:
             m = min(E, self.nsteps - k * E)
             t0 = time.perf_counter()
             g.adj_restore(k)
-            g.masset()
             for j in range(m):
                 g.adj_record(base + j)
                 g.run(1)
@@ -96,7 +97,6 @@ class Trajectory:
             t1 = time.perf_counter()
             for j in range(m - 1, -1, -1):
                 g.adj_restore(base + j)
-                g.masset()
                 g.findf(False)
                 g.adj_step(gpar)
             g.sync()
@@ -128,19 +128,15 @@ class DiscModel:
         R, phi, g1, g2 = disc.disc_draws(self.n, w.seed, w.rtrunc)
         self.R, self.phi, self.g1, self.g2 = R, phi, g1, g2
         self.k_sig, self.k_phi, self.vc2, self.c = disc.velocity_terms(R, w.rtrunc, w.rcore)
-        self.x0 = R * np.cos(phi) * self.ls
-        self.y0 = R * np.sin(phi) * self.ls
+        self.cp, self.sp = np.cos(phi), np.sin(phi)
+        self.x0 = R * self.cp * self.ls
+        self.y0 = R * self.sp * self.ls
         self.iz0 = np.where(R * self.ls > self.rgrid, 2, 1).astype(np.int32)
 
     def velocities(self, Q, md):
         """Internal-unit initial velocities (numpy FP64) for Q, md."""
-        sR = Q * md * self.k_sig
-        vm = np.sqrt(np.maximum(self.vc2 + sR * sR * self.c, 0.0))
-        vR = sR * self.g1
-        vphi = vm + sR * self.k_phi * self.g2
-        f = self.ls * self.w.dt
-        cp, sp = np.cos(self.phi), np.sin(self.phi)
-        return (vR * cp - vphi * sp) * f, (vR * sp + vphi * cp) * f
+        return ic_velocities(self.k_sig, self.k_phi, self.vc2, self.c, self.g1, self.g2,
+                             self.cp, self.sp, Q, md, self.ls * self.w.dt)
 
     def grid(self, md):
         """A fresh grid for disc mass md: particle mass and halo force scale
@@ -172,11 +168,14 @@ class DiscModel:
         g.close()
         return a2_loss(out["x"], out["y"], self.ls, sel=sel)[0]
 
-    def gradient(self, Q, md, nsteps, every=20):
+    def gradient(self, Q, md, nsteps, every=20, profile=False):
         """L and its gradient: dL/dvx0, dL/dvy0 (internal units), dL/dQ,
-        dL/dmd, through the adjoint kernels."""
+        dL/dmd, through the adjoint kernels.  profile: HIP events around
+        the adjoint particle kernels (result 'kernel_ms': scatter, gather)."""
         vx, vy = self.velocities(Q, md)
         g = self.grid(md)
+        if profile:
+            g.set_profile(True)
         tr = Trajectory(g, nsteps, every)
         out = tr.forward(self.x0, self.y0, vx, vy, self.iz0)
         L, gx, gy = a2_loss(out["x"], out["y"], self.ls)
@@ -185,31 +184,47 @@ class DiscModel:
         lam = np.zeros((4, self.n))
         lam[0], lam[1] = gx, gy
         lam0, gpar = tr.backward(lam)
+        kms = g.adj_get_profile() if profile else None
         g.close()
         lvx, lvy = lam0[2], lam0[3]
         dQ, dmd_ic = self._ic_chain(Q, md, lvx, lvy)
         dmd = dmd_ic + gpar[0] / md + gpar[1]
         return dict(loss=L, dvx=lvx, dvy=lvy, dQ=dQ, dmd=dmd, gpar=gpar,
-                    seconds=dict(tr.seconds), sel=sel)
+                    seconds=dict(tr.seconds), sel=sel, kernel_ms=kms)
 
     def _ic_chain(self, Q, md, lvx, lvy):
         """dL/dQ and the initial-velocity part of dL/dmd from the velocity
         adjoints: both parameters enter only through sigma_R = Q md k_sig."""
-        sR = Q * md * self.k_sig
-        vphi2 = self.vc2 + sR * sR * self.c
-        vm = np.sqrt(np.maximum(vphi2, 0.0))
-        # d/d sR of (vR, vphi)
-        dvR = self.g1
-        with np.errstate(divide="ignore", invalid="ignore"):
-            dvm = np.where(vphi2 > 0, sR * self.c / np.where(vm > 0, vm, 1.0), 0.0)
-        dvphi = dvm + self.k_phi * self.g2
-        f = self.ls * self.w.dt
-        cp, sp = np.cos(self.phi), np.sin(self.phi)
-        dvx = (dvR * cp - dvphi * sp) * f
-        dvy = (dvR * sp + dvphi * cp) * f
-        # sR_i = Q md k_sig_i: dsR_i/dQ = sR_i / Q, dsR_i/dmd = sR_i / md
-        dQ = float(np.dot(lvx, dvx * (sR / Q)) + np.dot(lvy, dvy * (sR / Q)))
+        dvx, dvy = ic_velocity_dq(self.k_sig, self.k_phi, self.vc2, self.c, self.g1, self.g2,
+                                  self.cp, self.sp, Q, md, self.ls * self.w.dt)
+        dQ = float(np.dot(lvx, dvx) + np.dot(lvy, dvy))
         return dQ, dQ * Q / md
+
+
+def ic_velocities(k_sig, k_phi, vc2, c, g1, g2, cp, sp, Q, md, f):
+    """Internal-unit initial velocities of the exponential disc (the
+    formulas of disc.exponential_disc on its draws and terms): sigma_R =
+    Q md k_sig, vR = sigma_R g1, vphi = sqrt(max(vc2 + sigma_R^2 c, 0)) +
+    sigma_R k_phi g2, rotated by (cp, sp) and scaled by f = lscale ts."""
+    sR = Q * md * k_sig
+    vm = np.sqrt(np.maximum(vc2 + sR * sR * c, 0.0))
+    vR = sR * g1
+    vphi = vm + sR * k_phi * g2
+    return (vR * cp - vphi * sp) * f, (vR * sp + vphi * cp) * f
+
+
+def ic_velocity_dq(k_sig, k_phi, vc2, c, g1, g2, cp, sp, Q, md, f):
+    """d/dQ of ic_velocities (d/dmd is Q/md times it)."""
+    sR = Q * md * k_sig
+    vphi2 = vc2 + sR * sR * c
+    vm = np.sqrt(np.maximum(vphi2, 0.0))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dvm = np.where(vphi2 > 0, sR * c / np.where(vm > 0, vm, 1.0), 0.0)
+    dvphi = dvm + k_phi * g2           # d/d sigma_R
+    dvx = (g1 * cp - dvphi * sp) * f
+    dvy = (g1 * sp + dvphi * cp) * f
+    ds = sR / Q                        # d sigma_R / dQ
+    return dvx * ds, dvy * ds
 
 
 def torch_function():

@@ -22,6 +22,7 @@
 // the forward state (positions, (u, t), field) is the context's.
 #include "galhip_ctx.h"
 
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -33,7 +34,7 @@ template <typename R> struct AdjArgs {
     const uint8_t *zone;
     const unsigned *id;
     long long n;
-    double *lam;                      // [4][n] in load order: x, y, vx, vy
+    double *lam;                      // [4][n] in storage order: x, y, vx, vy
     double *gx, *gy;                  // [mesh]
     const double *rbar;               // [mesh]
     const typename Vec2<R>::T *fld;   // forward (F_x, F_y)
@@ -100,28 +101,143 @@ __device__ __forceinline__ double fixforce_deriv(double r, int nfr, double dr, d
            inv_dr;
 }
 
+// dst[k][ids[i]] = src[k][i] (storage order -> load order), k = 0..3
+__global__ __launch_bounds__(GH_BLOCK) void adj_to_load_kernel(const double *__restrict__ src,
+                                                               const unsigned *__restrict__ ids,
+                                                               double *__restrict__ dst,
+                                                               long long n) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const long long p = ids[i];
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k * n + p] = src[k * n + i];
+}
+
+// dst[k][i] = src[k][ids[i]] (load order -> storage order)
+__global__ __launch_bounds__(GH_BLOCK) void adj_to_store_kernel(const double *__restrict__ src,
+                                                                const unsigned *__restrict__ ids,
+                                                                double *__restrict__ dst,
+                                                                long long n) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const long long p = ids[i];
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k * n + i] = src[k * n + p];
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+constexpr int ADJ_TILE = 1024;   // LDS window cells (x 2 components x 8 B)
+
 // la = lv + lx stored as the new lv; on-grid particles scatter la with their
-// weights (the transpose of grdacc's interpolation)
+// weights (the transpose of grdacc's interpolation).  Particles are kept
+// ordered by cell, so a block's corners fall in a small window of rings x
+// spokes: the block accumulates them in LDS (FP64 LDS atomics) and adds
+// each touched cell to the grid with one global atomic; a block whose
+// window exceeds the tile adds every corner globally.
 template <typename R>
 __global__ __launch_bounds__(GH_BLOCK) void adj_scatter_kernel(AdjArgs<R> a) {
-    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
-    if (i >= a.n) return;
-    const long long n = a.n, p = a.id[i];
-    const double lax = a.lam[2 * n + p] + a.lam[p];
-    const double lay = a.lam[3 * n + p] + a.lam[n + p];
-    a.lam[2 * n + p] = lax;
-    a.lam[3 * n + p] = lay;
-    if (a.zone[i] != 1) return;
-    const typename Vec2<R>::T cu = a.ut[i];
-    double w[4], wu[4], wt[4];
-    int ring, spoke, ic[4];
-    cic_grad((double)cu.x, (double)cu.y, a.nr, a.na, ring, spoke, w, wu, wt);
-    corners_rs(ring, spoke, a.na, ic);
-    for (int q = 0; q < 4; q++)
-        if (ic[q] < a.mesh) {
-            atomicAdd(&a.gx[ic[q]], w[q] * lax);
-            atomicAdd(&a.gy[ic[q]], w[q] * lay);
+    __shared__ double tile[2][ADJ_TILE];
+    __shared__ int s_first, s_sref, s_rmin, s_rmax, s_dmin, s_dmax;
+    const int tid = threadIdx.x, lane = tid & 63, na = a.na, half = na >> 1;
+    const long long i = (long long)blockIdx.x * GH_BLOCK + tid;
+    if (tid == 0) {
+        s_first = INT_MAX;
+        s_rmin = INT_MAX;
+        s_rmax = -1;
+        s_dmin = INT_MAX;
+        s_dmax = INT_MIN;
+    }
+    const bool in = i < a.n;
+    const long long n = a.n, p = i;
+    double lax = 0.0, lay = 0.0;
+    if (in) {
+        lax = a.lam[2 * n + p] + a.lam[p];
+        lay = a.lam[3 * n + p] + a.lam[n + p];
+        a.lam[2 * n + p] = lax;
+        a.lam[3 * n + p] = lay;
+    }
+    const bool on = in && a.zone[i] == 1;
+    double w[4] = {0.0, 0.0, 0.0, 0.0}, wu[4], wt[4];
+    int ring = 0, spoke = 0;
+    if (on) {
+        const typename Vec2<R>::T cu = a.ut[i];
+        cic_grad((double)cu.x, (double)cu.y, a.nr, na, ring, spoke, w, wu, wt);
+    }
+    __syncthreads();
+    // reference spoke: the first on-grid particle's
+    if (on) atomicMin(&s_first, tid);
+    __syncthreads();
+    if (tid == s_first) s_sref = spoke;
+    __syncthreads();
+    if (s_first == INT_MAX) return;   // no on-grid particle: nothing to add
+    const int sref = s_sref;
+    int d = spoke - sref;
+    if (d < -half) d += na;
+    else if (d >= half) d -= na;
+    {
+        const int rmin = wave_min_i(on ? ring : INT_MAX), rmax = wave_max_i(on ? ring : -1);
+        const int dmin = wave_min_i(on ? d : INT_MAX), dmax = wave_max_i(on ? d : INT_MIN);
+        if (lane == 0 && rmax >= 0) {
+            atomicMin(&s_rmin, rmin);
+            atomicMax(&s_rmax, rmax);
+            atomicMin(&s_dmin, dmin);
+            atomicMax(&s_dmax, dmax);
         }
+    }
+    __syncthreads();
+    const int r0 = s_rmin, d0 = s_dmin;
+    const int tw = s_dmax - d0 + 2, th = s_rmax - r0 + 2;
+    const bool use_tile = th * tw <= ADJ_TILE;
+    const int tlen = use_tile ? th * tw : 0;
+    for (int j = tid; j < tlen; j += GH_BLOCK) {
+        tile[0][j] = 0.0;
+        tile[1][j] = 0.0;
+    }
+    __syncthreads();
+    if (on) {
+        if (use_tile) {
+            const int b = (ring - r0) * tw + (d - d0);
+            const int ti[4] = {b, b + 1, b + tw, b + tw + 1};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                atomicAdd(&tile[0][ti[q]], w[q] * lax);
+                atomicAdd(&tile[1][ti[q]], w[q] * lay);
+            }
+        } else {
+            int ic[4];
+            corners_rs(ring, spoke, na, ic);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (ic[q] < a.mesh) {
+                    atomicAdd(&a.gx[ic[q]], w[q] * lax);
+                    atomicAdd(&a.gy[ic[q]], w[q] * lay);
+                }
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < tlen; j += GH_BLOCK) {
+        const double vx = tile[0][j], vy = tile[1][j];
+        if (vx == 0.0 && vy == 0.0) continue;
+        const int row = j / tw, col = j - row * tw;
+        const int rg = r0 + row;
+        if (rg >= a.nr) continue;   // past the last ring: grdacc reads 0 there
+        int sp = sref + d0 + col;
+        sp %= na;
+        if (sp < 0) sp += na;
+        const int c = rg * na + sp;
+        atomicAdd(&a.gx[c], vx);
+        atomicAdd(&a.gy[c], vy);
+    }
 }
 
 // lx += J_direct^T la + pmass sum_q rb(c_q) grad w_q; block sums of the
@@ -133,7 +249,7 @@ __global__ __launch_bounds__(GH_BLOCK) void adj_gather_kernel(AdjArgs<R> a) {
     const long long i = (long long)blockIdx.x * GH_BLOCK + tid;
     double p0 = 0.0, p1 = 0.0;
     if (i < a.n && a.zone[i] == 1) {
-        const long long n = a.n, p = a.id[i];
+        const long long n = a.n, p = i;
         const double lax = a.lam[2 * n + p], lay = a.lam[3 * n + p];
         const typename Vec4<R>::T st = a.xv[i];
         const typename Vec2<R>::T cu = a.ut[i];
@@ -339,14 +455,27 @@ void adj_free(galhip_ctx *c) {
     dfree(c->adj_part);
     dfree(c->fd_val);
     dfree(c->fd_der);
+    dfree(c->adj_tmp);
+    dfree(c->adj_id);
     for (auto &s : c->adj_slots) {
         dfree(s.xv);
         dfree(s.ut);
         dfree(s.zone);
         dfree(s.id);
         dfree(s.counters);
+        dfree(s.mass);
+        dfree(s.rng);
     }
     c->adj_slots.clear();
+    for (auto e : c->adj_ev) (void)hipEventDestroy(e);
+    c->adj_ev.clear();
+}
+
+static hipEvent_t adj_event(galhip_ctx *c) {
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    c->adj_ev.push_back(e);
+    return e;
 }
 
 static int adj_check(galhip_ctx *c) {
@@ -362,7 +491,11 @@ static int adj_ensure(galhip_ctx *c) {
     if (c->adj_lam && c->adj_n == c->n) return 0;
     dfree(c->adj_lam);
     dfree(c->adj_part);
+    dfree(c->adj_tmp);
+    dfree(c->adj_id);
     int rc = dalloc(&c->adj_lam, sizeof(double) * 4 * (size_t)(c->n > 0 ? c->n : 1));
+    rc |= dalloc(&c->adj_tmp, sizeof(double) * 4 * (size_t)(c->n > 0 ? c->n : 1));
+    rc |= dalloc(&c->adj_id, sizeof(unsigned) * (size_t)(c->n > 0 ? c->n : 1));
     const long long nb = (c->n + GH_BLOCK - 1) / GH_BLOCK;
     rc |= dalloc(&c->adj_part, sizeof(double) * 2 * (size_t)(nb > 0 ? nb : 1) + 16);
     if (!c->adj_g) {
@@ -383,18 +516,29 @@ extern "C" int galhip_adj_record(galhip_ctx *c, int32_t slot) {
     if ((size_t)slot >= c->adj_slots.size()) c->adj_slots.resize(slot + 1);
     galhip_ctx::AdjSlot &s = c->adj_slots[slot];
     const size_t es = esize(c), n = (size_t)c->n;
+    const size_t cells = (size_t)c->mesh;
     if (s.n != c->n || !s.xv) {
         dfree(s.xv);
         dfree(s.ut);
         dfree(s.zone);
         dfree(s.id);
         dfree(s.counters);
+        dfree(s.mass);
+        dfree(s.rng);
         rc = dalloc(&s.xv, 4 * es * (n ? n : 1)) | dalloc(&s.ut, 2 * es * (n ? n : 1)) |
              dalloc(&s.zone, n ? n : 1) | dalloc(&s.id, sizeof(unsigned) * (n ? n : 1)) |
-             dalloc(&s.counters, sizeof(long long) * 4);
+             dalloc(&s.counters, sizeof(long long) * 4) |
+             dalloc(&s.mass, sizeof(long long) * cells * c->nz) |
+             dalloc(&s.rng, sizeof(int) * 2 * GH_MAXZ);
         if (rc) return rc;
         s.n = c->n;
     }
+    for (int z = 0; z < c->nz; z++)
+        HIPCHK(hipMemcpyAsync(s.mass + z * cells, c->mfx[z], sizeof(long long) * cells,
+                              hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(s.rng, c->jrad, sizeof(int) * GH_MAXZ, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(s.rng + GH_MAXZ, c->krad, sizeof(int) * GH_MAXZ,
+                          hipMemcpyDeviceToDevice, c->s));
     if (n) {
         HIPCHK(hipMemcpyAsync(s.xv, c->xv, 4 * es * n, hipMemcpyDeviceToDevice, c->s));
         HIPCHK(hipMemcpyAsync(s.ut, c->ut, 2 * es * n, hipMemcpyDeviceToDevice, c->s));
@@ -404,6 +548,7 @@ extern "C" int galhip_adj_record(galhip_ctx *c, int32_t slot) {
     HIPCHK(hipMemcpyAsync(s.counters, c->counters, sizeof(long long) * 4,
                           hipMemcpyDeviceToDevice, c->s));
     s.istep = c->istep;
+    s.order_ver = c->order_ver;
     return 0;
 }
 
@@ -424,8 +569,17 @@ extern "C" int galhip_adj_restore(galhip_ctx *c, int32_t slot) {
     }
     HIPCHK(hipMemcpyAsync(c->counters, s.counters, sizeof(long long) * 4,
                           hipMemcpyDeviceToDevice, c->s));
+    const size_t cells = (size_t)c->mesh;
+    for (int z = 0; z < c->nz; z++)
+        HIPCHK(hipMemcpyAsync(c->mfx[z], s.mass + z * cells, sizeof(long long) * cells,
+                              hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->jrad, s.rng, sizeof(int) * GH_MAXZ, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->krad, s.rng + GH_MAXZ, sizeof(int) * GH_MAXZ,
+                          hipMemcpyDeviceToDevice, c->s));
     c->istep = s.istep;
+    c->order_ver = s.order_ver;
     c->have_field = 0;
+    c->masscl_done = 0;
     c->steps_since_sort = 0;
     return 0;
 }
@@ -437,9 +591,33 @@ extern "C" int galhip_adj_set(galhip_ctx *c, const double *lam) {
     rc = adj_ensure(c);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->s));
-    if (c->n)
-        HIPCHK(hipMemcpy(c->adj_lam, lam, sizeof(double) * 4 * (size_t)c->n,
-                         hipMemcpyHostToDevice));
+    const long long n = c->n;
+    if (n) {
+        HIPCHK(hipMemcpy(c->adj_tmp, lam, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice));
+        const unsigned nb = (unsigned)((n + GH_BLOCK - 1) / GH_BLOCK);
+        hipLaunchKernelGGL(adj_to_store_kernel, dim3(nb), dim3(GH_BLOCK), 0, c->s,
+                           (const double *)c->adj_tmp, (const unsigned *)c->id, c->adj_lam, n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c->adj_id, c->id, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    }
+    c->adj_ver = c->order_ver;
+    return 0;
+}
+
+// bring the adjoints to the current storage order
+static int adj_relayout(galhip_ctx *c) {
+    if (c->adj_ver == c->order_ver || c->n == 0) return 0;
+    const long long n = c->n;
+    const unsigned nb = (unsigned)((n + GH_BLOCK - 1) / GH_BLOCK);
+    hipLaunchKernelGGL(adj_to_load_kernel, dim3(nb), dim3(GH_BLOCK), 0, c->s,
+                       (const double *)c->adj_lam, (const unsigned *)c->adj_id, c->adj_tmp, n);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(adj_to_store_kernel, dim3(nb), dim3(GH_BLOCK), 0, c->s,
+                       (const double *)c->adj_tmp, (const unsigned *)c->id, c->adj_lam, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->adj_id, c->id, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, c->s));
+    c->adj_ver = c->order_ver;
     return 0;
 }
 
@@ -448,10 +626,15 @@ extern "C" int galhip_adj_get(galhip_ctx *c, double *lam) {
     if (rc) return rc;
     if (!lam) return gh_fail(GALHIP_EINVAL, "null argument");
     if (!c->adj_lam || c->adj_n != c->n) return gh_fail(GALHIP_EINVAL, "no adjoint set");
-    HIPCHK(hipStreamSynchronize(c->s));
-    if (c->n)
-        HIPCHK(hipMemcpy(lam, c->adj_lam, sizeof(double) * 4 * (size_t)c->n,
-                         hipMemcpyDeviceToHost));
+    const long long n = c->n;
+    if (n) {
+        const unsigned nb = (unsigned)((n + GH_BLOCK - 1) / GH_BLOCK);
+        hipLaunchKernelGGL(adj_to_load_kernel, dim3(nb), dim3(GH_BLOCK), 0, c->s,
+                           (const double *)c->adj_lam, (const unsigned *)c->adj_id, c->adj_tmp, n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->s));
+        HIPCHK(hipMemcpy(lam, c->adj_tmp, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost));
+    }
     return 0;
 }
 
@@ -537,18 +720,25 @@ template <typename R> static AdjArgs<R> adj_args(galhip_ctx *c) {
 
 template <typename R> static int adj_step_impl(galhip_ctx *c, double *gpar) {
     const long long nb = (c->n + GH_BLOCK - 1) / GH_BLOCK;
+    int rc0 = adj_relayout(c);
+    if (rc0) return rc0;
     HIPCHK(hipMemsetAsync(c->adj_g, 0, sizeof(double) * 2 * (size_t)c->mesh, c->s));
     AdjArgs<R> a = adj_args<R>(c);
+    const bool prof = c->prof && nb > 0;
+    if (prof) HIPCHK(hipEventRecord(adj_event(c), c->s));
     if (nb > 0) {
         hipLaunchKernelGGL(adj_scatter_kernel<R>, dim3((unsigned)nb), dim3(GH_BLOCK), 0, c->s, a);
         HIPCHK(hipGetLastError());
     }
+    if (prof) HIPCHK(hipEventRecord(adj_event(c), c->s));
     int rc = adj_solve(c);
     if (rc) return rc;
+    if (prof) HIPCHK(hipEventRecord(adj_event(c), c->s));
     if (nb > 0) {
         hipLaunchKernelGGL(adj_gather_kernel<R>, dim3((unsigned)nb), dim3(GH_BLOCK), 0, c->s, a);
         HIPCHK(hipGetLastError());
     }
+    if (prof) HIPCHK(hipEventRecord(adj_event(c), c->s));
     double *dout = c->adj_part + 2 * (nb > 0 ? nb : 1);
     hipLaunchKernelGGL(adj_sum_kernel, dim3(1), dim3(GH_BLOCK), 0, c->s,
                        (const double *)c->adj_part, (int)nb, dout);
@@ -585,5 +775,26 @@ extern "C" int galhip_adj_field(galhip_ctx *c, const double *g, double *rbar) {
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipMemcpy(rbar, c->adj_rbar, sizeof(double) * (size_t)c->mesh, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int galhip_adj_get_profile(galhip_ctx *c, double *ms, int64_t *steps) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!ms) return gh_fail(GALHIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(c->s));
+    double t[2] = {0.0, 0.0};
+    const size_t k = c->adj_ev.size() / 4;
+    for (size_t i = 0; i < k; i++)
+        for (int q = 0; q < 2; q++) {
+            float v = 0.f;
+            HIPCHK(hipEventElapsedTime(&v, c->adj_ev[4 * i + 2 * q], c->adj_ev[4 * i + 2 * q + 1]));
+            t[q] += v;
+        }
+    ms[0] = t[0];
+    ms[1] = t[1];
+    if (steps) *steps = (int64_t)k;
+    for (auto e : c->adj_ev) (void)hipEventDestroy(e);
+    c->adj_ev.clear();
     return 0;
 }

@@ -446,6 +446,7 @@ template <typename R> static int sort_impl(galhip_ctx *c) {
     std::swap(c->ut, c->ut2);
     std::swap(c->zone, c->zone2);
     std::swap(c->id, c->id2);
+    c->order_ver++;
     c->steps_since_sort = 0;
     return 0;
 }
@@ -498,6 +499,7 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
     }
     HIPCHK(hipMemcpy(c->counters, cnt, sizeof cnt, hipMemcpyHostToDevice));
     c->qbits = qbits_for(n);
+    c->order_ver++;
     if (n == 0) return 0;
     {
         // interleave (x, y, vx, vy) per particle

@@ -93,7 +93,13 @@ struct galhip_ctx {
     // FP64 grid work arrays, the fixed force's parameter derivative table
     // and recorded particle states
     double *adj_lam, *adj_g, *adj_rbar, *adj_wbar, *adj_part;
+    double *adj_tmp;                  // [4][n] work copy for re-layouts
     long long adj_n;
+    // particle order: order_ver changes whenever the storage order does;
+    // the adjoints are kept in the storage order of version adj_ver,
+    // whose load-order index is adj_id
+    long long order_ver, adj_ver;
+    unsigned *adj_id;
     double *fd_val, *fd_der, fd_dr;
     int nfd;
     struct AdjSlot {
@@ -101,10 +107,13 @@ struct galhip_ctx {
         uint8_t *zone;
         unsigned *id;
         long long *counters;
-        long long n;
+        long long *mass;              // zone sums [nz][mesh]
+        int *rng;                     // jrad, krad [2][GH_MAXZ]
+        long long n, order_ver;
         int istep;
     };
     std::vector<AdjSlot> adj_slots;
+    std::vector<hipEvent_t> adj_ev;   // profiling: scatter and gather start/end per step
 };
 
 static size_t esize(const galhip_ctx *c) { return c->prec == GALHIP_FP64 ? 8 : 4; }

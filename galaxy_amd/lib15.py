@@ -241,6 +241,14 @@ class P2DGrid:
         """One backward step; gpar (2 FP64) accumulates the parameter terms."""
         self._c(self._lib.galhip_adj_step(self._h, _ptr(gpar)), "adj_step")
 
+    def adj_get_profile(self):
+        """(scatter ms, gather ms, steps) since the last call (profiling on)."""
+        ms = np.zeros(2)
+        k = C.c_int64()
+        self._c(self._lib.galhip_adj_get_profile(self._h, _ptr(ms), C.byref(k)),
+                "adj_get_profile")
+        return float(ms[0]), float(ms[1]), k.value
+
     def adj_field(self, g):
         """K^T: g[2][mesh] -> rbar[mesh] (FP64)."""
         g = np.ascontiguousarray(g, np.float64)

@@ -332,10 +332,10 @@ int galhip_set_graph(galhip_ctx *ctx, int32_t on);
  * device per particle in load order: lam[4][n] = adjoints of x, y, vx, vy.
  *
  * galhip_adj_record / galhip_adj_restore save and restore the particle
- * state (positions, velocities, grid coordinates, zones, order) in device
- * memory slot `slot` (checkpoints, and the positions of each step of a
- * recomputed segment).  After a restore, masset + findf rebuild the step's
- * mass and field exactly (integer mass sums do not depend on order). */
+ * state (positions, velocities, grid coordinates, zones, order) with the
+ * zone mass sums and ring ranges in device memory slot `slot` (checkpoints,
+ * and the state before each step of a recomputed segment).  After a
+ * restore, findf rebuilds that step's field exactly. */
 int galhip_adj_record(galhip_ctx *ctx, int32_t slot);
 int galhip_adj_restore(galhip_ctx *ctx, int32_t slot);
 int galhip_adj_set(galhip_ctx *ctx, const double *lam);
@@ -356,6 +356,10 @@ int galhip_adj_step(galhip_ctx *ctx, double *gpar);
  * rbar[mesh] (adjoint of the combined, scaled mass), with the ring ranges
  * of the last findf (FP64 tables). */
 int galhip_adj_field(galhip_ctx *ctx, const double *g, double *rbar);
+/* With profiling on (galhip_set_profile), galhip_adj_step records HIP
+ * events around its particle kernels: ms[0] the summed scatter time, ms[1]
+ * the gather time (milliseconds) over the steps since the last call. */
+int galhip_adj_get_profile(galhip_ctx *ctx, double *ms, int64_t *steps);
 
 #ifdef __cplusplus
 }
