@@ -275,7 +275,8 @@ def main_p3d(args):
     print(json.dumps(line), flush=True)
 
 
-def run_gpu_ens(w, precision, scale, steps, warmup, sort_interval, graph, parts):
+def run_gpu_ens(w, precision, scale, steps, warmup, sort_interval, graph, parts,
+                contraction="exact"):
     """The ensemble workload: timed run (HIP graphs when graph is set), then
     a profiled pass of the same length with events around every push."""
     from galaxy_amd import configs
@@ -283,6 +284,8 @@ def run_gpu_ens(w, precision, scale, steps, warmup, sort_interval, graph, parts)
     g = configs.make_ensemble(w, precision=precision, scale=scale, parts=parts)
     g.set_sort_interval(sort_interval)
     g.set_graph(graph)
+    if contraction == "mfma":
+        g.set_contraction("mfma")
     log("[bench] ensemble %s set-up %.1f s (%d members, N=%d)"
         % (precision, time.time() - t0, g.nmem, g.n))
     if warmup:
@@ -306,7 +309,8 @@ def run_gpu_ens(w, precision, scale, steps, warmup, sort_interval, graph, parts)
     members = g.unload()
     a2s = [a2(m["x"], m["y"], w.lscale()) for m in members]
     res = dict(seconds=t1 - t0, pushes=pushes, push_ms=push_ms, launches=launches,
-               pushes_prof=pushes_p, a2=a2s, e0=e0, e1=e1, n=g.n, nmem=g.nmem)
+               pushes_prof=pushes_p, a2=a2s, e0=e0, e1=e1, n=g.n, nmem=g.nmem,
+               contraction=g.get_contraction() if contraction == "mfma" else None)
     log("[bench] ensemble %s: %d steps in %.4f s, %d pushes; profiled pass: push kernel "
         "%.3f ms over %d launches" % (precision, steps, t1 - t0, pushes, push_ms, launches))
     g.close()
@@ -324,7 +328,7 @@ def main_ens(args):
     parts = configs.ensemble_particles(w, counts, rgrid)
     log("[bench] ensemble particles drawn in %.1f s" % (time.time() - t0))
     mixed = run_gpu_ens(w, "fp32", scale, args.steps, args.warmup, args.sort_interval,
-                        args.graph, parts)
+                        args.graph, parts, args.contraction)
     fp64 = None
     if not args.no_fp64:
         fp64 = run_gpu_ens(w, "fp64", scale, args.steps, args.warmup, args.sort_interval,
@@ -348,8 +352,11 @@ def main_ens(args):
         cpu["sample"] = "member %d's disc: " % i + cpu["sample"]
         log("[bench] ensemble cpu baseline: %.3g pushes/s (%s)" % (cpu["value"], cpu["sample"]))
     extra = dict(members=mixed["nmem"], n_total=mixed["n"], graph=bool(args.graph),
-                 sort_interval=args.sort_interval,
+                 sort_interval=args.sort_interval, contraction=args.contraction,
                  push_share=round(mixed["push_ms"] * 1e-3 / mixed["seconds"], 3))
+    if mixed["contraction"] is not None:
+        _, var, us = mixed["contraction"]
+        extra.update(gemm_variant=int(var), gemm_variant_us=[round(float(u), 2) for u in us])
     if fp64:
         v64 = fp64["pushes"] / fp64["seconds"]
         d_m = [(b - a) / abs(a) for a, b in zip(mixed["e0"], mixed["e1"])]
@@ -488,6 +495,9 @@ def main():
     ap.add_argument("--workload", default="C2", choices=("C2", "C3", "C4", "C5"),
                     help="C2: 2-D grid, the headline (default); C3: 3-D grid; "
                          "C4: ensemble of 64 2-D discs")
+    ap.add_argument("--contraction", default=None, choices=("exact", "mfma"),
+                    help="ensemble radial contraction: FP32 dot products in the synthesis "
+                         "(exact) or split-BF16 GEMM on MFMA (default for C4)")
     ap.add_argument("--graph", type=int, default=None, choices=(0, 1),
                     help="run the timed steps as HIP graphs (default: 1 for C4, 0 otherwise)")
     ap.add_argument("--cpu-n3", type=int, default=4_000_000)
@@ -513,6 +523,8 @@ def main():
             args.sort_interval = 10
         if args.graph is None:
             args.graph = 1
+        if args.contraction is None:
+            args.contraction = "mfma"
         main_ens(args)
         return
     if args.workload == "C3":

@@ -101,6 +101,8 @@ SIGNATURES = {
     "galhip_ens_get_noff": [_vp, _vp],
     "galhip_ens_integrals": [_vp, _i32, _vp],
     "galhip_set_graph": [_vp, _i32],
+    "galhip_set_contraction": [_vp, _i32],
+    "galhip_get_contraction": [_vp, _vp, _vp, _vp],
     "galhip_adj_record": [_vp, _i32],
     "galhip_adj_restore": [_vp, _i32],
     "galhip_adj_set": [_vp, _vp],

@@ -54,6 +54,9 @@ template <typename R> hipError_t launch_analyse(const AnaArgs<R> &, hipStream_t)
 template <typename R> hipError_t launch_synth(const SynArgs<R> &, hipStream_t);
 hipError_t launch_green(double *, const double *, int, int, int, int, double,
                         double, int, hipStream_t);
+hipError_t launch_split_bf16(const float *, unsigned short *, unsigned short *, int, int, int,
+                             hipStream_t);
+hipError_t launch_ens_gemm(const EnsGemmArgs &, int, hipStream_t);
 hipError_t launch_d2f(const double *, float *, long long, hipStream_t);
 hipError_t launch_fill_i64(long long *, long long, long long, hipStream_t);
 hipError_t launch_fill_i32(int *, int, long long, hipStream_t);
@@ -273,6 +276,7 @@ extern "C" int galhip_free(galhip_ctx *c) {
     dfree(c->Gd);
     dfree(c->fr_val); dfree(c->fr_der);
     dfree(c->m_qbits_d); dfree(c->m_pmass_d); dfree(c->m_off_d); dfree(c->blk_d);
+    dfree(c->ghi); dfree(c->glo); dfree(c->wpre);
     drop_graphs(c);
     adj_free(c);
     for (auto e : c->ev_free) (void)hipEventDestroy(e);
@@ -296,6 +300,7 @@ extern "C" int galhip_geometry(const galhip_ctx *c, double *out) {
 // ---- Green's tables ------------------------------------------------------------
 
 static int green_to_work(galhip_ctx *c) {
+    c->gsplit_valid = 0;
     if (c->prec == GALHIP_FP32) {
         long long n = 3LL * c->ng * c->nr * c->nr;
         HIPCHK(launch_d2f(c->Gd, (float *)c->G, n, c->s));
@@ -784,16 +789,117 @@ extern "C" int galhip_masset(galhip_ctx *c) {
     return c->prec == GALHIP_FP32 ? masset_impl<float>(c) : masset_impl<double>(c);
 }
 
+// The MFMA contraction's arguments for a solve with ntypes field types.
+static EnsGemmArgs gemm_args(galhip_ctx *c, int ntypes) {
+    EnsGemmArgs g;
+    memset(&g, 0, sizeof g);
+    g.ghi = c->ghi;
+    g.glo = c->glo;
+    g.coef = (const float *)c->coef;
+    g.ranges = c->ranges;
+    g.lg = c->lg_d;
+    g.W = c->wpre;
+    g.nr = c->nr;
+    g.nrp = c->nrp;
+    g.ng = c->ng;
+    g.ncoef = c->ncoef;
+    g.nmem = c->nmem;
+    g.mmax = c->mmax;
+    g.ntypes = ntypes;
+    return g;
+}
+
+// Before an MFMA solve: the bf16 split of the current tables, and the GEMM
+// variant -- tuned once per context by timing each variant on this
+// context's shapes (HIP events, 5 launches each after one warm-up).
+static int mfma_prepare(galhip_ctx *c) {
+    if (!c->gsplit_valid) {
+        HIPCHK(launch_split_bf16((const float *)c->G, c->ghi, c->glo, c->ng, c->nr, c->nrp, c->s));
+        c->gsplit_valid = 1;
+    }
+    if (c->gemm_variant < 0) {
+        const EnsGemmArgs g = gemm_args(c, 2);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        int best = 0;
+        for (int v = 0; v < 3; v++) {
+            HIPCHK(launch_ens_gemm(g, v, c->s));
+            HIPCHK(hipEventRecord(e0, c->s));
+            for (int r = 0; r < 5; r++) HIPCHK(launch_ens_gemm(g, v, c->s));
+            HIPCHK(hipEventRecord(e1, c->s));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            c->gemm_us[v] = 1e3 * ms / 5.0;
+            if (c->gemm_us[v] < c->gemm_us[best]) best = v;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->gemm_variant = best;
+    }
+    return 0;
+}
+
+// analyse, [MFMA contraction], synthesis
+template <typename R>
+static int launch_solve(galhip_ctx *c, const AnaArgs<R> &a, SynArgs<R> &b, int lpot) {
+    HIPCHK(launch_analyse<R>(a, c->s));
+    if (c->contract == 1) {
+        HIPCHK(launch_ens_gemm(gemm_args(c, lpot ? 3 : 2), c->gemm_variant, c->s));
+        b.wpre = c->wpre;
+    }
+    HIPCHK(launch_synth<R>(b, c->s));
+    return 0;
+}
+
 template <typename R>
 static int solve_impl(galhip_ctx *c, int lpot, int explicit_zones, int jzone,
                       int kzone, int masscl_z) {
     if (!c->have_green) return gh_fail(GALHIP_EINVAL, "P2CONV: Green function not set (call galhip_greenm)");
+    if (c->contract == 1) {
+        int rc = mfma_prepare(c);
+        if (rc) return rc;
+    }
     AnaArgs<R> a = ana_args<R>(c, lpot, explicit_zones, jzone, kzone);
-    HIPCHK(launch_analyse<R>(a, c->s));
     SynArgs<R> b = syn_args<R>(c, lpot, masscl_z);
-    HIPCHK(launch_synth<R>(b, c->s));
+    int rc = launch_solve<R>(c, a, b, lpot);
+    if (rc) return rc;
     c->have_field = 1;
     c->pot_valid = lpot != 0;
+    return 0;
+}
+
+extern "C" int galhip_set_contraction(galhip_ctx *c, int32_t mode) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (mode < 0 || mode > 4) return gh_fail(GALHIP_EINVAL, "contraction mode %d", mode);
+    if (mode >= 1 && (c->p3d || c->prec != GALHIP_FP32))
+        return gh_fail(GALHIP_EINVAL, "the MFMA contraction is built for FP32 p2d grids");
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (mode >= 1 && !c->ghi) {
+        c->nrp = (c->nr + 31) / 32 * 32;
+        const size_t ng = (size_t)3 * c->ng * c->nrp * c->nrp;
+        int rc = dalloc(&c->ghi, sizeof(unsigned short) * ng) |
+                 dalloc(&c->glo, sizeof(unsigned short) * ng) |
+                 dalloc(&c->wpre, sizeof(float) * (size_t)c->nmem * 3 * 2 * c->ng * c->nr);
+        if (rc) return rc;
+        HIPCHK(hipMemset(c->wpre, 0, sizeof(float) * (size_t)c->nmem * 3 * 2 * c->ng * c->nr));
+        c->gsplit_valid = 0;
+        c->gemm_variant = -1;
+    }
+    c->contract = mode >= 1 ? 1 : 0;
+    if (mode >= 2) c->gemm_variant = mode - 2;   // a fixed variant: no tuning
+    drop_graphs(c);
+    return 0;
+}
+
+extern "C" int galhip_get_contraction(const galhip_ctx *c, int32_t *mode, int32_t *variant,
+                                      double *us) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (mode) *mode = c->contract;
+    if (variant) *variant = c->gemm_variant;
+    if (us)
+        for (int v = 0; v < 3; v++) us[v] = c->gemm_variant >= 0 ? c->gemm_us[v] : 0.0;
     return 0;
 }
 
@@ -925,8 +1031,13 @@ extern "C" int galhip_integrals(galhip_ctx *c, int32_t centred, double *out) {
 // schedule is periodic; a sort swaps the particle arrays).
 template <typename R> static int graph_step(galhip_ctx *c) {
     if (!c->have_green) return gh_fail(GALHIP_EINVAL, "P2CONV: Green function not set (call galhip_greenm)");
+    if (c->contract == 1) {
+        int rc = mfma_prepare(c);
+        if (rc) return rc;
+    }
     const AnaArgs<R> a = ana_args<R>(c, 0, 0, 0, 0);
-    const SynArgs<R> b = syn_args<R>(c, 0, c->mzone);
+    SynArgs<R> b = syn_args<R>(c, 0, c->mzone);
+    if (c->contract == 1) b.wpre = c->wpre;
     c->have_field = 1;
     c->pot_valid = 0;
     set_mzone(c);
@@ -936,6 +1047,8 @@ template <typename R> static int graph_step(galhip_ctx *c) {
     std::string key((const char *)&a, sizeof a);
     key.append((const char *)&b, sizeof b);
     key.append((const char *)&p, sizeof p);
+    key.append((const char *)&c->contract, sizeof c->contract);
+    key.append((const char *)&c->gemm_variant, sizeof c->gemm_variant);
     hipGraphExec_t ex = nullptr;
     for (auto &g : c->graphs)
         if (g.first == key) {
@@ -950,6 +1063,8 @@ template <typename R> static int graph_step(galhip_ctx *c) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
         hipError_t e = launch_analyse<R>(a, c->s);
+        if (e == hipSuccess && c->contract == 1)
+            e = launch_ens_gemm(gemm_args(c, 2), c->gemm_variant, c->s);
         if (e == hipSuccess) e = launch_synth<R>(b, c->s);
         if (e == hipSuccess) e = launch_push<R>(p, c->s);
         hipError_t e2 = hipStreamEndCapture(c->s, &g);

@@ -114,6 +114,13 @@ struct galhip_ctx {
     };
     std::vector<AdjSlot> adj_slots;
     std::vector<hipEvent_t> adj_ev;   // profiling: scatter and gather start/end per step
+    // radial contraction: 0 in the synthesis kernel (default), 1 on MFMA
+    // with split-BF16 operands (FP32 grids); the bf16 split of G, padded to
+    // nrp rings; the GEMM output; the tuned GEMM variant (-1: not yet)
+    int contract, nrp, gsplit_valid, gemm_variant;
+    unsigned short *ghi, *glo;
+    float *wpre;
+    double gemm_us[3];
 };
 
 static size_t esize(const galhip_ctx *c) { return c->prec == GALHIP_FP64 ? 8 : 4; }

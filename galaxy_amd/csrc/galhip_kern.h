@@ -323,6 +323,17 @@ template <typename R> struct SynArgs {
     int *jrad, *krad;
     int masscl_z;
     int nmem;                         // grids (blockIdx.z); 1 for one grid
+    const float *wpre;                // contraction done beforehand, or NULL
+};
+
+// the ensemble's MFMA contraction (galhip_solve.hip)
+struct EnsGemmArgs {
+    const unsigned short *ghi, *glo;  // bf16 split of G, [3][ng][nrp][nrp]
+    const float *coef;                // [nmem][ncoef][nr]
+    const int *ranges;                // [nmem][4]
+    const int *lg;
+    float *W;                         // [nmem][3][2][ng][nr]
+    int nr, nrp, ng, ncoef, nmem, mmax, ntypes;
 };
 
 }  // namespace gh

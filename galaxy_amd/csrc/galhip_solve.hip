@@ -118,7 +118,9 @@ __global__ __launch_bounds__(SOLVE_BLOCK) void analyse_kernel(AnaArgs<R> a) {
 // grid (nr, SYN_SPLIT, nmem): block (ir, q, mem) forms the ring's
 // contraction (small) and synthesises spokes [q na/SYN_SPLIT,
 // (q+1) na/SYN_SPLIT) of grid mem.
-template <typename R>
+// PRE: the contraction was done beforehand (the ensemble's MFMA GEMM):
+// W is read from a.wpre[mem][type][cos/sin][m][ring] instead.
+template <typename R, bool PRE>
 __global__ __launch_bounds__(SOLVE_BLOCK) void synth_kernel(SynArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     __shared__ R W[3][2][MAX_NG];
@@ -152,7 +154,15 @@ __global__ __launch_bounds__(SOLVE_BLOCK) void synth_kernel(SynArgs<R> a) {
     const int wave = tid >> 6, lane = tid & 63, nw = SOLVE_BLOCK / 64;
     const int mmax = a.mmax;
     const int maz = min(mmax - 2, a.ng - 1);
-    for (int task = wave; task < ntypes * a.ng; task += nw) {
+    if (PRE) {
+        for (int t = tid; t < ntypes * 2 * a.ng; t += SOLVE_BLOCK) {
+            const int type = t / (2 * a.ng), r = t - type * 2 * a.ng, cs = r / a.ng;
+            const int m = r - cs * a.ng;
+            W[type][cs][m] = (R)a.wpre[((((size_t)mem * 3 + type) * 2 + cs) * a.ng + m) * nr +
+                                       ir - 1];
+        }
+    }
+    for (int task = wave; task < (PRE ? 0 : ntypes * a.ng); task += nw) {
         const int type = task / a.ng, m = task - type * a.ng;
         const bool azim = type == 1;
         const bool valid = !a.lg[m] && (!azim || (m >= 1 && m <= maz));
@@ -331,8 +341,181 @@ hipError_t launch_analyse(const AnaArgs<R> &a, hipStream_t s) {
 template <typename R>
 hipError_t launch_synth(const SynArgs<R> &a, hipStream_t s) {
     if (a.ng > MAX_NG) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(synth_kernel<R>, dim3(a.nr, SYN_SPLIT, a.nmem), dim3(SOLVE_BLOCK), 0, s,
-                       a);
+    if (a.wpre)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(synth_kernel<R, true>), dim3(a.nr, SYN_SPLIT, a.nmem),
+                           dim3(SOLVE_BLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(synth_kernel<R, false>), dim3(a.nr, SYN_SPLIT, a.nmem),
+                           dim3(SOLVE_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---- the ensemble's radial contraction on MFMA (split BF16, FP32 sums) ----
+//
+// For every (type, harmonic m) the contraction of all members is one GEMM:
+//   W[field ring][column] = sum over source rings of G[ring][source] B[source][column],
+// columns = (member, cos/sin) pairs, B = the members' coefficients in the
+// arrangement p2conv uses for the type (F_R and the potential: (Mc, Ms);
+// F_phi: (Ms, -Mc)), zero outside each member's source range.  Each operand
+// x is split into bf16 hi = rn(x) and lo = rn(x - hi); the product is
+// hi*hi + hi*lo + lo*hi accumulated in FP32 by the matrix cores (the
+// lo*lo term is below the split's own 2^-17 representation error).
+
+typedef short gh_v4s __attribute__((ext_vector_type(4)));
+typedef float gh_v4f __attribute__((ext_vector_type(4)));
+typedef float gh_v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ unsigned short bf16_rn(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) {
+    return __uint_as_float((unsigned)h << 16);
+}
+
+// G (FP32 [3][ng][nr][nr]) -> hi, lo bf16 [3][ng][nrp][nrp], zero padded
+__global__ void split_bf16_kernel(const float *__restrict__ G, unsigned short *__restrict__ hi,
+                                  unsigned short *__restrict__ lo, int ng, int nr, int nrp) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    const long long tot = 3LL * ng * nrp * nrp;
+    if (i >= tot) return;
+    const int k = (int)(i % nrp);
+    const long long r1 = i / nrp;
+    const int row = (int)(r1 % nrp);
+    const long long tm = r1 / nrp;
+    float g = 0.f;
+    if (row < nr && k < nr) g = G[(tm * nr + row) * nr + k];
+    const unsigned short h = bf16_rn(g);
+    hi[i] = h;
+    lo[i] = bf16_rn(g - bf16_to_f(h));
+}
+
+// element (source ring k, column col) of the (type, m) right-hand side
+__device__ __forceinline__ float ens_rhs(const EnsGemmArgs &a, int type, int m, bool pair,
+                                         int off, int k, int col) {
+    const int mem = col >> 1, cs = col & 1;
+    if (mem >= a.nmem || k >= a.nr) return 0.f;
+    const int jrs = a.ranges[4 * mem], krs = a.ranges[4 * mem + 1];
+    if (jrs < 1 || k + 1 < jrs || k + 1 > krs) return 0.f;
+    const float *cf = a.coef + (size_t)mem * a.ncoef * a.nr;
+    const float mc = cf[(size_t)off * a.nr + k];
+    const float ms = pair ? cf[(size_t)(off + 1) * a.nr + k] : 0.f;
+    if (type == 1) return cs == 0 ? ms : -mc;
+    return cs == 0 ? mc : ms;
+}
+
+__device__ __forceinline__ bool ens_valid(const EnsGemmArgs &a, int type, int m) {
+    const int maz = min(a.mmax - 2, a.ng - 1);
+    return !a.lg[m] && (type != 1 || (m >= 1 && m <= maz));
+}
+
+__device__ __forceinline__ void ens_store(const EnsGemmArgs &a, int type, int m, int row,
+                                          int col, float v) {
+    if (row >= a.nr || col >= 2 * a.nmem) return;
+    const int mem = col >> 1, cs = col & 1;
+    a.W[((((size_t)mem * 3 + type) * 2 + cs) * a.ng + m) * a.nr + row] = v;
+}
+
+// 16 x 16 output tiles, K = 16 per instruction (v_mfma_f32_16x16x16_bf16):
+// lane l holds A row l%16 and B column l%16 at k = 4(l/16)..+3, and D
+// column l%16 at rows 4(l/16)..+3.  WPB wavefronts per block, each its own
+// tile along the columns.  grid (column tiles / WPB, nrp/16, ntypes*ng).
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void ens_gemm16_kernel(EnsGemmArgs a) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tn = blockIdx.x * WPB + w, tm = blockIdx.y;
+    const int type = blockIdx.z / a.ng, m = blockIdx.z - type * a.ng;
+    if (tn * 16 >= 2 * a.nmem) return;
+    const bool valid = ens_valid(a, type, m);
+    const bool pair = m >= 1 && m < a.mmax - 1;
+    const int off = m == 0 ? 0 : 2 * m - 1;
+    gh_v4f acc = {0.f, 0.f, 0.f, 0.f};
+    const int col = tn * 16 + (lane & 15), kq = 4 * (lane >> 4);
+    if (valid) {
+        const size_t rowoff = ((size_t)(type * a.ng + m) * a.nrp + tm * 16 + (lane & 15)) * a.nrp;
+        for (int k0 = 0; k0 < a.nrp; k0 += 16) {
+            gh_v4s ah, al, bh, bl;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                ah[j] = (short)a.ghi[rowoff + k0 + kq + j];
+                al[j] = (short)a.glo[rowoff + k0 + kq + j];
+                const float b = ens_rhs(a, type, m, pair, off, k0 + kq + j, col);
+                const unsigned short h = bf16_rn(b);
+                bh[j] = (short)h;
+                bl[j] = (short)bf16_rn(b - bf16_to_f(h));
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) ens_store(a, type, m, tm * 16 + kq + i, col, acc[i]);
+}
+
+// 32 x 32 output tiles, K = 8 per instruction (v_mfma_f32_32x32x8_bf16):
+// lane l holds A row l%32 and B column l%32 at k = 4(l/32)..+3, and D
+// column l%32 at rows 8(i/4) + 4(l/32) + i%4, i = 0..15.
+// grid (column tiles, nrp/32, ntypes*ng), one wavefront per block.
+__global__ __launch_bounds__(64) void ens_gemm32_kernel(EnsGemmArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int tn = blockIdx.x, tm = blockIdx.y;
+    const int type = blockIdx.z / a.ng, m = blockIdx.z - type * a.ng;
+    const bool valid = ens_valid(a, type, m);
+    const bool pair = m >= 1 && m < a.mmax - 1;
+    const int off = m == 0 ? 0 : 2 * m - 1;
+    gh_v16f acc;
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = 0.f;
+    const int col = tn * 32 + (lane & 31), kq = 4 * (lane >> 5);
+    if (valid) {
+        const size_t rowoff = ((size_t)(type * a.ng + m) * a.nrp + tm * 32 + (lane & 31)) * a.nrp;
+        for (int k0 = 0; k0 < a.nrp; k0 += 8) {
+            gh_v4s ah, al, bh, bl;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                ah[j] = (short)a.ghi[rowoff + k0 + kq + j];
+                al[j] = (short)a.glo[rowoff + k0 + kq + j];
+                const float b = ens_rhs(a, type, m, pair, off, k0 + kq + j, col);
+                const unsigned short h = bf16_rn(b);
+                bh[j] = (short)h;
+                bl[j] = (short)bf16_rn(b - bf16_to_f(h));
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh, acc, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        ens_store(a, type, m, tm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i]);
+}
+
+hipError_t launch_split_bf16(const float *G, unsigned short *hi, unsigned short *lo, int ng,
+                             int nr, int nrp, hipStream_t s) {
+    const long long tot = 3LL * ng * nrp * nrp;
+    hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((tot + GH_BLOCK - 1) / GH_BLOCK)),
+                       dim3(GH_BLOCK), 0, s, G, hi, lo, ng, nr, nrp);
+    return hipGetLastError();
+}
+
+// variant: 0 = 16x16 tiles, one wavefront per block; 1 = 16x16, four
+// wavefronts per block; 2 = 32x32 tiles
+hipError_t launch_ens_gemm(const EnsGemmArgs &a, int variant, hipStream_t s) {
+    const int ncol = 2 * a.nmem;
+    const unsigned z = (unsigned)(a.ntypes * a.ng);
+    if (variant == 2) {
+        hipLaunchKernelGGL(ens_gemm32_kernel, dim3((ncol + 31) / 32, a.nrp / 32, z), dim3(64), 0,
+                           s, a);
+    } else if (variant == 1) {
+        const int tn = (ncol + 15) / 16;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(ens_gemm16_kernel<4>), dim3((tn + 3) / 4, a.nrp / 16, z),
+                           dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(ens_gemm16_kernel<1>),
+                           dim3((ncol + 15) / 16, a.nrp / 16, z), dim3(64), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -213,6 +213,23 @@ class P2DGrid:
         """Run the steps of run() as replayed HIP graphs."""
         self._c(self._lib.galhip_set_graph(self._h, int(bool(on))), "set_graph")
 
+    def set_contraction(self, mode, variant=None):
+        """0 / "exact": FP32 dot products in the synthesis (default);
+        1 / "mfma": split-BF16 GEMM on the matrix cores, tile variant tuned
+        on the first solve, or fixed by variant (0, 1, 2)."""
+        m = 1 if mode in (1, "mfma") else 0
+        if m and variant is not None:
+            m = 2 + int(variant)
+        self._c(self._lib.galhip_set_contraction(self._h, m), "set_contraction")
+
+    def get_contraction(self):
+        """(mode, tuned variant, per-variant GEMM microseconds)."""
+        mode, var = C.c_int32(), C.c_int32()
+        us = np.zeros(3)
+        self._c(self._lib.galhip_get_contraction(self._h, C.byref(mode), C.byref(var),
+                                                 _ptr(us)), "get_contraction")
+        return mode.value, var.value, us
+
     # -- adjoint (one zone, one sector) ------------------------------------------
     def adj_record(self, slot):
         self._c(self._lib.galhip_adj_record(self._h, int(slot)), "adj_record")

@@ -317,6 +317,23 @@ int galhip_ens_get_noff(const galhip_ctx *ctx, int64_t *noff);
 /* galhip_integrals of every member: out[nmem][9]. */
 int galhip_ens_integrals(galhip_ctx *ctx, int32_t centred, double *out);
 
+/* Where the radial Green's contraction runs (FP32 p2d grids and
+ * ensembles): mode 0 (default) in the synthesis kernel, one FP32 dot product
+ * per (ring, type, harmonic) -- the arithmetic that makes an ensemble member
+ * bit-identical to its lone disc; mode 1 as one GEMM per (type, harmonic)
+ * over all members' cos/sin columns on the matrix cores, operands split
+ * into bf16 hi + lo and hi*hi + hi*lo + lo*hi accumulated in FP32 (relative
+ * error ~1e-5 of the field).  The GEMM's tile shape is chosen on the first
+ * solve by timing the built variants (0: 16x16 tiles, one wavefront per
+ * block; 1: 16x16, four per block; 2: 32x32 tiles); modes 2, 3, 4 select
+ * variant 0, 1, 2 without tuning (variants 0 and 1 sum in the same order,
+ * variant 2 in another, so a fixed variant makes runs reproducible from
+ * process to process).  galhip_get_contraction returns the mode (0 or 1),
+ * the variant (-1: not tuned yet) and each variant's measured time in
+ * microseconds (us[3], 0 when not tuned). */
+int galhip_set_contraction(galhip_ctx *ctx, int32_t mode);
+int galhip_get_contraction(const galhip_ctx *ctx, int32_t *mode, int32_t *variant, double *us);
+
 /* Run galhip_run's steps as HIP graphs (p2d grids and ensembles): each
  * step's three launches are captured once per distinct set of launch
  * arguments and replayed.  Results are identical to on = 0.  While

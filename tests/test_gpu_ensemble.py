@@ -205,3 +205,73 @@ def test_graph_replay_equals_direct_launches(kind):
             g.integrals()
     A.close()
     B.close()
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_mfma_contraction_close_to_exact():
+    """The split-BF16 MFMA contraction against the FP32 dot products: every
+    member's F_x, F_y and potential within 5e-5 of the field's maximum; a
+    graph-replayed run equals the direct run with the same contraction bit
+    for bit; a lone grid with the MFMA contraction agrees as well."""
+    _say("mfma contraction")
+    E, parts = _ensemble("fp32")
+    M, _ = _ensemble("fp32")
+    M.set_contraction("mfma")
+    for g in (E, M):
+        g.masset()
+        g.findf(True)
+    mode, var, us = M.get_contraction()
+    _say("mfma variant %d, variant times (us) %s" % (var, us))
+    assert mode == 1 and var in (0, 1, 2) and np.all(us > 0)
+    for i in range(len(COUNTS)):
+        if COUNTS[i] == 0:
+            continue
+        fe, fm = E.get_field(i), M.get_field(i)
+        for t in range(3):
+            err = _rel(fm[t], fe[t])
+            assert err < 5e-5, (i, t, err)
+    # each variant on its own
+    for v in range(3):
+        V, _ = _ensemble("fp32")
+        V.set_contraction("mfma", variant=v)
+        V.masset()
+        V.findf(True)
+        assert V.get_contraction()[1] == v
+        for i in range(len(COUNTS)):
+            if COUNTS[i]:
+                fe, fv = E.get_field(i), V.get_field(i)
+                for t in range(3):
+                    assert _rel(fv[t], fe[t]) < 5e-5, (v, i, t, _rel(fv[t], fe[t]))
+        V.close()
+    # graph replay with the GEMM in the step (the variant M tuned to)
+    A, _ = _ensemble("fp32")
+    A.set_contraction("mfma", variant=var)
+    A.set_graph(True)
+    for g in (A, M):
+        g.tmcent(True) if g is M else (g.masset(), g.findf(True), g.tmcent(True))
+    for k in range(2):
+        A.run(7)
+        M.run(7)
+        ua, um = A.unload(), M.unload()
+        for i in range(len(COUNTS)):
+            for key in ("x", "y", "vx", "vy", "iz"):
+                _same(ua[i][key], um[i][key], "mfma graph run %d member %d %s" % (k, i, key))
+    for g in (E, M, A):
+        g.close()
+    # a lone grid
+    from tests._model import gpu_model
+    P = gpu_model(precision="fp32", n=30000)
+    Q = gpu_model(precision="fp32", n=30000)
+    Q.set_contraction("mfma")
+    for g in (P, Q):
+        g.masset()
+        g.findf(True)
+    fp, fq = P.get_field(), Q.get_field()
+    for t in range(3):
+        assert _rel(fq[t], fp[t]) < 5e-5, (t, _rel(fq[t], fp[t]))
+    P.close()
+    Q.close()
