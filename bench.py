@@ -94,13 +94,13 @@ def a2(x, y, lscale, r1=1.0, r2=3.0):
     return complex(c)
 
 
-def run_gpu(w, precision, n, steps, warmup, sort_interval, graph=0):
+def run_gpu(w, precision, n, steps, warmup, sort_interval, graph=0, compensated=False):
     """Wall-clock rate over `steps` steps without events on the stream, then
     the same number of further steps with HIP events around every push
     launch (the push kernel's own time for the roofline)."""
     from galaxy_amd import configs
     t0 = time.time()
-    g = configs.make_grid(w, precision=precision, n=n)
+    g = configs.make_grid(w, precision=precision, n=n, compensated=compensated)
     g.set_sort_interval(sort_interval)
     g.set_graph(graph)
     log("[bench] %s set-up %.1f s (N=%d)" % (precision, time.time() - t0, n))
@@ -498,6 +498,8 @@ def main():
     ap.add_argument("--contraction", default=None, choices=("exact", "mfma"),
                     help="ensemble radial contraction: FP32 dot products in the synthesis "
                          "(exact) or split-BF16 GEMM on MFMA (default for C4)")
+    ap.add_argument("--compensated", action="store_true",
+                    help="mixed form with compensated position updates (C2)")
     ap.add_argument("--graph", type=int, default=None, choices=(0, 1),
                     help="run the timed steps as HIP graphs (default: 1 for C4, 0 otherwise)")
     ap.add_argument("--cpu-n3", type=int, default=4_000_000)
@@ -540,7 +542,8 @@ def main():
     w = configs.C2
     n = args.n or w.n
     graph = args.graph or 0
-    mixed = run_gpu(w, "fp32", n, args.steps, args.warmup, args.sort_interval, graph)
+    mixed = run_gpu(w, "fp32", n, args.steps, args.warmup, args.sort_interval, graph,
+                    args.compensated)
     fp64 = None
     if not args.no_fp64:
         fp64 = run_gpu(w, "fp64", n, args.steps, args.warmup, args.sort_interval, graph)
@@ -567,6 +570,7 @@ def main():
         pushes_per_step=mixed["pushes"] / args.steps,
         sort_interval=args.sort_interval,
         graph=bool(graph),
+        compensated=bool(args.compensated),
     )
     if fp64:
         v64 = fp64["pushes"] / fp64["seconds"]

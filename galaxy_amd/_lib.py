@@ -102,6 +102,8 @@ SIGNATURES = {
     "galhip_ens_integrals": [_vp, _i32, _vp],
     "galhip_set_graph": [_vp, _i32],
     "galhip_set_contraction": [_vp, _i32],
+    "galhip_set_compensated": [_vp, _i32],
+    "galhip_get_position_lo": [_vp, _vp, _vp],
     "galhip_get_contraction": [_vp, _vp, _vp, _vp],
     "galhip_adj_record": [_vp, _i32],
     "galhip_adj_restore": [_vp, _i32],

@@ -162,11 +162,13 @@ def fixpot(w: Workload, rgrid=None):
     return disc.halo_potential_table(rg, w.lscale(), w.dt, md=w.md, rcore=w.rcore)
 
 
-def make_grid(w: Workload, precision="fp32", n=None, seed=None, green=True):
+def make_grid(w: Workload, precision="fp32", n=None, seed=None, green=True, compensated=False):
     """A ready P2DGrid for workload w: Green's tables, halo table, particles
-    loaded and assigned (masset)."""
+    loaded and assigned (masset); compensated: compensated position updates."""
     from .lib15 import P2DGrid
     g = P2DGrid(precision=precision, **grid_kwargs(w, n))
+    if compensated:
+        g.set_compensated(True)
     if green:
         g.greenm()
     dr, val, der = fixforce(w, g.rgrid)

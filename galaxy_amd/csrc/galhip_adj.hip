@@ -483,6 +483,7 @@ static int adj_check(galhip_ctx *c) {
     if (c->p3d || c->ens) return gh_fail(GALHIP_EINVAL, "the adjoint is built for one p2d grid");
     if (c->nz != 1 || c->nsect != 1)
         return gh_fail(GALHIP_EINVAL, "the adjoint needs one zone and one sector");
+    if (c->comp) return gh_fail(GALHIP_EINVAL, "the adjoint differentiates the plain position update");
     return 0;
 }
 

@@ -47,6 +47,8 @@ template <typename R> struct PermArgs {
     unsigned *iout;
     const unsigned *perm;
     long long n;
+    const typename Vec2<R>::T *lo_in;
+    typename Vec2<R>::T *lo_out;
 };
 template <typename R>
 hipError_t launch_permute_all(const PermArgs<R> &, hipStream_t);
@@ -79,6 +81,7 @@ int gh_fail(int code, const char *fmt, ...) {
 }
 
 static void free_particles(galhip_ctx *c) {
+    dfree(c->xlo); dfree(c->xlo2);
     dfree(c->xv); dfree(c->ut); dfree(c->zone); dfree(c->id);
     dfree(c->xv2); dfree(c->ut2); dfree(c->zone2); dfree(c->id2);
     dfree(c->key); dfree(c->key2); dfree(c->perm); dfree(c->perm2); dfree(c->sort_tmp);
@@ -399,6 +402,7 @@ static int ensure_particles(galhip_ctx *c, long long n) {
         nullptr, tmp, c->key, c->key2, c->perm, c->perm2, (int)n, 0, 32, c->s);
     if (e != hipSuccess) return gh_fail(GALHIP_EHIP, "radix sort sizing: %s", hipGetErrorString(e));
     rc = dalloc(&c->sort_tmp, tmp);
+    if (c->comp) rc |= dalloc(&c->xlo, 2 * es * n) | dalloc(&c->xlo2, 2 * es * n);
     if (rc) { free_particles(c); return rc; }
     c->sort_tmp_bytes = tmp;
     c->cap = c->n = n;
@@ -446,7 +450,10 @@ template <typename R> static int sort_impl(galhip_ctx *c) {
     p.iout = c->id2;
     p.perm = c->perm2;
     p.n = n;
+    p.lo_in = (const typename Vec2<R>::T *)c->xlo;
+    p.lo_out = (typename Vec2<R>::T *)c->xlo2;
     HIPCHK(launch_permute_all<R>(p, c->s));
+    if (c->xlo) std::swap(c->xlo, c->xlo2);
     std::swap(c->xv, c->xv2);
     std::swap(c->ut, c->ut2);
     std::swap(c->zone, c->zone2);
@@ -518,6 +525,7 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
     }
     HIPCHK(hipMemcpy(c->zone, z.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+    if (c->xlo) HIPCHK(hipMemset(c->xlo, 0, 2 * es * n));
     c->qbits = qbits_for(n);
     rc = compute_ut(c);
     if (rc) return rc;
@@ -682,6 +690,7 @@ template <typename R> static PushArgs<R> push_args(galhip_ctx *c) {
         a.qbits_m = c->m_qbits_d;
         a.nblocks = c->nblk;
     }
+    a.xlo = (typename Vec2<R>::T *)c->xlo;
     return a;
 }
 
@@ -695,6 +704,7 @@ template <typename R> static PushArgs<R> member_args(galhip_ctx *c, int m) {
     a.nblocks = 0;
     const long long o = c->m_off[m];
     const size_t mo = (size_t)m * c->mesh;
+    if (a.xlo) a.xlo += o;
     a.xv = (typename Vec4<R>::T *)c->xv + o;
     a.ut = (typename Vec2<R>::T *)c->ut + o;
     a.zone = c->zone + o;
@@ -1412,6 +1422,7 @@ extern "C" int galhip_ens_restrt(galhip_ctx *c, const int64_t *count, const void
     }
     HIPCHK(hipMemcpy(c->zone, z.data(), n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+    if (c->xlo) HIPCHK(hipMemset(c->xlo, 0, 2 * es * n));
     rc = compute_ut(c);
     if (rc) return rc;
     rc = galhip_sort(c);
@@ -1484,4 +1495,50 @@ extern "C" int galhip_ens_integrals(galhip_ctx *c, int32_t centred, double *out)
         return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
     return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out)
                                   : integrals_impl<double>(c, centred != 0, out);
+}
+
+// ---- compensated positions ----------------------------------------------------------
+
+extern "C" int galhip_set_compensated(galhip_ctx *c, int32_t on) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (on && (c->p3d || c->prec != GALHIP_FP32))
+        return gh_fail(GALHIP_EINVAL, "compensated positions are built for FP32 p2d grids");
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (on && !c->xlo && c->cap > 0) {
+        const size_t b = 2 * sizeof(float) * (size_t)c->cap;
+        int rc = dalloc(&c->xlo, b) | dalloc(&c->xlo2, b);
+        if (rc) return rc;
+        HIPCHK(hipMemset(c->xlo, 0, b));
+    }
+    if (!on) {
+        dfree(c->xlo);
+        dfree(c->xlo2);
+    }
+    c->comp = on != 0;
+    drop_graphs(c);
+    return 0;
+}
+
+extern "C" int galhip_get_position_lo(const galhip_ctx *c, void *xlo, void *ylo) {
+    if (!c || !xlo || !ylo) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (c->p3d) return gh_fail(GALHIP_EINVAL, "p2d grids only");
+    HIPCHK(hipStreamSynchronize(c->s));
+    const long long n = c->n;
+    const size_t es = esize(c);
+    if (n <= 0) return 0;
+    if (!c->xlo) {
+        memset(xlo, 0, es * n);
+        memset(ylo, 0, es * n);
+        return 0;
+    }
+    std::vector<unsigned> ids(n);
+    std::vector<float> lo(2 * n);
+    HIPCHK(hipMemcpy(ids.data(), c->id, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lo.data(), c->xlo, sizeof(float) * 2 * n, hipMemcpyDeviceToHost));
+    float *xo = (float *)xlo, *yo = (float *)ylo;
+    for (long long i = 0; i < n; i++) {
+        xo[ids[i]] = lo[2 * i];
+        yo[ids[i]] = lo[2 * i + 1];
+    }
+    return 0;
 }

@@ -117,6 +117,9 @@ struct galhip_ctx {
     // radial contraction: 0 in the synthesis kernel (default), 1 on MFMA
     // with split-BF16 operands (FP32 grids); the bf16 split of G, padded to
     // nrp rings; the GEMM output; the tuned GEMM variant (-1: not yet)
+    // compensated positions (FP32, opt-in): low words of (x, y), sort copy
+    int comp;
+    void *xlo, *xlo2;
     int contract, nrp, gsplit_valid, gemm_variant;
     unsigned short *ghi, *glo;
     float *wpre;

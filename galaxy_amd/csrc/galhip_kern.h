@@ -271,6 +271,8 @@ template <typename R> struct PushArgs {
     const int4 *blk;
     const int *qbits_m;
     int nblocks;
+    // compensated positions (FP32, opt-in): the low words of (x, y), or NULL
+    typename Vec2<R>::T *xlo;
 };
 
 // grdacc-only / weight-only argument block (parity entry points)

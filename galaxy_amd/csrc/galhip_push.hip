@@ -82,6 +82,18 @@ __device__ __forceinline__ void chstep(R tfac, R ox, R oy, R &ovx, R &ovy,
     ny = oy + nvy;
 }
 
+// Compensated position update: the position is hi + lo, and the new
+// (hi, lo) = TwoSum(hi, v + lo) (Knuth's two-sum, exact without fused
+// operations), so each update's rounding error is carried, not dropped.
+template <typename R>
+__device__ __forceinline__ R comp_add(R x, R v, R lo, R &lo_out) {
+    const R y = v + lo;
+    const R s = x + y;
+    const R bp = s - x;
+    lo_out = (x - (s - bp)) + (y - bp);
+    return s;
+}
+
 __device__ __forceinline__ long long wave_sum64(long long v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -100,7 +112,8 @@ __device__ __forceinline__ int wave_max(int v) {
 
 
 // ENS: an ensemble launch, the block's particles and member from a.blk.
-template <typename R, bool ENS>
+// COMP: compensated positions (a.xlo holds the low words).
+template <typename R, bool ENS, bool COMP>
 // Wavefronts per SIMD, measured: FP32 at least 5 (96 VGPRs, no spill; 4
 // waves at the unconstrained 102 VGPRs is 8 % slower, 6 waves spill); FP64
 // at least 4 (128 VGPRs, a few spills, 4 % faster than 3 waves without).
@@ -150,6 +163,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     // offsets (uniform base, per-lane offset: no 64-bit address arithmetic)
     R4 *const xvb = a.xv + base;
     R2 *const utb = a.ut + base;
+    R2 *const lob = COMP ? a.xlo + base : nullptr;
     uint8_t *const zb = a.zone + base;
     const int nblk = ENS ? eb.z : (int)min((long long)(GH_BLOCK * PPT), a.n - base);
     int zin[PPT];
@@ -363,6 +377,12 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
             }
             R nvx = ovx + ax_, nvy = ovy + ay_;
             R nx = ox + nvx, ny = oy + nvy;
+            R2 lo = zero2<R>(), elo = zero2<R>();
+            if (COMP) {
+                lo = lob[j];
+                nx = comp_add(ox, nvx, lo.x, elo.x);
+                ny = comp_add(oy, nvy, lo.y, elo.y);
+            }
             int znew = z0;
             R rnew = sqrt(nx * nx + ny * ny);
             bool off = rnew > a.rgrid;
@@ -373,6 +393,10 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
                     if (a.offanal && z0 < nz) {
                         R tfac = tsfac(a, nz) / tsfac(a, z0);
                         chstep(tfac, ox, oy, ovx, ovy, ax_, ay_, nx, ny, nvx, nvy);
+                        if (COMP) {
+                            nx = comp_add(ox, nvx, lo.x, elo.x);
+                            ny = comp_add(oy, nvy, lo.y, elo.y);
+                        }
                         rnew = sqrt(nx * nx + ny * ny);
                     }
                 }
@@ -387,11 +411,16 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
                 if (kz != znew) {
                     R tfac = tsfac(a, kz) / tsfac(a, znew);
                     chstep(tfac, ox, oy, ovx, ovy, ax_, ay_, nx, ny, nvx, nvy);
+                    if (COMP) {
+                        nx = comp_add(ox, nvx, lo.x, elo.x);
+                        ny = comp_add(oy, nvy, lo.y, elo.y);
+                    }
                     rnew = sqrt(nx * nx + ny * ny);
                     znew = kz;
                 }
             }
             xvb[j] = mk4(nx, ny, nvx, nvy);
+            if (COMP) lob[j] = elo;
             if (znew != z0) zb[j] = (uint8_t)znew;
             if (znew >= 1 && znew <= nz) {
                 R u, t;
@@ -838,6 +867,8 @@ template <typename R> struct PermArgs {
     unsigned *iout;
     const unsigned *perm;
     long long n;
+    const typename Vec2<R>::T *lo_in;   // compensated positions' low words, or NULL
+    typename Vec2<R>::T *lo_out;
 };
 
 template <typename R>
@@ -849,6 +880,7 @@ __global__ __launch_bounds__(GH_BLOCK) void permute_all_kernel(PermArgs<R> p) {
     p.ut_out[i] = p.ut_in[s];
     p.zout[i] = p.zin[s];
     p.iout[i] = p.iin[s];
+    if (p.lo_in) p.lo_out[i] = p.lo_in[s];
 }
 
 // ---- host launchers ---------------------------------------------------------
@@ -857,16 +889,27 @@ static inline unsigned nblocks(long long n, long long per) {
     return (unsigned)((n + per - 1) / per);
 }
 
+template <typename R, bool COMP>
+static void launch_push_c(const PushArgs<R> &a, hipStream_t s) {
+    if (a.blk) {
+        if (a.nblocks > 0)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(push_kernel<R, true, COMP>), dim3(a.nblocks),
+                               dim3(GH_BLOCK), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(push_kernel<R, false, COMP>),
+                           dim3(nblocks(a.n, GH_BLOCK * PPT)), dim3(GH_BLOCK), 0, s, a);
+    }
+}
+
 template <typename R>
 hipError_t launch_push(const PushArgs<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    if (a.blk) {
-        if (a.nblocks > 0)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(push_kernel<R, true>), dim3(a.nblocks),
-                               dim3(GH_BLOCK), 0, s, a);
+    if constexpr (sizeof(R) == 4) {
+        if (a.xlo) launch_push_c<R, true>(a, s);
+        else launch_push_c<R, false>(a, s);
     } else {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(push_kernel<R, false>),
-                           dim3(nblocks(a.n, GH_BLOCK * PPT)), dim3(GH_BLOCK), 0, s, a);
+        if (a.xlo) return hipErrorInvalidValue;   // compensated positions are FP32 only
+        launch_push_c<R, false>(a, s);
     }
     return hipGetLastError();
 }

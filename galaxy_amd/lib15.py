@@ -222,6 +222,18 @@ class P2DGrid:
             m = 2 + int(variant)
         self._c(self._lib.galhip_set_contraction(self._h, m), "set_contraction")
 
+    def set_compensated(self, on=True):
+        """Compensated position updates (FP32): low words of (x, y) carried."""
+        self._c(self._lib.galhip_set_compensated(self._h, int(bool(on))), "set_compensated")
+
+    def get_position_lo(self):
+        """Low words of (x, y) in load order (zeros when not compensated)."""
+        xl = np.zeros(self.n, self.dtype)
+        yl = np.zeros(self.n, self.dtype)
+        self._c(self._lib.galhip_get_position_lo(self._h, _ptr(xl), _ptr(yl)),
+                "get_position_lo")
+        return xl, yl
+
     def get_contraction(self):
         """(mode, tuned variant, per-variant GEMM microseconds)."""
         mode, var = C.c_int32(), C.c_int32()

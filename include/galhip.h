@@ -334,6 +334,16 @@ int galhip_ens_integrals(galhip_ctx *ctx, int32_t centred, double *out);
 int galhip_set_contraction(galhip_ctx *ctx, int32_t mode);
 int galhip_get_contraction(const galhip_ctx *ctx, int32_t *mode, int32_t *variant, double *us);
 
+/* Compensated position updates (FP32 p2d grids and ensembles; off by
+ * default, which keeps the reference's plain FP32 x <- x + v): each
+ * particle carries low words of (x, y), and the update forms the new
+ * (x, lo) = TwoSum(x, v + lo), so the rounding error of every drift is
+ * kept instead of dropped.  Forces, cells and masses use x.  Low words start
+ * at zero at load; galhip_get_position_lo copies them in load order (zeros
+ * when off).  No counterpart in the reference. */
+int galhip_set_compensated(galhip_ctx *ctx, int32_t on);
+int galhip_get_position_lo(const galhip_ctx *ctx, void *xlo, void *ylo);
+
 /* Run galhip_run's steps as HIP graphs (p2d grids and ensembles): each
  * step's three launches are captured once per distinct set of launch
  * arguments and replayed.  Results are identical to on = 0.  While
