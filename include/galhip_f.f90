@@ -273,5 +273,145 @@ module galhip_f
        integer(c_int64_t), value :: n
        type(c_ptr), value :: x, y, z, acc
      end function galhip_grdacc3
+     ! ensembles of p2d discs (no counterpart in the reference)
+     integer(c_int) function galhip_ensset(p, nmem, pmass, ctx) bind(c, name='galhip_ensset')
+       import :: c_int, c_ptr, c_int32_t, c_double, galhip_p2d_params
+       type(galhip_p2d_params), intent(in) :: p
+       integer(c_int32_t), value :: nmem
+       real(c_double), intent(in) :: pmass(*)
+       type(c_ptr), intent(out) :: ctx
+     end function galhip_ensset
+     integer(c_int) function galhip_ens_restrt(ctx, count, x, y, vx, vy, iz) &
+          bind(c, name='galhip_ens_restrt')
+       import :: c_int, c_ptr, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(in) :: count(*)
+       type(c_ptr), value :: x, y, vx, vy
+       integer(c_int32_t), intent(in) :: iz(*)
+     end function galhip_ens_restrt
+     integer(c_int) function galhip_ens_unload(ctx, x, y, vx, vy, iz) &
+          bind(c, name='galhip_ens_unload')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: x, y, vx, vy, iz
+     end function galhip_ens_unload
+     integer(c_int) function galhip_ens_get_mass(ctx, member, mfx, mozfx, jrad, krad, qbits) &
+          bind(c, name='galhip_ens_get_mass')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: member
+       type(c_ptr), value :: mfx, mozfx, jrad, krad, qbits
+     end function galhip_ens_get_mass
+     integer(c_int) function galhip_ens_get_field(ctx, member, fld) &
+          bind(c, name='galhip_ens_get_field')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: member
+       type(c_ptr), value :: fld
+     end function galhip_ens_get_field
+     integer(c_int) function galhip_ens_set_field(ctx, member, fld) &
+          bind(c, name='galhip_ens_set_field')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: member
+       type(c_ptr), value :: fld
+     end function galhip_ens_set_field
+     integer(c_int) function galhip_ens_get_ranges(ctx, member, r4) &
+          bind(c, name='galhip_ens_get_ranges')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: member
+       integer(c_int32_t), intent(out) :: r4(4)
+     end function galhip_ens_get_ranges
+     integer(c_int) function galhip_ens_get_noff(ctx, noff) bind(c, name='galhip_ens_get_noff')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: noff(*)
+     end function galhip_ens_get_noff
+     integer(c_int) function galhip_ens_integrals(ctx, centred, out) &
+          bind(c, name='galhip_ens_integrals')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: centred
+       real(c_double), intent(out) :: out(*)
+     end function galhip_ens_integrals
+     ! execution and precision controls
+     integer(c_int) function galhip_set_graph(ctx, on) bind(c, name='galhip_set_graph')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on
+     end function galhip_set_graph
+     integer(c_int) function galhip_set_contraction(ctx, mode) &
+          bind(c, name='galhip_set_contraction')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: mode
+     end function galhip_set_contraction
+     integer(c_int) function galhip_get_contraction(ctx, mode, variant, us) &
+          bind(c, name='galhip_get_contraction')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: mode, variant
+       real(c_double), intent(out) :: us(3)
+     end function galhip_get_contraction
+     integer(c_int) function galhip_set_compensated(ctx, on) &
+          bind(c, name='galhip_set_compensated')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on
+     end function galhip_set_compensated
+     integer(c_int) function galhip_get_position_lo(ctx, xlo, ylo) &
+          bind(c, name='galhip_get_position_lo')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: xlo, ylo
+     end function galhip_get_position_lo
+     ! adjoint of the p2d step (no counterpart in the reference)
+     integer(c_int) function galhip_adj_record(ctx, slot) bind(c, name='galhip_adj_record')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: slot
+     end function galhip_adj_record
+     integer(c_int) function galhip_adj_restore(ctx, slot) bind(c, name='galhip_adj_restore')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: slot
+     end function galhip_adj_restore
+     integer(c_int) function galhip_adj_set(ctx, lam) bind(c, name='galhip_adj_set')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: lam(*)
+     end function galhip_adj_set
+     integer(c_int) function galhip_adj_get(ctx, lam) bind(c, name='galhip_adj_get')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: lam(*)
+     end function galhip_adj_get
+     integer(c_int) function galhip_set_fixforce_dpar(ctx, n, dr, val, der) &
+          bind(c, name='galhip_set_fixforce_dpar')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n
+       real(c_double), value :: dr
+       real(c_double), intent(in) :: val(*), der(*)
+     end function galhip_set_fixforce_dpar
+     integer(c_int) function galhip_adj_step(ctx, gpar) bind(c, name='galhip_adj_step')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: gpar(2)
+     end function galhip_adj_step
+     integer(c_int) function galhip_adj_field(ctx, g, rbar) bind(c, name='galhip_adj_field')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: g(*)
+       real(c_double), intent(out) :: rbar(*)
+     end function galhip_adj_field
+     integer(c_int) function galhip_adj_get_profile(ctx, ms, steps) &
+          bind(c, name='galhip_adj_get_profile')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ms(2)
+       integer(c_int64_t), intent(out) :: steps
+     end function galhip_adj_get_profile
   end interface
 end module galhip_f

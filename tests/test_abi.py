@@ -67,3 +67,11 @@ def test_p2dset_rejects_grids_coarse_against_softening():
     p.softl = width / 5.5
     rc = lib.galhip_p2dset(ctypes.byref(p), ctypes.byref(ctx))
     assert rc == -3 and b"SETGRD" in lib.galhip_last_error()
+
+
+def test_fortran_module_declares_every_entry_point():
+    """include/galhip_f.f90 binds every function galhip.h declares."""
+    text = open(os.path.join(ROOT, "include", "galhip_f.f90")).read()
+    bound = set(re.findall(r"bind\(c,\s*name='(galhip_\w+)'\)", text))
+    missing = set(declared()) - bound - {"galhip_last_error"}
+    assert not missing, sorted(missing)
