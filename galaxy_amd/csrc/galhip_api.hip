@@ -33,9 +33,8 @@ hipError_t launch_key(const typename Vec2<R>::T *, const uint8_t *, long long,
                       int, int, int, int, unsigned, unsigned *, unsigned *,
                       hipStream_t);
 template <typename R>
-hipError_t launch_key_ens(const typename Vec2<R>::T *, const uint8_t *, long long, int, int,
-                          int, int, unsigned, const long long *, int, unsigned *, unsigned *,
-                          hipStream_t);
+hipError_t launch_key_ens(const typename Vec2<R>::T *, const uint8_t *, int, int, int, int, int,
+                          unsigned, const int4 *, unsigned *, unsigned *, hipStream_t);
 template <typename R> struct PermArgs {
     const typename Vec4<R>::T *xv_in;
     typename Vec4<R>::T *xv_out;
@@ -278,7 +277,7 @@ extern "C" int galhip_free(galhip_ctx *c) {
     if (c->G != (void *)c->Gd) dfree(c->G);
     dfree(c->Gd);
     dfree(c->fr_val); dfree(c->fr_der);
-    dfree(c->m_qbits_d); dfree(c->m_pmass_d); dfree(c->m_off_d); dfree(c->blk_d);
+    dfree(c->m_qbits_d); dfree(c->m_pmass_d); dfree(c->blk_d);
     dfree(c->ghi); dfree(c->glo); dfree(c->wpre);
     drop_graphs(c);
     adj_free(c);
@@ -427,9 +426,8 @@ template <typename R> static int sort_impl(galhip_ctx *c) {
         int mbits = 0;
         while ((1LL << mbits) < (long long)c->nmem) mbits++;
         if (kbits + mbits > 32) return gh_fail(GALHIP_EINVAL, "ensemble sort keys exceed 32 bits");
-        HIPCHK(launch_key_ens<R>((const typename Vec2<R>::T *)c->ut, c->zone, n, c->nr, c->na,
-                                 c->nz, kbits, kmax, c->m_off_d, c->nmem, c->key, c->perm,
-                                 c->s));
+        HIPCHK(launch_key_ens<R>((const typename Vec2<R>::T *)c->ut, c->zone, c->nblk, c->nr,
+                                 c->na, c->nz, kbits, kmax, c->blk_d, c->key, c->perm, c->s));
         end_bit = kbits + mbits;
     } else {
         HIPCHK(launch_key<R>((const typename Vec2<R>::T *)c->ut, c->zone, n, c->nr,
@@ -1385,15 +1383,11 @@ extern "C" int galhip_ens_restrt(galhip_ctx *c, const int64_t *count, const void
         c->m_qbits[m] = qbits_for(count[m]);
     }
     dfree(c->blk_d);
-    dfree(c->m_off_d);
-    rc = dalloc(&c->blk_d, sizeof(int4) * std::max<size_t>(blk.size(), 1)) |
-         dalloc(&c->m_off_d, sizeof(long long) * (c->nmem + 1));
+    rc = dalloc(&c->blk_d, sizeof(int4) * std::max<size_t>(blk.size(), 1));
     if (rc) return rc;
     c->nblk = (int)blk.size();
     if (!blk.empty())
         HIPCHK(hipMemcpy(c->blk_d, blk.data(), sizeof(int4) * blk.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->m_off_d, c->m_off.data(), sizeof(long long) * (c->nmem + 1),
-                     hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->m_qbits_d, c->m_qbits.data(), sizeof(int) * c->nmem, hipMemcpyHostToDevice));
     size_t es = esize(c);
     std::vector<uint8_t> z(n);

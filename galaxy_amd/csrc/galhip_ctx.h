@@ -83,8 +83,7 @@ struct galhip_ctx {
     std::vector<int> m_qbits;         // fixed-point scale per member
     int *m_qbits_d;
     double *m_pmass_d;
-    long long *m_off_d;
-    int4 *blk_d;                      // push blocks: (member, first, count, 0)
+    int4 *blk_d;                      // push and key blocks: (member, first, count, 0)
     int nblk;
     // HIP graphs of galhip_run's steps, keyed by their launch arguments
     int graph;

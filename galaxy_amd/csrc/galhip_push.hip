@@ -824,35 +824,30 @@ __global__ __launch_bounds__(GH_BLOCK) void key_kernel(
 
 // Ensemble sort key: (member << kbits) | the member's cell key (off-grid
 // particles last within their member), so that one stable sort groups the
-// members in order and cells within each.  Member m holds particles
-// [off[m], off[m + 1]).
+// members in order and cells within each.  One block per entry of the push
+// block table, which never straddles two members.
 template <typename R>
 __global__ __launch_bounds__(GH_BLOCK) void key_ens_kernel(
     const typename Vec2<R>::T *__restrict__ ut, const uint8_t *__restrict__ zone,
-    long long n, int nr, int na, int nz, int kbits, unsigned kmax,
-    const long long *__restrict__ off, int nmem, unsigned *__restrict__ key,
-    unsigned *__restrict__ idx) {
-    long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    // the last member whose first particle is at or before i
-    int lo = 0, hi = nmem - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= i) lo = mid;
-        else hi = mid - 1;
+    int nr, int na, int nz, int kbits, unsigned kmax, const int4 *__restrict__ blk,
+    unsigned *__restrict__ key, unsigned *__restrict__ idx) {
+    // block b: the push block table's (member, first particle, count)
+    const int4 b = blk[blockIdx.x];
+    for (int j = threadIdx.x; j < b.z; j += GH_BLOCK) {
+        const long long i = (long long)b.y + j;
+        unsigned k = kmax;
+        const int z = zone[i];
+        if (z >= 1 && z <= nz) {
+            R w[4];
+            int ring, spoke;
+            const typename Vec2<R>::T c = ut[i];
+            cic(c.x, c.y, nr, na, ring, spoke, w);
+            k = (unsigned)(ring * na + spoke);
+            if (k >= kmax) k = kmax - 1;
+        }
+        key[i] = ((unsigned)b.x << kbits) | k;
+        idx[i] = (unsigned)i;
     }
-    unsigned k = kmax;
-    int z = zone[i];
-    if (z >= 1 && z <= nz) {
-        R w[4];
-        int ring, spoke;
-        typename Vec2<R>::T c = ut[i];
-        cic(c.x, c.y, nr, na, ring, spoke, w);
-        k = (unsigned)(ring * na + spoke);
-        if (k >= kmax) k = kmax - 1;
-    }
-    key[i] = ((unsigned)lo << kbits) | k;
-    idx[i] = (unsigned)i;
 }
 
 // one pass moving every particle array to sorted order
@@ -971,13 +966,12 @@ hipError_t launch_key(const typename Vec2<R>::T *ut, const uint8_t *zone,
 }
 
 template <typename R>
-hipError_t launch_key_ens(const typename Vec2<R>::T *ut, const uint8_t *zone, long long n,
-                          int nr, int na, int nz, int kbits, unsigned kmax,
-                          const long long *off, int nmem, unsigned *key, unsigned *idx,
-                          hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(key_ens_kernel<R>, dim3(nblocks(n, GH_BLOCK)), dim3(GH_BLOCK), 0, s,
-                       ut, zone, n, nr, na, nz, kbits, kmax, off, nmem, key, idx);
+hipError_t launch_key_ens(const typename Vec2<R>::T *ut, const uint8_t *zone, int nblk,
+                          int nr, int na, int nz, int kbits, unsigned kmax, const int4 *blk,
+                          unsigned *key, unsigned *idx, hipStream_t s) {
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(key_ens_kernel<R>, dim3(nblk), dim3(GH_BLOCK), 0, s, ut, zone, nr, na,
+                       nz, kbits, kmax, blk, key, idx);
     return hipGetLastError();
 }
 
@@ -1004,9 +998,9 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
                                       int, int, unsigned, unsigned *,         \
                                       unsigned *, hipStream_t);               \
     template hipError_t launch_key_ens<R>(const typename Vec2<R>::T *, const uint8_t *, \
-                                          long long, int, int, int, int, unsigned,    \
-                                          const long long *, int, unsigned *,         \
-                                          unsigned *, hipStream_t);                   \
+                                          int, int, int, int, int, unsigned,          \
+                                          const int4 *, unsigned *, unsigned *,       \
+                                          hipStream_t);                               \
     template hipError_t launch_permute_all<R>(const PermArgs<R> &, hipStream_t);
 GH_INST(float)
 GH_INST(double)
