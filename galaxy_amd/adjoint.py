@@ -30,26 +30,29 @@ def a2_loss(x, y, lscale, r1=1.0, r2=3.0, sel=None):
     """L = |A_2|^2 over r1 <= R <= r2 (R = r / lscale) and dL/dx, dL/dy
     (membership of the annulus held fixed), FP64.  sel: a given membership
     mask instead (a finite-difference check of that derivative at large N,
-    where single particles crossing the annulus edges make L jump)."""
+    where single particles crossing the annulus edges make L jump).
+
+    With c = cos 2 phi = (x^2 - y^2)/r^2, s = sin 2 phi = 2xy/r^2 and
+    A = (sum c - i sum s)/count = Ar + i Ai: L = Ar^2 + Ai^2 and
+    dL/dphi_j = -4 (Ar s_j + Ai c_j)/count; dphi/dx = -y/r^2, dphi/dy = x/r^2."""
     x = np.asarray(x, np.float64)
     y = np.asarray(y, np.float64)
     r2_ = x * x + y * y
-    R = np.sqrt(r2_) / lscale
     if sel is None:
+        R = np.sqrt(r2_) / lscale
         sel = (R >= r1) & (R <= r2)
     cnt = max(int(sel.sum()), 1)
-    # cos 2 phi, sin 2 phi from the coordinates
     with np.errstate(invalid="ignore", divide="ignore"):
         inv = np.where(r2_ > 0, 1.0 / r2_, 0.0)
-    c2, s2 = (x * x - y * y) * inv, 2.0 * x * y * inv
-    A = (c2[sel].sum() - 1j * s2[sel].sum()) / cnt
-    L = abs(A) ** 2
-    # dL/dphi = 2 Re(conj(A) dA/dphi), dA/dphi_j = -2i e^{-2i phi_j} / cnt
-    dphi = np.where(sel, 2.0 * np.real(np.conj(A) * (-2j) * (c2 - 1j * s2)) / cnt, 0.0)
-    with np.errstate(invalid="ignore", divide="ignore"):
-        gx = np.where(r2_ > 0, -dphi * y / r2_, 0.0)
-        gy = np.where(r2_ > 0, dphi * x / r2_, 0.0)
-    return L, gx, gy
+    c2 = (x * x - y * y) * inv
+    s2 = 2.0 * x * y * inv
+    ar = c2[sel].sum() / cnt
+    ai = -s2[sel].sum() / cnt
+    L = ar * ar + ai * ai
+    dphi = (-4.0 / cnt) * (ar * s2 + ai * c2)
+    dphi[~sel] = 0.0
+    dphi *= inv
+    return L, -dphi * y, dphi * x
 
 
 class Trajectory:
