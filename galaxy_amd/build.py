@@ -29,6 +29,9 @@ FLAGS = [
 ]
 
 
+DRIVER = os.path.join(HERE, "galhip_galaxy")
+
+
 def build(verbose=False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -37,7 +40,22 @@ def build(verbose=False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+def build_driver(verbose=False) -> str:
+    """The native host driver (csrc/galhip_galaxy.cpp): plain C++ against the
+    C ABI, linked to the in-tree libgalhip.so."""
+    tmp = DRIVER + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "galhip_galaxy.cpp"), "-L", HERE, "-lgalhip",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

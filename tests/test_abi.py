@@ -75,3 +75,20 @@ def test_fortran_module_declares_every_entry_point():
     bound = set(re.findall(r"bind\(c,\s*name='(galhip_\w+)'\)", text))
     missing = set(declared()) - bound - {"galhip_last_error"}
     assert not missing, sorted(missing)
+
+
+DRIVER = os.path.join(ROOT, "galaxy_amd", "galhip_galaxy")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_rejects_bad_input(tmp_path):
+    """The native driver's argument and file checks (before any GPU call)."""
+    import subprocess
+    r = subprocess.run([DRIVER], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([DRIVER, str(tmp_path / "none.pcs"), str(tmp_path / "o.pcs"),
+                        "nr=8", "na=16", "hash=2", "soft=1", "lscale=1", "ts=0.1", "steps=1"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot read" in r.stderr
+    r = subprocess.run([DRIVER, "a", "b", "steps=1", "bogus=3"], capture_output=True, text=True)
+    assert r.returncode == 2 and "unknown key" in r.stderr
