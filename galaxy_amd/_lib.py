@@ -81,6 +81,7 @@ SIGNATURES = {
     "galhip_set_fixpot": [_vp, _i32, _dbl, _vp, _vp],
     "galhip_integrals": [_vp, _i32, _vp],
     "galhip_set_sort_interval": [_vp, _i32],
+    "galhip_set_pmass": [_vp, _dbl],
     "galhip_sort": [_vp],
     "galhip_get_order": [_vp, _vp],
     "galhip_set_profile": [_vp, _i32],
@@ -111,6 +112,8 @@ SIGNATURES = {
     "galhip_adj_get": [_vp, _vp],
     "galhip_set_fixforce_dpar": [_vp, _i32, _dbl, _vp, _vp],
     "galhip_adj_step": [_vp, _vp],
+    "galhip_adj_get_par": [_vp, _vp],
+    "galhip_adj_seed_a2": [_vp, _dbl, _dbl, _dbl, _vp],
     "galhip_adj_field": [_vp, _vp, _vp],
     "galhip_adj_get_profile": [_vp, _vp, _vp],
 }

@@ -20,6 +20,7 @@ gpar[1] (the fixed force's parameter derivative table).
 from __future__ import annotations
 
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -65,10 +66,12 @@ class Trajectory:
         self.nck = (self.nsteps + self.every - 1) // self.every
         self.seconds = dict(forward=0.0, recompute=0.0, backward=0.0)
 
-    def forward(self, x, y, vx, vy, iz):
-        """Load, run, return the final state (dict in load order)."""
+    def forward(self, x, y, vx, vy, iz, unload=True):
+        """Load, run, return the final state (dict in load order; None with
+        unload False: the state stays on the device)."""
         g = self.g
         g.set_sort_interval(10)
+        g.set_step(0)
         g.restrt(x, y, vx, vy, iz)
         g.masset()
         g.sync()
@@ -79,14 +82,15 @@ class Trajectory:
             g.run(1)
         g.sync()
         self.seconds["forward"] += time.perf_counter() - t0
-        return g.unload()
+        return g.unload() if unload else None
 
-    def backward(self, lam):
-        """lam [4][n]: adjoints of the final (x, y, vx, vy).  Returns the
-        adjoints of the initial state and gpar [2]."""
+    def backward(self, lam=None):
+        """lam [4][n]: adjoints of the final (x, y, vx, vy); None when they
+        were set on the device after forward (P2DGrid.adj_seed_a2).  Returns
+        the adjoints of the initial state and gpar [2]."""
         g, E = self.g, self.every
-        g.adj_set(lam)
-        gpar = np.zeros(2, np.float64)
+        if lam is not None:
+            g.adj_set(lam)
         base = self.nck
         g.sync()
         for k in range(self.nck - 1, -1, -1):
@@ -101,12 +105,12 @@ class Trajectory:
             for j in range(m - 1, -1, -1):
                 g.adj_restore(base + j)
                 g.findf(False)
-                g.adj_step(gpar)
+                g.adj_step()
             g.sync()
             t2 = time.perf_counter()
             self.seconds["recompute"] += t1 - t0
             self.seconds["backward"] += t2 - t1
-        return g.adj_get(), gpar
+        return g.adj_get(), g.adj_get_par()
 
 
 class DiscModel:
@@ -156,6 +160,25 @@ class DiscModel:
         g.set_fixforce_dpar(dr, v1 - v0, d1 - d0)
         return g
 
+    def _grid_for(self, md):
+        """The model's own grid, kept between gradient calls (its device
+        buffers and adjoint slots are allocated once), set to disc mass md."""
+        g = getattr(self, "_g", None)
+        if g is None:
+            g = self._g = self.grid(md)
+        elif md != self._g_md:
+            w, ls, ts = self.w, self.ls, self.w.dt
+            g.set_pmass(md * ls ** 3 * ts * ts / self.n)
+            g.set_fixforce(*disc.halo_table(self.rgrid, ls, ts, md=md, rcore=w.rcore))
+        self._g_md = md
+        return g
+
+    def close(self):
+        """Frees the grid the gradient calls keep."""
+        if getattr(self, "_g", None) is not None:
+            self._g.close()
+            self._g = None
+
     def loss(self, Q, md, nsteps):
         """Forward only: L for (Q, md) (no checkpoints)."""
         vx, vy = self.velocities(Q, md)
@@ -171,37 +194,45 @@ class DiscModel:
         g.close()
         return a2_loss(out["x"], out["y"], self.ls, sel=sel)[0]
 
-    def gradient(self, Q, md, nsteps, every=20, profile=False):
+    def gradient(self, Q, md, nsteps, every=20, profile=False, with_sel=False):
         """L and its gradient: dL/dvx0, dL/dvy0 (internal units), dL/dQ,
-        dL/dmd, through the adjoint kernels.  profile: HIP events around
-        the adjoint particle kernels (result 'kernel_ms': scatter, gather)."""
+        dL/dmd, through the adjoint kernels.  The loss and its adjoint seed
+        are formed on the device (galhip_adj_seed_a2); d(v0)/dQ is computed
+        on the host in a worker thread while the GPU runs.  profile: HIP
+        events around the adjoint particle kernels (result 'kernel_ms':
+        scatter, gather).  with_sel: also return the final state's annulus
+        membership (load order; 'sel')."""
+        t0 = time.perf_counter()
         vx, vy = self.velocities(Q, md)
-        g = self.grid(md)
+        pool = ThreadPoolExecutor(max_workers=1)
+        dv = pool.submit(ic_velocity_dq, self.k_sig, self.k_phi, self.vc2, self.c, self.g1,
+                         self.g2, self.cp, self.sp, Q, md, self.ls * self.w.dt)
+        g = self._grid_for(md)
         if profile:
             g.set_profile(True)
         tr = Trajectory(g, nsteps, every)
-        out = tr.forward(self.x0, self.y0, vx, vy, self.iz0)
-        L, gx, gy = a2_loss(out["x"], out["y"], self.ls)
-        R = np.hypot(out["x"].astype(np.float64), out["y"].astype(np.float64)) / self.ls
-        sel = (R >= 1.0) & (R <= 3.0)
-        lam = np.zeros((4, self.n))
-        lam[0], lam[1] = gx, gy
-        lam0, gpar = tr.backward(lam)
-        kms = g.adj_get_profile() if profile else None
-        g.close()
+        tr.seconds["setup"] = time.perf_counter() - t0
+        tr.forward(self.x0, self.y0, vx, vy, self.iz0, unload=False)
+        sel = None
+        if with_sel:
+            out = g.unload()
+            R = np.hypot(out["x"].astype(np.float64), out["y"].astype(np.float64)) / self.ls
+            sel = (R >= 1.0) & (R <= 3.0)
+        L, _ = g.adj_seed_a2(self.ls, 1.0, 3.0)
+        lam0, gpar = tr.backward()
+        kms = None
+        if profile:
+            kms = g.adj_get_profile()
+            g.set_profile(False)
+        t1 = time.perf_counter()
         lvx, lvy = lam0[2], lam0[3]
-        dQ, dmd_ic = self._ic_chain(Q, md, lvx, lvy)
-        dmd = dmd_ic + gpar[0] / md + gpar[1]
+        dvx, dvy = dv.result()
+        pool.shutdown()
+        dQ = float(np.dot(lvx, dvx) + np.dot(lvy, dvy))
+        dmd = dQ * Q / md + gpar[0] / md + gpar[1]
+        tr.seconds["finish"] = time.perf_counter() - t1
         return dict(loss=L, dvx=lvx, dvy=lvy, dQ=dQ, dmd=dmd, gpar=gpar,
                     seconds=dict(tr.seconds), sel=sel, kernel_ms=kms)
-
-    def _ic_chain(self, Q, md, lvx, lvy):
-        """dL/dQ and the initial-velocity part of dL/dmd from the velocity
-        adjoints: both parameters enter only through sigma_R = Q md k_sig."""
-        dvx, dvy = ic_velocity_dq(self.k_sig, self.k_phi, self.vc2, self.c, self.g1, self.g2,
-                                  self.cp, self.sp, Q, md, self.ls * self.w.dt)
-        dQ = float(np.dot(lvx, dvx) + np.dot(lvy, dvy))
-        return dQ, dQ * Q / md
 
 
 def ic_velocities(k_sig, k_phi, vc2, c, g1, g2, cp, sp, Q, md, f):

@@ -421,7 +421,8 @@ __global__ __launch_bounds__(GH_BLOCK) void adj_rbar_kernel(AdjSolveArgs a) {
     }
 }
 
-// out[q] = sum over blocks of part[b][q] (q = 0, 1), blocks in order
+// out[q] = sum over blocks of part[b][q] (q = 0, 1), blocks in order, and
+// out[2 + q] += out[q] (the sums since the adjoints were set)
 __global__ __launch_bounds__(GH_BLOCK) void adj_sum_kernel(const double *part, int nb,
                                                            double *out) {
     __shared__ double red[GH_BLOCK / 64];
@@ -436,9 +437,112 @@ __global__ __launch_bounds__(GH_BLOCK) void adj_sum_kernel(const double *part, i
             double s = 0.0;
             for (int w = 0; w < GH_BLOCK / 64; w++) s += red[w];
             out[q] = s;
+            out[2 + q] += s;
         }
         __syncthreads();
     }
+}
+
+// The seed of L = |A_2|^2 over r1 <= R <= r2 (R = r / lscale, equal masses):
+// A_2 = (sum c - i sum s) / count with c = cos 2 phi = (x^2 - y^2) / r^2 and
+// s = sin 2 phi = 2 x y / r^2, all in FP64 from the stored positions.
+struct A2Terms {
+    double c, s, inv, x, y;
+    bool in;
+};
+
+template <typename R>
+__device__ __forceinline__ A2Terms a2_terms(const typename Vec4<R>::T p, double lscale,
+                                            double r1, double r2) {
+    A2Terms t;
+    t.x = (double)p.x;
+    t.y = (double)p.y;
+    const double rr = t.x * t.x + t.y * t.y;
+    const double rad = sqrt(rr) / lscale;
+    t.in = rad >= r1 && rad <= r2;
+    t.inv = rr > 0.0 ? 1.0 / rr : 0.0;
+    t.c = (t.x * t.x - t.y * t.y) * t.inv;
+    t.s = 2.0 * t.x * t.y * t.inv;
+    return t;
+}
+
+// part[3 b + q]: block b's sums of c, s and the count over the annulus
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void a2_part_kernel(const typename Vec4<R>::T *__restrict__ xv,
+                                                           long long n, double lscale, double r1,
+                                                           double r2, double *__restrict__ part) {
+    __shared__ double red[3][GH_BLOCK / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long i = (long long)blockIdx.x * GH_BLOCK + tid;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        const A2Terms t = a2_terms<R>(xv[i], lscale, r1, r2);
+        if (t.in) {
+            v[0] = t.c;
+            v[1] = t.s;
+            v[2] = 1.0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        if (lane == 0) red[q][wave] = v[q];
+    }
+    __syncthreads();
+    if (tid < 3) {
+        double s = 0.0;
+        for (int w = 0; w < GH_BLOCK / 64; w++) s += red[tid][w];
+        part[3 * (long long)blockIdx.x + tid] = s;
+    }
+}
+
+// tot[0..3] = L, Re A_2, Im A_2, count (at least 1) from the block sums, in order
+__global__ __launch_bounds__(GH_BLOCK) void a2_total_kernel(const double *part, int nb,
+                                                            double *tot) {
+    __shared__ double red[3][GH_BLOCK / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int b = tid; b < nb; b += GH_BLOCK)
+#pragma unroll
+        for (int q = 0; q < 3; q++) v[q] += part[3 * (long long)b + q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+        if (lane == 0) red[q][wave] = v[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s[3];
+        for (int q = 0; q < 3; q++) {
+            s[q] = 0.0;
+            for (int w = 0; w < GH_BLOCK / 64; w++) s[q] += red[q][w];
+        }
+        const double cnt = s[2] > 1.0 ? s[2] : 1.0;
+        const double ar = s[0] / cnt, ai = -s[1] / cnt;
+        tot[0] = ar * ar + ai * ai;
+        tot[1] = ar;
+        tot[2] = ai;
+        tot[3] = cnt;
+    }
+}
+
+// lam = (dL/dx, dL/dy, 0, 0), storage order, the annulus membership held fixed:
+// dL/dphi = -4 (Re A_2 s + Im A_2 c) / count, dphi/dx = -y / r^2, dphi/dy = x / r^2
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void a2_seed_kernel(const typename Vec4<R>::T *__restrict__ xv,
+                                                           long long n, double lscale, double r1,
+                                                           double r2, const double *__restrict__ tot,
+                                                           double *__restrict__ lam) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const A2Terms t = a2_terms<R>(xv[i], lscale, r1, r2);
+    const double ar = tot[1], ai = tot[2], cnt = tot[3];
+    double dphi = t.in ? (-4.0 / cnt) * (ar * t.s + ai * t.c) : 0.0;
+    dphi *= t.inv;
+    lam[i] = -dphi * t.y;
+    lam[n + i] = dphi * t.x;
+    lam[2 * n + i] = 0.0;
+    lam[3 * n + i] = 0.0;
 }
 
 }  // namespace gh
@@ -478,6 +582,14 @@ static hipEvent_t adj_event(galhip_ctx *c) {
     return e;
 }
 
+// after the per-block partial sums (2 per block for a step, 3 for the A_2 seed):
+// [0..1] the last step's parameter sums, [2..3] their totals since the adjoints
+// were set, [4..7] the A_2 seed's L, Re A_2, Im A_2, count
+static double *adj_tail(galhip_ctx *c) {
+    const long long nb = (c->n + GH_BLOCK - 1) / GH_BLOCK;
+    return c->adj_part + 3 * (nb > 0 ? nb : 1);
+}
+
 static int adj_check(galhip_ctx *c) {
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     if (c->p3d || c->ens) return gh_fail(GALHIP_EINVAL, "the adjoint is built for one p2d grid");
@@ -498,7 +610,7 @@ static int adj_ensure(galhip_ctx *c) {
     rc |= dalloc(&c->adj_tmp, sizeof(double) * 4 * (size_t)(c->n > 0 ? c->n : 1));
     rc |= dalloc(&c->adj_id, sizeof(unsigned) * (size_t)(c->n > 0 ? c->n : 1));
     const long long nb = (c->n + GH_BLOCK - 1) / GH_BLOCK;
-    rc |= dalloc(&c->adj_part, sizeof(double) * 2 * (size_t)(nb > 0 ? nb : 1) + 16);
+    rc |= dalloc(&c->adj_part, sizeof(double) * (3 * (size_t)(nb > 0 ? nb : 1) + 8));
     if (!c->adj_g) {
         rc |= dalloc(&c->adj_g, sizeof(double) * 2 * (size_t)c->mesh);
         rc |= dalloc(&c->adj_rbar, sizeof(double) * (size_t)c->mesh);
@@ -507,6 +619,7 @@ static int adj_ensure(galhip_ctx *c) {
     if (rc) return rc;
     c->adj_n = c->n;
     HIPCHK(hipMemset(c->adj_lam, 0, sizeof(double) * 4 * (size_t)(c->n > 0 ? c->n : 1)));
+    HIPCHK(hipMemset(adj_tail(c), 0, sizeof(double) * 4));
     return 0;
 }
 
@@ -600,9 +713,60 @@ extern "C" int galhip_adj_set(galhip_ctx *c, const double *lam) {
                            (const double *)c->adj_tmp, (const unsigned *)c->id, c->adj_lam, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(c->adj_id, c->id, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
     }
+    HIPCHK(hipMemsetAsync(adj_tail(c), 0, sizeof(double) * 4, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
     c->adj_ver = c->order_ver;
+    return 0;
+}
+
+template <typename R> static int adj_seed_a2_impl(galhip_ctx *c, double lscale, double r1,
+                                                  double r2, double *out) {
+    const long long n = c->n;
+    const long long nb = (n + GH_BLOCK - 1) / GH_BLOCK;
+    double *tail = adj_tail(c);
+    const typename Vec4<R>::T *xv = (const typename Vec4<R>::T *)c->xv;
+    if (nb > 0)
+        hipLaunchKernelGGL(a2_part_kernel<R>, dim3((unsigned)nb), dim3(GH_BLOCK), 0, c->s, xv, n,
+                           lscale, r1, r2, c->adj_part);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(a2_total_kernel, dim3(1), dim3(GH_BLOCK), 0, c->s,
+                       (const double *)c->adj_part, (int)nb, tail + 4);
+    HIPCHK(hipGetLastError());
+    if (nb > 0) {
+        hipLaunchKernelGGL(a2_seed_kernel<R>, dim3((unsigned)nb), dim3(GH_BLOCK), 0, c->s, xv, n,
+                           lscale, r1, r2, (const double *)(tail + 4), c->adj_lam);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c->adj_id, c->id, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, c->s));
+    }
+    HIPCHK(hipMemsetAsync(tail, 0, sizeof(double) * 4, c->s));
+    double h[4];
+    HIPCHK(hipMemcpyAsync(h, tail + 4, sizeof h, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->adj_ver = c->order_ver;
+    if (out) memcpy(out, h, sizeof h);
+    return 0;
+}
+
+extern "C" int galhip_adj_seed_a2(galhip_ctx *c, double lscale, double r1, double r2,
+                                  double *out) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!(lscale > 0.0) || !(r1 >= 0.0) || !(r2 >= r1))
+        return gh_fail(GALHIP_EINVAL, "need lscale > 0 and 0 <= r1 <= r2");
+    rc = adj_ensure(c);
+    if (rc) return rc;
+    return c->prec == GALHIP_FP32 ? adj_seed_a2_impl<float>(c, lscale, r1, r2, out)
+                                  : adj_seed_a2_impl<double>(c, lscale, r1, r2, out);
+}
+
+extern "C" int galhip_adj_get_par(galhip_ctx *c, double *gpar) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!gpar) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (!c->adj_lam || c->adj_n != c->n) return gh_fail(GALHIP_EINVAL, "no adjoint set");
+    HIPCHK(hipMemcpyAsync(gpar, adj_tail(c) + 2, sizeof(double) * 2, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
     return 0;
 }
 
@@ -740,14 +904,14 @@ template <typename R> static int adj_step_impl(galhip_ctx *c, double *gpar) {
         HIPCHK(hipGetLastError());
     }
     if (prof) HIPCHK(hipEventRecord(adj_event(c), c->s));
-    double *dout = c->adj_part + 2 * (nb > 0 ? nb : 1);
+    double *dout = adj_tail(c);
     hipLaunchKernelGGL(adj_sum_kernel, dim3(1), dim3(GH_BLOCK), 0, c->s,
                        (const double *)c->adj_part, (int)nb, dout);
     HIPCHK(hipGetLastError());
-    double h[2];
-    HIPCHK(hipMemcpyAsync(h, dout, sizeof h, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
     if (gpar) {
+        double h[2];
+        HIPCHK(hipMemcpyAsync(h, dout, sizeof h, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
         gpar[0] += h[0];
         gpar[1] += h[1];
     }

@@ -51,6 +51,11 @@ template <typename R> struct PermArgs {
 };
 template <typename R>
 hipError_t launch_permute_all(const PermArgs<R> &, hipStream_t);
+template <typename R>
+hipError_t launch_interleave(const R *, typename Vec4<R>::T *, unsigned *, long long, hipStream_t);
+template <typename R>
+hipError_t launch_deinterleave(const typename Vec4<R>::T *, const uint8_t *, const unsigned *, R *,
+                               int *, long long, hipStream_t);
 template <typename R> hipError_t launch_analyse(const AnaArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_synth(const SynArgs<R> &, hipStream_t);
 hipError_t launch_green(double *, const double *, int, int, int, int, double,
@@ -481,7 +486,34 @@ extern "C" int galhip_set_sort_interval(galhip_ctx *c, int32_t k) {
     return 0;
 }
 
+extern "C" int galhip_set_pmass(galhip_ctx *c, double pmass) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble members keep the masses of galhip_ensset");
+    if (!(pmass > 0.0)) return gh_fail(GALHIP_EINVAL, "particle mass not positive");
+    c->pmass = pmass;
+    if (!c->m_pmass.empty()) c->m_pmass[0] = pmass;
+    c->have_field = 0;
+    c->pot_valid = 0;
+    return 0;
+}
+
 static int compute_ut(galhip_ctx *c);
+
+// the four planar host arrays through the second particle buffer (free
+// outside a sort) into interleaved storage, and the load-order ids
+static int load_xv(galhip_ctx *c, long long n, const void *x, const void *y, const void *vx,
+                   const void *vy) {
+    const size_t es = esize(c);
+    unsigned char *stage = (unsigned char *)c->xv2;
+    const void *src[4] = {x, y, vx, vy};
+    for (int k = 0; k < 4; k++)
+        HIPCHK(hipMemcpy(stage + k * es * n, src[k], es * n, hipMemcpyHostToDevice));
+    if (c->prec == GALHIP_FP32)
+        HIPCHK(launch_interleave<float>((const float *)stage, (float4 *)c->xv, c->id, n, c->s));
+    else
+        HIPCHK(launch_interleave<double>((const double *)stage, (double4 *)c->xv, c->id, n, c->s));
+    return 0;
+}
 
 extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
                              const void *y, const void *vx, const void *vy,
@@ -497,32 +529,21 @@ extern "C" int galhip_restrt(galhip_ctx *c, int64_t n, const void *x,
     if (rc) return rc;
     size_t es = esize(c);
     std::vector<uint8_t> z(n);
-    std::vector<unsigned> ids(n);
     long long cnt[4] = {0, 0, 0, 0};   // counters: [0] noff, [1] pushes
     for (long long i = 0; i < n; i++) {
         int v = iz[i];
         if (v < 1 || v > c->nz + 1)
             return gh_fail(GALHIP_EINVAL, "particle %lld has zone %d outside 1..%d", i, v, c->nz + 1);
         z[i] = (uint8_t)v;
-        ids[i] = (unsigned)i;
         if (v == c->nz + 1) cnt[0]++;    // /admin/ noff: the off-grid list
     }
     HIPCHK(hipMemcpy(c->counters, cnt, sizeof cnt, hipMemcpyHostToDevice));
     c->qbits = qbits_for(n);
     c->order_ver++;
     if (n == 0) return 0;
-    {
-        // interleave (x, y, vx, vy) per particle
-        std::vector<unsigned char> buf(4 * es * n);
-        const unsigned char *src[4] = {(const unsigned char *)x, (const unsigned char *)y,
-                                       (const unsigned char *)vx, (const unsigned char *)vy};
-        for (long long i = 0; i < n; i++)
-            for (int k = 0; k < 4; k++)
-                memcpy(buf.data() + (4 * i + k) * es, src[k] + i * es, es);
-        HIPCHK(hipMemcpy(c->xv, buf.data(), 4 * es * n, hipMemcpyHostToDevice));
-    }
+    rc = load_xv(c, n, x, y, vx, vy);
+    if (rc) return rc;
     HIPCHK(hipMemcpy(c->zone, z.data(), n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
     if (c->xlo) HIPCHK(hipMemset(c->xlo, 0, 2 * es * n));
     c->qbits = qbits_for(n);
     rc = compute_ut(c);
@@ -549,23 +570,22 @@ static int unload_all(galhip_ctx *c, void *x, void *y, void *vx, void *vy, int32
     HIPCHK(hipStreamSynchronize(c->s));
     long long n = c->n;
     if (n <= 0) return 0;
-    std::vector<unsigned> ids(n);
-    HIPCHK(hipMemcpy(ids.data(), c->id, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    // planar, in load order, into the second particle buffer and the sort
+    // keys (both free outside a sort), then straight to the caller's arrays
     void *dst[4] = {x, y, vx, vy};
-    size_t es = esize(c);
-    std::vector<unsigned char> buf(4 * es * n);
-    HIPCHK(hipMemcpy(buf.data(), c->xv, 4 * es * n, hipMemcpyDeviceToHost));
-    for (int k = 0; k < 4; k++) {
-        if (!dst[k]) continue;
-        unsigned char *d = (unsigned char *)dst[k];
-        for (long long i = 0; i < n; i++)
-            memcpy(d + (size_t)ids[i] * es, buf.data() + (4 * i + k) * es, es);
-    }
-    if (iz) {
-        std::vector<uint8_t> z(n);
-        HIPCHK(hipMemcpy(z.data(), c->zone, n, hipMemcpyDeviceToHost));
-        for (long long i = 0; i < n; i++) iz[ids[i]] = z[i];
-    }
+    const size_t es = esize(c);
+    unsigned char *stage = (unsigned char *)c->xv2;
+    int *zs = (int *)c->key;
+    if (c->prec == GALHIP_FP32)
+        HIPCHK(launch_deinterleave<float>((const float4 *)c->xv, c->zone, c->id, (float *)stage,
+                                          zs, n, c->s));
+    else
+        HIPCHK(launch_deinterleave<double>((const double4 *)c->xv, c->zone, c->id,
+                                           (double *)stage, zs, n, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int k = 0; k < 4; k++)
+        if (dst[k]) HIPCHK(hipMemcpy(dst[k], stage + k * es * n, es * n, hipMemcpyDeviceToHost));
+    if (iz) HIPCHK(hipMemcpy(iz, zs, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -1391,7 +1411,6 @@ extern "C" int galhip_ens_restrt(galhip_ctx *c, const int64_t *count, const void
     HIPCHK(hipMemcpy(c->m_qbits_d, c->m_qbits.data(), sizeof(int) * c->nmem, hipMemcpyHostToDevice));
     size_t es = esize(c);
     std::vector<uint8_t> z(n);
-    std::vector<unsigned> ids(n);
     std::vector<long long> cnt(4 * (size_t)c->nmem, 0);   // per member: [0] noff
     for (int m = 0; m < c->nmem; m++)
         for (long long i = c->m_off[m]; i < c->m_off[m + 1]; i++) {
@@ -1399,23 +1418,14 @@ extern "C" int galhip_ens_restrt(galhip_ctx *c, const int64_t *count, const void
             if (v < 1 || v > c->nz + 1)
                 return gh_fail(GALHIP_EINVAL, "particle %lld has zone %d outside 1..%d", i, v, c->nz + 1);
             z[i] = (uint8_t)v;
-            ids[i] = (unsigned)i;
             if (v == c->nz + 1) cnt[4 * m]++;
         }
     HIPCHK(hipMemcpy(c->counters, cnt.data(), sizeof(long long) * cnt.size(), hipMemcpyHostToDevice));
     drop_graphs(c);
     if (n == 0) return 0;
-    {
-        std::vector<unsigned char> buf(4 * es * n);
-        const unsigned char *src[4] = {(const unsigned char *)x, (const unsigned char *)y,
-                                       (const unsigned char *)vx, (const unsigned char *)vy};
-        for (long long i = 0; i < n; i++)
-            for (int k = 0; k < 4; k++)
-                memcpy(buf.data() + (4 * i + k) * es, src[k] + i * es, es);
-        HIPCHK(hipMemcpy(c->xv, buf.data(), 4 * es * n, hipMemcpyHostToDevice));
-    }
+    rc = load_xv(c, n, x, y, vx, vy);
+    if (rc) return rc;
     HIPCHK(hipMemcpy(c->zone, z.data(), n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
     if (c->xlo) HIPCHK(hipMemset(c->xlo, 0, 2 * es * n));
     rc = compute_ut(c);
     if (rc) return rc;

@@ -878,6 +878,43 @@ __global__ __launch_bounds__(GH_BLOCK) void permute_all_kernel(PermArgs<R> p) {
     if (p.lo_in) p.lo_out[i] = p.lo_in[s];
 }
 
+// load: planar host arrays (x, y, vx, vy, es*n bytes each, staged one after
+// another in s) -> interleaved storage; ids = load order
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void interleave_kernel(const R *__restrict__ s,
+                                                              typename Vec4<R>::T *__restrict__ xv,
+                                                              unsigned *__restrict__ id,
+                                                              long long n) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    typename Vec4<R>::T v;
+    v.x = s[i];
+    v.y = s[n + i];
+    v.z = s[2 * n + i];
+    v.w = s[3 * n + i];
+    xv[i] = v;
+    id[i] = (unsigned)i;
+}
+
+// unload: interleaved storage -> planar arrays in load order (d[k n + id]),
+// zones widened to int32
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void deinterleave_kernel(const typename Vec4<R>::T *__restrict__ xv,
+                                                                const uint8_t *__restrict__ zone,
+                                                                const unsigned *__restrict__ id,
+                                                                R *__restrict__ d,
+                                                                int *__restrict__ iz, long long n) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const long long p = id[i];
+    const typename Vec4<R>::T v = xv[i];
+    d[p] = v.x;
+    d[n + p] = v.y;
+    d[2 * n + p] = v.z;
+    d[3 * n + p] = v.w;
+    iz[p] = zone[i];
+}
+
 // ---- host launchers ---------------------------------------------------------
 
 static inline unsigned nblocks(long long n, long long per) {
@@ -976,6 +1013,25 @@ hipError_t launch_key_ens(const typename Vec2<R>::T *ut, const uint8_t *zone, in
 }
 
 template <typename R>
+hipError_t launch_interleave(const R *stage, typename Vec4<R>::T *xv, unsigned *id, long long n,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(interleave_kernel<R>, dim3(nblocks(n, GH_BLOCK)), dim3(GH_BLOCK), 0, s,
+                       stage, xv, id, n);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_deinterleave(const typename Vec4<R>::T *xv, const uint8_t *zone,
+                               const unsigned *id, R *stage, int *iz, long long n,
+                               hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(deinterleave_kernel<R>, dim3(nblocks(n, GH_BLOCK)), dim3(GH_BLOCK), 0, s,
+                       xv, zone, id, stage, iz, n);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
     if (p.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(permute_all_kernel<R>, dim3(nblocks(p.n, GH_BLOCK)),
@@ -1001,7 +1057,12 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
                                           int, int, int, int, int, unsigned,          \
                                           const int4 *, unsigned *, unsigned *,       \
                                           hipStream_t);                               \
-    template hipError_t launch_permute_all<R>(const PermArgs<R> &, hipStream_t);
+    template hipError_t launch_permute_all<R>(const PermArgs<R> &, hipStream_t); \
+    template hipError_t launch_interleave<R>(const R *, typename Vec4<R>::T *, unsigned *, \
+                                             long long, hipStream_t);                      \
+    template hipError_t launch_deinterleave<R>(const typename Vec4<R>::T *, const uint8_t *, \
+                                               const unsigned *, R *, int *, long long,      \
+                                               hipStream_t);
 GH_INST(float)
 GH_INST(double)
 

@@ -206,6 +206,11 @@ class P2DGrid:
         self._c(self._lib.galhip_get_order(self._h, _ptr(ids)), "get_order")
         return ids
 
+    def set_pmass(self, pmass):
+        """Particle mass in place of the creation parameter (single grids);
+        the last field is invalidated."""
+        self._c(self._lib.galhip_set_pmass(self._h, float(pmass)), "set_pmass")
+
     def set_sort_interval(self, k):
         self._c(self._lib.galhip_set_sort_interval(self._h, int(k)), "set_sort_interval")
 
@@ -266,9 +271,26 @@ class P2DGrid:
         self._c(self._lib.galhip_set_fixforce_dpar(self._h, len(val), float(dr), _ptr(val),
                                                     _ptr(der)), "set_fixforce_dpar")
 
-    def adj_step(self, gpar):
-        """One backward step; gpar (2 FP64) accumulates the parameter terms."""
-        self._c(self._lib.galhip_adj_step(self._h, _ptr(gpar)), "adj_step")
+    def adj_step(self, gpar=None):
+        """One backward step; gpar (2 FP64) accumulates the parameter terms.
+        With gpar None the call does not wait for the GPU; the totals stay
+        on the device (adj_get_par)."""
+        self._c(self._lib.galhip_adj_step(self._h, None if gpar is None else _ptr(gpar)),
+                "adj_step")
+
+    def adj_get_par(self):
+        """The parameter sums' totals since the adjoints were set (2 FP64)."""
+        gpar = np.zeros(2, np.float64)
+        self._c(self._lib.galhip_adj_get_par(self._h, _ptr(gpar)), "adj_get_par")
+        return gpar
+
+    def adj_seed_a2(self, lscale, r1=1.0, r2=3.0):
+        """Adjoints of L = |A_2|^2 over r1 <= r / lscale <= r2, formed on the
+        device.  Returns (L, A_2) with A_2 complex."""
+        out = np.zeros(4, np.float64)
+        self._c(self._lib.galhip_adj_seed_a2(self._h, float(lscale), float(r1), float(r2),
+                                             _ptr(out)), "adj_seed_a2")
+        return float(out[0]), complex(out[1], out[2])
 
     def adj_get_profile(self):
         """(scatter ms, gather ms, steps) since the last call (profiling on)."""

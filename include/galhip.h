@@ -344,6 +344,13 @@ int galhip_get_contraction(const galhip_ctx *ctx, int32_t *mode, int32_t *varian
 int galhip_set_compensated(galhip_ctx *ctx, int32_t on);
 int galhip_get_position_lo(const galhip_ctx *ctx, void *xlo, void *ylo);
 
+/* Sets the particle mass of a single grid (p2d or p3d, not an ensemble)
+ * in place of the creation parameter's (setrun.f:866 fixes it for a
+ * run), so one context can serve models that differ only in the disc
+ * mass.  The mass sums are scaled by it at every field solve; the last
+ * field is invalidated (findf again before stepping). */
+int galhip_set_pmass(galhip_ctx *ctx, double pmass);
+
 /* Run galhip_run's steps as HIP graphs (p2d grids and ensembles): each
  * step's three launches are captured once per distinct set of launch
  * arguments and replayed.  Results are identical to on = 0.  While
@@ -377,8 +384,21 @@ int galhip_set_fixforce_dpar(galhip_ctx *ctx, int32_t n, double dr, const double
  * la . a_grid (the derivative for a factor on the particle mass, and so on
  * the grid force), gpar[1] += sum of la . (dF/dp)(r) x / r (the fixed
  * force's parameter derivative table), la being the adjoint of the step's
- * acceleration. */
+ * acceleration.  gpar may be NULL: the sums are kept on the device
+ * (galhip_adj_get_par) and the call does not wait for the GPU. */
 int galhip_adj_step(galhip_ctx *ctx, double *gpar);
+/* gpar[2]: the totals of galhip_adj_step's two parameter sums over the
+ * steps since the adjoints were last set (galhip_adj_set or
+ * galhip_adj_seed_a2). */
+int galhip_adj_get_par(galhip_ctx *ctx, double *gpar);
+/* Sets the adjoints for the loss L = |A_2|^2 of the current particles, as
+ * galhip_adj_set would with lam = (dL/dx, dL/dy, 0, 0), computed on the
+ * device: A_2 = sum e^{-2 i phi} / count over the particles with
+ * r1 <= R <= r2 (R = sqrt(x^2 + y^2) / lscale, equal masses, the m = 2
+ * coefficient of densty.f's DANL analysis), the annulus membership held
+ * fixed in the derivative; FP64.  out[4] (may be NULL) = L, Re A_2,
+ * Im A_2, count (at least 1).  Clears the parameter sums. */
+int galhip_adj_seed_a2(galhip_ctx *ctx, double lscale, double r1, double r2, double *out);
 /* The transposed field solve alone: g[2][mesh] (adjoints of F_x, F_y) ->
  * rbar[mesh] (adjoint of the combined, scaled mass), with the ring ranges
  * of the last findf (FP64 tables). */

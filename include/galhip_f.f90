@@ -212,6 +212,11 @@ module galhip_f
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: k
      end function galhip_set_sort_interval
+     integer(c_int) function galhip_set_pmass(ctx, pmass) bind(c, name='galhip_set_pmass')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: pmass
+     end function galhip_set_pmass
      integer(c_int) function galhip_sort(ctx) bind(c, name='galhip_sort')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
@@ -400,6 +405,18 @@ module galhip_f
        type(c_ptr), value :: ctx
        real(c_double), intent(inout) :: gpar(2)
      end function galhip_adj_step
+     integer(c_int) function galhip_adj_get_par(ctx, gpar) bind(c, name='galhip_adj_get_par')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: gpar(2)
+     end function galhip_adj_get_par
+     integer(c_int) function galhip_adj_seed_a2(ctx, lscale, r1, r2, out) &
+          bind(c, name='galhip_adj_seed_a2')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: lscale, r1, r2
+       real(c_double), intent(out) :: out(4)
+     end function galhip_adj_seed_a2
      integer(c_int) function galhip_adj_field(ctx, g, rbar) bind(c, name='galhip_adj_field')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

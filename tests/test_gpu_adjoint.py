@@ -118,3 +118,77 @@ def test_torch_harness():
     assert abs(L.item() - gr["loss"]) <= 1e-12 * gr["loss"]
     assert abs(Q.grad.item() - gr["dQ"]) <= 1e-8 * abs(gr["dQ"]), (Q.grad.item(), gr["dQ"])
     assert abs(md.grad.item() - gr["dmd"]) <= 1e-8 * abs(gr["dmd"]), (md.grad.item(), gr["dmd"])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_device_seed_matches_host_loss(prec):
+    """galhip_adj_seed_a2 against a2_loss on the unloaded particles: L and
+    A_2 to 1e-12, the seeded adjoints (read back in load order) to 1e-12 of
+    their maximum.  Then the backward pass from the device seed with the
+    parameter sums kept on the device, against the pass from adj_set of the
+    host seed with the sums taken step by step: initial adjoints and gpar
+    to 1e-10."""
+    _say("device A_2 seed", prec)
+    m = adjoint.DiscModel(W, precision=prec)
+    vx, vy = m.velocities(W.Q, W.md)
+    res = []
+    for dev in (True, False):
+        g = m.grid(W.md)
+        tr = adjoint.Trajectory(g, W.steps, 5)
+        out = tr.forward(m.x0, m.y0, vx, vy, m.iz0)
+        L, gx, gy = adjoint.a2_loss(out["x"], out["y"], m.ls)
+        if dev:
+            Ld, a2 = g.adj_seed_a2(m.ls, 1.0, 3.0)
+            assert abs(Ld - L) <= 1e-12 * L, (Ld, L)
+            assert abs(abs(a2) ** 2 - L) <= 1e-12 * L
+            lam = g.adj_get()
+            for k, ref in ((0, gx), (1, gy)):
+                err = np.abs(lam[k] - ref).max()
+                assert err <= 1e-12 * np.abs(ref).max(), (k, err)
+            assert not lam[2:].any()
+            lam0, gpar = tr.backward()
+        else:
+            lam = np.zeros((4, m.n))
+            lam[0], lam[1] = gx, gy
+            g.adj_set(lam)
+            gpar = np.zeros(2)
+            base = tr.nck
+            for k in range(tr.nck - 1, -1, -1):
+                mm = min(tr.every, tr.nsteps - k * tr.every)
+                g.adj_restore(k)
+                for j in range(mm):
+                    g.adj_record(base + j)
+                    g.run(1)
+                for j in range(mm - 1, -1, -1):
+                    g.adj_restore(base + j)
+                    g.findf(False)
+                    g.adj_step(gpar)
+            assert np.array_equal(g.adj_get_par(), gpar)
+            lam0 = g.adj_get()
+        g.close()
+        res.append((lam0, gpar))
+    (a, pa), (b, pb) = res
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+    assert np.all(np.abs(pa - pb) <= 1e-10 * np.abs(pb)), (pa, pb)
+
+
+def test_model_grid_reuse():
+    """DiscModel keeps its grid between gradient calls (galhip_set_pmass and
+    a new halo table when md changes): a second call at another md against
+    a fresh model's call.  The forward run is deterministic, so the loss is
+    bit-identical; the adjoint scatter sums with FP64 atomics in no fixed
+    order, so the gradients agree to rounding (1e-10 here)."""
+    _say("model grid reuse")
+    md2 = 1.15 * W.md
+    m = adjoint.DiscModel(W, precision="fp32")
+    m.gradient(W.Q, W.md, W.steps, every=5)
+    a = m.gradient(1.1 * W.Q, md2, W.steps, every=5)
+    b = adjoint.DiscModel(W, precision="fp32").gradient(1.1 * W.Q, md2, W.steps, every=5)
+    assert a["loss"] == b["loss"], (a["loss"], b["loss"])
+    for k in ("dQ", "dmd"):
+        assert abs(a[k] - b[k]) <= 1e-10 * abs(b[k]), (k, a[k], b[k])
+    for k in ("dvx", "dvy", "gpar"):
+        d = np.abs(a[k] - b[k]).max()
+        _say("%s: max |reused - fresh| %.3g of max %.3g" % (k, d, np.abs(b[k]).max()))
+        assert d <= 1e-10 * np.abs(b[k]).max(), k
+    m.close()
