@@ -204,30 +204,29 @@ class DiscModel:
         membership (load order; 'sel')."""
         t0 = time.perf_counter()
         vx, vy = self.velocities(Q, md)
-        pool = ThreadPoolExecutor(max_workers=1)
-        dv = pool.submit(ic_velocity_dq, self.k_sig, self.k_phi, self.vc2, self.c, self.g1,
-                         self.g2, self.cp, self.sp, Q, md, self.ls * self.w.dt)
-        g = self._grid_for(md)
-        if profile:
-            g.set_profile(True)
-        tr = Trajectory(g, nsteps, every)
-        tr.seconds["setup"] = time.perf_counter() - t0
-        tr.forward(self.x0, self.y0, vx, vy, self.iz0, unload=False)
-        sel = None
-        if with_sel:
-            out = g.unload()
-            R = np.hypot(out["x"].astype(np.float64), out["y"].astype(np.float64)) / self.ls
-            sel = (R >= 1.0) & (R <= 3.0)
-        L, _ = g.adj_seed_a2(self.ls, 1.0, 3.0)
-        lam0, gpar = tr.backward()
-        kms = None
-        if profile:
-            kms = g.adj_get_profile()
-            g.set_profile(False)
-        t1 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=1) as pool:
+            dv = pool.submit(ic_velocity_dq, self.k_sig, self.k_phi, self.vc2, self.c,
+                             self.g1, self.g2, self.cp, self.sp, Q, md, self.ls * self.w.dt)
+            g = self._grid_for(md)
+            if profile:
+                g.set_profile(True)
+            tr = Trajectory(g, nsteps, every)
+            tr.seconds["setup"] = time.perf_counter() - t0
+            tr.forward(self.x0, self.y0, vx, vy, self.iz0, unload=False)
+            sel = None
+            if with_sel:
+                out = g.unload()
+                R = np.hypot(out["x"].astype(np.float64), out["y"].astype(np.float64)) / self.ls
+                sel = (R >= 1.0) & (R <= 3.0)
+            L, _ = g.adj_seed_a2(self.ls, 1.0, 3.0)
+            lam0, gpar = tr.backward()
+            kms = None
+            if profile:
+                kms = g.adj_get_profile()
+                g.set_profile(False)
+            t1 = time.perf_counter()
+            dvx, dvy = dv.result()
         lvx, lvy = lam0[2], lam0[3]
-        dvx, dvy = dv.result()
-        pool.shutdown()
         dQ = float(np.dot(lvx, dvx) + np.dot(lvy, dvy))
         dmd = dQ * Q / md + gpar[0] / md + gpar[1]
         tr.seconds["finish"] = time.perf_counter() - t1
