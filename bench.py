@@ -68,7 +68,7 @@ P3_PUSH_HBM_WRITE_B = 44.8
 # the same for push_kernel<float, true> on the 64-disc ensemble (mean of the
 # dispatches in profiles/push_kernel_ens-64x128x256-n8.6e7_traffic.txt)
 ENS_PUSH_HBM_READ_B = 26.7
-ENS_PUSH_HBM_WRITE_B = 27.5
+ENS_PUSH_HBM_WRITE_B = 29.5
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # adj_scatter_kernel<float>: algorithmic bytes per particle -- id (4), zone
 # (1), (u, t) (8), the four FP64 adjoints read (32) and the two velocity
@@ -487,7 +487,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=None, help="particles (default 10M)")
     ap.add_argument("--sort-interval", type=int, default=None,
-                    help="steps between re-sorts (default 10 for C2, 5 for C3)")
+                    help="steps between re-sorts (default 10 for C2, 5 for C3, 15 for C4)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--cpu-n", type=int, default=2_000_000)
     ap.add_argument("--cpu-steps", type=int, default=60)
@@ -524,7 +524,7 @@ def main():
         if args.steps == 500:
             args.steps = 300
         if args.sort_interval is None:
-            args.sort_interval = 10
+            args.sort_interval = 15
         if args.graph is None:
             args.graph = 1
         if args.contraction is None:
