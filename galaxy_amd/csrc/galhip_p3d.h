@@ -76,6 +76,13 @@ template <typename R>
 hipError_t launch_p3_key(const P3Args<R> &, int kshift, unsigned kmax, unsigned *key,
                          unsigned *perm, hipStream_t);
 template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &, hipStream_t);
+template <typename R>
+hipError_t launch_p3_interleave(const R *s4, const R *s2, typename Vec4<R>::T *xv,
+                                typename Vec2<R>::T *vw, unsigned *id, long long n, hipStream_t);
+template <typename R>
+hipError_t launch_p3_deinterleave(const typename Vec4<R>::T *xv, const typename Vec2<R>::T *vw,
+                                  const uint8_t *zone, const unsigned *id, R *d4, R *d2, int *iz,
+                                  long long n, hipStream_t);
 template <typename R> hipError_t launch_p3_massgn(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_push(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &, hipStream_t);

@@ -714,6 +714,51 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_permute_kernel(P3PermArgs<R> p) 
     p.iout[i] = p.iin[j];
 }
 
+// load: planar host arrays staged on the device (x, y, z, vx in s4 and vy, vz
+// in s2, n values each, one after another) -> interleaved storage; ids = load
+// order
+template <typename R>
+__global__ __launch_bounds__(P3_PBLOCK) void p3_interleave_kernel(const R *__restrict__ s4,
+                                                                  const R *__restrict__ s2,
+                                                                  typename Vec4<R>::T *__restrict__ xv,
+                                                                  typename Vec2<R>::T *__restrict__ vw,
+                                                                  unsigned *__restrict__ id,
+                                                                  long long n) {
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
+    if (i >= n) return;
+    typename Vec4<R>::T a;
+    a.x = s4[i];
+    a.y = s4[n + i];
+    a.z = s4[2 * n + i];
+    a.w = s4[3 * n + i];
+    typename Vec2<R>::T b;
+    b.x = s2[i];
+    b.y = s2[n + i];
+    xv[i] = a;
+    vw[i] = b;
+    id[i] = (unsigned)i;
+}
+
+// unload: interleaved storage -> planar arrays in load order, zones widened
+template <typename R>
+__global__ __launch_bounds__(P3_PBLOCK) void p3_deinterleave_kernel(
+    const typename Vec4<R>::T *__restrict__ xv, const typename Vec2<R>::T *__restrict__ vw,
+    const uint8_t *__restrict__ zone, const unsigned *__restrict__ id, R *__restrict__ d4,
+    R *__restrict__ d2, int *__restrict__ iz, long long n) {
+    const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
+    if (i >= n) return;
+    const long long p = id[i];
+    const typename Vec4<R>::T a = xv[i];
+    const typename Vec2<R>::T b = vw[i];
+    d4[p] = a.x;
+    d4[n + p] = a.y;
+    d4[2 * n + p] = a.z;
+    d4[3 * n + p] = a.w;
+    d2[p] = b.x;
+    d2[n + p] = b.y;
+    iz[p] = zone[i];
+}
+
 // ---- field solve ------------------------------------------------------------------
 
 // grid (ngz, nr): block (plane, ring-1)
@@ -1139,6 +1184,26 @@ hipError_t launch_p3_key(const P3Args<R> &a, int kshift, unsigned kmax, unsigned
     return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_p3_interleave(const R *s4, const R *s2, typename Vec4<R>::T *xv,
+                                typename Vec2<R>::T *vw, unsigned *id, long long n,
+                                hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_interleave_kernel<R>, dim3(p3_blocks(n)), dim3(P3_PBLOCK), 0, s, s4,
+                       s2, xv, vw, id, n);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_p3_deinterleave(const typename Vec4<R>::T *xv, const typename Vec2<R>::T *vw,
+                                  const uint8_t *zone, const unsigned *id, R *d4, R *d2,
+                                  int *iz, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(p3_deinterleave_kernel<R>, dim3(p3_blocks(n)), dim3(P3_PBLOCK), 0, s, xv,
+                       vw, zone, id, d4, d2, iz, n);
+    return hipGetLastError();
+}
+
 template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &p, hipStream_t s) {
     if (p.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(p3_permute_kernel<R>, dim3(p3_blocks(p.n)), dim3(P3_PBLOCK), 0, s, p);
@@ -1214,7 +1279,14 @@ hipError_t launch_p3_green(double *G, const double *twd, const double *twzd, con
     template bool p3_solve_fits<R>(const P3SolveArgs<R> &);                            \
     template hipError_t launch_p3_key<R>(const P3Args<R> &, int, unsigned, unsigned *,  \
                                          unsigned *, hipStream_t);                     \
-    template hipError_t launch_p3_permute<R>(const P3PermArgs<R> &, hipStream_t);
+    template hipError_t launch_p3_permute<R>(const P3PermArgs<R> &, hipStream_t);          \
+    template hipError_t launch_p3_interleave<R>(const R *, const R *, typename Vec4<R>::T *,  \
+                                                typename Vec2<R>::T *, unsigned *, long long, \
+                                                hipStream_t);                                 \
+    template hipError_t launch_p3_deinterleave<R>(const typename Vec4<R>::T *,               \
+                                                  const typename Vec2<R>::T *, const uint8_t *, \
+                                                  const unsigned *, R *, R *, int *, long long, \
+                                                  hipStream_t);
 P3_INST(float)
 P3_INST(double)
 

@@ -299,30 +299,33 @@ extern "C" int galhip_restrt3(galhip_ctx *c, int64_t n, const void *x, const voi
     if (rc) return rc;
     const size_t es = esize(c);
     std::vector<uint8_t> zz(n);
-    std::vector<unsigned> ids(n);
     long long cnt[4] = {0, 0, 0, 0};
     for (long long i = 0; i < n; i++) {
         const int v = iz[i];
         if (v < 1 || v > c->nz + 1)
             return gh_fail(GALHIP_EINVAL, "particle %lld has zone %d outside 1..%d", i, v, c->nz + 1);
         zz[i] = (uint8_t)v;
-        ids[i] = (unsigned)i;
         if (v == c->nz + 1) cnt[0]++;
     }
     HIPCHK(hipMemcpy(c->counters, cnt, sizeof cnt, hipMemcpyHostToDevice));
     if (n > 0) {
-        std::vector<unsigned char> b4(4 * es * n), b2(2 * es * n);
-        const unsigned char *s4[4] = {(const unsigned char *)x, (const unsigned char *)y,
-                                      (const unsigned char *)z, (const unsigned char *)vx};
-        const unsigned char *s2[2] = {(const unsigned char *)vy, (const unsigned char *)vz};
-        for (long long i = 0; i < n; i++) {
-            for (int k = 0; k < 4; k++) memcpy(b4.data() + (4 * i + k) * es, s4[k] + i * es, es);
-            for (int k = 0; k < 2; k++) memcpy(b2.data() + (2 * i + k) * es, s2[k] + i * es, es);
-        }
-        HIPCHK(hipMemcpy(c->xv, b4.data(), 4 * es * n, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->vw, b2.data(), 2 * es * n, hipMemcpyHostToDevice));
+        // the planar arrays through the second particle buffers (free outside
+        // a sort), interleaved on the device
+        unsigned char *t4 = (unsigned char *)c->xv2, *t2 = (unsigned char *)c->vw2;
+        const void *s4[4] = {x, y, z, vx};
+        const void *s2[2] = {vy, vz};
+        for (int k = 0; k < 4; k++)
+            HIPCHK(hipMemcpy(t4 + k * es * n, s4[k], es * n, hipMemcpyHostToDevice));
+        for (int k = 0; k < 2; k++)
+            HIPCHK(hipMemcpy(t2 + k * es * n, s2[k], es * n, hipMemcpyHostToDevice));
+        if (c->prec == GALHIP_FP32)
+            HIPCHK(launch_p3_interleave<float>((const float *)t4, (const float *)t2,
+                                               (float4 *)c->xv, (float2 *)c->vw, c->id, n, c->s));
+        else
+            HIPCHK(launch_p3_interleave<double>((const double *)t4, (const double *)t2,
+                                                (double4 *)c->xv, (double2 *)c->vw, c->id, n,
+                                                c->s));
         HIPCHK(hipMemcpy(c->zone, zz.data(), n, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->id, ids.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
     }
     c->qbits = qbits_for(n);
     if (c->prec == GALHIP_FP32) HIPCHK(launch_p3_ut<float>(p3_args<float>(c), c->s));
@@ -340,29 +343,26 @@ extern "C" int galhip_unload3(galhip_ctx *c, void *x, void *y, void *z, void *vx
     HIPCHK(hipStreamSynchronize(c->s));
     const long long n = c->n;
     if (n == 0) return 0;
+    // planar, in load order, into the second particle buffers and the sort
+    // keys (all free outside a sort), then straight to the caller's arrays
     const size_t es = esize(c);
-    std::vector<unsigned> ids(n);
-    HIPCHK(hipMemcpy(ids.data(), c->id, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
-    std::vector<unsigned char> b4(4 * es * n), b2(2 * es * n);
-    HIPCHK(hipMemcpy(b4.data(), c->xv, 4 * es * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(b2.data(), c->vw, 2 * es * n, hipMemcpyDeviceToHost));
+    unsigned char *t4 = (unsigned char *)c->xv2, *t2 = (unsigned char *)c->vw2;
+    int *zs = (int *)c->key;
+    if (c->prec == GALHIP_FP32)
+        HIPCHK(launch_p3_deinterleave<float>((const float4 *)c->xv, (const float2 *)c->vw, c->zone,
+                                             c->id, (float *)t4, (float *)t2, zs, n, c->s));
+    else
+        HIPCHK(launch_p3_deinterleave<double>((const double4 *)c->xv, (const double2 *)c->vw,
+                                              c->zone, c->id, (double *)t4, (double *)t2, zs, n,
+                                              c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
     void *d4[4] = {x, y, z, vx};
     void *d2[2] = {vy, vz};
-    for (int k = 0; k < 4; k++) {
-        if (!d4[k]) continue;
-        unsigned char *d = (unsigned char *)d4[k];
-        for (long long i = 0; i < n; i++) memcpy(d + (size_t)ids[i] * es, b4.data() + (4 * i + k) * es, es);
-    }
-    for (int k = 0; k < 2; k++) {
-        if (!d2[k]) continue;
-        unsigned char *d = (unsigned char *)d2[k];
-        for (long long i = 0; i < n; i++) memcpy(d + (size_t)ids[i] * es, b2.data() + (2 * i + k) * es, es);
-    }
-    if (iz) {
-        std::vector<uint8_t> zz(n);
-        HIPCHK(hipMemcpy(zz.data(), c->zone, n, hipMemcpyDeviceToHost));
-        for (long long i = 0; i < n; i++) iz[ids[i]] = zz[i];
-    }
+    for (int k = 0; k < 4; k++)
+        if (d4[k]) HIPCHK(hipMemcpy(d4[k], t4 + k * es * n, es * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 2; k++)
+        if (d2[k]) HIPCHK(hipMemcpy(d2[k], t2 + k * es * n, es * n, hipMemcpyDeviceToHost));
+    if (iz) HIPCHK(hipMemcpy(iz, zs, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     return 0;
 }
 
