@@ -6,8 +6,10 @@
 // as loadup does (SRC/lib15/loadup.f:99,168-169: x lscale, v / gvfac with
 // gvfac = 1/(lscale ts), FP32), assigns zones (zoneno.f:40-55, off-grid
 // beyond rgrid) and the zone step factors (adjvls.f), runs, and writes the
-// end state as a .pcs file in model units (velocities over their zone's
-// factor, times gvfac) with the time advanced.
+// end state as dmp2pcs does (SRC/progs/dmp2pcs.f:89-100: masset, findf,
+// then unload with its tmcent(.false.), unload.f:45, so the velocities are
+// synchronous again) as a .pcs file in model units (velocities over their
+// zone's factor, times gvfac) with the time advanced.
 //
 // usage: galhip_galaxy in.pcs out.pcs nr=N na=N hash=N soft=S lscale=L ts=T
 //        steps=K [nstep=1,2,4] [zbr=0,r2,r3] [sort=K] [tsoft=1] [halo=file]
@@ -193,6 +195,12 @@ int run(const Pcs &in, Pcs &out, galhip_p2d_params &p, double lscale, double ts,
         std::fflush(stdout);
         CALL(galhip_run(c, k));
     }
+    // dmp2pcs's end (dmp2pcs.f:89-100): masset, a full findf, then unload,
+    // whose tmcent(.false.) (unload.f:45) turns the leap-frog's half-step
+    // velocities back into synchronous ones before they are written
+    CALL(galhip_masset(c));
+    CALL(galhip_findf(c, 1));
+    CALL(galhip_tmcent(c, 0));
     CALL(galhip_sync(c));
     CALL(galhip_unload(c, x.data(), y.data(), vx.data(), vy.data(), iz.data()));
     int64_t noff = 0;

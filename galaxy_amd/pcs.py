@@ -142,3 +142,15 @@ def start_state(grid, coords, lscale, ts, zbr, nstep):
     grid.findf(False)
     grid.tmcent(True)
     return iz
+
+
+def end_state(grid):
+    """dmp2pcs's path before the particles are written (dmp2pcs.f:89-100):
+    assign every particle's mass, solve for the full field with the
+    potential, and undo the time-centring as unload does (unload.f:45), so
+    the velocities are synchronous with the positions again.  Returns the
+    particles in load order (P2DGrid.unload)."""
+    grid.masset()
+    grid.findf(True)
+    grid.tmcent(False)
+    return grid.unload()

@@ -1,7 +1,9 @@
 """The native driver (galaxy_amd/galhip_galaxy, csrc/galhip_galaxy.cpp):
-pcs2dmp's start and galaxy's loop through the C ABI from C++, against the
-same calls made from Python (galaxy_amd.pcs.start_state + P2DGrid.run) on
-the same .pcs file: the end states must be bit-identical."""
+pcs2dmp's start, galaxy's loop and dmp2pcs's end through the C ABI from
+C++, against the same calls made from Python (galaxy_amd.pcs.start_state +
+P2DGrid.run + pcs.end_state) on the same .pcs file: the end states must be
+bit-identical.  With no steps, the written velocities are the input's to
+rounding (time-centring undone, SRC/lib15/unload.f:45)."""
 import os
 import subprocess
 
@@ -18,10 +20,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "galaxy_amd", "galhip_galaxy")
 
 
-def test_driver_matches_python_path(tmp_path):
-    print("[case] native driver", flush=True)
+def _run_driver(tmp_path, steps):
     w = SMALL
-    n, steps = 20000, 12
+    n = 20000
     R, phi, vR, vphi = disc.exponential_disc(n, 21, Q=w.Q, md=w.md)
     coords = np.stack([R * np.cos(phi), R * np.sin(phi),
                        vR * np.cos(phi) - vphi * np.sin(phi),
@@ -51,7 +52,7 @@ def test_driver_matches_python_path(tmp_path):
     g.set_sort_interval(3)
     pcs.start_state(g, coords, ls, ts, kw["zbr"], w.nstep)
     g.run(steps)
-    u = g.unload()
+    u = pcs.end_state(g)
     g.close()
     f32 = np.float32
     lsf, tsf = f32(ls), f32(ts)
@@ -65,3 +66,24 @@ def test_driver_matches_python_path(tmp_path):
     assert np.array_equal(out["coords"], ref)
     assert out["t"] == np.float32(steps * ts)
     assert out["pm"] == pm and list(out["nsp"]) == [n, 0, 0]
+    return coords, out
+
+
+def test_driver_matches_python_path(tmp_path):
+    print("[case] native driver, 12 steps", flush=True)
+    _run_driver(tmp_path, 12)
+
+
+def test_driver_no_steps_returns_input(tmp_path):
+    """pcs in, no steps, pcs out: tmcent(.true.) then tmcent(.false.) with
+    the same field, so positions come back to one unit-conversion rounding
+and velocities come back to
+    within a few FP32 roundings of the loaded ones (the half-step kick is
+    added and subtracted, each rounded, and the unit conversions round)."""
+    print("[case] native driver, no steps", flush=True)
+    coords, out = _run_driver(tmp_path, 0)
+    c = out["coords"]
+    dx = np.abs(c[:, :2] - coords[:, :2]) / np.abs(coords[:, :2]).max()
+    assert dx.max() <= 2.5e-7, dx.max()   # x lscale / lscale in FP32
+    dv = np.abs(c[:, 2:] - coords[:, 2:]).max()
+    assert dv <= 4e-7 * np.abs(coords[:, 2:]).max(), dv
