@@ -56,29 +56,35 @@ def _record(payload: bytes) -> bytes:
 def read_dmp(path) -> dict:
     with open(path, "rb") as f:
         recs = _records(f.read())
-    # irun, istep, noff, ncen(mcmp), nshort, nrsup, suppl, n, numprocs,
-    # angoff(3, mcmp), amoff(3, mcmp), rguard: 9 + 7 mcmp words
-    r4 = np.frombuffer(recs[3], "<i4")
+    # hedrec's header: two records and one per mass component (hedrec.f),
+    # then the step record: irun, istep, noff, ncen(mcmp), nshort, nrsup,
+    # suppl, n, numprocs, angoff(3, mcmp), amoff(3, mcmp), rguard: 9 + 7 mcmp
+    # words (a component record is 38 words, which that form cannot take)
+    h = 3
+    while h < len(recs) and not (len(recs[h]) >= 64 and len(recs[h]) % 4 == 0
+                                 and (len(recs[h]) // 4 - 9) % 7 == 0):
+        h += 1
+    if h >= len(recs) - 3:
+        raise ValueError("not a dump: no step record")
+    r4 = np.frombuffer(recs[h], "<i4")
     mcmp = (r4.size - 9) // 7
-    if r4.size < 16 or 9 + 7 * mcmp != r4.size:
-        raise ValueError("not a dump: unexpected step record")
     irun, istep, noff = int(r4[0]), int(r4[1]), int(r4[2])
     if r4[mcmp + 5] != 0:
         raise NotImplementedError("dumps with supplementary-force tables")
-    lst = np.frombuffer(recs[4], "<i4")
+    lst = np.frombuffer(recs[h + 1], "<i4")
     numprocs, myid, nbod, nwpp = (int(v) for v in lst[:4])
     heads = lst[4:].copy()
     if numprocs != 1 or nwpp < 6:
         raise NotImplementedError("single-process 2-D dumps only")
     nlists = len(heads)
     nz = nlists - 1
-    grd1 = np.frombuffer(recs[6], "<f4").copy()
+    grd1 = np.frombuffer(recs[h + 3], "<f4").copy()
     mesh = grd1.size
     lstep = np.zeros((2, nz), np.int32)
     moz = np.zeros((nz, mesh), np.float32)
     mss = np.zeros((nz, mesh), np.float32)
     mss[0] = grd1
-    k = 7
+    k = h + 4
     for z in range(1, nz):
         a, b = recs[k], recs[k + 1]
         lstep[0, z] = np.frombuffer(a[:4], "<i4")[0]
@@ -103,7 +109,7 @@ def read_dmp(path) -> dict:
             seen += 1
     if np.any(iz == 0):
         raise ValueError("particles on no list")
-    return dict(header=recs[:3], steprec=recs[3], rng=recs[5], irun=irun,
+    return dict(header=recs[:h], steprec=recs[h], rng=recs[h + 2], irun=irun,
                 istep=istep, noff=noff, nbod=nbod, nwpp=nwpp, nz=nz,
                 lstep=lstep, grdmss=mss, grdmoz=moz,
                 x=pf[:, 0].copy(), y=pf[:, 1].copy(), vx=pf[:, 2].copy(),

@@ -90,6 +90,9 @@ for _s, _p, _t in (("f", _f32p, C.c_float), ("d", _f64p, C.c_double)):
     _sig("orc_push_" + _s, C.c_int, [C.POINTER(StepCfg), C.c_int, _p, _p, _p,
                                      _p, _i32p, _p, _p, _p, _i64p, C.c_int,
                                      _i32p, _i32p, _u8p])
+    _sig("orc_push_par_" + _s, C.c_int, [C.POINTER(StepCfg), C.c_int, _p, _p, _p,
+                                         _p, _i32p, _p, _p, _i64p, C.c_int,
+                                         _i32p, _i32p, _u8p])
     _sig("orc_tmcent_" + _s, None, [C.POINTER(StepCfg), C.c_int, _p, _p, _p, _p,
                                     _i32p, _p, _p, C.c_int])
 _sig("orc_fixforce", C.c_double, [C.c_double, C.c_int, C.c_double, _f64p, _f64p])
@@ -413,10 +416,13 @@ class P2D:
             c.fr_val, c.fr_der = self._fv.ctypes.data, self._fd.ctypes.data
         return c
 
-    def step(self, field=None):
+    def step(self, field=None, parallel=False):
         """step (SRC/lib15/step.f:135-290) for a single grid, no perturber,
         no guard zones.  ``field`` (2, mesh) in the working precision
-        overrides the solved Cartesian field (used to inject the GPU's)."""
+        overrides the solved Cartesian field (used to inject the GPU's).
+        ``parallel``: the particle loop on every host thread (OpenMP); the
+        fixed-point sums are the serial loop's exactly, the reference-order
+        REAL sums ``grdmss`` are not formed (needs ``use_fx``)."""
         self._set_mzone()
         self._masscl(True)
         if field is None:
@@ -424,11 +430,19 @@ class P2D:
         fx = np.ascontiguousarray(field[0], self.dt)
         fy = np.ascontiguousarray(field[1], self.dt)
         c = self.stepcfg()
-        fn = _lib.orc_push_f if self.prec == "f" else _lib.orc_push_d
         self.moved = np.zeros(len(self.x), np.uint8)
-        dn = fn(C.byref(c), len(self.x), self.x, self.y, self.vx, self.vy,
-                self.iz, fx, fy, self.grdmss.reshape(-1), self.mfx.reshape(-1),
-                self.qbits, self.jrad, self.krad, self.moved)
+        if parallel:
+            if not self.use_fx:
+                raise ValueError("the parallel loop forms the fixed-point sums only")
+            fn = _lib.orc_push_par_f if self.prec == "f" else _lib.orc_push_par_d
+            dn = fn(C.byref(c), len(self.x), self.x, self.y, self.vx, self.vy,
+                    self.iz, fx, fy, self.mfx.reshape(-1), self.qbits, self.jrad,
+                    self.krad, self.moved)
+        else:
+            fn = _lib.orc_push_f if self.prec == "f" else _lib.orc_push_d
+            dn = fn(C.byref(c), len(self.x), self.x, self.y, self.vx, self.vy,
+                    self.iz, fx, fy, self.grdmss.reshape(-1), self.mfx.reshape(-1),
+                    self.qbits, self.jrad, self.krad, self.moved)
         self.noff += dn
         self.istep += 1
         self._zoncmb_scaled()

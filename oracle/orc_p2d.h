@@ -83,6 +83,16 @@ int orc_push_d(const orc_stepcfg *c, int n, double *x, double *y, double *vx,
                double *vy, int32_t *iz, const double *fx, const double *fy,
                double *grdmss, int64_t *mfx, int qbits,
                int32_t *jrad, int32_t *krad, uint8_t *moved);
+/* The particle loop on every host thread: fixed-point sums only, equal to
+ * orc_push_*'s (integer sums commute); no order-dependent REAL sums. */
+int orc_push_par_f(const orc_stepcfg *c, int n, float *x, float *y, float *vx,
+                   float *vy, int32_t *iz, const float *fx, const float *fy,
+                   int64_t *mfx, int qbits, int32_t *jrad, int32_t *krad,
+                   uint8_t *moved);
+int orc_push_par_d(const orc_stepcfg *c, int n, double *x, double *y, double *vx,
+                   double *vy, int32_t *iz, const double *fx, const double *fy,
+                   int64_t *mfx, int qbits, int32_t *jrad, int32_t *krad,
+                   uint8_t *moved);
 void orc_tmcent_f(const orc_stepcfg *c, int n, const float *x, const float *y,
                   float *vx, float *vy, const int32_t *iz, const float *fx,
                   const float *fy, int start);
