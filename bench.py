@@ -13,14 +13,25 @@ re-sorted by cell every --sort-interval steps inside the timed region.
 value = particle pushes per second of the mixed form (FP32 particles and
 grids, FP32 contraction): one push = one stpgrp application; with zones,
 zone-k particles are pushed every nstep(k) steps.  The plain FP64 form runs
-the same steps from the same particles; the JSON line carries its rate, the
-ratio, and the m = 2 amplitude of both end states.
+the same steps from the same particles, and the mixed form with compensated
+position updates beside it; the JSON line carries their rates and the
+ratio.  An accuracy leg outside the timed region, independent of --steps,
+runs both forms over the workload's own 500 steps from the same particles:
+the mixed solve's relative force error against the FP64 solve at the
+start, each form's energy and Lz drift over the 500 steps and their
+difference, and the m = 2 amplitude of both end states.
 
 roofline: the push kernel, algorithmic bytes = pushes x 32 B (x, y, vx, vy
 read and written in FP32) over its summed HIP-event time on the library's
 stream, against 8 TB/s; traffic = measured HBM bytes per launch (per-push
-bytes from rocprofv3 counters times the pushes per launch).  cpu_baseline: the oracle's FP32 restatement on one
-host core over a bounded sample of the same workload.
+bytes from rocprofv3 counters times the pushes per launch); step_frac = the
+whole step's algorithmic bytes (pushes x 32 B, the two field types' FP32
+tables, the zone mass sums, the field written and read) over the wall-clock
+step time.  cpu_baseline: the reference's own galaxy program (built from
+its Fortran sources into oracle/_ref; serial, one core) on a bounded sample
+of the same workload (1e6 particles, 20 steps), or the oracle's FP32
+restatement where that build is absent; extra.cpu_port_parallel: the
+oracle's particle loop on every host thread at the full 1e7 particles.
 
 --workload C4 measures the ensemble of 64 independent discs on 128 x 256
 grids (N_i log-uniform in [1e5, 4e6], Q_i in [1, 2], 300 steps) advanced by
@@ -70,6 +81,12 @@ P3_PUSH_HBM_WRITE_B = 44.8
 ENS_PUSH_HBM_READ_B = 26.7
 ENS_PUSH_HBM_WRITE_B = 29.5
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# algorithmic grid bytes of one C2 step besides the particles: the F_R and
+# F_phi tables in FP32 (2 x 17 harmonics x 256 x 256 x 4 B, read once by the
+# synthesis), the three zones' int64 mass sums read by the analysis
+# (3 x 131072 x 8 B) and the FP32 (F_x, F_y) field written by the synthesis
+# and read by the push (2 x 131072 x 8 B)
+STEP_GRID_BYTES = 2 * 17 * 256 * 256 * 4 + 3 * 131072 * 8 + 2 * 131072 * 8
 # adj_scatter_kernel<float>: algorithmic bytes per particle -- id (4), zone
 # (1), (u, t) (8), the four FP64 adjoints read (32) and the two velocity
 # adjoints written (16); the grid sums stay in LDS and L2.  Measured HBM
@@ -139,6 +156,51 @@ def drift(r):
     return (r["e1"] - r["e0"]) / abs(r["e0"])
 
 
+def accuracy_leg(w, n, steps, sort_interval):
+    """The mixed and FP64 forms from the same initial particles, outside the
+    timed region and independent of --steps: the relative force error of
+    the mixed solve against the FP64 solve at the start (each form assigns
+    its own mass; F_x, F_y and the potential, error over the grid's
+    largest value), then `steps` steps of each (the workload's own length):
+    the energy and Lz drift of each form, their difference, and the m = 2
+    amplitude of both end states."""
+    from galaxy_amd import configs
+    from galaxy_amd.lib15 import P2DGrid
+    t0 = time.time()
+    rgrid = P2DGrid(precision="fp32", **configs.grid_kwargs(w, n)).rgrid
+    parts = configs.particles(w, n, None, rgrid)
+    fields, res = {}, {}
+    for prec in ("fp32", "fp64"):
+        g = P2DGrid(precision=prec, **configs.grid_kwargs(w, n))
+        g.greenm()
+        g.set_fixforce(*configs.fixforce(w, g.rgrid))
+        g.set_fixpot(*configs.fixpot(w, g.rgrid))
+        g.restrt(*parts)
+        g.set_sort_interval(sort_interval)
+        g.masset()
+        g.findf(True)
+        fields[prec] = g.get_field().astype(np.float64)
+        e0 = g.integrals()
+        g.run(steps)
+        g.findf(True)
+        e1 = g.integrals()
+        out = g.unload()
+        res[prec] = dict(drift=(e1["energy"] - e0["energy"]) / abs(e0["energy"]),
+                         lz=(e1["lz"] - e0["lz"]) / abs(e0["lz"]),
+                         a2=a2(out["x"], out["y"], w.lscale()))
+        g.close()
+    ferr = [float(np.abs(fields["fp32"][t] - fields["fp64"][t]).max()
+                  / np.abs(fields["fp64"][t]).max()) for t in range(3)]
+    m, d = res["fp32"], res["fp64"]
+    log("[bench] accuracy leg: %d steps of both forms in %.1f s" % (steps, time.time() - t0))
+    return dict(accuracy_steps=steps,
+                force_err_vs_fp64=max(ferr), force_err_by_type=ferr,
+                a2_mixed=abs(m["a2"]), a2_fp64=abs(d["a2"]), m2_err=abs(m["a2"] - d["a2"]),
+                energy_drift_mixed=m["drift"], energy_drift_fp64=d["drift"],
+                energy_drift_err=abs(m["drift"] - d["drift"]),
+                lz_drift_mixed=m["lz"], lz_drift_fp64=d["lz"])
+
+
 def cpu_baseline(w, n, steps):
     """The oracle (FP32 restatement, one core) on n particles of workload w."""
     from galaxy_amd import configs
@@ -161,6 +223,64 @@ def cpu_baseline(w, n, steps):
                 kind="port",
                 sample="oracle FP32, %s geometry, N=%d, %d steps (%d pushes, %.1f s)"
                        % (w.name, n, steps, pushes, dt))
+
+
+def cpu_reference(w, n, steps):
+    """The reference's own galaxy program (built from its sources into
+    oracle/_ref, serial: one core) on n particles of workload w's disc at
+    its grid, harmonics and zones; None when the build is absent.  Pushes
+    per step from the zone populations of its first and last dumps."""
+    import shutil
+    import tempfile
+    from oracle import refrun
+    if not refrun.available():
+        return None
+    d = tempfile.mkdtemp(prefix="galaxy_ref_")
+    try:
+        refrun.write_run(d, w, n, steps)
+        r = refrun.run_galaxy(d)
+        per = refrun.pushes_per_step(d, 101, w.nstep)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    pushes = per * r["steps"]
+    return dict(value=pushes / r["seconds"], unit="particle-steps/s", cores=1,
+                kind="reference",
+                sample="reference galaxy (flang build of SRC/, serial), %s geometry, N=%d, "
+                       "%d steps, %.0f pushes per step, %.1f s by its own step lines "
+                       "(set-up %.1f s outside; its Green's table re-read from disk every "
+                       "solve is inside)" % (w.name, n, r["steps"], per, r["seconds"],
+                                            r["setup_seconds"]))
+
+
+def cpu_port_parallel(w, n, steps):
+    """The oracle's particle loop on every host thread (OpenMP) at the full
+    workload size, with the GPU's FP64 Green's tables (the oracle's own
+    build of them takes minutes at this grid on one core)."""
+    from galaxy_amd import configs
+    from galaxy_amd.lib15 import P2DGrid
+    from oracle import oracle as orc
+    g = P2DGrid(precision="fp64", **configs.grid_kwargs(w, n))
+    g.greenm()
+    G = g.get_green()
+    g.close()
+    kw = configs.grid_kwargs(w, n)
+    m = orc.P2D(kw["nr"], kw["na"], kw["hash_"], kw["softl"], tsoft=1, nsect=1,
+                nzones=kw["nzones"], nstep=kw["nstep"], zbr=kw["zbr"], pmass=kw["pmass"],
+                offanal=True, prec="f", use_fx=True, green_tables=G)
+    m.fixforce = configs.fixforce(w, float(m.rgrid))
+    m.load(*configs.particles(w, n, None, float(m.rgrid)))
+    m.masset()
+    pushes = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.findf()
+        m.step(parallel=True)
+        pushes += int(m.moved.sum())
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=pushes / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample="oracle FP32 particle loop on %d OpenMP threads, %s, N=%d, %d steps "
+                       "(%d pushes, %.1f s)" % (threads, w.name, n, steps, pushes, dt))
 
 
 def run_gpu3(w, precision, n, steps, warmup, sort_interval, want_green=False):
@@ -489,7 +609,12 @@ def main():
     ap.add_argument("--sort-interval", type=int, default=None,
                     help="steps between re-sorts (default 10 for C2, 5 for C3, 15 for C4)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 leg")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the accuracy leg (C2: both forms over the workload's 500 steps)")
     ap.add_argument("--cpu-n", type=int, default=2_000_000)
+    ap.add_argument("--cpu-ref-n", type=int, default=1_000_000,
+                    help="particles of the reference's own CPU run (C2)")
+    ap.add_argument("--cpu-ref-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -549,6 +674,10 @@ def main():
     fp64 = None
     if not args.no_fp64:
         fp64 = run_gpu(w, "fp64", n, args.steps, args.warmup, args.sort_interval, graph)
+    comp = None
+    if not args.compensated and not args.no_fp64:
+        # the same mixed form with compensated position updates, beside it
+        comp = run_gpu(w, "fp32", n, args.steps, args.warmup, args.sort_interval, graph, True)
     value = mixed["pushes"] / mixed["seconds"]
     ms_per_step = 1e3 * mixed["seconds"] / args.steps
     bytes_per_push = 32.0
@@ -562,10 +691,22 @@ def main():
                 bytes_per_push=bytes_per_push,
                 avg_launch_us=round(1e3 * mixed["push_ms"] / max(mixed["launches"], 1), 2))
     cpu = None
+    cpu_par = None
     if not args.no_cpu:
-        cpu = cpu_baseline(w, args.cpu_n, args.cpu_steps)
+        cpu = cpu_reference(w, args.cpu_ref_n, args.cpu_ref_steps)
+        if cpu is None:
+            cpu = cpu_baseline(w, args.cpu_n, args.cpu_steps)
         log("[bench] cpu baseline: %.3g pushes/s (%s)" % (cpu["value"], cpu["sample"]))
+        cpu_par = cpu_port_parallel(w, n, 4)
+        log("[bench] cpu port, all threads: %.3g pushes/s (%s)"
+            % (cpu_par["value"], cpu_par["sample"]))
     in_kernel = mixed["pushes_prof"] / (mixed["push_ms"] * 1e-3)
+    # the step-level fraction: (pushes x 32 B + the two field types' FP32
+    # tables + zone mass sums + field written and read) per step over the
+    # wall-clock step time
+    step_bytes = (mixed["pushes"] / args.steps * bytes_per_push + STEP_GRID_BYTES)
+    roof["step_frac"] = round(step_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
+    roof["step_bytes"] = round(step_bytes)
     extra = dict(
         n_steps_rate=n * args.steps / mixed["seconds"],
         push_valu_frac=round(in_kernel * PUSH_VALU_PER_PUSH / VALU_PEAK_LANE_OPS, 3),
@@ -577,13 +718,16 @@ def main():
     if fp64:
         v64 = fp64["pushes"] / fp64["seconds"]
         extra.update(fp64_value=v64, mixed_over_fp64=value / v64,
-                     fp64_ms_per_step=1e3 * fp64["seconds"] / args.steps,
-                     a2_mixed=abs(mixed["a2"]), a2_fp64=abs(fp64["a2"]),
-                     m2_err=abs(mixed["a2"] - fp64["a2"]),
-                     energy_drift_mixed=drift(mixed), energy_drift_fp64=drift(fp64),
-                     energy_drift_err=abs(drift(mixed) - drift(fp64)),
-                     lz_drift_mixed=(mixed["lz1"] - mixed["lz0"]) / abs(mixed["lz0"]),
-                     lz_drift_fp64=(fp64["lz1"] - fp64["lz0"]) / abs(fp64["lz0"]))
+                     fp64_ms_per_step=1e3 * fp64["seconds"] / args.steps)
+    if comp:
+        extra.update(compensated_value=comp["pushes"] / comp["seconds"],
+                     compensated_ms_per_step=1e3 * comp["seconds"] / args.steps,
+                     compensated_push_us=round(1e3 * comp["push_ms"] / max(comp["launches"], 1),
+                                               2))
+    if cpu_par:
+        extra["cpu_port_parallel"] = cpu_par
+    if not args.no_accuracy:
+        extra.update(accuracy_leg(w, n, w.steps, args.sort_interval))
     line = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

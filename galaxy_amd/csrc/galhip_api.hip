@@ -60,7 +60,7 @@ template <typename R> hipError_t launch_analyse(const AnaArgs<R> &, hipStream_t)
 template <typename R> hipError_t launch_synth(const SynArgs<R> &, hipStream_t);
 hipError_t launch_green(double *, const double *, int, int, int, int, double,
                         double, int, hipStream_t);
-hipError_t launch_split_bf16(const float *, unsigned short *, unsigned short *, int, int, int,
+hipError_t launch_split_bf16(const double *, unsigned short *, int, int, int,
                              hipStream_t);
 hipError_t launch_ens_gemm(const EnsGemmArgs &, int, hipStream_t);
 hipError_t launch_d2f(const double *, float *, long long, hipStream_t);
@@ -283,7 +283,7 @@ extern "C" int galhip_free(galhip_ctx *c) {
     dfree(c->Gd);
     dfree(c->fr_val); dfree(c->fr_der);
     dfree(c->m_qbits_d); dfree(c->m_pmass_d); dfree(c->blk_d);
-    dfree(c->ghi); dfree(c->glo); dfree(c->wpre);
+    dfree(c->gsp); dfree(c->wpre);
     drop_graphs(c);
     adj_free(c);
     for (auto e : c->ev_free) (void)hipEventDestroy(e);
@@ -821,8 +821,7 @@ extern "C" int galhip_masset(galhip_ctx *c) {
 static EnsGemmArgs gemm_args(galhip_ctx *c, int ntypes) {
     EnsGemmArgs g;
     memset(&g, 0, sizeof g);
-    g.ghi = c->ghi;
-    g.glo = c->glo;
+    g.gsp = c->gsp;
     g.coef = (const float *)c->coef;
     g.ranges = c->ranges;
     g.lg = c->lg_d;
@@ -842,7 +841,7 @@ static EnsGemmArgs gemm_args(galhip_ctx *c, int ntypes) {
 // context's shapes (HIP events, 5 launches each after one warm-up).
 static int mfma_prepare(galhip_ctx *c) {
     if (!c->gsplit_valid) {
-        HIPCHK(launch_split_bf16((const float *)c->G, c->ghi, c->glo, c->ng, c->nr, c->nrp, c->s));
+        HIPCHK(launch_split_bf16(c->Gd, c->gsp, c->ng, c->nr, c->nrp, c->s));
         c->gsplit_valid = 1;
     }
     if (c->gemm_variant < 0) {
@@ -904,11 +903,10 @@ extern "C" int galhip_set_contraction(galhip_ctx *c, int32_t mode) {
     if (mode >= 1 && (c->p3d || c->prec != GALHIP_FP32))
         return gh_fail(GALHIP_EINVAL, "the MFMA contraction is built for FP32 p2d grids");
     HIPCHK(hipStreamSynchronize(c->s));
-    if (mode >= 1 && !c->ghi) {
+    if (mode >= 1 && !c->gsp) {
         c->nrp = (c->nr + 31) / 32 * 32;
         const size_t ng = (size_t)3 * c->ng * c->nrp * c->nrp;
-        int rc = dalloc(&c->ghi, sizeof(unsigned short) * ng) |
-                 dalloc(&c->glo, sizeof(unsigned short) * ng) |
+        int rc = dalloc(&c->gsp, 3 * sizeof(unsigned short) * ng) |
                  dalloc(&c->wpre, sizeof(float) * (size_t)c->nmem * 3 * 2 * c->ng * c->nr);
         if (rc) return rc;
         HIPCHK(hipMemset(c->wpre, 0, sizeof(float) * (size_t)c->nmem * 3 * 2 * c->ng * c->nr));

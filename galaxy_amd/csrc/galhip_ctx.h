@@ -120,7 +120,7 @@ struct galhip_ctx {
     int comp;
     void *xlo, *xlo2;
     int contract, nrp, gsplit_valid, gemm_variant;
-    unsigned short *ghi, *glo;
+    unsigned short *gsp;          // three bf16 words of G (FP64 master), word-major
     float *wpre;
     double gemm_us[3];
 };

@@ -330,7 +330,7 @@ template <typename R> struct SynArgs {
 
 // the ensemble's MFMA contraction (galhip_solve.hip)
 struct EnsGemmArgs {
-    const unsigned short *ghi, *glo;  // bf16 split of G, [3][ng][nrp][nrp]
+    const unsigned short *gsp;        // bf16 words of G, [3 words][3][ng][nrp][nrp]
     const float *coef;                // [nmem][ncoef][nr]
     const int *ranges;                // [nmem][4]
     const int *lg;

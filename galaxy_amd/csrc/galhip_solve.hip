@@ -357,11 +357,17 @@ hipError_t launch_synth(const SynArgs<R> &a, hipStream_t s) {
 // columns = (member, cos/sin) pairs, B = the members' coefficients in the
 // arrangement p2conv uses for the type (F_R and the potential: (Mc, Ms);
 // F_phi: (Ms, -Mc)), zero outside each member's source range.  Each operand
-// x is split into bf16 hi = rn(x) and lo = rn(x - hi); the product is
-// hi*hi + hi*lo + lo*hi accumulated in FP32 by the matrix cores (the
-// lo*lo term is below the split's own 2^-17 representation error).
+// x is split into three bf16 words, x1 = rn(x), x2 = rn(x - x1),
+// x3 = rn(x - x1 - x2), which hold x to about 2^-24 of itself (G's split is
+// taken from the FP64 master table, B's from the FP32 coefficients, which
+// it holds exactly).  The product keeps the six terms whose word orders sum
+// to at most 4 (x1 y1, x1 y2, x2 y1, x1 y3, x2 y2, x3 y1); each is exact in
+// FP32 and the matrix cores accumulate them in FP32.  The dropped terms are
+// below 2^-24 of the product, so the result has the error of an FP32 dot
+// product.  The instructions are gfx950's K = 32 (16 x 16 tiles) and
+// K = 16 (32 x 32 tiles) bf16 forms.
 
-typedef short gh_v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 gh_v8bf __attribute__((ext_vector_type(8)));
 typedef float gh_v4f __attribute__((ext_vector_type(4)));
 typedef float gh_v16f __attribute__((ext_vector_type(16)));
 
@@ -373,10 +379,14 @@ __device__ __forceinline__ unsigned short bf16_rn(float x) {
 __device__ __forceinline__ float bf16_to_f(unsigned short h) {
     return __uint_as_float((unsigned)h << 16);
 }
+__device__ __forceinline__ __bf16 bf16_bits(unsigned short h) {
+    return __builtin_bit_cast(__bf16, h);
+}
 
-// G (FP32 [3][ng][nr][nr]) -> hi, lo bf16 [3][ng][nrp][nrp], zero padded
-__global__ void split_bf16_kernel(const float *__restrict__ G, unsigned short *__restrict__ hi,
-                                  unsigned short *__restrict__ lo, int ng, int nr, int nrp) {
+// G (FP64 master [3][ng][nr][nr]) -> three bf16 words [3][3][ng][nrp][nrp]
+// (word-major), zero padded to nrp rings
+__global__ void split_bf16_kernel(const double *__restrict__ G, unsigned short *__restrict__ w,
+                                  int ng, int nr, int nrp) {
     const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
     const long long tot = 3LL * ng * nrp * nrp;
     if (i >= tot) return;
@@ -384,11 +394,15 @@ __global__ void split_bf16_kernel(const float *__restrict__ G, unsigned short *_
     const long long r1 = i / nrp;
     const int row = (int)(r1 % nrp);
     const long long tm = r1 / nrp;
-    float g = 0.f;
+    double g = 0.0;
     if (row < nr && k < nr) g = G[(tm * nr + row) * nr + k];
-    const unsigned short h = bf16_rn(g);
-    hi[i] = h;
-    lo[i] = bf16_rn(g - bf16_to_f(h));
+    const unsigned short h1 = bf16_rn((float)g);
+    const double g1 = g - (double)bf16_to_f(h1);
+    const unsigned short h2 = bf16_rn((float)g1);
+    const double g2 = g1 - (double)bf16_to_f(h2);
+    w[i] = h1;
+    w[tot + i] = h2;
+    w[2 * tot + i] = bf16_rn((float)g2);
 }
 
 // element (source ring k, column col) of the (type, m) right-hand side
@@ -417,10 +431,35 @@ __device__ __forceinline__ void ens_store(const EnsGemmArgs &a, int type, int m,
     a.W[((((size_t)mem * 3 + type) * 2 + cs) * a.ng + m) * a.nr + row] = v;
 }
 
-// 16 x 16 output tiles, K = 16 per instruction (v_mfma_f32_16x16x16_bf16):
-// lane l holds A row l%16 and B column l%16 at k = 4(l/16)..+3, and D
-// column l%16 at rows 4(l/16)..+3.  WPB wavefronts per block, each its own
-// tile along the columns.  grid (column tiles / WPB, nrp/16, ntypes*ng).
+// One lane's eight consecutive k of a G row (three words) and of a B column
+// (split here), starting at k0 + kq.
+struct SplitOps {
+    gh_v8bf a1, a2, a3, b1, b2, b3;
+};
+
+__device__ __forceinline__ void load_ops(const EnsGemmArgs &a, SplitOps &o, size_t rowoff,
+                                         size_t wstride, int type, int m, bool pair, int off,
+                                         int k, int col) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        o.a1[j] = bf16_bits(a.gsp[rowoff + k + j]);
+        o.a2[j] = bf16_bits(a.gsp[wstride + rowoff + k + j]);
+        o.a3[j] = bf16_bits(a.gsp[2 * wstride + rowoff + k + j]);
+        const float b = ens_rhs(a, type, m, pair, off, k + j, col);
+        const unsigned short h1 = bf16_rn(b);
+        const float r1 = b - bf16_to_f(h1);
+        const unsigned short h2 = bf16_rn(r1);
+        o.b1[j] = bf16_bits(h1);
+        o.b2[j] = bf16_bits(h2);
+        o.b3[j] = bf16_bits(bf16_rn(r1 - bf16_to_f(h2)));
+    }
+}
+
+// 16 x 16 output tiles, K = 32 per instruction (v_mfma_f32_16x16x32_bf16):
+// lane l holds A row l%16 and B column l%16 at eight k of its lane group
+// l/16 (the same k for A and B), and D column l%16 at rows 4(l/16)..+3.
+// WPB wavefronts per block, each its own tile along the columns.
+// grid (column tiles / WPB, nrp/16, ntypes*ng).
 template <int WPB>
 __global__ __launch_bounds__(64 * WPB) void ens_gemm16_kernel(EnsGemmArgs a) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -430,33 +469,30 @@ __global__ __launch_bounds__(64 * WPB) void ens_gemm16_kernel(EnsGemmArgs a) {
     const bool valid = ens_valid(a, type, m);
     const bool pair = m >= 1 && m < a.mmax - 1;
     const int off = m == 0 ? 0 : 2 * m - 1;
-    gh_v4f acc = {0.f, 0.f, 0.f, 0.f};
-    const int col = tn * 16 + (lane & 15), kq = 4 * (lane >> 4);
+    gh_v4f acc = {0.f, 0.f, 0.f, 0.f}, lo = acc;   // x1 y1 terms; the five corrections
+    const int col = tn * 16 + (lane & 15), kq = 8 * (lane >> 4);
     if (valid) {
+        const size_t wstride = 3ull * a.ng * a.nrp * a.nrp;
         const size_t rowoff = ((size_t)(type * a.ng + m) * a.nrp + tm * 16 + (lane & 15)) * a.nrp;
-        for (int k0 = 0; k0 < a.nrp; k0 += 16) {
-            gh_v4s ah, al, bh, bl;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                ah[j] = (short)a.ghi[rowoff + k0 + kq + j];
-                al[j] = (short)a.glo[rowoff + k0 + kq + j];
-                const float b = ens_rhs(a, type, m, pair, off, k0 + kq + j, col);
-                const unsigned short h = bf16_rn(b);
-                bh[j] = (short)h;
-                bl[j] = (short)bf16_rn(b - bf16_to_f(h));
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+        for (int k0 = 0; k0 < a.nrp; k0 += 32) {
+            SplitOps o;
+            load_ops(a, o, rowoff, wstride, type, m, pair, off, k0 + kq, col);
+            lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.a3, o.b1, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.a2, o.b2, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.a1, o.b3, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.a2, o.b1, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.a1, o.b2, lo, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.a1, o.b1, acc, 0, 0, 0);
         }
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) ens_store(a, type, m, tm * 16 + kq + i, col, acc[i]);
+    for (int i = 0; i < 4; i++)
+        ens_store(a, type, m, tm * 16 + 4 * (lane >> 4) + i, col, acc[i] + lo[i]);
 }
 
-// 32 x 32 output tiles, K = 8 per instruction (v_mfma_f32_32x32x8_bf16):
-// lane l holds A row l%32 and B column l%32 at k = 4(l/32)..+3, and D
-// column l%32 at rows 8(i/4) + 4(l/32) + i%4, i = 0..15.
+// 32 x 32 output tiles, K = 16 per instruction (v_mfma_f32_32x32x16_bf16):
+// lane l holds A row l%32 and B column l%32 at eight k of its lane group
+// l/32, and D column l%32 at rows 8(i/4) + 4(l/32) + i%4, i = 0..15.
 // grid (column tiles, nrp/32, ntypes*ng), one wavefront per block.
 __global__ __launch_bounds__(64) void ens_gemm32_kernel(EnsGemmArgs a) {
     const int lane = threadIdx.x & 63;
@@ -465,38 +501,35 @@ __global__ __launch_bounds__(64) void ens_gemm32_kernel(EnsGemmArgs a) {
     const bool valid = ens_valid(a, type, m);
     const bool pair = m >= 1 && m < a.mmax - 1;
     const int off = m == 0 ? 0 : 2 * m - 1;
-    gh_v16f acc;
+    gh_v16f acc, lo;   // x1 y1 terms; the five corrections
 #pragma unroll
-    for (int i = 0; i < 16; i++) acc[i] = 0.f;
-    const int col = tn * 32 + (lane & 31), kq = 4 * (lane >> 5);
+    for (int i = 0; i < 16; i++) acc[i] = lo[i] = 0.f;
+    const int col = tn * 32 + (lane & 31), kq = 8 * (lane >> 5);
     if (valid) {
+        const size_t wstride = 3ull * a.ng * a.nrp * a.nrp;
         const size_t rowoff = ((size_t)(type * a.ng + m) * a.nrp + tm * 32 + (lane & 31)) * a.nrp;
-        for (int k0 = 0; k0 < a.nrp; k0 += 8) {
-            gh_v4s ah, al, bh, bl;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                ah[j] = (short)a.ghi[rowoff + k0 + kq + j];
-                al[j] = (short)a.glo[rowoff + k0 + kq + j];
-                const float b = ens_rhs(a, type, m, pair, off, k0 + kq + j, col);
-                const unsigned short h = bf16_rn(b);
-                bh[j] = (short)h;
-                bl[j] = (short)bf16_rn(b - bf16_to_f(h));
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh, acc, 0, 0, 0);
+        for (int k0 = 0; k0 < a.nrp; k0 += 16) {
+            SplitOps o;
+            load_ops(a, o, rowoff, wstride, type, m, pair, off, k0 + kq, col);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.a3, o.b1, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.a2, o.b2, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.a1, o.b3, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.a2, o.b1, lo, 0, 0, 0);
+            lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.a1, o.b2, lo, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.a1, o.b1, acc, 0, 0, 0);
         }
     }
 #pragma unroll
     for (int i = 0; i < 16; i++)
-        ens_store(a, type, m, tm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col, acc[i]);
+        ens_store(a, type, m, tm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), col,
+                  acc[i] + lo[i]);
 }
 
-hipError_t launch_split_bf16(const float *G, unsigned short *hi, unsigned short *lo, int ng,
-                             int nr, int nrp, hipStream_t s) {
+hipError_t launch_split_bf16(const double *G, unsigned short *w, int ng, int nr, int nrp,
+                             hipStream_t s) {
     const long long tot = 3LL * ng * nrp * nrp;
     hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)((tot + GH_BLOCK - 1) / GH_BLOCK)),
-                       dim3(GH_BLOCK), 0, s, G, hi, lo, ng, nr, nrp);
+                       dim3(GH_BLOCK), 0, s, G, w, ng, nr, nrp);
     return hipGetLastError();
 }
 

@@ -321,9 +321,10 @@ int galhip_ens_integrals(galhip_ctx *ctx, int32_t centred, double *out);
  * ensembles): mode 0 (default) in the synthesis kernel, one FP32 dot product
  * per (ring, type, harmonic) -- the arithmetic that makes an ensemble member
  * bit-identical to its lone disc; mode 1 as one GEMM per (type, harmonic)
- * over all members' cos/sin columns on the matrix cores, operands split
- * into bf16 hi + lo and hi*hi + hi*lo + lo*hi accumulated in FP32 (relative
- * error ~1e-5 of the field).  The GEMM's tile shape is chosen on the first
+ * over all members' cos/sin columns on the matrix cores, each operand split
+ * into three bf16 words (the tables from their FP64 master), the six
+ * leading cross products accumulated in FP32 -- the accuracy of the FP32
+ * dot products, not bit-identical to them.  The GEMM's tile shape is chosen on the first
  * solve by timing the built variants (0: 16x16 tiles, one wavefront per
  * block; 1: 16x16, four per block; 2: 32x32 tiles); modes 2, 3, 4 select
  * variant 0, 1, 2 without tuning (variants 0 and 1 sum in the same order,

@@ -58,3 +58,50 @@ def test_kt_mode_gpu(precision):
 
     _check(*kt_mode(advance))
     g.close()
+
+
+def _gpu_kt(precision):
+    from galaxy_amd.lib15 import P2DGrid
+    kw, ff, parts, _ = kt_setup()
+    g = P2DGrid(precision=precision, **kw)
+    g.greenm()
+    g.set_fixforce(*ff)
+    g.restrt(*parts)
+    g.masset()
+
+    def advance(an):
+        g.findf(False)
+        rho = g.get_rho() if an else None
+        g.step()
+        return rho
+
+    out = kt_mode(advance)
+    g.close()
+    return out
+
+
+@pytest.mark.gpu
+def test_kt_growth_rate_vs_fp64_oracle(orc):
+    """The growing mode's eigenfrequency (pattern speed x m and growth rate)
+    fitted to the GPU's FP32 (mixed) and FP64 runs, each within 1 % of the
+    same fit to the FP64 oracle's run from the same particles: the whole
+    step reproduces the oracle's m = 2 growth, not only linear theory's."""
+    kw, ff, parts, _ = kt_setup()
+    m = orc.P2D(fixforce=ff, prec="d", use_fx=True, **kw)
+    m.load(*parts)
+    m.masset()
+
+    def advance(an):
+        m.findf(False)
+        rho = m.rho.copy() if an else None
+        m.step()
+        return rho
+
+    ref, _, _ = kt_mode(advance)
+    for precision in ("fp32", "fp64"):
+        print("[kt growth] %s" % precision, flush=True)
+        om, res, amp = _gpu_kt(precision)
+        print("[kt growth] %s: %s against the FP64 oracle's %s (residual %.3f)"
+              % (precision, om, ref, res), flush=True)
+        assert abs(om.real - ref.real) <= 0.01 * ref.real, (precision, om, ref)
+        assert abs(om.imag - ref.imag) <= 0.01 * ref.imag, (precision, om, ref)
