@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgalhip.so")
 SOURCES = ["galhip_api.hip", "galhip_push.hip", "galhip_solve.hip", "galhip_p3d.hip",
-           "galhip_p3d_api.hip", "galhip_adj.hip"]
+           "galhip_p3d_api.hip", "galhip_adj.hip", "galhip_diag.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -200,6 +200,21 @@ class P2DGrid:
         d["energy"] = d["ke"] + 0.5 * d["w_grid"] + d["w_fixed"]
         return d
 
+    def lgspia(self, np_, nm, lscale):
+        """Log-spiral coefficients of the particles (lgspia, normalised as
+        phyprp does): (cos parts, sin parts), each [np_ * nm] in the
+        reference's order (j - 1) nm + (im - 1); on an ensemble one pair per
+        member."""
+        k = int(np_) * int(nm)
+        nblk = getattr(self, "nmem", 1)
+        out = np.zeros(nblk * 2 * k)
+        self._c(self._lib.galhip_lgspia(self._h, int(np_), int(nm), float(lscale), _ptr(out)),
+                "lgspia")
+        out = out.reshape(nblk, 2, k)
+        if nblk == 1:
+            return out[0, 0], out[0, 1]
+        return [(o[0], o[1]) for o in out]
+
     def get_order(self):
         """Load-order index of the particle in each storage slot."""
         ids = np.empty(self.n, np.int32)

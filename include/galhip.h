@@ -198,6 +198,21 @@ int galhip_set_fixpot(galhip_ctx *ctx, int32_t n, double dr, const double *val,
  * spherical radius; energy = out[7]/2 + out[9]/2 + out[10]. */
 int galhip_integrals(galhip_ctx *ctx, int32_t centred, double *out);
 
+/* Log-spiral coefficients of the particles (lgspia, SRC/lib15/lgspia.f:1,
+ * normalised by the particle count as phyprp does, phyprp.f:182-197): for
+ * harmonic index im = 1..nm and pitch index j = 1..np,
+ *   A[j][im] = sum exp(i im (nsect theta + k_j a)) / count,
+ *   a = nsect ln(R / lscale) / 4,
+ * R the particle's cylindrical radius in grid units, over the particles on
+ * the grid and, when offanal is set, off it (equal masses); FP64 sums in a
+ * fixed order.  Index j = jc + k (jc = np/2 + 1) holds pitch k >= 0 and its
+ * mirror np + 1 - j pitch -k, so k_j = j - jc for odd np; for even np index
+ * np/2 is never filled and is 0.  out[2][np * nm]: the cos parts, then the
+ * sin parts, each in the reference's order (j - 1) nm + (im - 1).  np * nm <= 512.  On a p3d context
+ * R and theta are those of (x, y); on an ensemble out holds nmem such
+ * blocks, member by member. */
+int galhip_lgspia(galhip_ctx *ctx, int32_t np, int32_t nm, double lscale, double *out);
+
 /* ---- performance controls ---- */
 
 /* Regroup particles by cell every k steps of galhip_run (0 = never), and

@@ -505,3 +505,49 @@ class P2D:
         fn = _lib.orc_tmcent_f if self.prec == "f" else _lib.orc_tmcent_d
         fn(C.byref(c), len(self.x), self.x, self.y, self.vx, self.vy, self.iz,
            fx, fy, int(bool(start)))
+
+
+# ---------------------------------------------------------------------------
+# log-spiral coefficients
+
+
+def lgspia(x, y, iz, nzones, offanal, nsect, np_, nm, lscale):
+    """Log-spiral coefficients of the particles (SRC/lib15/lgspia.f:44-110,
+    normalised by the population's weight as phyprp.f:182-197 does): for
+    harmonic index im = 1..nm and pitch index j = 1..np_,
+        A[j, im] = sum_i exp(i im (nsect theta_i + k_j a_i)) / count,
+    a_i = nsect ln(R_i / lscale) / 4, over equal-mass particles on the grid
+    (and off it when offanal), in FP64 with the reference's recurrences
+    (powers of e^{i nsect theta} and e^{i a} by repeated complex products).
+    Pitch index j = jc + k (jc = np_ // 2 + 1, k >= 0) holds pitch k, and
+    its mirror np_ + 1 - j pitch -k from the conjugate: k_j = j - jc for odd
+    np_; for even np_ index np_ / 2 is never filled and stays zero.  Returns (cos part, sin part),
+    each [np_ * nm] in the reference's order (j - 1) nm + (im - 1)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    iz = np.asarray(iz)
+    use = (iz >= 1) & (iz <= nzones) if not offanal else (iz >= 1)
+    rc = np.sqrt(x * x + y * y)
+    use &= rc > 0
+    x, y, rc = x[use], y[use], rc[use]
+    z1 = (x + 1j * y) / rc
+    zs = z1.copy()
+    for _ in range(2, nsect + 1):
+        zs = zs * z1
+    a = 0.25 * nsect * np.log(rc / lscale)
+    w1 = np.cos(a) + 1j * np.sin(a)
+    A = np.zeros((np_, nm), np.complex128)
+    jc = np_ // 2 + 1
+    zm = np.ones_like(zs)
+    wm = np.ones_like(w1)
+    for im in range(1, nm + 1):
+        zm = zm * zs
+        wm = wm * w1
+        A[jc - 1, im - 1] = zm.sum()
+        g = np.ones_like(wm)
+        for j in range(jc + 1, np_ + 1):
+            g = g * wm
+            A[j - 1, im - 1] = (g * zm).sum()
+            A[np_ - j, im - 1] = (np.conj(g) * zm).sum()
+    A /= max(int(use.sum()), 1)
+    return A.real.reshape(-1), A.imag.reshape(-1)

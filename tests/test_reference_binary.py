@@ -5,6 +5,9 @@ with this image's flang into oracle/_ref (oracle/Makefile.ref).
     EXAMPLES/p2d/run100.dmp0 with the shipped run100.dat reach the shipped
     run100.dmp10 (written by the reference author's own build) to 1e-6,
     every particle on the same zone list.
+  - The log-spiral coefficients the reference's analysis writes (its .res
+    LGSP records, lgspia.f / phyprp.f) at steps 0 and 500 equal the
+    oracle's on the dumps of those steps to its FP32 summation error.
   - The reference's FP32 Green's tables (the .grt file its p2grnm writes,
     records in p2gwrt's order: F_R for m = 0..ng-1, F_phi for m = 1..,
     the potential) equal the oracle's FP64 tables to its FP32 rounding.
@@ -137,3 +140,29 @@ def test_oracle_matches_reference_steps(tmp_path, orc, name, n, steps, tol):
         assert errs[k] <= 5 * tol, (k, errs[k])
     assert np.array_equal(m.iz, d1["iz"])
     shutil.rmtree(d, ignore_errors=True)
+
+
+def test_lgspia_matches_reference(run100, orc):
+    """LGSP records of the run's .res (ipp, 'LGSP', istep, np, nm; then the
+    cos and sin parts, phyprp.f:182-197) at steps 0 and 500 against the
+    oracle on the particles of the shipped dmp0 and of the run's dmp10
+    (run100: two sectors, lscale 10, offgrid t).  The values are ~1e-6 (a
+    quiet start); the reference's FP32 sums of 15000 unit vectors carry
+    ~1e-8, the tolerance is 5e-8 absolute."""
+    import struct
+    recs = dmp._records(open(os.path.join(run100, "run100.res"), "rb").read())
+    lg = {}
+    for i, r in enumerate(recs):
+        if len(r) == 20 and r[4:8] == b"LGSP":
+            _, istep, np_, nm = struct.unpack("<i4xiii", r)
+            lg[istep] = (np_, nm, np.frombuffer(recs[i + 1], "<f4").astype(np.float64))
+    for istep, path in ((0, os.path.join(EX, "run100.dmp0")),
+                        (500, os.path.join(run100, "run100.dmp10"))):
+        np_, nm, v = lg[istep]
+        d = dmp.read_dmp(path)
+        assert d["istep"] == istep
+        c, s = orc.lgspia(d["x"], d["y"], d["iz"], d["nz"], True, 2, np_, nm, 10.0)
+        k = np_ * nm
+        assert np.abs(v).max() > 1e-6
+        assert np.abs(c - v[:k]).max() <= 5e-8, istep
+        assert np.abs(s - v[k:]).max() <= 5e-8, istep
