@@ -269,8 +269,9 @@ def test_push_fp32_tracks_fp64():
 
 def test_quadratic_weights():
     """jmass 3 (weight.f:251-300): the 27 weights are a partition of unity,
-    non-negative, and interpolate linear functions of (u, t, zg) exactly
-    about the central cell (the TSC property)."""
+    non-negative, interpolate linear functions of (u, t, zg) exactly about
+    the central cell, and have the second moments of the triangular-shaped
+    cloud (1/4 per direction, no cross terms)."""
     m = _model(jmass=3)
     rng = np.random.default_rng(17)
     n = 3000
@@ -297,6 +298,14 @@ def test_quadratic_weights():
     ct = np.where(t[:, None] - ct > na / 2, ct + na, ct)
     f = lambda a, b, c: 0.5 - 1.25 * a + 0.5 * b + 2.0 * c
     assert np.allclose((w * f(cu, ct, cz)).sum(1), f(u, t, zg), atol=1e-9)
+    # second moments about the particle: 1/4 in each direction, no cross
+    # terms -- the triangular-shaped cloud (three unit boxes convolved).
+    # With the zeroth and first moments this fixes the three 1-D weights
+    # of each direction, so a wrong coefficient of the quadratic fails here.
+    du, dt, dz = cu - u[:, None], ct - t[:, None], cz - zg[:, None]
+    for a, b, want in ((du, du, 0.25), (dt, dt, 0.25), (dz, dz, 0.25),
+                       (du, dt, 0.0), (du, dz, 0.0), (dt, dz, 0.0)):
+        assert np.allclose((w * a * b).sum(1), want, atol=1e-9)
 
 
 def test_quadratic_massgn_and_geometry():
@@ -318,3 +327,85 @@ def test_quadratic_massgn_and_geometry():
     cell, _, _ = m.weight(x, y, z)
     ring = cell // (m.ngz * m.na)
     assert m.jrad[0] == ring.min() and m.krad[0] == ring.max() + 2
+
+
+def _ring_particles(m, n, rng, zones=True):
+    """Particles inside the grid's volume, a few off it, two zones."""
+    R = rng.uniform(0.3, 0.85 * m.rgrid, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    z = rng.uniform(-0.8 * m.zm, 0.8 * m.zm, n)
+    x, y = R * np.cos(ph), R * np.sin(ph)
+    vx, vy, vz = rng.normal(0, 0.1, n), rng.normal(0, 0.1, n), rng.normal(0, 0.05, n)
+    iz = np.where(np.sqrt(R * R + z * z) >= m.zbr[1], 2, 1) if zones else np.ones(n, int)
+    iz = np.asarray(iz, np.int32)
+    iz[:5] = m.nzones + 1
+    x[:5] *= 5.0
+    return x, y, z, vx, vy, vz, iz
+
+
+@pytest.mark.parametrize("jmass", [2, 3])
+def test_tmcent3_closed_form(jmass):
+    """orc3_tmcent with a uniform field F0 and a constant radial fixed force
+    c (a table of a constant, which cubic Hermite reproduces exactly): every
+    on-grid particle's velocity moves by -(F0 + c x / r) s^2 / 2, s the step
+    factor of its zone (getacc.f:142-151, tmcgrp.f:26-36); off-grid ones do
+    not move; tmcent(False) takes it back."""
+    m = _model(jmass=jmass, nzones=2, nstep=(1, 2), zbr=(0.0, 3.0))
+    rng = np.random.default_rng(5)
+    x, y, z, vx, vy, vz, iz = _ring_particles(m, 500, rng)
+    F0 = np.array([0.013, -0.027, 0.041])
+    m.fld[:3] = F0[:, None]
+    cval = -0.05
+    m.fixforce = (0.5, np.full(200, cval), np.zeros(200))
+    m.load(x, y, z, vx, vy, vz, iz)
+    m.tmcent(True)
+    r = np.sqrt(x * x + y * y + z * z)
+    s2 = np.where(iz == 2, 4.0, 1.0)
+    on = iz <= m.nzones
+    for d, (v0, p) in enumerate(((vx, x), (vy, y), (vz, z))):
+        want = np.where(on, v0 - 0.5 * (F0[d] + cval * p / r) * s2, v0)
+        got = (m.vx, m.vy, m.vz)[d]
+        assert np.abs(got - want).max() < 1e-14, d
+    m.tmcent(False)
+    for got, v0 in ((m.vx, vx), (m.vy, vy), (m.vz, vz)):
+        assert np.abs(got - v0).max() < 1e-14
+
+
+@pytest.mark.parametrize("jmass", [2, 3])
+def test_integrals3_closed_form(jmass):
+    """oracle3.integrals (icheck.f:24-70 on the 3-D grid) against sums
+    written out here: mass, m x position, momentum, sum m v^2 and Lz with
+    velocities over the zone's step factor (the off-grid list's that of the
+    last zone, with which it moves); a uniform potential Phi0 gives
+    sum m Phi_grid = pmass x (particles on the grid) x Phi0 (partition of
+    unity of either scheme); a fixed potential a + b r tabulated on a
+    Hermite table gives pmass sum (a + b r) over every particle; centred
+    with a uniform field adds F0 s^2 / 2 to each on-grid velocity before the
+    step factor."""
+    m = _model(jmass=jmass, nzones=2, nstep=(1, 2), zbr=(0.0, 3.0))
+    rng = np.random.default_rng(9)
+    x, y, z, vx, vy, vz, iz = _ring_particles(m, 400, rng)
+    m.pmass = 0.37
+    m.load(x, y, z, vx, vy, vz, iz)
+    phi0, F0 = -1.75, np.array([0.01, 0.02, -0.03])
+    m.fld[3] = phi0
+    m.fld[:3] = F0[:, None]
+    a, b, dr = 0.4, -0.3, 0.25
+    rk = dr * np.arange(4000)                    # past every particle's radius
+    assert rk[-1] > np.sqrt(x * x + y * y + z * z).max()
+    fp = (dr, a + b * rk, np.full(4000, b))
+    # step factor of the zone; the off-grid list moves with the last zone
+    s = np.where(iz >= 2, 2.0, 1.0)
+    on = iz <= m.nzones
+    for centred in (False, True):
+        out = m.integrals(fixpot=fp, centred=centred)
+        kick = np.where(on, 0.5 * s * s, 0.0) if centred else np.zeros(len(x))
+        v = [(vx + kick * F0[0]) / s, (vy + kick * F0[1]) / s, (vz + kick * F0[2]) / s]
+        pm = 0.37
+        want = np.array([pm * len(x), pm * x.sum(), pm * y.sum(), pm * z.sum(),
+                         pm * v[0].sum(), pm * v[1].sum(), pm * v[2].sum(),
+                         pm * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2).sum(),
+                         pm * (x * v[1] - y * v[0]).sum(),
+                         pm * on.sum() * phi0,
+                         pm * (a + b * np.sqrt(x * x + y * y + z * z)).sum()])
+        assert np.allclose(out, want, rtol=1e-13, atol=1e-13), (centred, out - want)
