@@ -88,7 +88,7 @@ def test_c2_zone_cycle_bit_exact(orc, green_c2):
     w = configs.C2
     _say("C2 zone cycle fp32, N = 2e5")
     m, parts = oracle_model(orc, w, prec="f", n=200_000, G=green_c2)
-    g = gpu_model(w, precision="fp32", parts=parts)
+    g = gpu_model(w, precision="fp32", n=200_000, parts=parts)
     _chain(m, g, 5, True)
     g.close()
 
@@ -100,7 +100,7 @@ def test_c2_force_error_vs_fp64_oracle(orc, green_c2):
     w = configs.C2
     _say("C2 force error, N = 2e5")
     m, parts = oracle_model(orc, w, prec="d", n=200_000, G=green_c2)
-    g = gpu_model(w, precision="fp32", parts=parts)
+    g = gpu_model(w, precision="fp32", n=200_000, parts=parts)
     m.masset()
     g.masset()
     errs = []
