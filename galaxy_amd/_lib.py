@@ -80,6 +80,7 @@ SIGNATURES = {
     "galhip_get_noff": [_vp, _vp],
     "galhip_set_fixpot": [_vp, _i32, _dbl, _vp, _vp],
     "galhip_integrals": [_vp, _i32, _vp],
+    "galhip_integrals_noalloc": [_vp, _i32, _vp],
     "galhip_lgspia": [_vp, _i32, _i32, _dbl, _vp],
     "galhip_set_sort_interval": [_vp, _i32],
     "galhip_set_pmass": [_vp, _dbl],

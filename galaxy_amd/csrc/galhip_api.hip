@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1002,17 +1003,28 @@ extern "C" int galhip_tmcent(galhip_ctx *c, int32_t start) {
     return c->prec == GALHIP_FP32 ? tmcent_impl<float>(c, start) : tmcent_impl<double>(c, start);
 }
 
-// out[nmem][9]: each grid's integrals (one grid: nmem = 1), member by
-// member with the one-grid kernels, so that a member's sums are those of
-// the same disc in its own context.
-template <typename R> static int integrals_impl(galhip_ctx *c, int centred, double *out) {
+// Partial-sum rows of the p2d integrals kernels: 9 per block of the
+// largest grid, then the 9 totals of every grid.
+static long long integrals_nb(const galhip_ctx *c) {
     long long nbmax = 1;
     for (int m = 0; m < c->nmem; m++) {
         const long long nm = c->ens ? c->m_off[m + 1] - c->m_off[m] : c->n;
         nbmax = std::max(nbmax, (nm + GH_BLOCK - 1) / GH_BLOCK);
     }
-    double *part = nullptr, *dout = nullptr;
-    int rc = dalloc(&part, sizeof(double) * 9 * nbmax) |
+    return nbmax;
+}
+
+// out[nmem][9]: each grid's integrals (one grid: nmem = 1), member by
+// member with the one-grid kernels, so that a member's sums are those of
+// the same disc in its own context.  buf: a device buffer of
+// 9 * (integrals_nb(c) + nmem) doubles, or null to allocate one here.
+template <typename R> static int integrals_impl(galhip_ctx *c, int centred, double *out,
+                                                double *buf) {
+    const long long nbmax = integrals_nb(c);
+    double *part = buf, *dout = buf ? buf + 9 * nbmax : nullptr;
+    int rc = 0;
+    if (!buf)
+        rc = dalloc(&part, sizeof(double) * 9 * nbmax) |
              dalloc(&dout, sizeof(double) * 9 * c->nmem);
     if (!rc) {
         hipError_t e = hipSuccess;
@@ -1027,8 +1039,10 @@ template <typename R> static int integrals_impl(galhip_ctx *c, int centred, doub
         if (e == hipSuccess) e = hipStreamSynchronize(c->s);
         if (e != hipSuccess) rc = gh_fail(GALHIP_EHIP, "integrals: %s", hipGetErrorString(e));
     }
-    dfree(part);
-    dfree(dout);
+    if (!buf) {
+        dfree(part);
+        dfree(dout);
+    }
     if (rc) return rc;
     // particle sums times the particle mass; no potential: NaN
     for (int m = 0; m < c->nmem; m++) {
@@ -1039,14 +1053,71 @@ template <typename R> static int integrals_impl(galhip_ctx *c, int centred, doub
     return 0;
 }
 
-extern "C" int galhip_integrals(galhip_ctx *c, int32_t centred, double *out) {
+static int integrals_check(const galhip_ctx *c, int32_t centred, const double *out) {
     if (!c || !out) return gh_fail(GALHIP_EINVAL, "null argument");
     if (centred && !c->have_field)
         return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
-    if (c->p3d) return p3_integrals(c, centred != 0, out);
     if (c->ens) return gh_fail(GALHIP_EINVAL, "ensemble context: use galhip_ens_integrals");
-    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out)
-                                  : integrals_impl<double>(c, centred != 0, out);
+    return 0;
+}
+
+extern "C" int galhip_integrals(galhip_ctx *c, int32_t centred, double *out) {
+    int rc = integrals_check(c, centred, out);
+    if (rc) return rc;
+    if (c->p3d) return p3_integrals(c, centred != 0, out);
+    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out, nullptr)
+                                  : integrals_impl<double>(c, centred != 0, out, nullptr);
+}
+
+// ---- the integrals without allocation ------------------------------------------
+//
+// galhip_integrals_noalloc keeps its partial sums in `scratch`, a file-scope
+// device array of GH_SCRATCH_DOUBLES doubles (12 MiB) that every GPU the
+// library runs on holds until the process ends.  In host code the array's
+// name is only a stand-in for the symbol, not a device address: the address
+// comes from hipGetSymbolAddress, taken once per device (with that device
+// current) on the entry point's first call there, and only that pointer is
+// handed to kernels.  Every call on one GPU shares that GPU's array, so calls
+// that overlap on one GPU (two host threads, or two contexts' streams) race.
+#define GH_SCRATCH_DOUBLES (3 << 19)
+#define GH_SCRATCH_MAXDEV 64
+static __device__ __align__(256) double scratch[GH_SCRATCH_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch[GH_SCRATCH_MAXDEV];
+
+static int scratch_on_device(double **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= GH_SCRATCH_MAXDEV)
+        return gh_fail(GALHIP_EINVAL, "device %d: at most %d devices", dev, GH_SCRATCH_MAXDEV);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return gh_fail(GALHIP_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                           hipGetErrorString(e));
+        g_scratch[dev] = (double *)p;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
+extern "C" int galhip_integrals_noalloc(galhip_ctx *c, int32_t centred, double *out) {
+    int rc = integrals_check(c, centred, out);
+    if (rc) return rc;
+    const long long need = c->p3d ? p3_integrals_doubles(c) : 9 * (integrals_nb(c) + 1);
+    if (need > GH_SCRATCH_DOUBLES)
+        return gh_fail(GALHIP_ERANGE,
+                       "integrals_noalloc: %lld partial sums exceed the %d of the static "
+                       "scratch (12 MiB; use galhip_integrals)", need, GH_SCRATCH_DOUBLES);
+    double *buf = nullptr;
+    rc = scratch_on_device(&buf);
+    if (rc) return rc;
+    if (c->p3d) return p3_integrals(c, centred != 0, out, buf);
+    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out, buf)
+                                  : integrals_impl<double>(c, centred != 0, out, buf);
 }
 
 // One step of galhip_run -- findf with the fused masscl, then step -- as a
@@ -1495,8 +1566,8 @@ extern "C" int galhip_ens_integrals(galhip_ctx *c, int32_t centred, double *out)
     if (!c->ens) return gh_fail(GALHIP_EINVAL, "not an ensemble context");
     if (centred && !c->have_field)
         return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
-    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out)
-                                  : integrals_impl<double>(c, centred != 0, out);
+    return c->prec == GALHIP_FP32 ? integrals_impl<float>(c, centred != 0, out, nullptr)
+                                  : integrals_impl<double>(c, centred != 0, out, nullptr);
 }
 
 // ---- compensated positions ----------------------------------------------------------

@@ -173,4 +173,5 @@ int p3_get_field(const galhip_ctx *c, void *fld);
 int p3_set_field(galhip_ctx *c, const void *fld);
 int p3_unload(galhip_ctx *c, void *x, void *y, void *vx, void *vy, int32_t *iz);
 int p3_sort(galhip_ctx *c);
-int p3_integrals(galhip_ctx *c, int centred, double *out);
+int p3_integrals(galhip_ctx *c, int centred, double *out, double *buf = nullptr);
+long long p3_integrals_doubles(const galhip_ctx *c);

@@ -548,10 +548,15 @@ int p3_tmcent(galhip_ctx *c, int start) {
 
 // ---- global integrals ------------------------------------------------------------------------
 
-template <typename R> static int p3_integrals_impl(galhip_ctx *c, int centred, double *out) {
+// part and dout: device buffers of p3_integrals_doubles(c) doubles in all
+// (partials, then the P3_NINT totals), or null to allocate them here.
+template <typename R>
+static int p3_integrals_impl(galhip_ctx *c, int centred, double *out, double *buf) {
     const long long nb = c->n > 0 ? (c->n + 511) / 512 : 0;
-    double *part = nullptr, *dout = nullptr;
-    int rc = dalloc(&part, sizeof(double) * P3_NINT * (nb > 0 ? nb : 1)) |
+    double *part = buf, *dout = buf ? buf + P3_NINT * (nb > 0 ? nb : 1) : nullptr;
+    int rc = 0;
+    if (!buf)
+        rc = dalloc(&part, sizeof(double) * P3_NINT * (nb > 0 ? nb : 1)) |
              dalloc(&dout, sizeof(double) * P3_NINT);
     if (!rc) {
         hipError_t e = launch_p3_integrals<R>(p3_args<R>(c), c->pot_valid, c->fp_val, c->fp_der,
@@ -561,17 +566,24 @@ template <typename R> static int p3_integrals_impl(galhip_ctx *c, int centred, d
         if (e == hipSuccess) e = hipStreamSynchronize(c->s);
         if (e != hipSuccess) rc = gh_fail(GALHIP_EHIP, "integrals: %s", hipGetErrorString(e));
     }
-    dfree(part);
-    dfree(dout);
+    if (!buf) {
+        dfree(part);
+        dfree(dout);
+    }
     if (rc) return rc;
     for (int q = 0; q < P3_NINT; q++) out[q] *= c->pmass;
     if (!c->pot_valid) out[9] = NAN;
     return 0;
 }
 
-int p3_integrals(galhip_ctx *c, int centred, double *out) {
-    return c->prec == GALHIP_FP32 ? p3_integrals_impl<float>(c, centred, out)
-                                  : p3_integrals_impl<double>(c, centred, out);
+long long p3_integrals_doubles(const galhip_ctx *c) {
+    const long long nb = c->n > 0 ? (c->n + 511) / 512 : 0;
+    return P3_NINT * ((nb > 0 ? nb : 1) + 1);
+}
+
+int p3_integrals(galhip_ctx *c, int centred, double *out, double *buf) {
+    return c->prec == GALHIP_FP32 ? p3_integrals_impl<float>(c, centred, out, buf)
+                                  : p3_integrals_impl<double>(c, centred, out, buf);
 }
 
 // ---- fields ------------------------------------------------------------------------------------

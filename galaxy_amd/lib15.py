@@ -186,14 +186,15 @@ class P2DGrid:
         self._c(self._lib.galhip_set_fixpot(self._h, len(val), float(dr), _ptr(val),
                                              _ptr(der)), "set_fixpot")
 
-    def integrals(self, centred=True):
+    def integrals(self, centred=True, noalloc=False):
         """icheck's global integrals (see galhip.h): a dict with mass, cm,
         momentum, ke, lz, w_grid (sum m Phi_grid; nan unless the last findf
         formed the potential), w_fixed and the energy ke + w_grid/2 + w_fixed.
-        centred: velocities time-centred with the current field, as icheck."""
+        centred: velocities time-centred with the current field, as icheck.
+        noalloc: galhip_integrals_noalloc (static device scratch)."""
         out = np.zeros(9, np.float64)
-        self._c(self._lib.galhip_integrals(self._h, int(bool(centred)), _ptr(out)),
-                "integrals")
+        fn = self._lib.galhip_integrals_noalloc if noalloc else self._lib.galhip_integrals
+        self._c(fn(self._h, int(bool(centred)), _ptr(out)), "integrals")
         d = dict(mass=out[0], cm=out[1:3] / out[0] if out[0] else out[1:3],
                  momentum=out[3:5], ke=0.5 * out[5], lz=out[6], w_grid=out[7],
                  w_fixed=out[8])
@@ -486,12 +487,13 @@ class P3DGrid(P2DGrid):
                                          _ptr(acc)), "grdacc3")
         return acc
 
-    def integrals(self, centred=True):
+    def integrals(self, centred=True, noalloc=False):
         """icheck's global integrals on the 3-D grid: mass, cm (3), momentum
-        (3), ke, lz, w_grid (nan without a potential), w_fixed, energy."""
+        (3), ke, lz, w_grid (nan without a potential), w_fixed, energy.
+        noalloc: galhip_integrals_noalloc (static device scratch)."""
         out = np.zeros(11, np.float64)
-        self._c(self._lib.galhip_integrals(self._h, int(bool(centred)), _ptr(out)),
-                "integrals")
+        fn = self._lib.galhip_integrals_noalloc if noalloc else self._lib.galhip_integrals
+        self._c(fn(self._h, int(bool(centred)), _ptr(out)), "integrals")
         d = dict(mass=out[0], cm=out[1:4] / out[0] if out[0] else out[1:4],
                  momentum=out[4:7], ke=0.5 * out[7], lz=out[8], w_grid=out[9],
                  w_fixed=out[10])

@@ -197,6 +197,17 @@ int galhip_set_fixpot(galhip_ctx *ctx, int32_t n, double dr, const double *val,
  * [8] Lz, [9] sum m Phi_grid (NaN as above), [10] sum m Phi_fixed of the
  * spherical radius; energy = out[7]/2 + out[9]/2 + out[10]. */
 int galhip_integrals(galhip_ctx *ctx, int32_t centred, double *out);
+/* galhip_integrals without any device allocation: the same kernels and
+ * results (bit for bit), with the partial sums in a static 12 MiB device
+ * array that each GPU holds until the process ends.  No counterpart in the
+ * reference (its icheck sums on the host); it is for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Limit: at most 3 x 2^19 partial sums, i.e.
+ * 9 x (ceil(n / 256) + 1) on a p2d grid (n up to about 44.7M particles) and
+ * 11 x (ceil(n / 512) + 1) on a p3d grid (about 73.2M); larger contexts get
+ * GALHIP_ERANGE.  Every call on one GPU uses that GPU's one array: calls
+ * that overlap on a GPU (other host threads, other contexts) race. */
+int galhip_integrals_noalloc(galhip_ctx *ctx, int32_t centred, double *out);
 
 /* Log-spiral coefficients of the particles (lgspia, SRC/lib15/lgspia.f:1,
  * normalised by the particle count as phyprp does, phyprp.f:182-197): for

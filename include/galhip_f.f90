@@ -206,6 +206,13 @@ module galhip_f
        integer(c_int32_t), value :: centred
        real(c_double), intent(out) :: out(9)
      end function galhip_integrals
+     integer(c_int) function galhip_integrals_noalloc(ctx, centred, out) &
+          bind(c, name='galhip_integrals_noalloc')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: centred
+       real(c_double), intent(out) :: out(*)
+     end function galhip_integrals_noalloc
      integer(c_int) function galhip_lgspia(ctx, np, nm, lscale, out) &
           bind(c, name='galhip_lgspia')
        import :: c_int, c_ptr, c_int32_t, c_double
