@@ -505,8 +505,9 @@ def main_adj(args):
     """--workload C5: the differentiated run (128 x 256 grid, N = 1e6, Q = 1.2,
     200 steps): |A_2|^2 at the end and its gradient in every initial
     velocity, Q and the disc mass by the adjoint kernels with checkpoints
-    every 20 steps; the FP64 forward's gradient and a central difference
-    along one random velocity direction as the checks."""
+    every 20 steps; the FP64 forward's gradient, the FP64 tangent-linear run
+    and central differences along one random velocity direction as the
+    checks."""
     from galaxy_amd import adjoint, configs
     w = configs.C5
     n = args.n or w.n
@@ -551,8 +552,18 @@ def main_adj(args):
         vx, vy = m64.velocities(w.Q, w.md)
         vs = float(np.sqrt(np.mean(vx * vx + vy * vy)))
         ad = float(np.dot(g64["dvx"], dx) + np.dot(g64["dvy"], dy))
+        # the FP64 tangent-linear run along the same direction: an
+        # independent forward-mode reference for the adjoint's number
+        d0 = np.zeros((4, n))
+        d0[2], d0[3] = dx, dy
+        t0 = time.perf_counter()
+        tl, _, _ = m64.tangent(w.Q, w.md, args.steps, d0)
+        t_tl = time.perf_counter() - t0
         fds = []
-        for hr in (1e-6, 1e-7):
+        # h = 1e-4 |v| (the config's step) crosses CIC cell edges: the map is
+        # piecewise smooth in the positions, so the difference there measures
+        # the kinks as much as the derivative
+        for hr in (1e-4, 1e-6, 1e-7):
             h = hr * vs
             # the annulus membership of the unperturbed end state held fixed,
             # as the adjoint holds it (at this N single particles crossing
@@ -560,12 +571,15 @@ def main_adj(args):
             lp = m64.loss_v(vx + h * dx, vy + h * dy, w.md, args.steps, sel=g64["sel"])
             lm = m64.loss_v(vx - h * dx, vy - h * dy, w.md, args.steps, sel=g64["sel"])
             fd = (lp - lm) / (2 * h)
-            fds.append(dict(h_rel=hr, central_difference=fd, rel_err=abs(ad - fd) / abs(fd)))
+            fds.append(dict(h_rel=hr, central_difference=fd, rel_err=abs(ad - fd) / abs(fd),
+                            rel_err_tangent=abs(tl - fd) / abs(fd)))
         extra.update(fp64_seconds=round(t64, 3), loss_fp64=g64["loss"],
                      grad_v_rel_err_fp32_vs_fp64=float(np.linalg.norm(v32 - v64)
                                                        / np.linalg.norm(v64)),
                      dQ_fp64=g64["dQ"], dmd_fp64=g64["dmd"],
-                     fd_direction=dict(adjoint=ad, checks=fds))
+                     fd_direction=dict(adjoint=ad, tangent_linear=tl,
+                                       tangent_vs_adjoint=abs(tl - ad) / abs(ad),
+                                       tangent_seconds=round(t_tl, 3), checks=fds))
     value = n * args.steps / t32
     cpu = None
     if not args.no_cpu:

@@ -118,6 +118,10 @@ SIGNATURES = {
     "galhip_adj_seed_a2": [_vp, _dbl, _dbl, _dbl, _vp],
     "galhip_adj_field": [_vp, _vp, _vp],
     "galhip_adj_get_profile": [_vp, _vp, _vp],
+    "galhip_tl_set": [_vp, _vp],
+    "galhip_tl_get": [_vp, _vp],
+    "galhip_tl_step": [_vp],
+    "galhip_tl_field": [_vp, _vp, _vp],
 }
 
 _lib = None

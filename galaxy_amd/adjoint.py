@@ -113,6 +113,30 @@ class Trajectory:
         return g.adj_get(), g.adj_get_par()
 
 
+    def tangent(self, x, y, vx, vy, iz, d0):
+        """FP64 tangent-linear run (galhip_tl_*): the forward run from the
+        same state with the perturbation d0 [4][n] (load order) carried
+        along step by step; re-sorted every 10 steps as the forward run is.
+        Returns the final perturbation [4][n]; the final state stays on the
+        device."""
+        g = self.g
+        g.set_sort_interval(0)
+        g.set_step(0)
+        g.restrt(x, y, vx, vy, iz)
+        g.masset()
+        g.tl_set(d0)
+        t0 = time.perf_counter()
+        for s in range(self.nsteps):
+            g.findf(False)
+            g.tl_step()
+            g.step()
+            if (s + 1) % 10 == 0:
+                g.sort()
+        g.sync()
+        self.seconds["tangent"] = self.seconds.get("tangent", 0.0) + time.perf_counter() - t0
+        return g.tl_get()
+
+
 class DiscModel:
     """The differentiated workload: a seeded exponential disc whose initial
     velocities depend on Toomre Q and the disc mass md (sigma_R = Q md
@@ -232,6 +256,22 @@ class DiscModel:
         tr.seconds["finish"] = time.perf_counter() - t1
         return dict(loss=L, dvx=lvx, dvy=lvy, dQ=dQ, dmd=dmd, gpar=gpar,
                     seconds=dict(tr.seconds), sel=sel, kernel_ms=kms)
+
+
+    def tangent(self, Q, md, nsteps, d0):
+        """Directional derivative of L along the initial-state perturbation
+        d0 [4][n] ((x, y, vx, vy), load order, internal units) by the FP64
+        tangent-linear run: dL = <dL/dx_N, dx_N>, the seed dL/dx_N formed
+        on the device as the adjoint's (galhip_adj_seed_a2).  Returns
+        (dL, L, the final perturbation)."""
+        vx, vy = self.velocities(Q, md)
+        g = self._grid_for(md)
+        tr = Trajectory(g, nsteps)
+        d = tr.tangent(self.x0, self.y0, vx, vy, self.iz0, d0)
+        L, _ = g.adj_seed_a2(self.ls, 1.0, 3.0)
+        lam = g.adj_get()
+        dL = float(np.dot(lam[0], d[0]) + np.dot(lam[1], d[1]))
+        return dL, L, d
 
 
 def ic_velocities(k_sig, k_phi, vc2, c, g1, g2, cp, sp, Q, md, f):

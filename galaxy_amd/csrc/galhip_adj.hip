@@ -545,6 +545,211 @@ __global__ __launch_bounds__(GH_BLOCK) void a2_seed_kernel(const typename Vec4<R
     lam[3 * n + i] = 0.0;
 }
 
+
+// ---- tangent-linear (forward-mode) step ---------------------------------------------
+//
+// The derivative of the same map applied to a perturbation (dx, dv) of the
+// state before the step, in FP64 along the forward run:
+//   drho = pmass sum_i sum_q (grad w_q(x_i) . dx_i) [c_q]      tl_rho_kernel
+//   dF   = K drho                                               tl_ana + tl_syn
+//   da_i = sum_q w_q dF(c_q) + sum_q (grad w_q . dx_i) F(c_q) + J_halo dx_i
+//   dv'  = dv + da,  dx' = dx + dv'                             tl_push_kernel
+// with the cells, ring ranges and the off-grid choice held fixed, as the
+// adjoint holds them.  The solve is built as the transpose of the adjoint's
+// (adj_wbar_kernel, adj_rbar_kernel), which is itself the transpose of the
+// forward solve, so  <J dz, lam> = <dz, J^T lam>  holds step by step to
+// rounding.  Perturbations are kept in load order (indexed through the
+// particles' ids), so re-sorts leave them in place.
+
+template <typename R> struct TlArgs {
+    const typename Vec4<R>::T *xv;
+    const typename Vec2<R>::T *ut;
+    const uint8_t *zone;
+    const unsigned *id;
+    long long n;
+    double *d;                        // [4][n] load order: dx, dy, dvx, dvy
+    double *rho;                      // [mesh] perturbed density
+    const double *fx, *fy;            // [mesh] its field
+    const typename Vec2<R>::T *fld;   // forward (F_x, F_y)
+    const double *fr_val, *fr_der;
+    double fr_dr, fr_inv_dr;
+    int nfr;
+    double pmass, inv_alpha, tscale;
+    int nr, na, mesh;
+};
+
+// grad u . d and grad t . d at (x, y): u = log(1 + r) / alpha,
+// t = atan2(y, x) na / (2 pi)
+__device__ __forceinline__ void tl_dut(double x, double y, double dx, double dy,
+                                       double inv_alpha, double tscale, double &du, double &dt) {
+    const double r2 = x * x + y * y, r = sqrt(r2);
+    du = 0.0;
+    dt = 0.0;
+    if (r > 0.0) {
+        du = inv_alpha * (x * dx + y * dy) / (r * (1.0 + r));
+        dt = tscale * (x * dy - y * dx) / r2;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void tl_rho_kernel(TlArgs<R> a) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= a.n || a.zone[i] != 1) return;
+    const long long n = a.n, p = a.id[i];
+    const typename Vec4<R>::T st = a.xv[i];
+    const typename Vec2<R>::T cu = a.ut[i];
+    double w[4], wu[4], wt[4], du, dt;
+    int ring, spoke, ic[4];
+    cic_grad((double)cu.x, (double)cu.y, a.nr, a.na, ring, spoke, w, wu, wt);
+    corners_rs(ring, spoke, a.na, ic);
+    tl_dut((double)st.x, (double)st.y, a.d[p], a.d[n + p], a.inv_alpha, a.tscale, du, dt);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (ic[q] < a.mesh) atomicAdd(&a.rho[ic[q]], a.pmass * (wu[q] * du + wt[q] * dt));
+}
+
+struct TlSolveArgs {
+    const double *rho;                // [mesh]
+    double *m;                        // [2 (cos, sin)][ng][nr]
+    double *fx, *fy;                  // [mesh]
+    const double *G;                  // FP64 tables [3][ng][nr][nr]
+    const double *twd;
+    const int *lg, *ranges;
+    int nr, na, ng, mmax;
+};
+
+// block per source ring: the forward transform of the perturbed density
+// over the active harmonics (the transpose of adj_rbar_kernel's synthesis):
+// Mc[m] = sum_j rho_j C_m(j), Ms[m] = -sum_j rho_j sin(2 pi m j / na), C_m
+// = 1 (m = 0), cos (2m < na), (-1)^j (2m = na); zero off the source range
+__global__ __launch_bounds__(GH_BLOCK) void tl_ana_kernel(TlSolveArgs a) {
+    __shared__ double row[ADJ_MAX_NA];
+    const int is = blockIdx.x + 1, tid = threadIdx.x, na = a.na, nr = a.nr, ng = a.ng;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int jrs = a.ranges[0], krs = a.ranges[1];
+    const bool live = jrs >= 1 && is >= jrs && is <= krs;
+    if (live)
+        for (int j = tid; j < na; j += GH_BLOCK) row[j] = a.rho[(size_t)(is - 1) * na + j];
+    __syncthreads();
+    for (int m = wave; m < ng; m += GH_BLOCK / 64) {
+        double c = 0.0, s = 0.0;
+        if (live && !a.lg[m]) {
+            for (int j = lane; j < na; j += 64) {
+                const int k = (int)(((long long)m * j) % na);
+                c += row[j] * a.twd[2 * k];
+                s += row[j] * a.twd[2 * k + 1];
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                c += __shfl_xor(c, o, 64);
+                s += __shfl_xor(s, o, 64);
+            }
+        }
+        if (lane == 0) {
+            const bool sine = m >= 1 && m < a.mmax - 1 && 2 * m < na;
+            a.m[(size_t)m * nr + is - 1] = c;
+            a.m[((size_t)ng + m) * nr + is - 1] = sine ? -s : 0.0;
+        }
+    }
+}
+
+// block per field ring: the Green's contraction of the coefficients (F_R:
+// G_R (Mc, Ms), F_phi: G_phi (Ms, -Mc)), the synthesis over the active
+// harmonics and polcat's rotation to (F_x, F_y) -- the transposes of
+// adj_rbar_kernel's contraction and adj_wbar_kernel; zero off the field range
+__global__ __launch_bounds__(GH_BLOCK) void tl_syn_kernel(TlSolveArgs a) {
+    __shared__ double wc[2][64], ws[2][64];
+    const int ir = blockIdx.x + 1, tid = threadIdx.x, na = a.na, nr = a.nr, ng = a.ng;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int jrs = a.ranges[0], krs = a.ranges[1], jrf = a.ranges[2], krf = a.ranges[3];
+    const bool live = jrs >= 1 && ir >= jrf && ir <= krf;
+    const int maz = min(a.mmax - 2, ng - 1);
+    for (int task = wave; task < 2 * ng; task += GH_BLOCK / 64) {
+        const int type = task / ng, m = task - type * ng;
+        const bool valid = live && !a.lg[m] && (type == 0 || (m >= 1 && m <= maz));
+        double c = 0.0, s = 0.0;
+        if (valid) {
+            const double *g = a.G + (((size_t)type * ng + m) * nr + (ir - 1)) * nr;
+            const double *mc = a.m + (size_t)m * nr, *ms = a.m + ((size_t)ng + m) * nr;
+            for (int is = jrs + lane; is <= krs; is += 64) {
+                c += g[is - 1] * mc[is - 1];
+                s += g[is - 1] * ms[is - 1];
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                c += __shfl_xor(c, o, 64);
+                s += __shfl_xor(s, o, 64);
+            }
+        }
+        if (lane == 0) {
+            // F_R: (wc, ws) = G (Mc, Ms); F_phi: (wc, ws) = G (Ms, -Mc)
+            double vc = type == 0 ? c : s, vs = type == 0 ? s : -c;
+            // synthesis weights: 1/na (m = 0 or 2m = na), 2/na (cos) and
+            // -2/na (sin) otherwise
+            const double al = (m == 0 || 2 * m >= na) ? 1.0 / na : 2.0 / na;
+            const double be = (m >= 1 && 2 * m < na) ? -2.0 / na : 0.0;
+            wc[type][m] = valid ? al * vc : 0.0;
+            ws[type][m] = valid ? be * vs : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < na; j += GH_BLOCK) {
+        double g0 = 0.0, g1 = 0.0;
+        if (live)
+            for (int m = 0; m < ng; m++) {
+                const int k = (int)(((long long)m * j) % na);
+                const double cm = a.twd[2 * k], sm = a.twd[2 * k + 1];
+                g0 += wc[0][m] * cm + ws[0][m] * sm;
+                g1 += wc[1][m] * cm + ws[1][m] * sm;
+            }
+        const double cj = a.twd[2 * j], sj = a.twd[2 * j + 1];
+        a.fx[(size_t)(ir - 1) * na + j] = g0 * cj - g1 * sj;
+        a.fy[(size_t)(ir - 1) * na + j] = g0 * sj + g1 * cj;
+    }
+}
+
+// da from the perturbed field, the weights' derivative against the forward
+// field and the halo's Jacobian; dv += da, dx += dv (off-grid: da = 0)
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void tl_push_kernel(TlArgs<R> a) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const long long n = a.n, p = a.id[i];
+    double dx = a.d[p], dy = a.d[n + p], dvx = a.d[2 * n + p], dvy = a.d[3 * n + p];
+    if (a.zone[i] == 1) {
+        const typename Vec4<R>::T st = a.xv[i];
+        const typename Vec2<R>::T cu = a.ut[i];
+        const double x = st.x, y = st.y;
+        double w[4], wu[4], wt[4], du, dt;
+        int ring, spoke, ic[4];
+        cic_grad((double)cu.x, (double)cu.y, a.nr, a.na, ring, spoke, w, wu, wt);
+        corners_rs(ring, spoke, a.na, ic);
+        tl_dut(x, y, dx, dy, a.inv_alpha, a.tscale, du, dt);
+        double ax = 0.0, ay = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (ic[q] < a.mesh) {
+                const typename Vec2<R>::T f = a.fld[ic[q]];
+                const double dw = wu[q] * du + wt[q] * dt;
+                ax += w[q] * a.fx[ic[q]] + dw * (double)f.x;
+                ay += w[q] * a.fy[ic[q]] + dw * (double)f.y;
+            }
+        const double r2 = x * x + y * y, r = sqrt(r2);
+        if (a.nfr > 0 && r > 0.0) {
+            const double f = fixforce(r, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val, a.fr_der);
+            const double fp = fixforce_deriv(r, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val, a.fr_der);
+            const double xd = x * dx + y * dy;
+            // d/dx of f(r) x / r applied to (dx, dy)
+            ax += fp * xd * x / r2 + f * (dx / r - xd * x / (r2 * r));
+            ay += fp * xd * y / r2 + f * (dy / r - xd * y / (r2 * r));
+        }
+        dvx += ax;
+        dvy += ay;
+    }
+    a.d[p] = dx + dvx;
+    a.d[n + p] = dy + dvy;
+    a.d[2 * n + p] = dvx;
+    a.d[3 * n + p] = dvy;
+}
+
 }  // namespace gh
 
 using namespace gh;
@@ -561,6 +766,11 @@ void adj_free(galhip_ctx *c) {
     dfree(c->fd_der);
     dfree(c->adj_tmp);
     dfree(c->adj_id);
+    dfree(c->tl_d);
+    dfree(c->tl_rho);
+    dfree(c->tl_f);
+    dfree(c->tl_m);
+    c->tl_n = 0;
     for (auto &s : c->adj_slots) {
         dfree(s.xv);
         dfree(s.ut);
@@ -961,5 +1171,143 @@ extern "C" int galhip_adj_get_profile(galhip_ctx *c, double *ms, int64_t *steps)
     if (steps) *steps = (int64_t)k;
     for (auto e : c->adj_ev) (void)hipEventDestroy(e);
     c->adj_ev.clear();
+    return 0;
+}
+
+// ---- tangent-linear host side -------------------------------------------------------
+
+static int tl_ensure(galhip_ctx *c) {
+    if (c->na > ADJ_MAX_NA || c->ng > 64)
+        return gh_fail(GALHIP_EINVAL, "grid too large for the tangent-linear solve");
+    if (!c->tl_rho) {
+        int rc = dalloc(&c->tl_rho, sizeof(double) * (size_t)c->mesh) |
+                 dalloc(&c->tl_f, sizeof(double) * 2 * (size_t)c->mesh) |
+                 dalloc(&c->tl_m, sizeof(double) * 2 * (size_t)c->ng * c->nr);
+        if (rc) return rc;
+    }
+    if (c->tl_d && c->tl_n == c->n) return 0;
+    dfree(c->tl_d);
+    c->tl_n = 0;
+    int rc = dalloc(&c->tl_d, sizeof(double) * 4 * (size_t)(c->n > 0 ? c->n : 1));
+    if (rc) return rc;
+    c->tl_n = c->n;
+    HIPCHK(hipMemset(c->tl_d, 0, sizeof(double) * 4 * (size_t)(c->n > 0 ? c->n : 1)));
+    return 0;
+}
+
+static TlSolveArgs tl_solve_args(galhip_ctx *c) {
+    TlSolveArgs a;
+    a.rho = c->tl_rho;
+    a.m = c->tl_m;
+    a.fx = c->tl_f;
+    a.fy = c->tl_f + c->mesh;
+    a.G = c->Gd;
+    a.twd = c->twd;
+    a.lg = c->lg_d;
+    a.ranges = c->ranges;
+    a.nr = c->nr;
+    a.na = c->na;
+    a.ng = c->ng;
+    a.mmax = c->mmax;
+    return a;
+}
+
+// (F_x, F_y) = K rho in FP64, with the ranges of the last solve
+static int tl_solve(galhip_ctx *c) {
+    const TlSolveArgs a = tl_solve_args(c);
+    hipLaunchKernelGGL(tl_ana_kernel, dim3(c->nr), dim3(GH_BLOCK), 0, c->s, a);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(tl_syn_kernel, dim3(c->nr), dim3(GH_BLOCK), 0, c->s, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <typename R> static TlArgs<R> tl_args(galhip_ctx *c) {
+    TlArgs<R> a;
+    memset(&a, 0, sizeof a);
+    a.xv = (const typename Vec4<R>::T *)c->xv;
+    a.ut = (const typename Vec2<R>::T *)c->ut;
+    a.zone = c->zone;
+    a.id = c->id;
+    a.n = c->n;
+    a.d = c->tl_d;
+    a.rho = c->tl_rho;
+    a.fx = c->tl_f;
+    a.fy = c->tl_f + c->mesh;
+    a.fld = (const typename Vec2<R>::T *)c->fld2;
+    a.fr_val = c->fr_val;
+    a.fr_der = c->fr_der;
+    a.fr_dr = c->fr_dr;
+    a.fr_inv_dr = c->nfr > 0 ? 1.0 / c->fr_dr : 0.0;
+    a.nfr = c->nfr;
+    a.pmass = c->pmass;
+    a.inv_alpha = 1.0 / c->alpha;
+    a.tscale = (double)c->na / (2.0 * 3.141592653589793238462643);
+    a.nr = c->nr;
+    a.na = c->na;
+    a.mesh = c->mesh;
+    return a;
+}
+
+extern "C" int galhip_tl_set(galhip_ctx *c, const double *d) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!d) return gh_fail(GALHIP_EINVAL, "null argument");
+    rc = tl_ensure(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->n) HIPCHK(hipMemcpy(c->tl_d, d, sizeof(double) * 4 * (size_t)c->n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int galhip_tl_get(galhip_ctx *c, double *d) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!d) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (!c->tl_d || c->tl_n != c->n) return gh_fail(GALHIP_EINVAL, "no tangent-linear state set");
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->n) HIPCHK(hipMemcpy(d, c->tl_d, sizeof(double) * 4 * (size_t)c->n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+template <typename R> static int tl_step_impl(galhip_ctx *c) {
+    const long long nb = (c->n + GH_BLOCK - 1) / GH_BLOCK;
+    HIPCHK(hipMemsetAsync(c->tl_rho, 0, sizeof(double) * (size_t)c->mesh, c->s));
+    const TlArgs<R> a = tl_args<R>(c);
+    if (nb > 0) {
+        hipLaunchKernelGGL(tl_rho_kernel<R>, dim3((unsigned)nb), dim3(GH_BLOCK), 0, c->s, a);
+        HIPCHK(hipGetLastError());
+    }
+    int rc = tl_solve(c);
+    if (rc) return rc;
+    if (nb > 0) {
+        hipLaunchKernelGGL(tl_push_kernel<R>, dim3((unsigned)nb), dim3(GH_BLOCK), 0, c->s, a);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int galhip_tl_step(galhip_ctx *c) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!c->have_field) return gh_fail(GALHIP_ERANGE, "tangent-linear step needs the step's field (findf)");
+    if (!c->have_green) return gh_fail(GALHIP_EINVAL, "Green function not set");
+    if (!c->tl_d || c->tl_n != c->n) return gh_fail(GALHIP_EINVAL, "no tangent-linear state set");
+    return c->prec == GALHIP_FP32 ? tl_step_impl<float>(c) : tl_step_impl<double>(c);
+}
+
+extern "C" int galhip_tl_field(galhip_ctx *c, const double *rho, double *f) {
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!rho || !f) return gh_fail(GALHIP_EINVAL, "null argument");
+    if (!c->have_green) return gh_fail(GALHIP_EINVAL, "Green function not set");
+    rc = tl_ensure(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(c->tl_rho, rho, sizeof(double) * (size_t)c->mesh, hipMemcpyHostToDevice));
+    rc = tl_solve(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(f, c->tl_f, sizeof(double) * 2 * (size_t)c->mesh, hipMemcpyDeviceToHost));
     return 0;
 }

@@ -101,6 +101,11 @@ struct galhip_ctx {
     unsigned *adj_id;
     double *fd_val, *fd_der, fd_dr;
     int nfd;
+    // tangent-linear (forward-mode) state: perturbations [4][n] of
+    // (x, y, vx, vy) in load order, the perturbed density [mesh], its
+    // field [2][mesh] and the solve's coefficients [2][ng][nr]
+    double *tl_d, *tl_rho, *tl_f, *tl_m;
+    long long tl_n;
     struct AdjSlot {
         void *xv, *ut;
         uint8_t *zone;

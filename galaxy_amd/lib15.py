@@ -323,6 +323,30 @@ class P2DGrid:
         self._c(self._lib.galhip_adj_field(self._h, _ptr(g), _ptr(rb)), "adj_field")
         return rb
 
+    # -- tangent-linear (forward mode) ---------------------------------------------
+    def tl_set(self, d):
+        """Perturbation [4][n] of (x, y, vx, vy), load order, FP64."""
+        d = np.ascontiguousarray(d, np.float64)
+        assert d.shape == (4, self.n)
+        self._c(self._lib.galhip_tl_set(self._h, _ptr(d)), "tl_set")
+
+    def tl_get(self):
+        d = np.zeros((4, self.n), np.float64)
+        self._c(self._lib.galhip_tl_get(self._h, _ptr(d)), "tl_get")
+        return d
+
+    def tl_step(self):
+        """Carry the perturbation through the coming step (call after findf,
+        before step)."""
+        self._c(self._lib.galhip_tl_step(self._h), "tl_step")
+
+    def tl_field(self, rho):
+        """K: rho[mesh] -> (F_x, F_y)[2][mesh] (FP64), the last solve's ranges."""
+        rho = np.ascontiguousarray(rho, np.float64)
+        f = np.zeros((2, self.mesh), np.float64)
+        self._c(self._lib.galhip_tl_field(self._h, _ptr(rho), _ptr(f)), "tl_field")
+        return f
+
     # -- group level ---------------------------------------------------------------
     def weight(self, x, y, old=True):
         x = np.ascontiguousarray(x, self.dtype)

@@ -435,6 +435,23 @@ int galhip_adj_field(galhip_ctx *ctx, const double *g, double *rbar);
  * the gather time (milliseconds) over the steps since the last call. */
 int galhip_adj_get_profile(galhip_ctx *ctx, double *ms, int64_t *steps);
 
+/* ---- tangent-linear (forward mode) ---- *
+ * No counterpart in the reference.  The derivative of the same step map as
+ * the adjoint's (one p2d grid, one zone, one sector, cells, ranges and the
+ * off-grid choice held fixed) applied to a perturbation, in FP64 along the
+ * forward run: an independent reference for the adjoint's gradients
+ * (<J d, lam> = <d, J^T lam>).  galhip_tl_set / galhip_tl_get: the
+ * perturbation d[4][n] of (x, y, vx, vy) in load order (re-sorts leave it
+ * in place).  galhip_tl_step carries it through the coming step: call it
+ * after findf and before step, at the state the step starts from.
+ * galhip_tl_field: the field solve's own map, rho[mesh] -> (F_x, F_y)
+ * f[2][mesh] in FP64 with the ring ranges of the last findf (equal to
+ * galhip_findf on the same density). */
+int galhip_tl_set(galhip_ctx *ctx, const double *d);
+int galhip_tl_get(galhip_ctx *ctx, double *d);
+int galhip_tl_step(galhip_ctx *ctx);
+int galhip_tl_field(galhip_ctx *ctx, const double *rho, double *f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -445,5 +445,25 @@ module galhip_f
        real(c_double), intent(out) :: ms(2)
        integer(c_int64_t), intent(out) :: steps
      end function galhip_adj_get_profile
+     integer(c_int) function galhip_tl_set(ctx, d) bind(c, name='galhip_tl_set')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: d(*)
+     end function galhip_tl_set
+     integer(c_int) function galhip_tl_get(ctx, d) bind(c, name='galhip_tl_get')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: d(*)
+     end function galhip_tl_get
+     integer(c_int) function galhip_tl_step(ctx) bind(c, name='galhip_tl_step')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function galhip_tl_step
+     integer(c_int) function galhip_tl_field(ctx, rho, f) bind(c, name='galhip_tl_field')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: rho(*)
+       real(c_double), intent(out) :: f(*)
+     end function galhip_tl_field
   end interface
 end module galhip_f
