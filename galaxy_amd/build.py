@@ -8,6 +8,13 @@ Flags:
                           IEEE FP32 division and square root
   -target-feature -packed-fp32-ops
                           no packed FP32 instructions
+  -amdgpu-atomic-optimizer-strategy=None
+                          atomics to one shared address stay single
+                          instructions: the kernels reduce within the
+                          wavefront themselves (DPP, ballots) and issue
+                          block-level atomics from one lane, where the
+                          compiler's default rewrite added a loop over the
+                          active lanes to every such atomic
 """
 import os
 import subprocess
@@ -25,6 +32,7 @@ FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
     "-Wno-unused-result", "-I", os.path.join(ROOT, "include"),
 ]
 

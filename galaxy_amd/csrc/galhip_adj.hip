@@ -38,11 +38,11 @@ template <typename R> struct AdjArgs {
     double *gx, *gy;                  // [mesh]
     const double *rbar;               // [mesh]
     const typename Vec2<R>::T *fld;   // forward (F_x, F_y)
-    const double *fr_val, *fr_der;    // fixed force table
-    double fr_dr, fr_inv_dr;
+    const double4 *fr_c;              // fixed force table (coefficients)
+    double fr_inv_dr;
     int nfr;
-    const double *fd_val, *fd_der;    // its derivative in the model parameter
-    double fd_dr, fd_inv_dr;
+    const double4 *fd_c;              // its derivative in the model parameter
+    double fd_inv_dr;
     int nfd;
     double pmass, inv_alpha, tscale;
     int nr, na, mesh;
@@ -86,21 +86,6 @@ __device__ __forceinline__ void cic_grad(double u, double t, int nr, int na, int
     }
 }
 
-// Derivative in r of fixforce's cubic Hermite interpolant (0 past the end,
-// where the force is held at the last entry).
-__device__ __forceinline__ double fixforce_deriv(double r, int nfr, double dr, double inv_dr,
-                                                 const double *__restrict__ val,
-                                                 const double *__restrict__ der) {
-    bool past;
-    const int k = fixforce_index(r, nfr, inv_dr, past);
-    if (past) return 0.0;
-    const double t = r * inv_dr - (double)k;
-    const double d00 = 6.0 * t * t - 6.0 * t, d10 = 3.0 * t * t - 4.0 * t + 1.0;
-    const double d01 = -6.0 * t * t + 6.0 * t, d11 = 3.0 * t * t - 2.0 * t;
-    return (d00 * val[k] + d10 * dr * der[k] + d01 * val[k + 1] + d11 * dr * der[k + 1]) *
-           inv_dr;
-}
-
 // dst[k][ids[i]] = src[k][i] (storage order -> load order), k = 0..3
 __global__ __launch_bounds__(GH_BLOCK) void adj_to_load_kernel(const double *__restrict__ src,
                                                                const unsigned *__restrict__ ids,
@@ -125,16 +110,8 @@ __global__ __launch_bounds__(GH_BLOCK) void adj_to_store_kernel(const double *__
     for (int k = 0; k < 4; k++) dst[k * n + i] = src[k * n + p];
 }
 
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
+__device__ __forceinline__ int wave_min_i(int v) { return wave_min_i32(v); }
+__device__ __forceinline__ int wave_max_i(int v) { return wave_max_i32(v); }
 
 constexpr int ADJ_TILE = 1024;   // LDS window cells (x 2 components x 8 B)
 
@@ -174,8 +151,11 @@ __global__ __launch_bounds__(GH_BLOCK) void adj_scatter_kernel(AdjArgs<R> a) {
         cic_grad((double)cu.x, (double)cu.y, a.nr, na, ring, spoke, w, wu, wt);
     }
     __syncthreads();
-    // reference spoke: the first on-grid particle's
-    if (on) atomicMin(&s_first, tid);
+    // reference spoke: the first on-grid particle's (one atomic per wave)
+    {
+        const unsigned long long bm = __ballot(on);
+        if (bm && lane == __ffsll((long long)bm) - 1) atomicMin(&s_first, tid);
+    }
     __syncthreads();
     if (tid == s_first) s_sref = spoke;
     __syncthreads();
@@ -278,16 +258,15 @@ __global__ __launch_bounds__(GH_BLOCK) void adj_gather_kernel(AdjArgs<R> a) {
             gx = su * cu_ * x - sv * ct * y;
             gy = su * cu_ * y + sv * ct * x;
             if (a.nfr > 0) {
-                const double f = fixforce(r, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val, a.fr_der);
-                const double fp = fixforce_deriv(r, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val,
-                                                 a.fr_der);
+                const double f = fixforce(r, a.nfr, a.fr_inv_dr, a.fr_c);
+                const double fp = fixforce_deriv(r, a.nfr, a.fr_inv_dr, a.fr_c);
                 const double xl = x * lax + y * lay;
                 // d/dx of f(r) x / r, transposed, applied to la
                 gx += fp * xl * x / r2 + f * (lax / r - xl * x / (r2 * r));
                 gy += fp * xl * y / r2 + f * (lay / r - xl * y / (r2 * r));
             }
             if (a.nfd > 0)
-                p1 = fixforce(r, a.nfd, a.fd_dr, a.fd_inv_dr, a.fd_val, a.fd_der) *
+                p1 = fixforce(r, a.nfd, a.fd_inv_dr, a.fd_c) *
                      (x * lax + y * lay) / r;
         }
         a.lam[p] += gx;
@@ -571,8 +550,8 @@ template <typename R> struct TlArgs {
     double *rho;                      // [mesh] perturbed density
     const double *fx, *fy;            // [mesh] its field
     const typename Vec2<R>::T *fld;   // forward (F_x, F_y)
-    const double *fr_val, *fr_der;
-    double fr_dr, fr_inv_dr;
+    const double4 *fr_c;
+    double fr_inv_dr;
     int nfr;
     double pmass, inv_alpha, tscale;
     int nr, na, mesh;
@@ -734,8 +713,8 @@ __global__ __launch_bounds__(GH_BLOCK) void tl_push_kernel(TlArgs<R> a) {
             }
         const double r2 = x * x + y * y, r = sqrt(r2);
         if (a.nfr > 0 && r > 0.0) {
-            const double f = fixforce(r, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val, a.fr_der);
-            const double fp = fixforce_deriv(r, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val, a.fr_der);
+            const double f = fixforce(r, a.nfr, a.fr_inv_dr, a.fr_c);
+            const double fp = fixforce_deriv(r, a.nfr, a.fr_inv_dr, a.fr_c);
             const double xd = x * dx + y * dy;
             // d/dx of f(r) x / r applied to (dx, dy)
             ax += fp * xd * x / r2 + f * (dx / r - xd * x / (r2 * r));
@@ -762,8 +741,7 @@ void adj_free(galhip_ctx *c) {
     dfree(c->adj_rbar);
     dfree(c->adj_wbar);
     dfree(c->adj_part);
-    dfree(c->fd_val);
-    dfree(c->fd_der);
+    dfree(c->fd_c);
     dfree(c->adj_tmp);
     dfree(c->adj_id);
     dfree(c->tl_d);
@@ -1019,14 +997,13 @@ extern "C" int galhip_set_fixforce_dpar(galhip_ctx *c, int32_t n, double dr, con
     if (n != 0 && (n < 2 || !(dr > 0.0) || !val || !der))
         return gh_fail(GALHIP_EINVAL, "galhip_set_fixforce_dpar: bad table");
     HIPCHK(hipStreamSynchronize(c->s));
-    dfree(c->fd_val);
-    dfree(c->fd_der);
+    dfree(c->fd_c);
     c->nfd = 0;
     if (n == 0) return 0;
-    int rc = dalloc(&c->fd_val, sizeof(double) * n) | dalloc(&c->fd_der, sizeof(double) * n);
+    const std::vector<double4> cf = radtab_coeffs(n, dr, val, der);
+    int rc = dalloc(&c->fd_c, sizeof(double4) * n);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(c->fd_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->fd_der, der, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fd_c, cf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
     c->nfd = n;
     c->fd_dr = dr;
     return 0;
@@ -1073,14 +1050,10 @@ template <typename R> static AdjArgs<R> adj_args(galhip_ctx *c) {
     a.gy = c->adj_g + c->mesh;
     a.rbar = c->adj_rbar;
     a.fld = (const typename Vec2<R>::T *)c->fld2;
-    a.fr_val = c->fr_val;
-    a.fr_der = c->fr_der;
-    a.fr_dr = c->fr_dr;
+    a.fr_c = c->fr_c;
     a.fr_inv_dr = c->nfr > 0 ? 1.0 / c->fr_dr : 0.0;
     a.nfr = c->nfr;
-    a.fd_val = c->fd_val;
-    a.fd_der = c->fd_der;
-    a.fd_dr = c->fd_dr;
+    a.fd_c = c->fd_c;
     a.fd_inv_dr = c->nfd > 0 ? 1.0 / c->fd_dr : 0.0;
     a.nfd = c->nfd;
     a.pmass = c->pmass;
@@ -1235,9 +1208,7 @@ template <typename R> static TlArgs<R> tl_args(galhip_ctx *c) {
     a.fx = c->tl_f;
     a.fy = c->tl_f + c->mesh;
     a.fld = (const typename Vec2<R>::T *)c->fld2;
-    a.fr_val = c->fr_val;
-    a.fr_der = c->fr_der;
-    a.fr_dr = c->fr_dr;
+    a.fr_c = c->fr_c;
     a.fr_inv_dr = c->nfr > 0 ? 1.0 / c->fr_dr : 0.0;
     a.nfr = c->nfr;
     a.pmass = c->pmass;

@@ -27,8 +27,8 @@ template <typename R> hipError_t launch_gather(const GatherArgs<R> &, hipStream_
 template <typename R> hipError_t launch_ut(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_tmcent(const PushArgs<R> &, R, hipStream_t);
 template <typename R>
-hipError_t launch_integrals(const PushArgs<R> &, const R *, const double *, const double *,
-                            int, double, int, double *, double *, hipStream_t);
+hipError_t launch_integrals(const PushArgs<R> &, const R *, const double4 *, int, double, int,
+                            double *, double *, hipStream_t);
 template <typename R>
 hipError_t launch_key(const typename Vec2<R>::T *, const uint8_t *, long long,
                       int, int, int, int, unsigned, unsigned *, unsigned *,
@@ -278,11 +278,11 @@ extern "C" int galhip_free(galhip_ctx *c) {
     free_particles(c);
     for (int z = 0; z < GH_MAXZ; z++) { dfree(c->mfx[z]); dfree(c->moz[z]); }
     dfree(c->jrad); dfree(c->krad); dfree(c->ranges); dfree(c->lg_d); dfree(c->counters);
-    dfree(c->fp_val); dfree(c->fp_der);
+    dfree(c->fp_c);
     dfree(c->rho); dfree(c->coef); dfree(c->fld2); dfree(c->pot); dfree(c->tw); dfree(c->twd);
     if (c->G != (void *)c->Gd) dfree(c->G);
     dfree(c->Gd);
-    dfree(c->fr_val); dfree(c->fr_der);
+    dfree(c->fr_c);
     dfree(c->m_qbits_d); dfree(c->m_pmass_d); dfree(c->blk_d);
     dfree(c->gsp); dfree(c->wpre);
     drop_graphs(c);
@@ -353,16 +353,15 @@ extern "C" int galhip_set_fixforce(galhip_ctx *c, int32_t n, double dr,
                                    const double *val, const double *der) {
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
-    dfree(c->fr_val);
-    dfree(c->fr_der);
+    dfree(c->fr_c);
     c->nfr = 0;
     if (n <= 0) return 0;
     if (n < 2 || !(dr > 0.0) || !val || !der)
         return gh_fail(GALHIP_EINVAL, "fixed force table needs n >= 2, dr > 0");
-    int rc = dalloc(&c->fr_val, sizeof(double) * n) | dalloc(&c->fr_der, sizeof(double) * n);
+    const std::vector<double4> cf = radtab_coeffs(n, dr, val, der);
+    int rc = dalloc(&c->fr_c, sizeof(double4) * n);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(c->fr_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->fr_der, der, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fr_c, cf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
     c->nfr = n;
     c->fr_dr = dr;
     return 0;
@@ -373,16 +372,15 @@ extern "C" int galhip_set_fixpot(galhip_ctx *c, int32_t n, double dr,
                                  const double *val, const double *der) {
     if (!c) return gh_fail(GALHIP_EINVAL, "null context");
     HIPCHK(hipStreamSynchronize(c->s));
-    dfree(c->fp_val);
-    dfree(c->fp_der);
+    dfree(c->fp_c);
     c->nfp = 0;
     if (n <= 0) return 0;
     if (n < 2 || !(dr > 0.0) || !val || !der)
         return gh_fail(GALHIP_EINVAL, "fixed potential table needs n >= 2, dr > 0");
-    int rc = dalloc(&c->fp_val, sizeof(double) * n) | dalloc(&c->fp_der, sizeof(double) * n);
+    const std::vector<double4> cf = radtab_coeffs(n, dr, val, der);
+    int rc = dalloc(&c->fp_c, sizeof(double4) * n);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(c->fp_val, val, sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->fp_der, der, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fp_c, cf.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
     c->nfp = n;
     c->fp_dr = dr;
     return 0;
@@ -690,9 +688,7 @@ template <typename R> static PushArgs<R> push_args(galhip_ctx *c) {
     a.jrad = c->jrad;
     a.krad = c->krad;
     a.counters = c->counters;
-    a.fr_val = c->fr_val;
-    a.fr_der = c->fr_der;
-    a.fr_dr = c->fr_dr;
+    a.fr_c = c->fr_c;
     a.fr_inv_dr = c->nfr > 0 ? 1.0 / c->fr_dr : 0.0;
     a.nfr = c->nfr;
     a.fixforce = c->nfr > 0;
@@ -1031,7 +1027,7 @@ template <typename R> static int integrals_impl(galhip_ctx *c, int centred, doub
         for (int m = 0; m < c->nmem && e == hipSuccess; m++) {
             PushArgs<R> a = member_args<R>(c, m);
             const R *pot = c->pot_valid ? (const R *)c->pot + (size_t)m * c->mesh : nullptr;
-            e = launch_integrals<R>(a, pot, c->fp_val, c->fp_der, c->nfp, c->fp_dr, centred,
+            e = launch_integrals<R>(a, pot, c->fp_c, c->nfp, c->fp_dr, centred,
                                     part, dout + 9 * m, c->s);
         }
         if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * 9 * c->nmem,
@@ -1459,10 +1455,10 @@ extern "C" int galhip_ens_restrt(galhip_ctx *c, const int64_t *count, const void
     c->tcent = 0;
     int rc = ensure_particles(c, n);
     if (rc) return rc;
-    // member ranges, push blocks of at most GH_BLOCK x 4 particles that never
+    // member ranges, push blocks of at most GH_PUSH_PER particles that never
     // straddle two members, fixed-point scales as a lone disc of that size
     // would have them
-    const int per = GH_BLOCK * 4;
+    const int per = GH_PUSH_PER;
     std::vector<int4> blk;
     c->m_off.assign(c->nmem + 1, 0);
     for (int m = 0; m < c->nmem; m++) {

@@ -51,8 +51,12 @@ struct galhip_ctx {
     long long *counters;
     void *rho, *coef, *fld2, *pot, *G, *tw;
     double *Gd, *twd;
-    double *fr_val, *fr_der, fr_dr;
-    double *fp_val, *fp_der, fp_dr;   // fixed potential table (diagnostics)
+    // fixed radial tables as cubic coefficients per interval (radtab_coeffs):
+    // the force, the potential (diagnostics)
+    double4 *fr_c;
+    double fr_dr;
+    double4 *fp_c;
+    double fp_dr;
     int nfp;
     int pot_valid;                    // the last solve formed the potential
     int nfr, qbits, ncoef;
@@ -99,7 +103,8 @@ struct galhip_ctx {
     // whose load-order index is adj_id
     long long order_ver, adj_ver;
     unsigned *adj_id;
-    double *fd_val, *fd_der, fd_dr;
+    double4 *fd_c;                    // parameter derivative of the fixed force
+    double fd_dr;
     int nfd;
     // tangent-linear (forward-mode) state: perturbations [4][n] of
     // (x, y, vx, vy) in load order, the perturbed density [mesh], its
@@ -131,6 +136,23 @@ struct galhip_ctx {
 };
 
 static size_t esize(const galhip_ctx *c) { return c->prec == GALHIP_FP64 ? 8 : 4; }
+
+// The cubic coefficients of the Hermite interpolant of (val, der) on
+// r_k = k dr, interval by interval (the device's fixed-table layout; see
+// fixforce_at), with the constant val[n - 1] as entry n - 1.
+static std::vector<double4> radtab_coeffs(int n, double dr, const double *val,
+                                          const double *der) {
+    std::vector<double4> c((size_t)n);
+    for (int k = 0; k + 1 < n; k++) {
+        const double v0 = val[k], v1 = val[k + 1];
+        const double c1 = dr * der[k];
+        const double c2 = (3.0 * (v1 - v0) - 2.0 * c1) - dr * der[k + 1];
+        const double c3 = (2.0 * (v0 - v1) + c1) + dr * der[k + 1];
+        c[k] = make_double4(v0, c1, c2, c3);
+    }
+    c[n - 1] = make_double4(val[n - 1], 0.0, 0.0, 0.0);
+    return c;
+}
 
 template <typename T> static int dalloc(T **p, size_t bytes) {
     *p = nullptr;

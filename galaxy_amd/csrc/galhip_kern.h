@@ -13,6 +13,15 @@
 
 #define GH_MAXZ 10
 #define GH_BLOCK 256
+// the p2d push kernel's launch shape: threads per block and particles per
+// thread (a block takes GH_PUSH_PER consecutive particles)
+#ifndef GH_PUSH_BLOCK
+#define GH_PUSH_BLOCK 256
+#endif
+#ifndef GH_PPT
+#define GH_PPT 4
+#endif
+#define GH_PUSH_PER (GH_PUSH_BLOCK * GH_PPT)
 // 2.d0 * pi of SRC/lib15/inc/pi.f
 #define GH_TWO_PI (2.0 * 3.141592653589793238462643)
 
@@ -29,6 +38,45 @@ __device__ __forceinline__ float2 mk2(float a, float b) { return make_float2(a, 
 __device__ __forceinline__ double2 mk2(double a, double b) { return make_double2(a, b); }
 __device__ __forceinline__ float4 mk4(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
 __device__ __forceinline__ double4 mk4(double a, double b, double c, double d) { return make_double4(a, b, c, d); }
+
+// ---- wavefront reductions -------------------------------------------------
+//
+// min / max / sum of a 32-bit integer over the 64 lanes of a wavefront, all
+// lanes active: DPP exchanges inside each 16-lane row (quad permutations
+// [1,0,3,2] and [2,3,0,1], the half-row mirror, the row mirror), then the
+// four rows' values read into scalar registers.  The result is uniform.
+// Four VALU steps and four lane reads instead of six LDS-routed shuffles.
+#define GH_DPP_QUAD_1032 0xB1
+#define GH_DPP_QUAD_2301 0x4E
+#define GH_DPP_ROW_HALF_MIRROR 0x141
+#define GH_DPP_ROW_MIRROR 0x140
+
+#define GH_DPP(v, ctrl) __builtin_amdgcn_update_dpp((v), (v), (ctrl), 0xF, 0xF, false)
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+    v = min(v, GH_DPP(v, GH_DPP_QUAD_1032));
+    v = min(v, GH_DPP(v, GH_DPP_QUAD_2301));
+    v = min(v, GH_DPP(v, GH_DPP_ROW_HALF_MIRROR));
+    v = min(v, GH_DPP(v, GH_DPP_ROW_MIRROR));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+    v = max(v, GH_DPP(v, GH_DPP_QUAD_1032));
+    v = max(v, GH_DPP(v, GH_DPP_QUAD_2301));
+    v = max(v, GH_DPP(v, GH_DPP_ROW_HALF_MIRROR));
+    v = max(v, GH_DPP(v, GH_DPP_ROW_MIRROR));
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+__device__ __forceinline__ int wave_sum_i32(int v) {
+    v = v + GH_DPP(v, GH_DPP_QUAD_1032);
+    v = v + GH_DPP(v, GH_DPP_QUAD_2301);
+    v = v + GH_DPP(v, GH_DPP_ROW_HALF_MIRROR);
+    v = v + GH_DPP(v, GH_DPP_ROW_MIRROR);
+    return (__builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16)) +
+           (__builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48));
+}
 
 // ---- per-particle geometry ------------------------------------------------
 
@@ -120,51 +168,57 @@ template <typename R>
 __device__ __forceinline__ long long quant(R w, int qbits) {
     return __double2ll_rn(ldexp((double)w, qbits));
 }
-
-// Table interval of the fixed radial force at radius r: k with
-// r_k <= r < r_k+1 (s = r * (1/dr)), clamped to the last interval.
-__device__ __forceinline__ int fixforce_index(double r, int nfr, double inv_dr,
-                                              bool &past) {
-    double s = r * inv_dr;
-    int k = (int)s;
-    past = k >= nfr - 1;
-    if (k < 0) k = 0;
-    if (past) k = nfr - 2;
-    return k;
+// FP32 weights, the same values: a normal w >= 0 is m 2^(e - 150) with the
+// 24-bit significand m and biased exponent e, so for 0 <= e - 150 + qbits
+// <= 39 the product w 2^qbits is the integer m << (e - 150 + qbits),
+// exact (nothing to round) and below 2^63.  Zero, negative and very small
+// weights (the shift out of range, as an unsigned compare) take the FP64
+// path.
+template <>
+__device__ __forceinline__ long long quant<float>(float w, int qbits) {
+    const unsigned b = __float_as_uint(w);
+    const unsigned sh = (b >> 23) + (unsigned)(qbits - 150);
+    if (sh <= 39u)
+        return (long long)((unsigned long long)((b & 0x7fffffu) | 0x800000u) << sh);
+    return __double2ll_rn(ldexp((double)w, qbits));
 }
 
-// Cubic Hermite interpolation of the tabulated fixed radial force on
-// r_k = k dr (s = r * (1/dr), the same formula as the oracle's) in the
-// interval k that fixforce_index gives.  Branch-free, so that the table
-// reads of several particles overlap: past the table end t = 1 selects
-// val[nfr - 1] exactly (the other Hermite basis values are then exact
-// zeros).  val and der hold the table from entry k0 on (a slice of it).
-__device__ __forceinline__ double fixforce_at(double r, int k, bool past, double dr,
-                                              double inv_dr,
-                                              const double *__restrict__ val,
-                                              const double *__restrict__ der,
-                                              int k0) {
-    double s = r * inv_dr;
-    double t = past ? 1.0 : s - (double)k;
-    double t2 = t * t;
-    double omt = 1.0 - t;
-    double omt2 = omt * omt;
-    double h00 = (1.0 + 2.0 * t) * omt2;
-    double h10 = t * omt2;
-    double h01 = t2 * (3.0 - 2.0 * t);
-    double h11 = t2 * (t - 1.0);
-    const int j = k - k0;
-    return h00 * val[j] + h10 * dr * der[j] + h01 * val[j + 1] +
-           h11 * dr * der[j + 1];
+// Fixed radial tables (the halo force, its potential, the force's
+// derivative in a model parameter) are the cubic Hermite interpolants of
+// (val, der) on r_k = k dr, held per interval k as the coefficients of
+//   f = c0 + t (c1 + t (c2 + t c3)),   t = r (1/dr) - k in [0, 1),
+// c0 = v_k, c1 = dr d_k, c2 = (3 (v_k+1 - v_k) - 2 c1) - dr d_k+1,
+// c3 = (2 (v_k - v_k+1) + c1) + dr d_k+1 (radtab_coeffs, on the host).
+// Entry n - 1 is the constant v_n-1, which holds past the table's end
+// (t = 0 there).  Evaluated with fused multiply-adds.
+
+// Interval of radius r: k with r_k <= r < r_k+1, n - 1 past the last.
+__device__ __forceinline__ int fixforce_index(double r, int n, double inv_dr) {
+    const int k = (int)(r * inv_dr);
+    return k >= n - 1 ? n - 1 : (k < 0 ? 0 : k);
 }
 
-__device__ __forceinline__ double fixforce(double r, int nfr, double dr,
-                                           double inv_dr,
-                                           const double *__restrict__ val,
-                                           const double *__restrict__ der) {
-    bool past;
-    const int k = fixforce_index(r, nfr, inv_dr, past);
-    return fixforce_at(r, k, past, dr, inv_dr, val, der, 0);
+// The table at r in interval k (from fixforce_index); c holds the table
+// from entry k0 on (a slice of it).
+__device__ __forceinline__ double fixforce_at(double r, int k, int n, double inv_dr,
+                                              const double4 *__restrict__ c, int k0) {
+    const double t = k == n - 1 ? 0.0 : r * inv_dr - (double)k;
+    const double4 q = c[k - k0];
+    return fma(fma(fma(q.w, t, q.z), t, q.y), t, q.x);
+}
+
+__device__ __forceinline__ double fixforce(double r, int n, double inv_dr,
+                                           const double4 *__restrict__ c) {
+    return fixforce_at(r, fixforce_index(r, n, inv_dr), n, inv_dr, c, 0);
+}
+
+// d/dr of the table at r (0 past the end, where the constant has no slope)
+__device__ __forceinline__ double fixforce_deriv(double r, int n, double inv_dr,
+                                                 const double4 *__restrict__ c) {
+    const int k = fixforce_index(r, n, inv_dr);
+    const double t = k == n - 1 ? 0.0 : r * inv_dr - (double)k;
+    const double4 q = c[k];
+    return fma(fma(3.0 * q.w, t, 2.0 * q.z), t, q.y) * inv_dr;
 }
 
 // ---- shared by the solve and particle kernels ---------------------------------
@@ -255,8 +309,8 @@ template <typename R> struct PushArgs {
     long long *mfx[GH_MAXZ];          // zone mass sums, fixed point
     int *jrad, *krad;                 // [nz] 1-based ring ranges
     long long *counters;              // [0] off-grid delta, [1] pushes
-    const double *fr_val, *fr_der;
-    double fr_dr, fr_inv_dr;
+    const double4 *fr_c;              // fixed force table (coefficients)
+    double fr_inv_dr;
     int nfr, fixforce;
     int nr, na, mesh, nz, mzone, offanal, nsect, qbits;
     UtConst k;

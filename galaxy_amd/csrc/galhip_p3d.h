@@ -15,8 +15,8 @@ template <typename R> struct P3Args {
     long long *mfx[GH_MAXZ];
     int *jrad, *krad;
     long long *counters;              // [0] off-grid count, [1] pushes
-    const double *fr_val, *fr_der;
-    double fr_dr, fr_inv_dr;
+    const double4 *fr_c;              // fixed force table (coefficients)
+    double fr_inv_dr;
     int nfr, fixforce;
     int nr, na, ngz, mesh, nz, mzone, offanal, nsect, qbits;
     int jmass;                        // 2 linear (8 corners), 3 quadratic (27)
@@ -89,8 +89,8 @@ template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &, hipStream_t
 template <typename R> hipError_t launch_p3_ut(const P3Args<R> &, hipStream_t);
 constexpr int P3_NINT = 11;   // entries of the p3d integrals
 template <typename R>
-hipError_t launch_p3_integrals(const P3Args<R> &, int with_pot, const double *fp_val,
-                               const double *fp_der, int nfp, double fp_dr, int centred,
+hipError_t launch_p3_integrals(const P3Args<R> &, int with_pot, const double4 *fp_c,
+                               int nfp, double fp_dr, int centred,
                                double *part, double *out, hipStream_t);
 template <typename R> hipError_t launch_p3_gather(const P3GatherArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_solve(const P3SolveArgs<R> &, hipStream_t);

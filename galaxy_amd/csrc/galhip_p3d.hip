@@ -227,7 +227,7 @@ __device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz, R 
             r2 = r2 + oy * oy;
             r2 = r2 + oz * oz;
             const R rr = sqrt(r2);
-            R fr = (R)fixforce((double)rr, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val, a.fr_der);
+            R fr = (R)fixforce((double)rr, a.nfr, a.fr_inv_dr, a.fr_c);
             if (fr != (R)0) {
                 fr = fr / rr;
                 sx = sx + fr * ox;
@@ -248,18 +248,17 @@ __device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz, R 
     }
 }
 
-__device__ __forceinline__ int p3_wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int p3_wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
+__device__ __forceinline__ int p3_wave_min(int v) { return wave_min_i32(v); }
+__device__ __forceinline__ int p3_wave_max(int v) { return wave_max_i32(v); }
 
-constexpr int P3_TILE = 4096;   // cells of a block's LDS deposit window
+#ifndef GH_P3_TILE
+#define GH_P3_TILE 4096
+#endif
+#ifndef GH_P3_TILE_REP
+#define GH_P3_TILE_REP 1
+#endif
+constexpr int P3_TILE = GH_P3_TILE;       // cells of a block's LDS deposit window
+constexpr int P3_TILE_REP = GH_P3_TILE_REP;   // window copies (by lane) against conflicts
 
 // Block-level fixed-point deposit.  The block's particles (sorted by cell,
 // so a block covers a few neighbouring cells) are accumulated into an LDS
@@ -337,8 +336,11 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
         if (NQ > mq) { q0 += (NQ - mq) / 2; NQ = mq; }
     }
     const int ncell = use ? NR * NP * NQ : 0;
-    for (int e = tid; e < ncell; e += P3_PBLOCK) tile[e] = 0;
+    for (int e = tid; e < ncell; e += P3_PBLOCK)
+#pragma unroll
+        for (int rep = 0; rep < P3_TILE_REP; rep++) tile[rep * P3_TILE + e] = 0;
     __syncthreads();
+    long long *const trep = tile + (lane & (P3_TILE_REP - 1)) * P3_TILE;
     if (on) {
         long long v[NW];
 #pragma unroll
@@ -355,7 +357,7 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
                     o = b + (q % 3 - 1) + ((q / 3) % 3 - 1) * NQ + (q / 9 - 1) * NP * NQ;
                 else
                     o = b + ((q & 1) ? 1 : 0) + ((q & 2) ? NQ : 0) + ((q & 4) ? NP * NQ : 0);
-                if (v[q] != 0) atomicAdd((unsigned long long *)&tile[o], (unsigned long long)v[q]);
+                if (v[q] != 0) atomicAdd((unsigned long long *)&trep[o], (unsigned long long)v[q]);
             }
         } else {
             int ic[NW];
@@ -373,7 +375,9 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
         const int base = ((r0 + dr) * a.ngz + (p0 + dp)) * a.na + q0;
         const long long *t = tile + row * NQ;
         for (int dq = lane; dq < NQ; dq += 64) {
-            const long long v = t[dq];
+            long long v = 0;
+#pragma unroll
+            for (int rep = 0; rep < P3_TILE_REP; rep++) v += t[rep * P3_TILE + dq];
             if (v != 0)
                 atomicAdd((unsigned long long *)&a.mfx[tz - 1][base + dq], (unsigned long long)v);
         }
@@ -395,7 +399,7 @@ __device__ __forceinline__ void flush_ranges(int nz, int nr, int lo, const int *
 template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
     __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
-    __shared__ long long tile[P3_TILE];
+    __shared__ long long tile[P3_TILE_REP * P3_TILE];
     const int tid = threadIdx.x;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
@@ -429,7 +433,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
     typedef typename Vec2<R>::T R2;
     __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
     __shared__ int s_dn, s_np;
-    __shared__ long long tile[P3_TILE];
+    __shared__ long long tile[P3_TILE_REP * P3_TILE];
     const int tid = threadIdx.x;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
@@ -510,10 +514,8 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
     }
     deposit3_block<R, J>(a, don, dz, dc, dw, tile, s_b, s_jr, s_kr);
     // block sums of the off-grid change and the pushes
-    for (int o = 32; o > 0; o >>= 1) {
-        dn += __shfl_xor(dn, o, 64);
-        np += __shfl_xor(np, o, 64);
-    }
+    dn = wave_sum_i32(dn);
+    np = wave_sum_i32(np);
     if ((tid & 63) == 0) {
         if (dn) atomicAdd(&s_dn, dn);
         if (np) atomicAdd(&s_np, np);
@@ -596,8 +598,7 @@ __device__ __forceinline__ double p3_wave_sumd(double v) {
 
 template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_integrals_kernel(P3Args<R> a, int with_pot,
-                                                                 const double *fp_val,
-                                                                 const double *fp_der, int nfp,
+                                                                 const double4 *fp_c, int nfp,
                                                                  double fp_dr, int centred,
                                                                  double *part) {
     typedef typename Vec4<R>::T R4;
@@ -646,8 +647,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_integrals_kernel(P3Args<R> a, in
                 v[9] = ph;
             }
             if (nfp > 0)
-                v[10] = fixforce(sqrt(x * x + y * y + zz * zz), nfp, fp_dr, 1.0 / fp_dr, fp_val,
-                                 fp_der);
+                v[10] = fixforce(sqrt(x * x + y * y + zz * zz), nfp, 1.0 / fp_dr, fp_c);
         }
     }
 #pragma unroll
@@ -1211,16 +1211,16 @@ template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &p, hipSt
 }
 
 template <typename R>
-hipError_t launch_p3_integrals(const P3Args<R> &a, int with_pot, const double *fp_val,
-                               const double *fp_der, int nfp, double fp_dr, int centred,
+hipError_t launch_p3_integrals(const P3Args<R> &a, int with_pot, const double4 *fp_c,
+                               int nfp, double fp_dr, int centred,
                                double *part, double *out, hipStream_t s) {
     const unsigned nb = a.n > 0 ? p3_blocks(a.n) : 0;
     if (nb && a.jmass == 3)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_integrals_kernel<R, 3>), dim3(nb), dim3(P3_PBLOCK), 0, s, a,
-                           with_pot, fp_val, fp_der, nfp, fp_dr, centred, part);
+                           with_pot, fp_c, nfp, fp_dr, centred, part);
     else if (nb)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(p3_integrals_kernel<R, 2>), dim3(nb), dim3(P3_PBLOCK), 0, s, a,
-                           with_pot, fp_val, fp_der, nfp, fp_dr, centred, part);
+                           with_pot, fp_c, nfp, fp_dr, centred, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(p3_integrals_sum_kernel, dim3(1), dim3(P3_BLOCK), 0, s, part, (int)nb,
@@ -1271,8 +1271,8 @@ hipError_t launch_p3_green(double *G, const double *twd, const double *twzd, con
     template hipError_t launch_p3_push<R>(const P3Args<R> &, hipStream_t);            \
     template hipError_t launch_p3_tmcent<R>(const P3Args<R> &, hipStream_t);          \
     template hipError_t launch_p3_ut<R>(const P3Args<R> &, hipStream_t);              \
-    template hipError_t launch_p3_integrals<R>(const P3Args<R> &, int, const double *,  \
-                                               const double *, int, double, int,       \
+    template hipError_t launch_p3_integrals<R>(const P3Args<R> &, int, const double4 *, \
+                                               int, double, int,                       \
                                                double *, double *, hipStream_t);       \
     template hipError_t launch_p3_gather<R>(const P3GatherArgs<R> &, hipStream_t);    \
     template hipError_t launch_p3_solve<R>(const P3SolveArgs<R> &, hipStream_t);      \

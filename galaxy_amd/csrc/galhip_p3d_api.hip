@@ -427,9 +427,7 @@ template <typename R> static P3Args<R> p3_args(galhip_ctx *c) {
     a.jrad = c->jrad;
     a.krad = c->krad;
     a.counters = c->counters;
-    a.fr_val = c->fr_val;
-    a.fr_der = c->fr_der;
-    a.fr_dr = c->fr_dr;
+    a.fr_c = c->fr_c;
     a.fr_inv_dr = c->nfr > 0 ? 1.0 / c->fr_dr : 0.0;
     a.nfr = c->nfr;
     a.fixforce = c->nfr > 0;
@@ -559,7 +557,7 @@ static int p3_integrals_impl(galhip_ctx *c, int centred, double *out, double *bu
         rc = dalloc(&part, sizeof(double) * P3_NINT * (nb > 0 ? nb : 1)) |
              dalloc(&dout, sizeof(double) * P3_NINT);
     if (!rc) {
-        hipError_t e = launch_p3_integrals<R>(p3_args<R>(c), c->pot_valid, c->fp_val, c->fp_der,
+        hipError_t e = launch_p3_integrals<R>(p3_args<R>(c), c->pot_valid, c->fp_c,
                                               c->nfp, c->fp_dr, centred, part, dout, c->s);
         if (e == hipSuccess)
             e = hipMemcpyAsync(out, dout, sizeof(double) * P3_NINT, hipMemcpyDeviceToHost, c->s);

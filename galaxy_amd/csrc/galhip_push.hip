@@ -36,9 +36,6 @@
 namespace gh {
 
 // Launch shape (tuned on MI355X; see the DESIGN notes of the deposit):
-#ifndef GH_PPT
-#define GH_PPT 4
-#endif
 #ifndef GH_TILE_CAP
 #define GH_TILE_CAP 1024
 #endif
@@ -47,6 +44,14 @@ namespace gh {
 #endif
 
 constexpr int PPT = GH_PPT;            // particles per thread
+constexpr int PB = GH_PUSH_BLOCK;      // threads per block
+// minimum wavefronts per SIMD of push_kernel (launch bounds), FP32 and FP64
+#ifndef GH_PUSH_WAVES32
+#define GH_PUSH_WAVES32 5
+#endif
+#ifndef GH_PUSH_WAVES64
+#define GH_PUSH_WAVES64 4
+#endif
 constexpr int TILE_CAP = GH_TILE_CAP;  // LDS tile, cells of a ring x spoke window
 constexpr int TILE_REP = GH_TILE_REP;  // replicas (lane & (REP-1)) against conflicts
 constexpr int FF_CAP = 256;            // staged fixed-force table entries
@@ -60,10 +65,12 @@ __device__ __forceinline__ R tsfac(const PushArgs<R> &a, int iz) {
 // zoneno for a position of known radius (zoneno.f:40-55)
 template <typename R>
 __device__ __forceinline__ int zone_of(const PushArgs<R> &a, R rad) {
+    // fully unrolled over the zone limit with a uniform guard, so that the
+    // zone radii are read from the argument block at fixed offsets
     int z = 1;
-    if (a.nz > 1)
-        for (int i = 1; i <= a.nz; i++)
-            if (rad >= a.zbr[i - 1]) z = i;
+#pragma unroll
+    for (int i = 2; i <= GH_MAXZ; i++)
+        if (i <= a.nz && rad >= a.zbr[i - 1]) z = i;
     return z;
 }
 
@@ -99,16 +106,8 @@ __device__ __forceinline__ long long wave_sum64(long long v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
+__device__ __forceinline__ int wave_min(int v) { return wave_min_i32(v); }
+__device__ __forceinline__ int wave_max(int v) { return wave_max_i32(v); }
 
 
 // ENS: an ensemble launch, the block's particles and member from a.blk.
@@ -117,7 +116,7 @@ template <typename R, bool ENS, bool COMP>
 // Wavefronts per SIMD, measured: FP32 at least 5 (96 VGPRs, no spill; 4
 // waves at the unconstrained 102 VGPRs is 8 % slower, 6 waves spill); FP64
 // at least 4 (128 VGPRs, a few spills, 4 % faster than 3 waves without).
-__global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(PushArgs<R> a) {
+__global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVES64) void push_kernel(PushArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     typedef typename Vec4<R>::T R4;
     __shared__ long long tile[TILE_REP][TILE_CAP];
@@ -126,12 +125,14 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     __shared__ int s_rmin, s_rmax, s_dmin, s_dmax;
     __shared__ int s_fref, s_frmin, s_frmax, s_fdmin, s_fdmax, s_kmin, s_kmax;
     // slice of the fixed-force table that the block's radii use
-    __shared__ double s_ffv[FF_CAP], s_ffd[FF_CAP];
+    __shared__ double4 s_ffc[FF_CAP];
+    // the logarithm and arc-tangent tables, read per particle by ut_of
+    __shared__ double s_tab[GH_MATH_TAB_N];
 
     const int tid = threadIdx.x, lane = tid & 63;
     // the block's particles and its grid (ensembles: member eb.x)
     const int4 eb = ENS ? a.blk[blockIdx.x] : make_int4(0, 0, 0, 0);
-    const long long base = ENS ? (long long)eb.y : (long long)blockIdx.x * (GH_BLOCK * PPT);
+    const long long base = ENS ? (long long)eb.y : (long long)blockIdx.x * (PB * PPT);
     const long long mo = ENS ? (long long)eb.x * a.mesh : 0;
     const R2 *const fld = a.fld + mo;
     int *const jrad = a.jrad + (ENS ? eb.x * GH_MAXZ : 0);
@@ -139,6 +140,8 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     long long *const counters = a.counters + (ENS ? eb.x * 4 : 0);
     const int qbits = ENS ? a.qbits_m[eb.x] : a.qbits;
     const int nz = a.nz, nlists = a.nz + 1, na = a.na, nr = a.nr;
+    for (int j = tid; j < GH_MATH_TAB_N; j += PB) s_tab[j] = GH_MATH_TAB[j];
+    const double *const mtab = s_tab;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
         s_kr[tid] = -1;
@@ -165,13 +168,13 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     R2 *const utb = a.ut + base;
     R2 *const lob = COMP ? a.xlo + base : nullptr;
     uint8_t *const zb = a.zone + base;
-    const int nblk = ENS ? eb.z : (int)min((long long)(GH_BLOCK * PPT), a.n - base);
+    const int nblk = ENS ? eb.z : (int)min((long long)(PB * PPT), a.n - base);
     int zin[PPT];
     bool mv[PPT];
     int any = 0;
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        const int j = k * GH_BLOCK + tid;
+        const int j = k * PB + tid;
         zin[k] = j < nblk ? (int)zb[j] : 0;
         bool off = zin[k] == nlists;
         mv[k] = zin[k] > 0 &&
@@ -190,7 +193,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     R2 cu[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        const int j = min(k * GH_BLOCK + tid, nblk - 1);
+        const int j = min(k * PB + tid, nblk - 1);
         st[k] = xvb[j];
         cu[k] = utb[j];
     }
@@ -211,13 +214,11 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     }
     R rr[PPT], fr[PPT];
     int kf[PPT];
-    bool kpast[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
         rr[k] = sqrt(st[k].x * st[k].x + st[k].y * st[k].y);
         fr[k] = 0;
-        kpast[k] = false;
-        kf[k] = a.fixforce ? fixforce_index((double)rr[k], a.nfr, a.fr_inv_dr, kpast[k]) : 0;
+        kf[k] = a.fixforce ? fixforce_index((double)rr[k], a.nfr, a.fr_inv_dr) : 0;
     }
     if (tid == 0) {
         int r = 0;
@@ -275,9 +276,9 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
         // rows past the last ring read as zero, as the global path does
         // (row, col) of element j advanced incrementally: one division per
         // thread instead of one per element
-        const int drow = GH_BLOCK / ftw, dcol = GH_BLOCK - drow * ftw;
+        const int drow = PB / ftw, dcol = PB - drow * ftw;
         int row = tid / ftw, col = tid - row * ftw;
-        for (int j = tid; j < fth * ftw; j += GH_BLOCK) {
+        for (int j = tid; j < fth * ftw; j += PB) {
             const int ring = fr0 + row;
             int spoke = fref + fd0 + col;
             row += drow;
@@ -292,14 +293,11 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
             ftile[j] = ring < nr ? fld[ring * na + spoke] : zero2<R>();
         }
     }
-    // the table entries k0 .. k1 + 1 that the block's grid particles read
-    const int k0 = s_kmin, nk = s_kmax + 2 - s_kmin;
+    // the table intervals k0 .. k1 that the block's grid particles read
+    const int k0 = s_kmin, nk = s_kmax + 1 - s_kmin;
     const bool use_ff = a.fixforce && s_kmax >= 0 && nk <= FF_CAP;
     if (use_ff)
-        for (int j = tid; j < nk; j += GH_BLOCK) {
-            s_ffv[j] = a.fr_val[k0 + j];
-            s_ffd[j] = a.fr_der[k0 + j];
-        }
+        for (int j = tid; j < nk; j += PB) s_ffc[j] = a.fr_c[k0 + j];
     __syncthreads();
     R ax[PPT], ay[PPT];
 #pragma unroll
@@ -338,32 +336,33 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
             const bool grid = gring[k] >= 0;
-            fr[k] = (R)fixforce_at((double)rr[k], grid ? kf[k] : k0, grid && kpast[k],
-                                   a.fr_dr, a.fr_inv_dr, s_ffv, s_ffd, k0);
+            fr[k] = (R)fixforce_at((double)rr[k], grid ? kf[k] : k0, a.nfr, a.fr_inv_dr,
+                                   s_ffc, k0);
         }
     } else if (a.fixforce) {
 #pragma unroll
         for (int k = 0; k < PPT; k++)
-            fr[k] = (R)fixforce_at((double)rr[k], kf[k], kpast[k], a.fr_dr,
-                                   a.fr_inv_dr, a.fr_val, a.fr_der, 0);
+            fr[k] = (R)fixforce_at((double)rr[k], kf[k], a.nfr, a.fr_inv_dr, a.fr_c, 0);
     }
 
     // deposit state kept per particle: ring, spoke, zone (0 = none), weights
     int dring[PPT], dspoke[PPT], dzone[PPT];
     R dw[PPT][4];
-    int my_pushes = 0, my_dnoff = 0;
+    // off-grid list changes: +1 leaving the grid, -1 returning
+    bool dn_plus[PPT], dn_minus[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
         dzone[k] = 0;
         dring[k] = 0;
         dspoke[k] = 0;
+        dn_plus[k] = false;
+        dn_minus[k] = false;
         if (mv[k]) {
-            const int j = k * GH_BLOCK + tid;
+            const int j = k * PB + tid;
             const int z0 = zin[k];
             const bool offlist = z0 == nlists;
             R ox = st[k].x, oy = st[k].y, ovx = st[k].z, ovy = st[k].w;
             R ax_ = ax[k], ay_ = ay[k];
-            my_pushes++;
             if (!offlist && fr[k] != (R)0) {
                 R f = fr[k] / rr[k];
                 ax_ = ax_ + f * ox;
@@ -389,7 +388,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
             if (!offlist) {
                 if (off) {
                     znew = nlists;
-                    my_dnoff++;
+                    dn_plus[k] = true;
                     if (a.offanal && z0 < nz) {
                         R tfac = tsfac(a, nz) / tsfac(a, z0);
                         chstep(tfac, ox, oy, ovx, ovy, ax_, ay_, nx, ny, nvx, nvy);
@@ -401,7 +400,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
                     }
                 }
             } else if (!off) {
-                my_dnoff--;
+                dn_minus[k] = true;
                 znew = nz;
             }
             if (a.mzone > 1 && !offlist && znew <= a.mzone) {
@@ -424,7 +423,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
             if (znew != z0) zb[j] = (uint8_t)znew;
             if (znew >= 1 && znew <= nz) {
                 R u, t;
-                ut_of(nx, ny, rnew, a.k, u, t);
+                ut_of(nx, ny, rnew, a.k, u, t, mtab);
                 utb[j] = mk2(u, t);
                 R tf = a.nsect > 1 ? (R)fmod(t, a.thmax) : t;
                 cic(u, tf, nr, na, dring[k], dspoke[k], dw[k]);
@@ -432,8 +431,20 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
             }
         }
     }
-    if (my_pushes) atomicAdd(&s_pushes, my_pushes);
-    if (my_dnoff) atomicAdd(&s_dnoff, my_dnoff);
+    // the wave's pushes and off-grid change by ballots (scalar counts),
+    // added to the block's counters by one lane
+    {
+        int wp = 0, wd = 0;
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            wp += __popcll(__ballot(mv[k]));
+            wd += __popcll(__ballot(dn_plus[k])) - __popcll(__ballot(dn_minus[k]));
+        }
+        if (lane == 0) {
+            if (wp) atomicAdd(&s_pushes, wp);
+            if (wd) atomicAdd(&s_dnoff, wd);
+        }
+    }
     // the tile's zone and reference spoke: thread 0's first deposit
     if (tid == 0) {
 #pragma unroll
@@ -488,7 +499,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
         atomicMin(&s_jr[zt - 1], r0);
         atomicMax(&s_kr[zt - 1], r1);
     }
-    for (int j = tid; j < tlen; j += GH_BLOCK) {
+    for (int j = tid; j < tlen; j += PB) {
 #pragma unroll
         for (int rep = 0; rep < TILE_REP; rep++) tile[rep][j] = 0;
     }
@@ -520,7 +531,7 @@ __global__ __launch_bounds__(GH_BLOCK, sizeof(R) == 4 ? 5 : 4) void push_kernel(
     // flush: one global atomic per touched cell
     for (int row = 0; row < th && use_tile; row++) {
         const int ring = r0 + row;
-        for (int col = tid; col < tw; col += GH_BLOCK) {
+        for (int col = tid; col < tw; col += PB) {
             const int j = row * tw + col;
             long long v = 0;
 #pragma unroll
@@ -667,8 +678,7 @@ __device__ __forceinline__ void getacc1(const PushArgs<R> &a, const typename Vec
         }
         if (a.fixforce) {
             const R rr = sqrt(st.x * st.x + st.y * st.y);
-            R fr = (R)fixforce((double)rr, a.nfr, a.fr_dr, a.fr_inv_dr, a.fr_val,
-                               a.fr_der);
+            R fr = (R)fixforce((double)rr, a.nfr, a.fr_inv_dr, a.fr_c);
             if (fr != (R)0) {
                 fr = fr / rr;
                 ax = ax + fr * st.x;
@@ -720,8 +730,7 @@ __device__ __forceinline__ double wave_sumd(double v) {
 
 template <typename R>
 __global__ __launch_bounds__(GH_BLOCK) void integrals_kernel(PushArgs<R> a, const R *pot,
-                                                             const double *fp_val,
-                                                             const double *fp_der, int nfp,
+                                                             const double4 *fp_c, int nfp,
                                                              double fp_dr, int centred,
                                                              double *part) {
     __shared__ double red[GH_BLOCK / 64][NINT];
@@ -762,7 +771,7 @@ __global__ __launch_bounds__(GH_BLOCK) void integrals_kernel(PushArgs<R> a, cons
                 v[7] = ph;
             }
             if (nfp > 0)
-                v[8] = fixforce(sqrt(x * x + y * y), nfp, fp_dr, 1.0 / fp_dr, fp_val, fp_der);
+                v[8] = fixforce(sqrt(x * x + y * y), nfp, 1.0 / fp_dr, fp_c);
         }
     }
 #pragma unroll
@@ -926,10 +935,10 @@ static void launch_push_c(const PushArgs<R> &a, hipStream_t s) {
     if (a.blk) {
         if (a.nblocks > 0)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(push_kernel<R, true, COMP>), dim3(a.nblocks),
-                               dim3(GH_BLOCK), 0, s, a);
+                               dim3(PB), 0, s, a);
     } else {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(push_kernel<R, false, COMP>),
-                           dim3(nblocks(a.n, GH_BLOCK * PPT)), dim3(GH_BLOCK), 0, s, a);
+                           dim3(nblocks(a.n, PB * PPT)), dim3(PB), 0, s, a);
     }
 }
 
@@ -971,13 +980,13 @@ hipError_t launch_gather(const GatherArgs<R> &a, hipStream_t s) {
 }
 
 template <typename R>
-hipError_t launch_integrals(const PushArgs<R> &a, const R *pot, const double *fp_val,
-                            const double *fp_der, int nfp, double fp_dr, int centred,
+hipError_t launch_integrals(const PushArgs<R> &a, const R *pot, const double4 *fp_c,
+                            int nfp, double fp_dr, int centred,
                             double *part, double *out, hipStream_t s) {
     const unsigned nb = a.n > 0 ? nblocks(a.n, GH_BLOCK) : 0;
     if (nb)
         hipLaunchKernelGGL(integrals_kernel<R>, dim3(nb), dim3(GH_BLOCK), 0, s, a, pot,
-                           fp_val, fp_der, nfp, fp_dr, centred, part);
+                           fp_c, nfp, fp_dr, centred, part);
     hipLaunchKernelGGL(integrals_sum_kernel, dim3(1), dim3(GH_BLOCK), 0, s,
                        (const double *)part, (int)nb, out);
     return hipGetLastError();
@@ -1046,9 +1055,8 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
     template hipError_t launch_gather<R>(const GatherArgs<R> &, hipStream_t); \
     template hipError_t launch_tmcent<R>(const PushArgs<R> &, R, hipStream_t); \
     template hipError_t launch_integrals<R>(const PushArgs<R> &, const R *,     \
-                                            const double *, const double *, int, \
-                                            double, int, double *, double *,   \
-                                            hipStream_t);                        \
+                                            const double4 *, int, double, int,   \
+                                            double *, double *, hipStream_t);    \
     template hipError_t launch_key<R>(const typename Vec2<R>::T *,           \
                                       const uint8_t *, long long, int, int,   \
                                       int, int, unsigned, unsigned *,         \

@@ -47,8 +47,14 @@ static inline float logf_cr(float v) { return (float)log((double)v); }
 float orc_alpha_f(int na) { return (float)(TWO_PI_D / (double)na); }
 double orc_alpha_d(int na) { return TWO_PI_D / (double)na; }
 
-/* Cubic Hermite interpolation of a tabulated radial force on r_k = k dr,
- * with s = r * (1/dr). */
+/* Cubic Hermite interpolation of a tabulated radial force on r_k = k dr
+ * (val, der = its derivative), with s = r * (1/dr) and t = s - k: the cubic
+ * written as c0 + t (c1 + t (c2 + t c3)) with c0 = v_k, c1 = dr d_k,
+ * c2 = (3 (v_k+1 - v_k) - 2 c1) - dr d_k+1, c3 = (2 (v_k - v_k+1) + c1) +
+ * dr d_k+1, evaluated by fused multiply-adds; past the last entry the
+ * constant val[nfr - 1].  (The Hermite form h00 v_k + h10 dr d_k + h01 v_k+1
+ * + h11 dr d_k+1 regrouped by powers of t; the library evaluates the same
+ * coefficients the same way, so the two agree to the bit.) */
 double orc_fixforce(double r, int nfr, double dr, const double *val,
                     const double *der)
 {
@@ -58,15 +64,11 @@ double orc_fixforce(double r, int nfr, double dr, const double *val,
     if (k >= nfr - 1) return val[nfr - 1];
     if (k < 0) k = 0;
     double t = s - (double)k;
-    double t2 = t * t;
-    double omt = 1.0 - t;
-    double omt2 = omt * omt;
-    double h00 = (1.0 + 2.0 * t) * omt2;
-    double h10 = t * omt2;
-    double h01 = t2 * (3.0 - 2.0 * t);
-    double h11 = t2 * (t - 1.0);
-    return h00 * val[k] + h10 * dr * der[k] + h01 * val[k + 1] +
-           h11 * dr * der[k + 1];
+    double v0 = val[k], v1 = val[k + 1];
+    double c1 = dr * der[k];
+    double c2 = (3.0 * (v1 - v0) - 2.0 * c1) - dr * der[k + 1];
+    double c3 = (2.0 * (v0 - v1) + c1) + dr * der[k + 1];
+    return fma(fma(fma(c3, t, c2), t, c1), t, v0);
 }
 
 /* ---------------- FFTPACK conventions, direct sums ---------------- */
