@@ -94,6 +94,9 @@ SIGNATURES = {
     "galhip_unload3": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "galhip_weight3": [_vp, _i64, _vp, _vp, _vp, _i32, _vp, _vp],
     "galhip_grdacc3": [_vp, _i64, _vp, _vp, _vp, _vp],
+    "galhip_getacc": [_vp, _i64, _vp, _vp, _vp, _vp],
+    "galhip_stpgrp": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "galhip_massgn": [_vp, _i64, _vp, _vp, _vp],
     "galhip_ensset": [C.POINTER(P2DParams), _i32, _vp, C.POINTER(_vp)],
     "galhip_ens_restrt": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "galhip_ens_unload": [_vp, _vp, _vp, _vp, _vp, _vp],

@@ -25,6 +25,10 @@ template <typename R> hipError_t launch_push(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_massgn(const PushArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_gather(const GatherArgs<R> &, hipStream_t);
 template <typename R> hipError_t launch_ut(const PushArgs<R> &, hipStream_t);
+template <typename R>
+hipError_t launch_getacc_group(const PushArgs<R> &, typename Vec2<R>::T *, hipStream_t);
+template <typename R>
+hipError_t launch_stpgrp_group(const PushArgs<R> &, const typename Vec2<R>::T *, hipStream_t);
 template <typename R> hipError_t launch_tmcent(const PushArgs<R> &, R, hipStream_t);
 template <typename R>
 hipError_t launch_integrals(const PushArgs<R> &, const R *, const double4 *, int, double, int,
@@ -1272,6 +1276,144 @@ extern "C" int galhip_grdacc(galhip_ctx *c, int64_t n, const void *x, const void
     HIPCHK(hipStreamSynchronize(c->s));
     return c->prec == GALHIP_FP32 ? gather_impl<float>(c, n, x, y, 0, nullptr, nullptr, acc)
                                   : gather_impl<double>(c, n, x, y, 0, nullptr, nullptr, acc);
+}
+
+// ---- group-level routines: getacc(jst), stpgrp(jst) + rezgrp(jst), massgn(jst) ----
+//
+// For parity work on groups of particles given by the caller: the group is
+// copied to device buffers (positions and velocities interleaved as the
+// context stores them, zones as bytes), run through the same device code as
+// the step, and copied back.  They act on the context's field, zone
+// schedule and zone mass sums as the reference's routines act on its commons.
+
+enum { GRP_GETACC, GRP_STPGRP, GRP_MASSGN };
+
+template <typename R>
+static int group_impl(galhip_ctx *c, int kind, int64_t n, void *x, void *y, void *vx, void *vy,
+                      void *acc, int32_t *iz) {
+    typedef typename Vec4<R>::T R4;
+    typedef typename Vec2<R>::T R2;
+    if (n <= 0) return 0;
+    for (int64_t i = 0; i < n; i++)
+        if (iz[i] < 1 || iz[i] > c->nz + 1)
+            return gh_fail(GALHIP_EINVAL, "group particle %lld: zone %d out of 1..%d",
+                           (long long)i, iz[i], c->nz + 1);
+    std::vector<R4> hx((size_t)n);
+    std::vector<uint8_t> hz((size_t)n);
+    const R *px = (const R *)x, *py = (const R *)y, *pvx = (const R *)vx, *pvy = (const R *)vy;
+    for (int64_t i = 0; i < n; i++) {
+        hx[i].x = px[i];
+        hx[i].y = py[i];
+        hx[i].z = pvx ? pvx[i] : (R)0;
+        hx[i].w = pvy ? pvy[i] : (R)0;
+        hz[i] = (uint8_t)iz[i];
+    }
+    std::vector<R2> ha;
+    if (kind == GRP_STPGRP) {
+        ha.resize((size_t)n);
+        const R *pa = (const R *)acc;
+        for (int64_t i = 0; i < n; i++) {
+            ha[i].x = pa[2 * i];
+            ha[i].y = pa[2 * i + 1];
+        }
+    }
+    R4 *dxv = nullptr;
+    R2 *dut = nullptr, *dacc = nullptr;
+    uint8_t *dz = nullptr;
+    int rc = dalloc(&dxv, sizeof(R4) * n) | dalloc(&dz, (size_t)n) |
+             dalloc(&dut, sizeof(R2) * n) | dalloc(&dacc, sizeof(R2) * n);
+    if (!rc) {
+        hipError_t e = hipMemcpy(dxv, hx.data(), sizeof(R4) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dz, hz.data(), (size_t)n, hipMemcpyHostToDevice);
+        if (e == hipSuccess && kind == GRP_STPGRP)
+            e = hipMemcpy(dacc, ha.data(), sizeof(R2) * n, hipMemcpyHostToDevice);
+        PushArgs<R> a = push_args<R>(c);
+        a.xv = dxv;
+        a.ut = dut;
+        a.zone = dz;
+        a.n = n;
+        a.xlo = nullptr;
+        if (e == hipSuccess) {
+            if (kind == GRP_GETACC) e = launch_getacc_group<R>(a, dacc, c->s);
+            else if (kind == GRP_STPGRP) e = launch_stpgrp_group<R>(a, dacc, c->s);
+            else e = launch_massgn<R>(a, c->s);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+        if (e == hipSuccess && kind == GRP_GETACC) {
+            ha.resize((size_t)n);
+            e = hipMemcpy(ha.data(), dacc, sizeof(R2) * n, hipMemcpyDeviceToHost);
+            R *pa = (R *)acc;
+            for (int64_t i = 0; e == hipSuccess && i < n; i++) {
+                pa[2 * i] = ha[i].x;
+                pa[2 * i + 1] = ha[i].y;
+            }
+        }
+        if (e == hipSuccess && kind == GRP_STPGRP) {
+            e = hipMemcpy(hx.data(), dxv, sizeof(R4) * n, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(hz.data(), dz, (size_t)n, hipMemcpyDeviceToHost);
+            R *wx = (R *)x, *wy = (R *)y, *wvx = (R *)vx, *wvy = (R *)vy;
+            for (int64_t i = 0; e == hipSuccess && i < n; i++) {
+                wx[i] = hx[i].x;
+                wy[i] = hx[i].y;
+                wvx[i] = hx[i].z;
+                wvy[i] = hx[i].w;
+                iz[i] = hz[i];
+            }
+        }
+        if (e != hipSuccess) rc = gh_fail(GALHIP_EHIP, "group kernel: %s", hipGetErrorString(e));
+    }
+    dfree(dxv);
+    dfree(dz);
+    dfree(dut);
+    dfree(dacc);
+    return rc;
+}
+
+static int group_check(galhip_ctx *c, int64_t n, const void *x, const void *y,
+                       const int32_t *iz) {
+    if (!c) return gh_fail(GALHIP_EINVAL, "null context");
+    if (c->p3d || c->ens) return gh_fail(GALHIP_EINVAL, "group routines: one p2d grid");
+    if (n < 0 || (n > 0 && (!x || !y || !iz))) return gh_fail(GALHIP_EINVAL, "bad arguments");
+    return 0;
+}
+
+extern "C" int galhip_getacc(galhip_ctx *c, int64_t n, const void *x, const void *y,
+                             const int32_t *iz, void *acc) {
+    int rc = group_check(c, n, x, y, iz);
+    if (rc) return rc;
+    if (n > 0 && !acc) return gh_fail(GALHIP_EINVAL, "bad arguments");
+    if (!c->have_field) return gh_fail(GALHIP_ERANGE, "GETACC: Accelerations at useless time");
+    HIPCHK(hipStreamSynchronize(c->s));
+    int32_t *zz = const_cast<int32_t *>(iz);   // read only for GETACC
+    return c->prec == GALHIP_FP32
+               ? group_impl<float>(c, GRP_GETACC, n, (void *)x, (void *)y, nullptr, nullptr, acc, zz)
+               : group_impl<double>(c, GRP_GETACC, n, (void *)x, (void *)y, nullptr, nullptr, acc,
+                                    zz);
+}
+
+extern "C" int galhip_stpgrp(galhip_ctx *c, int64_t n, void *x, void *y, void *vx, void *vy,
+                             const void *acc, int32_t *iz) {
+    int rc = group_check(c, n, x, y, iz);
+    if (rc) return rc;
+    if (n > 0 && (!vx || !vy || !acc)) return gh_fail(GALHIP_EINVAL, "bad arguments");
+    HIPCHK(hipStreamSynchronize(c->s));
+    set_mzone(c);
+    return c->prec == GALHIP_FP32
+               ? group_impl<float>(c, GRP_STPGRP, n, x, y, vx, vy, (void *)acc, iz)
+               : group_impl<double>(c, GRP_STPGRP, n, x, y, vx, vy, (void *)acc, iz);
+}
+
+extern "C" int galhip_massgn(galhip_ctx *c, int64_t n, const void *x, const void *y,
+                             const int32_t *iz) {
+    int rc = group_check(c, n, x, y, iz);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    int32_t *zz = const_cast<int32_t *>(iz);
+    return c->prec == GALHIP_FP32
+               ? group_impl<float>(c, GRP_MASSGN, n, (void *)x, (void *)y, nullptr, nullptr,
+                                   nullptr, zz)
+               : group_impl<double>(c, GRP_MASSGN, n, (void *)x, (void *)y, nullptr, nullptr,
+                                    nullptr, zz);
 }
 
 // grid m's zone masses, ring ranges and fixed-point scale

@@ -51,7 +51,10 @@ __device__ __forceinline__ double4 mk4(double a, double b, double c, double d) {
 #define GH_DPP_ROW_HALF_MIRROR 0x141
 #define GH_DPP_ROW_MIRROR 0x140
 
-#define GH_DPP(v, ctrl) __builtin_amdgcn_update_dpp((v), (v), (ctrl), 0xF, 0xF, false)
+// (every lane reads a lane of its own row, so nothing falls back to the
+// "old" value; mov_dpp lets the compiler fold the exchange into the min,
+// max or add that consumes it)
+#define GH_DPP(v, ctrl) __builtin_amdgcn_mov_dpp((v), (ctrl), 0xF, 0xF, true)
 
 __device__ __forceinline__ int wave_min_i32(int v) {
     v = min(v, GH_DPP(v, GH_DPP_QUAD_1032));

@@ -121,7 +121,10 @@ GH_HD double gh_atan2(double y, double x, const double *tab = GH_MATH_TAB) {
         r = neg_x ? PI_HI : 0.0;
     } else {
         bool swap = ay > ax;
-        double a = swap ? ax / ay : ay / ax;
+        // one quotient of the selected operands (a select of two quotients
+        // would evaluate both divisions)
+        const double num = swap ? ax : ay, den = swap ? ay : ax;
+        double a = num / den;
         int j = (int)(a * 16.0 + 0.5);
         double c = (double)j * 0.0625;
         double z = (a - c) / (1.0 + a * c);

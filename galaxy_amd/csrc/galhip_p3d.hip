@@ -60,7 +60,8 @@ __device__ __forceinline__ void ut3(R x, R y, const P3Args<R> &a, R &u, R &t) {
 // (weight.f:219-248): returns the 0-based cell (ju ngz + jz) na + jt and the
 // 8 weights in the reference's order (bit 0 = dt, bit 1 = dz, bit 2 = du).
 template <typename R>
-__device__ __forceinline__ int cell3(R u, R t, R z, const P3Args<R> &a, bool fold, R w[8]) {
+__device__ __forceinline__ int cell3(R u, R t, R z, const P3Args<R> &a, bool fold, R w[8],
+                                     int &ring, int &plane, int &spoke) {
     R zz = z + a.zm;
     if (fold) t = (R)fmod(t, a.thmax);
     int ju = (int)u;
@@ -80,21 +81,26 @@ __device__ __forceinline__ int cell3(R u, R t, R z, const P3Args<R> &a, bool fol
     w[5] = dt * (one - dz) * du;
     w[6] = (one - dt) * dz * du;
     w[7] = dt * dz * du;
+    ring = ju;
+    plane = jz;
+    spoke = jt;
     return (ju * a.ngz + jz) * a.na + jt;
 }
 
 template <typename R>
 __device__ __forceinline__ int weight3(R x, R y, R z, const P3Args<R> &a, bool fold, R w[8]) {
     R u, t;
+    int rg, pl, sp;
     ut3(x, y, a, u, t);
-    return cell3(u, t, z, a, fold, w);
+    return cell3(u, t, z, a, fold, w, rg, pl, sp);
 }
 
 // weight() for jmass = 3 given (u, t) (weight.f:281-300): the central cell
 // of the 27 (nearest node, ju >= 1, jt == na folded to spoke 0) and their
 // weights, spoke offset -1, 0, +1 fastest, then plane, then ring.
 template <typename R>
-__device__ __forceinline__ int cell3q(R u, R t, R z, const P3Args<R> &a, bool fold, R w[27]) {
+__device__ __forceinline__ int cell3q(R u, R t, R z, const P3Args<R> &a, bool fold, R w[27],
+                                      int &ring, int &plane, int &spoke) {
     R zz = z + a.zm;
     if (fold) t = (R)fmod(t, a.thmax);
     const R h = (R)0.5;
@@ -108,6 +114,9 @@ __device__ __forceinline__ int cell3q(R u, R t, R z, const P3Args<R> &a, bool fo
     if (jz > a.ngz - 2) jz = a.ngz - 2;
     int c = (ju * a.ngz + jz) * a.na + jt;
     if (jt == a.na) c -= a.na;
+    ring = ju;
+    plane = jz;
+    spoke = jt == a.na ? 0 : jt;
     R dp[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -130,15 +139,21 @@ __device__ __forceinline__ int cell3q(R u, R t, R z, const P3Args<R> &a, bool fo
 template <int J> struct P3NW { static constexpr int v = J == 3 ? 27 : 8; };
 
 template <typename R, int J>
+__device__ __forceinline__ int cellJ(R u, R t, R z, const P3Args<R> &a, bool fold, R *w,
+                                     int &ring, int &plane, int &spoke) {
+    if constexpr (J == 3) return cell3q<R>(u, t, z, a, fold, w, ring, plane, spoke);
+    else return cell3<R>(u, t, z, a, fold, w, ring, plane, spoke);
+}
+template <typename R, int J>
 __device__ __forceinline__ int cellJ(R u, R t, R z, const P3Args<R> &a, bool fold, R *w) {
-    if constexpr (J == 3) return cell3q<R>(u, t, z, a, fold, w);
-    else return cell3<R>(u, t, z, a, fold, w);
+    int rg, pl, sp;
+    return cellJ<R, J>(u, t, z, a, fold, w, rg, pl, sp);
 }
 
 // grdacc's corners (grdacc.f:193-203): c, c+1 (wrapped when c+1 is a
 // multiple of na), the next plane (+na), and the same one ring out.
-__device__ __forceinline__ void corners3(int c, int na, int ngz, int ic[8]) {
-    const bool wrap = (c + 1) % na == 0;
+__device__ __forceinline__ void corners3(int c, int spoke, int na, int ngz, int ic[8]) {
+    const bool wrap = spoke + 1 == na;   // (c + 1) mod na == 0
     ic[0] = c;
     ic[1] = wrap ? c + 1 - na : c + 1;
     ic[2] = c + na;
@@ -152,11 +167,11 @@ __device__ __forceinline__ void corners3(int c, int na, int ngz, int ic[8]) {
 // the 27 cells about a central cell c (grdacc.f:262-275, massgn.f:182-195):
 // spoke -1, 0, +1 (with the azimuthal wrap) fastest, then plane -1, 0, +1
 // (-+ na), then ring -1, 0, +1 (-+ ngz na), as flat indices
-__device__ __forceinline__ void corners27(int c, int na, int ngz, int ic[27]) {
+__device__ __forceinline__ void corners27(int c, int spoke, int na, int ngz, int ic[27]) {
     int m[3];
     m[1] = c;
-    m[0] = c % na == 0 ? c - 1 + na : c - 1;
-    m[2] = (c + 1) % na == 0 ? c + 1 - na : c + 1;
+    m[0] = spoke == 0 ? c - 1 + na : c - 1;          // c mod na == 0
+    m[2] = spoke + 1 == na ? c + 1 - na : c + 1;     // (c + 1) mod na == 0
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -165,9 +180,12 @@ __device__ __forceinline__ void corners27(int c, int na, int ngz, int ic[27]) {
             for (int i = 0; i < 3; i++) ic[9 * k + 3 * j + i] = m[i] + (j - 1) * na + (k - 1) * ngz * na;
 }
 
-template <int J> __device__ __forceinline__ void cornersJ(int c, int na, int ngz, int *ic) {
-    if constexpr (J == 3) corners27(c, na, ngz, ic);
-    else corners3(c, na, ngz, ic);
+// spoke: the central cell's spoke as cellJ gives it (c mod na, or na for a
+// linear cell at t == na)
+template <int J> __device__ __forceinline__ void cornersJ(int c, int spoke, int na, int ngz,
+                                                          int *ic) {
+    if constexpr (J == 3) corners27(c, spoke, na, ngz, ic);
+    else corners3(c, spoke, na, ngz, ic);
 }
 
 template <typename R>
@@ -208,8 +226,9 @@ __device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz, R 
     if (z0 != a.nz + 1) {
         R w[NW];
         int ic[NW];
-        const int c = cellJ<R, J>(u, t, oz, a, false, w);
-        cornersJ<J>(c, a.na, a.ngz, ic);
+        int rg, pl, sp;
+        const int c = cellJ<R, J>(u, t, oz, a, false, w, rg, pl, sp);
+        cornersJ<J>(c, sp, a.na, a.ngz, ic);
         R sx = (R)0, sy = (R)0, sz = (R)0;
 #pragma unroll
         for (int q = 0; q < NW; q++) {
@@ -276,19 +295,15 @@ constexpr int P3_TILE_REP = GH_P3_TILE_REP;   // window copies (by lane) against
 // maximum per zone.
 template <typename R, int J>
 __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int z, int c,
+                                               int ring, int plane, int spoke,
                                                const R *w, long long *tile, int *s_b,
                                                int *s_jr, int *s_kr) {
     constexpr int NW = P3NW<J>::v;
     constexpr int LO = J == 3 ? 1 : 0;   // corner cells below the central one
     const int tid = threadIdx.x, lane = tid & 63;
-    const int S = a.ngz * a.na;
-    int ring = 0, plane = 0, spoke = 0;
-    if (on) {
-        ring = c / S;
-        const int rem = c - ring * S;
-        plane = rem / a.na;
-        spoke = rem - plane * a.na;
-    }
+    // ring, plane, spoke: the cell's own indices from cellJ (a linear cell
+    // at t == na has spoke na: not a window candidate, deposited directly)
+    if (!on) ring = plane = spoke = 0;
     // the block's zone: the lowest among its deposits
     int zw = p3_wave_min(on ? z : INT_MAX);
     if (lane == 0 && zw != INT_MAX) atomicMin(&s_b[0], zw);
@@ -361,7 +376,7 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
             }
         } else {
             int ic[NW];
-            cornersJ<J>(c, a.na, a.ngz, ic);
+            cornersJ<J>(c, spoke, a.na, a.ngz, ic);
 #pragma unroll
             for (int q = 0; q < NW; q++)
                 if (v[q] != 0) deposit_global(a.mfx, a.nz, a.mesh, z, ic[q], v[q]);
@@ -409,7 +424,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
     __syncthreads();
     const long long i = (long long)blockIdx.x * P3_PBLOCK + tid;
     bool on = false;
-    int z = 0, c = 0;
+    int z = 0, c = 0, rg = 0, pl = 0, sp = 0;
     R w[P3NW<J>::v];
     if (i < a.n) {
         z = a.zone[i];
@@ -418,11 +433,11 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
         ut3(p.x, p.y, a, u, t);
         a.ut[i] = mk2(u, t);
         if (z >= 1 && z <= a.nz) {
-            c = cellJ<R, J>(u, t, p.z, a, a.nsect > 1, w);
+            c = cellJ<R, J>(u, t, p.z, a, a.nsect > 1, w, rg, pl, sp);
             on = true;
         }
     }
-    deposit3_block<R, J>(a, on, z, c, w, tile, s_b, s_jr, s_kr);
+    deposit3_block<R, J>(a, on, z, c, rg, pl, sp, w, tile, s_b, s_jr, s_kr);
     __syncthreads();
     flush_ranges(a.nz, a.nr, J == 3 ? 1 : 0, s_jr, s_kr, a.jrad, a.krad);
 }
@@ -447,7 +462,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
     const long long i = (long long)blockIdx.x * P3_PBLOCK + tid;
     int dn = 0, np = 0;
     bool don = false;
-    int dz = 0, dc = 0;
+    int dz = 0, dc = 0, drg = 0, dpl = 0, dsp = 0;
     R dw[P3NW<J>::v];
     if (i < a.n) {
         const int nz = a.nz, nlists = nz + 1;
@@ -506,13 +521,13 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
             a.ut[i] = mk2(un, tn);
             a.zone[i] = (uint8_t)znew;
             if (znew >= 1 && znew <= nz) {
-                dc = cellJ<R, J>(un, tn, nx[2], a, a.nsect > 1, dw);
+                dc = cellJ<R, J>(un, tn, nx[2], a, a.nsect > 1, dw, drg, dpl, dsp);
                 dz = znew;
                 don = true;
             }
         }
     }
-    deposit3_block<R, J>(a, don, dz, dc, dw, tile, s_b, s_jr, s_kr);
+    deposit3_block<R, J>(a, don, dz, dc, drg, dpl, dsp, dw, tile, s_b, s_jr, s_kr);
     // block sums of the off-grid change and the pushes
     dn = wave_sum_i32(dn);
     np = wave_sum_i32(np);
@@ -564,13 +579,14 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_gather_kernel(P3GatherArgs<R> b)
     R w[NW];
     R u, t;
     ut3(b.x[i], b.y[i], b.g, u, t);
-    const int c = cellJ<R, J>(u, t, b.z[i], b.g, b.fold != 0, w);
+    int rg, pl, sp;
+    const int c = cellJ<R, J>(u, t, b.z[i], b.g, b.fold != 0, w, rg, pl, sp);
     if (b.cell) b.cell[i] = c;
     if (b.wt)
         for (int q = 0; q < NW; q++) b.wt[NW * i + q] = w[q];
     if (b.acc) {
         int ic[NW];
-        cornersJ<J>(c, b.g.na, b.g.ngz, ic);
+        cornersJ<J>(c, sp, b.g.na, b.g.ngz, ic);
         R s[3] = {(R)0, (R)0, (R)0};
         for (int q = 0; q < NW; q++) {
             R4 f = (ic[q] >= 0 && ic[q] < b.g.mesh) ? b.fld[ic[q]] : mk4((R)0, (R)0, (R)0, (R)0);
@@ -639,8 +655,9 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_integrals_kernel(P3Args<R> a, in
                 constexpr int NW = P3NW<J>::v;
                 R w[NW];
                 int ic[NW];
-                const int c = cellJ<R, J>(ut.x, ut.y, p.z, a, false, w);
-                cornersJ<J>(c, a.na, a.ngz, ic);
+                int rg, pl, sp;
+                const int c = cellJ<R, J>(ut.x, ut.y, p.z, a, false, w, rg, pl, sp);
+                cornersJ<J>(c, sp, a.na, a.ngz, ic);
                 double ph = 0.0;
                 for (int k = 0; k < NW; k++)
                     if (ic[k] >= 0 && ic[k] < a.mesh) ph += (double)w[k] * (double)a.fld[ic[k]].w;

@@ -65,12 +65,17 @@ __device__ __forceinline__ R tsfac(const PushArgs<R> &a, int iz) {
 // zoneno for a position of known radius (zoneno.f:40-55)
 template <typename R>
 __device__ __forceinline__ int zone_of(const PushArgs<R> &a, R rad) {
-    // fully unrolled over the zone limit with a uniform guard, so that the
-    // zone radii are read from the argument block at fixed offsets
+    // the first two boundaries in uniform branches with the radii read at
+    // fixed offsets; only more than three zones take the loop
     int z = 1;
-#pragma unroll
-    for (int i = 2; i <= GH_MAXZ; i++)
-        if (i <= a.nz && rad >= a.zbr[i - 1]) z = i;
+    if (a.nz >= 2) {
+        if (rad >= a.zbr[1]) z = 2;
+        if (a.nz >= 3) {
+            if (rad >= a.zbr[2]) z = 3;
+            for (int i = 4; i <= a.nz; i++)
+                if (rad >= a.zbr[i - 1]) z = i;
+        }
+    }
     return z;
 }
 
@@ -709,6 +714,73 @@ __global__ __launch_bounds__(GH_BLOCK) void tmcent_kernel(PushArgs<R> a, R fac) 
     a.xv[i] = st;
 }
 
+// ---- group-level routines (parity use) --------------------------------------
+//
+// The lib15 routines that act on one group of particles -- getacc(jst),
+// stpgrp(jst) with offgrd/addoff and rezgrp(jst) -- on a caller's group (device
+// copies of its arrays), in the push kernel's operation order: the push fuses
+// the same steps, and massgn, for every particle of a step.
+
+// getacc(jst) (getacc.f:35-163) at the group's positions, their (u, t) formed
+// as the push's cache holds them
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void getacc_group_kernel(PushArgs<R> a,
+                                                                 typename Vec2<R>::T *acc) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const typename Vec4<R>::T st = a.xv[i];
+    const R rr = sqrt(st.x * st.x + st.y * st.y);
+    R u, t, ax, ay;
+    ut_of(st.x, st.y, rr, a.k, u, t);
+    getacc1(a, st, mk2(u, t), (int)a.zone[i], ax, ay);
+    acc[i] = mk2(ax, ay);
+}
+
+// stpgrp(jst) (stpgrp.f:27-56) with the given accelerations, the off-grid
+// test and list change (offgrd.f:55-59, addoff.f:30-70, with chstep to the
+// top zone's step when offanal), and rezgrp(jst) at the current mzone
+// (rezgrp.f:30-50, zoneno.f:40-55, chstep.f:22-31); zones updated in place.
+template <typename R>
+__global__ __launch_bounds__(GH_BLOCK) void stpgrp_group_kernel(PushArgs<R> a,
+                                                                 const typename Vec2<R>::T *acc) {
+    const long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const typename Vec4<R>::T st = a.xv[i];
+    const int nz = a.nz, nlists = nz + 1, z0 = a.zone[i];
+    const bool offlist = z0 == nlists;
+    R ox = st.x, oy = st.y, ovx = st.z, ovy = st.w;
+    R ax_ = acc[i].x, ay_ = acc[i].y;
+    R nvx = ovx + ax_, nvy = ovy + ay_;
+    R nx = ox + nvx, ny = oy + nvy;
+    int znew = z0;
+    R rnew = sqrt(nx * nx + ny * ny);
+    const bool off = rnew > a.rgrid;
+    if (!offlist) {
+        if (off) {
+            znew = nlists;
+            if (a.offanal && z0 < nz) {
+                const R tfac = tsfac(a, nz) / tsfac(a, z0);
+                chstep(tfac, ox, oy, ovx, ovy, ax_, ay_, nx, ny, nvx, nvy);
+                rnew = sqrt(nx * nx + ny * ny);
+            }
+        }
+    } else if (!off) {
+        znew = nz;
+    }
+    if (a.mzone > 1 && !offlist && znew <= a.mzone) {
+        int kz = zone_of(a, rnew);
+        if (kz > a.mzone) kz = a.mzone;
+        if (kz > znew) kz = zone_of(a, sqrt(ox * ox + oy * oy));
+        if (kz != znew) {
+            const R tfac = tsfac(a, kz) / tsfac(a, znew);
+            chstep(tfac, ox, oy, ovx, ovy, ax_, ay_, nx, ny, nvx, nvy);
+            znew = kz;
+        }
+    }
+    a.xv[i] = mk4(nx, ny, nvx, nvy);
+    a.zone[i] = (uint8_t)znew;
+}
+
 // ---- global integrals (icheck, SRC/lib15/icheck.f:24-70) ------------------
 //
 // Per particle, in FP64: 1, x, y, vx, vy, v^2, x vy - y vx, the grid
@@ -964,6 +1036,23 @@ hipError_t launch_massgn(const PushArgs<R> &a, hipStream_t s) {
 }
 
 template <typename R>
+hipError_t launch_getacc_group(const PushArgs<R> &a, typename Vec2<R>::T *acc, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(getacc_group_kernel<R>, dim3(nblocks(a.n, GH_BLOCK)), dim3(GH_BLOCK), 0,
+                       s, a, acc);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_stpgrp_group(const PushArgs<R> &a, const typename Vec2<R>::T *acc,
+                               hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(stpgrp_group_kernel<R>, dim3(nblocks(a.n, GH_BLOCK)), dim3(GH_BLOCK), 0,
+                       s, a, acc);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_ut(const PushArgs<R> &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(ut_kernel<R>, dim3(nblocks(a.n, GH_BLOCK)),
@@ -1052,6 +1141,11 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
     template hipError_t launch_push<R>(const PushArgs<R> &, hipStream_t);     \
     template hipError_t launch_massgn<R>(const PushArgs<R> &, hipStream_t);   \
     template hipError_t launch_ut<R>(const PushArgs<R> &, hipStream_t);       \
+    template hipError_t launch_getacc_group<R>(const PushArgs<R> &,           \
+                                               typename Vec2<R>::T *, hipStream_t); \
+    template hipError_t launch_stpgrp_group<R>(const PushArgs<R> &,           \
+                                               const typename Vec2<R>::T *,   \
+                                               hipStream_t);                  \
     template hipError_t launch_gather<R>(const GatherArgs<R> &, hipStream_t); \
     template hipError_t launch_tmcent<R>(const PushArgs<R> &, R, hipStream_t); \
     template hipError_t launch_integrals<R>(const PushArgs<R> &, const R *,     \

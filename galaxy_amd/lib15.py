@@ -366,6 +366,32 @@ class P2DGrid:
                 "grdacc")
         return acc
 
+    def getacc(self, x, y, iz):
+        """getacc(jst) on a group: acceleration [n][2] at (x, y) for lists iz."""
+        x = np.ascontiguousarray(x, self.dtype)
+        y = np.ascontiguousarray(y, self.dtype)
+        iz = np.ascontiguousarray(iz, np.int32)
+        acc = np.empty((len(x), 2), self.dtype)
+        self._c(self._lib.galhip_getacc(self._h, len(x), _ptr(x), _ptr(y), _ptr(iz), _ptr(acc)),
+                "getacc")
+        return acc
+
+    def stpgrp(self, x, y, vx, vy, acc, iz):
+        """stpgrp(jst) + offgrd + rezgrp on a group; returns new (x, y, vx, vy, iz)."""
+        x, y, vx, vy = (np.array(a, self.dtype) for a in (x, y, vx, vy))
+        acc = np.ascontiguousarray(acc, self.dtype)
+        iz = np.array(iz, np.int32)
+        self._c(self._lib.galhip_stpgrp(self._h, len(x), _ptr(x), _ptr(y), _ptr(vx), _ptr(vy),
+                                        _ptr(acc), _ptr(iz)), "stpgrp")
+        return x, y, vx, vy, iz
+
+    def massgn(self, x, y, iz):
+        """massgn(jst): add a group's mass to the zone sums."""
+        x = np.ascontiguousarray(x, self.dtype)
+        y = np.ascontiguousarray(y, self.dtype)
+        iz = np.ascontiguousarray(iz, np.int32)
+        self._c(self._lib.galhip_massgn(self._h, len(x), _ptr(x), _ptr(y), _ptr(iz)), "massgn")
+
     # -- grids -----------------------------------------------------------------------
     def get_mass(self):
         mfx = np.zeros((self.nzones, self.mesh), np.int64)

@@ -156,6 +156,26 @@ int galhip_weight(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
  * acc[n][2] interpolated from the current Cartesian field. */
 int galhip_grdacc(galhip_ctx *ctx, int64_t n, const void *x, const void *y,
                   void *acc);
+/* Group-level routines for parity work, on a caller's group of n particles
+ * (host arrays in the context's precision; iz the group's lists, 1..nz
+ * on the grid, nz + 1 off it), with the step's own device arithmetic:
+ *   galhip_getacc   getacc(jst) (SRC/lib15/getacc.f:1): grid acceleration at
+ *                   the particles' positions from the current field, the
+ *                   fixed radial force, the zone's step factor squared;
+ *                   acc[n][2] (x, y); zero off the grid.
+ *   galhip_stpgrp   stpgrp(jst) (SRC/lib15/stpgrp.f:1) with offgrd/addoff and
+ *                   rezgrp(jst) (SRC/lib15/rezgrp.f:1): v += acc, x += v, the
+ *                   off-grid test (with chstep to the top zone's step when
+ *                   offanal) and the zone change at the current istep's
+ *                   mzone; x, y, vx, vy and iz updated in place.  The
+ *                   context's off-grid count is not changed.
+ *   galhip_massgn   massgn(jst) (SRC/lib15/massgn.f:1): adds the group's
+ *                   weights to the context's zone mass sums and ring ranges. */
+int galhip_getacc(galhip_ctx *ctx, int64_t n, const void *x, const void *y, const int32_t *iz,
+                  void *acc);
+int galhip_stpgrp(galhip_ctx *ctx, int64_t n, void *x, void *y, void *vx, void *vy,
+                  const void *acc, int32_t *iz);
+int galhip_massgn(galhip_ctx *ctx, int64_t n, const void *x, const void *y, const int32_t *iz);
 
 /* Read back grids: mass sums as fixed-point int64 [nzones][mesh] (weight x
  * 2^qbits), the saved zone masses grdmoz, and the zone ring ranges jrad,

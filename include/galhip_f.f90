@@ -155,6 +155,30 @@ module galhip_f
        integer(c_int64_t), value :: n
        type(c_ptr), value :: x, y, acc
      end function galhip_grdacc
+     integer(c_int) function galhip_getacc(ctx, n, x, y, iz, acc) &
+          bind(c, name='galhip_getacc')
+       import :: c_int, c_ptr, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y, acc
+       integer(c_int32_t), intent(in) :: iz(*)
+     end function galhip_getacc
+     integer(c_int) function galhip_stpgrp(ctx, n, x, y, vx, vy, acc, iz) &
+          bind(c, name='galhip_stpgrp')
+       import :: c_int, c_ptr, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y, vx, vy, acc
+       integer(c_int32_t), intent(inout) :: iz(*)
+     end function galhip_stpgrp
+     integer(c_int) function galhip_massgn(ctx, n, x, y, iz) &
+          bind(c, name='galhip_massgn')
+       import :: c_int, c_ptr, c_int64_t, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       type(c_ptr), value :: x, y
+       integer(c_int32_t), intent(in) :: iz(*)
+     end function galhip_massgn
      integer(c_int) function galhip_get_mass(ctx, mfx, mozfx, jrad, krad, qbits) &
           bind(c, name='galhip_get_mass')
        import :: c_int, c_ptr, c_int32_t
