@@ -273,57 +273,66 @@ __device__ __forceinline__ int p3_wave_max(int v) { return wave_max_i32(v); }
 #ifndef GH_P3_TILE
 #define GH_P3_TILE 4096
 #endif
-#ifndef GH_P3_TILE_REP
-#define GH_P3_TILE_REP 1
+#ifndef GH_P3_PASSES
+#define GH_P3_PASSES 3
 #endif
 constexpr int P3_TILE = GH_P3_TILE;       // cells of a block's LDS deposit window
-constexpr int P3_TILE_REP = GH_P3_TILE_REP;   // window copies (by lane) against conflicts
+constexpr int P3_PASSES = GH_P3_PASSES;   // window passes before the direct fallback
+constexpr int P3_SB = 13;                 // ints of the block scratch s_b
 
 // Block-level fixed-point deposit.  The block's particles (sorted by cell,
 // so a block covers a few neighbouring cells) are accumulated into an LDS
 // window spanning the (ring, plane, spoke) box of their corners, then the
 // window's non-zero cells are added to the zone array with one global atomic
-// each.  The corners of a central cell are the next ring, plane and spoke
-// (jmass 2) or the cells one either side (jmass 3).  Particles whose
-// corners leave the box shape (the azimuthal wrap; the plane next to the
-// last or, for jmass 3, the first), or that belong to another zone than the
-// block's first, and those outside the window when the box is clipped to
-// P3_TILE cells, add their corners directly.  Integer sums: the result does
-// not depend on the grouping.  Every thread of the block must call it
-// (on = false for threads without a deposit).  s_b: 7 ints, s_b[0] =
-// INT_MAX on entry; s_jr/s_kr receive the block's central-ring minimum and
-// maximum per zone.
+// each (consecutive cells of a row, so the atomics of a wave coalesce).  The
+// corners of a central cell are the next ring, plane and spoke (jmass 2) or
+// the cells one either side (jmass 3); spokes wrap around the ring, so the
+// window's spoke axis runs past na (or below 0) and the flush folds it back.
+// A box larger than P3_TILE cells is clipped to at most 8 rings and 8
+// planes, then as many spokes as fit, placed about a reference particle:
+// the middle thread's, or the first pending one after it in thread order.
+// The particles it leaves out (outliers that have moved away from the
+// block's cells, or the far rows of a block in a sparse region) get the
+// next pass's window over what remains, up to P3_PASSES passes; the
+// reference particle is always inside its window.
+// Particles whose corners leave the row structure (the plane next to the
+// last or, for jmass 3, the first; a linear cell at t == na), those of
+// another zone than the block's lowest, and those still left after the last
+// pass add their corners to the zone array directly.  Integer sums: the
+// result does not depend on the grouping.  Every thread of the block must
+// call it (on = false for threads without a deposit).  s_b: P3_SB ints,
+// s_b[0] = INT_MAX on entry; s_jr/s_kr receive the block's central-ring
+// minimum and maximum per zone.
 template <typename R, int J>
 __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int z, int c,
                                                int ring, int plane, int spoke,
                                                const R *w, long long *tile, int *s_b,
-                                               int *s_jr, int *s_kr) {
+                                               unsigned long long *s_ref, int *s_jr,
+                                               int *s_kr) {
     constexpr int NW = P3NW<J>::v;
     constexpr int LO = J == 3 ? 1 : 0;   // corner cells below the central one
-    const int tid = threadIdx.x, lane = tid & 63;
-    // ring, plane, spoke: the cell's own indices from cellJ (a linear cell
-    // at t == na has spoke na: not a window candidate, deposited directly)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // reference key: thread order from the middle thread, then the cell
+    const unsigned long long rkey =
+        ((unsigned long long)((tid - P3_PBLOCK / 2) & (P3_PBLOCK - 1)) << 40) |
+        ((unsigned long long)(unsigned)ring << 26) | ((unsigned long long)(unsigned)plane << 13) |
+        (unsigned long long)(unsigned)spoke;
     if (!on) ring = plane = spoke = 0;
     // the block's zone: the lowest among its deposits
     int zw = p3_wave_min(on ? z : INT_MAX);
     if (lane == 0 && zw != INT_MAX) atomicMin(&s_b[0], zw);
     if (tid == 0) {
-        s_b[1] = s_b[3] = s_b[5] = INT_MAX;
-        s_b[2] = s_b[4] = s_b[6] = -1;
+        // two (min, max) x (ring, plane, spoke) box buffers, used by
+        // alternate passes so that one can be reset while the other is read
+#pragma unroll
+        for (int k = 1; k < P3_SB; k++) s_b[k] = (k & 1) ? INT_MAX : -1;
+        s_ref[0] = s_ref[1] = ~0ull;
     }
     __syncthreads();
     const int tz = s_b[0];
-    const bool cand = on && z == tz && plane - LO >= 0 && plane + 1 < a.ngz &&
-                      spoke - LO >= 0 && spoke + 1 < a.na && ring - LO >= 0;
+    bool pend = on && z == tz && plane - LO >= 0 && plane + 1 < a.ngz && spoke < a.na &&
+                ring - LO >= 0;
     {
-        const int r0 = p3_wave_min(cand ? ring : INT_MAX), r1 = p3_wave_max(cand ? ring : -1);
-        const int p0 = p3_wave_min(cand ? plane : INT_MAX), p1 = p3_wave_max(cand ? plane : -1);
-        const int q0 = p3_wave_min(cand ? spoke : INT_MAX), q1 = p3_wave_max(cand ? spoke : -1);
-        if (lane == 0 && r1 >= 0) {
-            atomicMin(&s_b[1], r0); atomicMax(&s_b[2], r1);
-            atomicMin(&s_b[3], p0); atomicMax(&s_b[4], p1);
-            atomicMin(&s_b[5], q0); atomicMax(&s_b[6], q1);
-        }
         // ring ranges: the block zone's by wave reduction, others directly
         const int jr = p3_wave_min(on && z == tz ? ring : INT_MAX);
         const int kr = p3_wave_max(on && z == tz ? ring : -1);
@@ -336,33 +345,64 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
             atomicMax(&s_kr[z - 1], ring);
         }
     }
-    __syncthreads();
-    // the window: the box of the candidates' corners, clipped about its
-    // centre to at most P3_TILE cells (8 rings, 8 planes, the rest spokes);
-    // candidates whose corners fall outside go to the zone array directly
-    int r0 = s_b[1] - LO, p0 = s_b[3] - LO, q0 = s_b[5] - LO;
-    int NR = s_b[2] - s_b[1] + 2 + LO, NP = s_b[4] - s_b[3] + 2 + LO;
-    int NQ = s_b[6] - s_b[5] + 2 + LO;
-    const bool use = s_b[2] >= 0;
-    if (use && NR * NP * NQ > P3_TILE) {
-        if (NR > 8) { r0 += (NR - 8) / 2; NR = 8; }
-        if (NP > 8) { p0 += (NP - 8) / 2; NP = 8; }
-        const int mq = P3_TILE / (NR * NP);
-        if (NQ > mq) { q0 += (NQ - mq) / 2; NQ = mq; }
-    }
-    const int ncell = use ? NR * NP * NQ : 0;
-    for (int e = tid; e < ncell; e += P3_PBLOCK)
+    long long v[NW];
 #pragma unroll
-        for (int rep = 0; rep < P3_TILE_REP; rep++) tile[rep * P3_TILE + e] = 0;
-    __syncthreads();
-    long long *const trep = tile + (lane & (P3_TILE_REP - 1)) * P3_TILE;
-    if (on) {
-        long long v[NW];
+    for (int q = 0; q < NW; q++) v[q] = on ? quant<R>(w[q], a.qbits) : 0;
+    bool direct = on && !pend;
+    for (int pass = 0;; pass++) {
+        int *const sb = s_b + 1 + 6 * (pass & 1);
+        {
+            const int r0 = p3_wave_min(pend ? ring : INT_MAX), r1 = p3_wave_max(pend ? ring : -1);
+            const int p0 = p3_wave_min(pend ? plane : INT_MAX), p1 = p3_wave_max(pend ? plane : -1);
+            const int q0 = p3_wave_min(pend ? spoke : INT_MAX), q1 = p3_wave_max(pend ? spoke : -1);
+            // the wave's first pending lane holds its lowest reference key
+            const unsigned long long bal = __ballot(pend);
+            const int fl = bal ? __ffsll((long long)bal) - 1 : 0;
+            const unsigned klo = __builtin_amdgcn_readlane((unsigned)rkey, fl);
+            const unsigned khi = __builtin_amdgcn_readlane((unsigned)(rkey >> 32), fl);
+            if (lane == 0 && r1 >= 0) {
+                atomicMin(&sb[0], r0); atomicMax(&sb[1], r1);
+                atomicMin(&sb[2], p0); atomicMax(&sb[3], p1);
+                atomicMin(&sb[4], q0); atomicMax(&sb[5], q1);
+                atomicMin(&s_ref[pass & 1], ((unsigned long long)khi << 32) | klo);
+            }
+        }
+        __syncthreads();
+        // the box of the pending corners, clipped from its low corner
+        int r0 = sb[0] - LO, p0 = sb[2] - LO, q0 = sb[4] - LO;
+        int NR = sb[1] - sb[0] + 2 + LO, NP = sb[3] - sb[2] + 2 + LO;
+        int NQ = sb[5] - sb[4] + 2 + LO;
+        const bool use = sb[1] >= 0;
+        const unsigned long long kref = s_ref[pass & 1];
+        if (tid == 0) {
+            int *const nb = s_b + 1 + 6 * ((pass + 1) & 1);   // last read in the previous pass
 #pragma unroll
-        for (int q = 0; q < NW; q++) v[q] = quant<R>(w[q], a.qbits);
-        const bool inside = use && cand && ring - LO >= r0 && ring - r0 + 1 < NR &&
-                            plane - LO >= p0 && plane - p0 + 1 < NP && spoke - LO >= q0 &&
-                            spoke - q0 + 1 < NQ;
+            for (int k = 0; k < 6; k++) nb[k] = (k & 1) ? -1 : INT_MAX;
+            s_ref[(pass + 1) & 1] = ~0ull;
+        }
+        if (use && NR * NP * NQ > P3_TILE) {
+            // about the reference cell, inside the box
+            const int rr = (int)((kref >> 26) & 0x1FFF), rp = (int)((kref >> 13) & 0x1FFF);
+            const int rq = (int)(kref & 0x1FFF);
+            if (NR > 8) {
+                r0 = min(max(rr - LO - 3, r0), r0 + NR - 8);
+                NR = 8;
+            }
+            if (NP > 8) {
+                p0 = min(max(rp - LO - 3, p0), p0 + NP - 8);
+                NP = 8;
+            }
+            const int mq = P3_TILE / (NR * NP);
+            if (NQ > mq) {
+                q0 = min(max(rq - LO - (mq - 2 - LO) / 2, q0), q0 + NQ - mq);
+                NQ = mq;
+            }
+        }
+        const int ncell = use ? NR * NP * NQ : 0;
+        for (int e = tid; e < ncell; e += P3_PBLOCK) tile[e] = 0;
+        __syncthreads();
+        const bool inside = pend && ring - LO >= r0 && ring - r0 + 1 < NR && plane - LO >= p0 &&
+                            plane - p0 + 1 < NP && spoke - LO >= q0 && spoke - q0 + 1 < NQ;
         if (inside) {
             const int b = ((ring - r0) * NP + (plane - p0)) * NQ + (spoke - q0);
 #pragma unroll
@@ -372,30 +412,38 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
                     o = b + (q % 3 - 1) + ((q / 3) % 3 - 1) * NQ + (q / 9 - 1) * NP * NQ;
                 else
                     o = b + ((q & 1) ? 1 : 0) + ((q & 2) ? NQ : 0) + ((q & 4) ? NP * NQ : 0);
-                if (v[q] != 0) atomicAdd((unsigned long long *)&trep[o], (unsigned long long)v[q]);
+                if (v[q] != 0) atomicAdd((unsigned long long *)&tile[o], (unsigned long long)v[q]);
             }
-        } else {
-            int ic[NW];
-            cornersJ<J>(c, spoke, a.na, a.ngz, ic);
-#pragma unroll
-            for (int q = 0; q < NW; q++)
-                if (v[q] != 0) deposit_global(a.mfx, a.nz, a.mesh, z, ic[q], v[q]);
+            pend = false;
         }
+        __syncthreads();
+        // flush: one wave per (ring, plane) row of the window; window spokes
+        // q0 + dq lie in [-1, na], folded back onto the ring
+        for (int row = wave; row < (use ? NR * NP : 0); row += P3_PBLOCK / 64) {
+            const int dr = row / NP, dp = row - dr * NP;
+            const int base = ((r0 + dr) * a.ngz + (p0 + dp)) * a.na;
+            const long long *t = tile + row * NQ;
+            for (int dq = lane; dq < NQ; dq += 64) {
+                const long long tv = t[dq];
+                int sp = q0 + dq;
+                sp = sp >= a.na ? sp - a.na : sp < 0 ? sp + a.na : sp;
+                if (tv != 0)
+                    atomicAdd((unsigned long long *)&a.mfx[tz - 1][base + sp], (unsigned long long)tv);
+            }
+        }
+        if (pass + 1 >= P3_PASSES) {
+            direct = direct || pend;
+            break;
+        }
+        // every thread reads the flag of the same barrier: a uniform exit
+        if (!__syncthreads_or(pend ? 1 : 0)) break;
     }
-    __syncthreads();
-    // flush: one wave per (ring, plane) row of the window
-    const int wave = tid >> 6;
-    for (int row = wave; row < (use ? NR * NP : 0); row += P3_PBLOCK / 64) {
-        const int dr = row / NP, dp = row - dr * NP;
-        const int base = ((r0 + dr) * a.ngz + (p0 + dp)) * a.na + q0;
-        const long long *t = tile + row * NQ;
-        for (int dq = lane; dq < NQ; dq += 64) {
-            long long v = 0;
+    if (direct) {
+        int ic[NW];
+        cornersJ<J>(c, spoke, a.na, a.ngz, ic);
 #pragma unroll
-            for (int rep = 0; rep < P3_TILE_REP; rep++) v += t[rep * P3_TILE + dq];
-            if (v != 0)
-                atomicAdd((unsigned long long *)&a.mfx[tz - 1][base + dq], (unsigned long long)v);
-        }
+        for (int q = 0; q < NW; q++)
+            if (v[q] != 0) deposit_global(a.mfx, a.nz, a.mesh, z, ic[q], v[q]);
     }
 }
 
@@ -413,8 +461,9 @@ __device__ __forceinline__ void flush_ranges(int nz, int nr, int lo, const int *
 
 template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
-    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
-    __shared__ long long tile[P3_TILE_REP * P3_TILE];
+    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[P3_SB];
+    __shared__ unsigned long long s_ref[2];
+    __shared__ long long tile[P3_TILE];
     const int tid = threadIdx.x;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
@@ -437,7 +486,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
             on = true;
         }
     }
-    deposit3_block<R, J>(a, on, z, c, rg, pl, sp, w, tile, s_b, s_jr, s_kr);
+    deposit3_block<R, J>(a, on, z, c, rg, pl, sp, w, tile, s_b, s_ref, s_jr, s_kr);
     __syncthreads();
     flush_ranges(a.nz, a.nr, J == 3 ? 1 : 0, s_jr, s_kr, a.jrad, a.krad);
 }
@@ -446,9 +495,10 @@ template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
     typedef typename Vec4<R>::T R4;
     typedef typename Vec2<R>::T R2;
-    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[7];
+    __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ], s_b[P3_SB];
+    __shared__ unsigned long long s_ref[2];
     __shared__ int s_dn, s_np;
-    __shared__ long long tile[P3_TILE_REP * P3_TILE];
+    __shared__ long long tile[P3_TILE];
     const int tid = threadIdx.x;
     if (tid < GH_MAXZ) {
         s_jr[tid] = INT_MAX;
@@ -527,7 +577,7 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
             }
         }
     }
-    deposit3_block<R, J>(a, don, dz, dc, drg, dpl, dsp, dw, tile, s_b, s_jr, s_kr);
+    deposit3_block<R, J>(a, don, dz, dc, drg, dpl, dsp, dw, tile, s_b, s_ref, s_jr, s_kr);
     // block sums of the off-grid change and the pushes
     dn = wave_sum_i32(dn);
     np = wave_sum_i32(np);
