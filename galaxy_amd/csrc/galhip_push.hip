@@ -55,6 +55,14 @@ constexpr int PB = GH_PUSH_BLOCK;      // threads per block
 constexpr int TILE_CAP = GH_TILE_CAP;  // LDS tile, cells of a ring x spoke window
 constexpr int TILE_REP = GH_TILE_REP;  // replicas (lane & (REP-1)) against conflicts
 constexpr int FF_CAP = 256;            // staged fixed-force table entries
+#ifndef GH_PUSH_PASSES
+#define GH_PUSH_PASSES 2               // deposit window passes before global atomics
+#endif
+
+// fields of a packed deposit cell (zone << 28 | ring << 16 | spoke)
+__device__ __forceinline__ int dc_zone(int c) { return (int)((unsigned)c >> 28); }
+__device__ __forceinline__ int dc_ring(int c) { return (c >> 16) & 0xFFF; }
+__device__ __forceinline__ int dc_spoke(int c) { return c & 0xFFFF; }
 
 template <typename R>
 __device__ __forceinline__ R tsfac(const PushArgs<R> &a, int iz) {
@@ -126,8 +134,9 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
     typedef typename Vec4<R>::T R4;
     __shared__ long long tile[TILE_REP][TILE_CAP];
     __shared__ int s_jr[GH_MAXZ], s_kr[GH_MAXZ];
-    __shared__ int s_dnoff, s_pushes, s_zt, s_sref;
-    __shared__ int s_rmin, s_rmax, s_dmin, s_dmax;
+    __shared__ int s_dnoff, s_pushes;
+    __shared__ int s_box[8];                 // two (ring min, max, offset min, max) boxes
+    __shared__ unsigned long long s_ref[2];  // two deposit-pass reference keys
     __shared__ int s_fref, s_frmin, s_frmax, s_fdmin, s_fdmax, s_kmin, s_kmax;
     // slice of the fixed-force table that the block's radii use
     __shared__ double4 s_ffc[FF_CAP];
@@ -154,11 +163,11 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
     if (tid == 0) {
         s_dnoff = 0;
         s_pushes = 0;
-        s_zt = 0;
-        s_rmin = INT_MAX;
-        s_rmax = -1;
-        s_dmin = INT_MAX;
-        s_dmax = INT_MIN;
+        s_box[0] = s_box[4] = INT_MAX;
+        s_box[1] = s_box[5] = -1;
+        s_box[2] = s_box[6] = INT_MAX;
+        s_box[3] = s_box[7] = INT_MIN;
+        s_ref[0] = s_ref[1] = ~0ull;
         s_frmin = INT_MAX;
         s_frmax = -1;
         s_fdmin = INT_MAX;
@@ -351,15 +360,14 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
     }
 
     // deposit state kept per particle: ring, spoke, zone (0 = none), weights
-    int dring[PPT], dspoke[PPT], dzone[PPT];
+    // dcell: zone << 28 | ring << 16 | spoke (zone 0: no deposit)
+    int dcell[PPT];
     R dw[PPT][4];
     // off-grid list changes: +1 leaving the grid, -1 returning
     bool dn_plus[PPT], dn_minus[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
-        dzone[k] = 0;
-        dring[k] = 0;
-        dspoke[k] = 0;
+        dcell[k] = 0;
         dn_plus[k] = false;
         dn_minus[k] = false;
         if (mv[k]) {
@@ -431,8 +439,9 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
                 ut_of(nx, ny, rnew, a.k, u, t, mtab);
                 utb[j] = mk2(u, t);
                 R tf = a.nsect > 1 ? (R)fmod(t, a.thmax) : t;
-                cic(u, tf, nr, na, dring[k], dspoke[k], dw[k]);
-                dzone[k] = znew;
+                int ring, spoke;
+                cic(u, tf, nr, na, ring, spoke, dw[k]);
+                dcell[k] = (int)(((unsigned)znew << 28) | ((unsigned)ring << 16) | (unsigned)spoke);
             }
         }
     }
@@ -450,107 +459,162 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
             if (wd) atomicAdd(&s_dnoff, wd);
         }
     }
-    // the tile's zone and reference spoke: thread 0's first deposit
-    if (tid == 0) {
+    // Deposit passes.  A pass takes a reference deposit -- the pending one
+    // first in particle order from the block's middle particle -- and the
+    // box of the pending deposits of the reference's zone (rings x spoke
+    // offsets from the reference spoke, which wrap around the ring); a box
+    // over TILE_CAP cells is clipped about the reference, which therefore
+    // always fits.  The deposits inside go to the LDS tile, which is then
+    // flushed with one global atomic per touched cell.  What the clip or a
+    // second zone leaves pending gets the next pass; after PUSH_PASSES
+    // passes (and for a cell on the last ring) the corners go to global
+    // atomics directly.
+    bool pend[PPT];
 #pragma unroll
-        for (int k = PPT - 1; k >= 0; k--)
-            if (dzone[k] > 0) {
-                s_zt = dzone[k];
-                s_sref = dspoke[k];
-            }
-    }
-    __syncthreads();
-    const int zt = s_zt, sref = s_sref;        // zt == 0: no tile
-    // ring and spoke-offset ranges of the tile zone's deposits: registers,
-    // then one wavefront reduction; other zones' ring ranges go to LDS
-    int rmin = INT_MAX, rmax = -1, dmin = INT_MAX, dmax = INT_MIN;
-    int dd[PPT];
+    for (int k = 0; k < PPT; k++) pend[k] = dcell[k] != 0 && dc_ring(dcell[k]) + 1 < nr;
+    // this thread's reference candidate: the lowest rotated particle index
+    // among its pending deposits; the wave's lowest goes to s_ref[slot]
+    auto offer_ref = [&](int slot) {
+        int myrot = INT_MAX, mypack = 0;
 #pragma unroll
-    for (int k = 0; k < PPT; k++) {
-        dd[k] = 0;
-        if (dzone[k] > 0) {
-            if (dzone[k] == zt) {
-                int d = dspoke[k] - sref;
-                if (d < -half) d += na;
-                else if (d >= half) d -= na;
-                dd[k] = d;
-                rmin = min(rmin, dring[k]);
-                rmax = max(rmax, dring[k]);
-                dmin = min(dmin, d);
-                dmax = max(dmax, d);
-            } else {
-                atomicMin(&s_jr[dzone[k] - 1], dring[k]);
-                atomicMax(&s_kr[dzone[k] - 1], dring[k]);
+        for (int k = 0; k < PPT; k++) {
+            const int rot = (k * PB + tid - PB * PPT / 2) & (PB * PPT - 1);
+            if (pend[k] && rot < myrot) {
+                myrot = rot;
+                mypack = dcell[k];
             }
         }
-    }
-    rmin = wave_min(rmin);
-    rmax = wave_max(rmax);
-    dmin = wave_min(dmin);
-    dmax = wave_max(dmax);
-    if (lane == 0 && rmax >= 0) {
-        atomicMin(&s_rmin, rmin);
-        atomicMax(&s_rmax, rmax);
-        atomicMin(&s_dmin, dmin);
-        atomicMax(&s_dmax, dmax);
-    }
+        const int wmin = wave_min(myrot);
+        if (wmin != INT_MAX) {
+            const unsigned long long bal = __ballot(myrot == wmin);
+            const int fl = __ffsll((long long)bal) - 1;
+            const int pk = __builtin_amdgcn_readlane(mypack, fl);
+            if (lane == 0)
+                atomicMin(&s_ref[slot], ((unsigned long long)(unsigned)wmin << 32) | (unsigned)pk);
+        }
+    };
+    offer_ref(0);
     __syncthreads();
-    // tile: rings [r0, r1 + 1] x spoke offsets [d0, d1 + 1] of zone zt
-    const int r0 = s_rmin, r1 = s_rmax, d0 = s_dmin, d1 = s_dmax;
-    const int tw = d1 - d0 + 2, th = r1 - r0 + 2;
-    const bool use_tile = zt > 0 && r1 >= 0 && r1 + 1 < nr && th * tw <= TILE_CAP;
-    const int tlen = use_tile ? th * tw : 0;
-    if (tid == 0 && zt > 0 && r1 >= 0) {
-        atomicMin(&s_jr[zt - 1], r0);
-        atomicMax(&s_kr[zt - 1], r1);
-    }
-    for (int j = tid; j < tlen; j += PB) {
+    for (int pass = 0;; pass++) {
+        const unsigned long long kref = s_ref[pass & 1];
+        if (kref == ~0ull) break;                  // nothing pending (uniform)
+        const int zt = (int)((kref >> 28) & 0xF), rref = (int)((kref >> 16) & 0xFFF);
+        const int sref = (int)(kref & 0xFFFF);
+        int *const bx = s_box + 4 * (pass & 1);
+        if (tid == 0) s_ref[(pass + 1) & 1] = ~0ull;   // last read in the previous pass
+        // ring and spoke-offset box of the zone's pending deposits
+        {
+            int rmin = INT_MAX, rmax = -1, dmin = INT_MAX, dmax = INT_MIN;
 #pragma unroll
-        for (int rep = 0; rep < TILE_REP; rep++) tile[rep][j] = 0;
-    }
-    __syncthreads();
-
-    long long *trep = tile[lane & (TILE_REP - 1)];
+            for (int k = 0; k < PPT; k++) {
+                if (pend[k] && dc_zone(dcell[k]) == zt) {
+                    const int ring = dc_ring(dcell[k]);
+                    int d = dc_spoke(dcell[k]) - sref;
+                    if (d < -half) d += na;
+                    else if (d >= half) d -= na;
+                    rmin = min(rmin, ring);
+                    rmax = max(rmax, ring);
+                    dmin = min(dmin, d);
+                    dmax = max(dmax, d);
+                }
+            }
+            rmin = wave_min(rmin);
+            rmax = wave_max(rmax);
+            dmin = wave_min(dmin);
+            dmax = wave_max(dmax);
+            if (lane == 0 && rmax >= 0) {
+                atomicMin(&bx[0], rmin);
+                atomicMax(&bx[1], rmax);
+                atomicMin(&bx[2], dmin);
+                atomicMax(&bx[3], dmax);
+            }
+        }
+        __syncthreads();
+        // tile: rings [r0, r0 + th) x spoke offsets [d0, d0 + tw) of zone zt
+        int r0 = bx[0], th = bx[1] - bx[0] + 2, d0 = bx[2], tw = bx[3] - bx[2] + 2;
+        if (tid == 0) {
+            atomicMin(&s_jr[zt - 1], bx[0]);
+            atomicMax(&s_kr[zt - 1], bx[1]);
+            int *const nb = s_box + 4 * ((pass + 1) & 1);   // last read in the previous pass
+            nb[0] = INT_MAX;
+            nb[1] = -1;
+            nb[2] = INT_MAX;
+            nb[3] = INT_MIN;
+        }
+        if (th * tw > TILE_CAP) {
+            if (th > 8) {
+                r0 = min(max(rref - 3, r0), r0 + th - 8);
+                th = 8;
+            }
+            const int mw = TILE_CAP / th;
+            if (tw > mw) {
+                d0 = min(max(-((mw - 2) >> 1), d0), d0 + tw - mw);
+                tw = mw;
+            }
+        }
+        const int tlen = th * tw;
+        for (int j = tid; j < tlen; j += PB) {
 #pragma unroll
-    for (int k = 0; k < PPT; k++) {
-        if (dzone[k] == 0) continue;
-        long long qv[4];
+            for (int rep = 0; rep < TILE_REP; rep++) tile[rep][j] = 0;
+        }
+        __syncthreads();
+        long long *trep = tile[lane & (TILE_REP - 1)];
 #pragma unroll
-        for (int q = 0; q < 4; q++) qv[q] = quant<R>(dw[k][q], qbits);
-        if (use_tile && dzone[k] == zt) {
-            const int b = (dring[k] - r0) * tw + (dd[k] - d0);
+        for (int k = 0; k < PPT; k++) {
+            if (!pend[k] || dc_zone(dcell[k]) != zt) continue;
+            const int ring = dc_ring(dcell[k]);
+            int d = dc_spoke(dcell[k]) - sref;
+            if (d < -half) d += na;
+            else if (d >= half) d -= na;
+            if (ring < r0 || ring - r0 + 1 >= th || d < d0 || d - d0 + 1 >= tw) continue;
+            const int b = (ring - r0) * tw + (d - d0);
             const int ti[4] = {b, b + 1, b + tw, b + tw + 1};
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (qv[q] != 0)
-                    atomicAdd((unsigned long long *)&trep[ti[q]], (unsigned long long)qv[q]);
-        } else {
-            int ic[4];
-            corners_rs(dring[k], dspoke[k], na, ic);
+            for (int q = 0; q < 4; q++) {
+                const long long qv = quant<R>(dw[k][q], qbits);
+                if (qv != 0) atomicAdd((unsigned long long *)&trep[ti[q]], (unsigned long long)qv);
+            }
+            pend[k] = false;
+        }
+        const bool last = pass + 1 >= GH_PUSH_PASSES;
+        if (!last) offer_ref((pass + 1) & 1);
+        __syncthreads();
+        // flush: one global atomic per touched cell
+        for (int row = 0; row < th; row++) {
+            const int ring = r0 + row;
+            for (int col = tid; col < tw; col += PB) {
+                const int j = row * tw + col;
+                long long v = 0;
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (qv[q] != 0) deposit_global(a.mfx, nz, a.mesh, dzone[k], ic[q], qv[q], mo);
+                for (int rep = 0; rep < TILE_REP; rep++) v += tile[rep][j];
+                if (v != 0) {
+                    int spoke = sref + d0 + col;
+                    if (spoke < 0) spoke += na;
+                    else if (spoke >= na) spoke -= na;
+                    if (spoke >= na) spoke -= na;
+                    atomicAdd((unsigned long long *)&a.mfx[zt - 1][mo + ring * na + spoke],
+                              (unsigned long long)v);
+                }
+            }
+        }
+        if (last) break;
+    }
+    // what no pass took (and cells on the last ring): global atomics
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const int z = dc_zone(dcell[k]), ring = dc_ring(dcell[k]);
+        if (z == 0 || (ring + 1 < nr && !pend[k])) continue;
+        atomicMin(&s_jr[z - 1], ring);
+        atomicMax(&s_kr[z - 1], ring);
+        int ic[4];
+        corners_rs(ring, dc_spoke(dcell[k]), na, ic);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long long qv = quant<R>(dw[k][q], qbits);
+            if (qv != 0) deposit_global(a.mfx, nz, a.mesh, z, ic[q], qv, mo);
         }
     }
     __syncthreads();
-    // flush: one global atomic per touched cell
-    for (int row = 0; row < th && use_tile; row++) {
-        const int ring = r0 + row;
-        for (int col = tid; col < tw; col += PB) {
-            const int j = row * tw + col;
-            long long v = 0;
-#pragma unroll
-            for (int rep = 0; rep < TILE_REP; rep++) v += tile[rep][j];
-            if (v != 0) {
-                int spoke = sref + d0 + col;
-                if (spoke < 0) spoke += na;
-                else if (spoke >= na) spoke -= na;
-                if (spoke >= na) spoke -= na;
-                atomicAdd((unsigned long long *)&a.mfx[zt - 1][mo + ring * na + spoke],
-                          (unsigned long long)v);
-            }
-        }
-    }
     if (tid < nz && s_kr[tid] >= 0) {
         atomicMin(&jrad[tid], s_jr[tid] + 1);
         int k = s_kr[tid] + 2;
