@@ -678,7 +678,7 @@ def main():
         main_p3d(args)
         return
     if args.sort_interval is None:
-        args.sort_interval = 10
+        args.sort_interval = 16
     from galaxy_amd import configs
     w = configs.C2
     n = args.n or w.n
