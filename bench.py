@@ -78,8 +78,8 @@ P3_PUSH_HBM_READ_B = 35.9
 P3_PUSH_HBM_WRITE_B = 39.3
 # the same for push_kernel<float, true> on the 64-disc ensemble (mean of the
 # dispatches in profiles/push_kernel_ens-64x128x256-n8.6e7_traffic.txt)
-ENS_PUSH_HBM_READ_B = 26.7
-ENS_PUSH_HBM_WRITE_B = 29.5
+ENS_PUSH_HBM_READ_B = 26.9
+ENS_PUSH_HBM_WRITE_B = 26.0
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # algorithmic grid bytes of one C2 step besides the particles: the F_R and
 # F_phi tables in FP32 (2 x 17 harmonics x 256 x 256 x 4 B, read once by the

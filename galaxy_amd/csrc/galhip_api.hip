@@ -9,7 +9,7 @@
 // produces them, and the solve kernels read them there.
 #include "galhip_ctx.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -405,7 +405,7 @@ static int ensure_particles(galhip_ctx *c, long long n) {
     rc |= dalloc(&c->perm, sizeof(unsigned) * n); rc |= dalloc(&c->perm2, sizeof(unsigned) * n);
     if (rc) { free_particles(c); return rc; }
     size_t tmp = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+    hipError_t e = rocprim::radix_sort_pairs(
         nullptr, tmp, c->key, c->key2, c->perm, c->perm2, (int)n, 0, 32, c->s);
     if (e != hipSuccess) return gh_fail(GALHIP_EHIP, "radix sort sizing: %s", hipGetErrorString(e));
     rc = dalloc(&c->sort_tmp, tmp);
@@ -442,7 +442,7 @@ template <typename R> static int sort_impl(galhip_ctx *c) {
                              c->na, c->nz, kshift, kmax, c->key, c->perm, c->s));
     }
     size_t tmp = c->sort_tmp_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->key, c->key2,
+    HIPCHK(rocprim::radix_sort_pairs(c->sort_tmp, tmp, c->key, c->key2,
                                               c->perm, c->perm2, (int)n, 0,
                                               end_bit, c->s));
     PermArgs<R> p;

@@ -7,7 +7,7 @@
 #include "galhip_ctx.h"
 #include "galhip_p3d.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cmath>
 #include <cstring>
@@ -273,7 +273,7 @@ static int p3_ensure(galhip_ctx *c, long long n) {
              dalloc(&c->perm2, sizeof(unsigned) * m);
     if (rc) { p3_free_particles(c); return rc; }
     size_t tmp = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->key, c->key2, c->perm,
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp, c->key, c->key2, c->perm,
                                                       c->perm2, (int)m, 0, 32, c->s);
     if (e != hipSuccess) {
         p3_free_particles(c);
@@ -379,7 +379,7 @@ template <typename R> static int p3_sort_impl(galhip_ctx *c) {
     int end_bit = 1;
     while ((1LL << end_bit) <= (long long)kmax) end_bit++;
     size_t tmp = c->sort_tmp_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->key, c->key2, c->perm,
+    HIPCHK(rocprim::radix_sort_pairs(c->sort_tmp, tmp, c->key, c->key2, c->perm,
                                               c->perm2, (int)n, 0, end_bit, c->s));
     P3PermArgs<R> p;
     p.xv_in = (const typename Vec4<R>::T *)c->xv;
