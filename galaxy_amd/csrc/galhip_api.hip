@@ -118,7 +118,8 @@ static int p2d_create(const galhip_p2d_params *p, int nmem, const double *pmass,
     if (p->precision != GALHIP_FP32 && p->precision != GALHIP_FP64)
         return gh_fail(GALHIP_EINVAL, "precision must be GALHIP_FP32 or GALHIP_FP64");
     int na = 2 * (p->na / 2);                       // setgrd.f:58
-    if (p->nr < 3 || na < 4 || na > 2048)
+    // nr <= 4096: the push packs a deposit's zone, ring and spoke in one word
+    if (p->nr < 3 || p->nr > 4096 || na < 4 || na > 2048)
         return gh_fail(GALHIP_EINVAL, "grid %d x %d out of range", p->nr, p->na);
     if (p->nzones < 1 || p->nzones > GALHIP_MAXZONES)
         return gh_fail(GALHIP_EINVAL, "nzones %d out of range", p->nzones);

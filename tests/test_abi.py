@@ -69,6 +69,24 @@ def test_p2dset_rejects_grids_coarse_against_softening():
     assert rc == -3 and b"SETGRD" in lib.galhip_last_error()
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgalhip.so not built")
+def test_p2dset_rejects_grids_out_of_range():
+    """Grids outside 3 <= nr <= 4096, 4 <= na <= 2048 return GALHIP_EINVAL
+    before any device call (nr <= 4096: the push kernel packs a deposit's
+    zone, ring and spoke into one 32-bit word)."""
+    from galaxy_amd import _lib
+    lib = _lib.load()
+    for nr, na in ((4097, 64), (2, 64), (64, 2050)):
+        p = _lib.P2DParams()
+        p.nr, p.na, p.hash, p.nsect, p.nskip = nr, na, 2, 1, 0
+        p.tsoft, p.nzones, p.offanal, p.pmass, p.softl = 1, 1, 1, 1.0, 0.1
+        p.nstep[0] = 1
+        p.precision = _lib.GALHIP_FP32
+        ctx = ctypes.c_void_p()
+        rc = lib.galhip_p2dset(ctypes.byref(p), ctypes.byref(ctx))
+        assert rc == -1 and b"out of range" in lib.galhip_last_error(), (nr, na, rc)
+
+
 def test_fortran_module_declares_every_entry_point():
     """include/galhip_f.f90 binds every function galhip.h declares."""
     text = open(os.path.join(ROOT, "include", "galhip_f.f90")).read()
