@@ -37,7 +37,8 @@ struct galhip_ctx {
     // device storage
     hipStream_t s;
     long long n, cap;
-    void *xv, *ut;                     // (x, y, vx, vy) and (u, t) per particle
+    void *xv, *ut;                     // (x, y, vx, vy) and (u, t) per particle; a p3d
+                                       // context keeps its particle records in xv (ut unused)
     uint8_t *zone;
     unsigned *id;
     void *xv2, *ut2;
@@ -69,7 +70,6 @@ struct galhip_ctx {
     // 3-D polar grid (p3d); p3d = 0 for the 2-D grid
     int p3d, ngz, nm3, jmass;
     double dzg, zm;                   // in the context's precision
-    void *vw, *vw2;                   // (vy, vz) per particle
     void *A3, *M3, *W3, *Z3;          // solve work arrays
     void *fld4;                       // [mesh] (F_x, F_y, F_z, potential)
     void *twz;                        // cos, sin of 2 pi k / (2 ngz)

@@ -5,10 +5,11 @@
 
 namespace gh {
 
+// A particle is one record of two R4 (32 B in FP32): rec[2 i] = (x, y, z, vx),
+// rec[2 i + 1] = (vy, vz, u, t), (u, t) the grid coordinates of the current
+// position; a sort's gather then touches one record per particle.
 template <typename R> struct P3Args {
-    typename Vec4<R>::T *xv;          // (x, y, z, vx)
-    typename Vec2<R>::T *vw;          // (vy, vz)
-    typename Vec2<R>::T *ut;          // (u, t) of the current position
+    typename Vec4<R>::T *rec;         // 2 per particle
     uint8_t *zone;
     long long n;
     const typename Vec4<R>::T *fld;   // (F_x, F_y, F_z, potential) per cell
@@ -58,12 +59,8 @@ template <typename R> struct P3SolveArgs {
 };
 
 template <typename R> struct P3PermArgs {
-    const typename Vec4<R>::T *xv_in;
-    typename Vec4<R>::T *xv_out;
-    const typename Vec2<R>::T *vw_in;
-    typename Vec2<R>::T *vw_out;
-    const typename Vec2<R>::T *ut_in;
-    typename Vec2<R>::T *ut_out;
+    const typename Vec4<R>::T *rec_in;   // 2 per particle
+    typename Vec4<R>::T *rec_out;
     const uint8_t *zin;
     uint8_t *zout;
     const unsigned *iin;
@@ -77,12 +74,12 @@ hipError_t launch_p3_key(const P3Args<R> &, int kshift, unsigned kmax, unsigned 
                          unsigned *perm, hipStream_t);
 template <typename R> hipError_t launch_p3_permute(const P3PermArgs<R> &, hipStream_t);
 template <typename R>
-hipError_t launch_p3_interleave(const R *s4, const R *s2, typename Vec4<R>::T *xv,
-                                typename Vec2<R>::T *vw, unsigned *id, long long n, hipStream_t);
+hipError_t launch_p3_interleave(const R *s4, const R *s2, typename Vec4<R>::T *rec,
+                                unsigned *id, long long n, hipStream_t);
 template <typename R>
-hipError_t launch_p3_deinterleave(const typename Vec4<R>::T *xv, const typename Vec2<R>::T *vw,
-                                  const uint8_t *zone, const unsigned *id, R *d4, R *d2, int *iz,
-                                  long long n, hipStream_t);
+hipError_t launch_p3_deinterleave(const typename Vec4<R>::T *rec, const uint8_t *zone,
+                                  const unsigned *id, R *d4, R *d2, int *iz, long long n,
+                                  hipStream_t);
 template <typename R> hipError_t launch_p3_massgn(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_push(const P3Args<R> &, hipStream_t);
 template <typename R> hipError_t launch_p3_tmcent(const P3Args<R> &, hipStream_t);

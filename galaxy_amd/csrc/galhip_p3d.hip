@@ -37,6 +37,12 @@ namespace gh {
 
 constexpr int P3_BLOCK = 256;    // solve kernels
 constexpr int P3_PBLOCK = 512;   // particle kernels: a larger block shares one deposit window
+
+// (u, t) of particle i: the second half of its record's second R4
+template <typename R>
+__device__ __forceinline__ void p3_set_ut(const P3Args<R> &a, long long i, R u, R t) {
+    reinterpret_cast<typename Vec2<R>::T *>(a.rec + 2 * i + 1)[1] = mk2(u, t);
+}
 constexpr int P3_MAX_NA = 2048;
 constexpr int P3_MAX_NM = 128;   // compact components (2 ng)
 constexpr int P3_MAX_NR = 1024;
@@ -477,10 +483,10 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_massgn_kernel(P3Args<R> a) {
     R w[P3NW<J>::v];
     if (i < a.n) {
         z = a.zone[i];
-        const typename Vec4<R>::T p = a.xv[i];
+        const typename Vec4<R>::T p = a.rec[2 * i];
         R u, t;
         ut3(p.x, p.y, a, u, t);
-        a.ut[i] = mk2(u, t);
+        p3_set_ut(a, i, u, t);
         if (z >= 1 && z <= a.nz) {
             c = cellJ<R, J>(u, t, p.z, a, a.nsect > 1, w, rg, pl, sp);
             on = true;
@@ -521,9 +527,10 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
         const bool move = offlist ? (a.offanal && a.mzone == nz) : z0 <= a.mzone;
         if (move) {
             np = 1;
-            const R4 p = a.xv[i];
-            const R2 q = a.vw[i];
-            const R2 ut0 = a.ut[i];
+            const R4 p = a.rec[2 * i];
+            const R4 q4 = a.rec[2 * i + 1];
+            const R2 q = mk2(q4.x, q4.y);
+            const R2 ut0 = mk2(q4.z, q4.w);
             R o[3] = {p.x, p.y, p.z};
             R ov[3] = {p.w, q.x, q.y};
             R acc[3], nv[3], nx[3];
@@ -566,9 +573,8 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_push_kernel(P3Args<R> a) {
             }
             R un, tn;
             ut3(nx[0], nx[1], a, un, tn);
-            a.xv[i] = mk4(nx[0], nx[1], nx[2], nv[0]);
-            a.vw[i] = mk2(nv[1], nv[2]);
-            a.ut[i] = mk2(un, tn);
+            a.rec[2 * i] = mk4(nx[0], nx[1], nx[2], nv[0]);
+            a.rec[2 * i + 1] = mk4(nv[1], nv[2], un, tn);
             a.zone[i] = (uint8_t)znew;
             if (znew >= 1 && znew <= nz) {
                 dc = cellJ<R, J>(un, tn, nx[2], a, a.nsect > 1, dw, drg, dpl, dsp);
@@ -598,26 +604,25 @@ template <typename R>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_ut_kernel(P3Args<R> a) {
     const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= a.n) return;
-    const typename Vec4<R>::T p = a.xv[i];
+    const typename Vec4<R>::T p = a.rec[2 * i];
     R u, t;
     ut3(p.x, p.y, a, u, t);
-    a.ut[i] = mk2(u, t);
+    p3_set_ut(a, i, u, t);
 }
 
 template <typename R, int J>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_tmcent_kernel(P3Args<R> a) {
     const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= a.n) return;
-    typename Vec4<R>::T p = a.xv[i];
-    typename Vec2<R>::T q = a.vw[i];
-    const typename Vec2<R>::T ut = a.ut[i];
+    typename Vec4<R>::T p = a.rec[2 * i];
+    typename Vec4<R>::T q = a.rec[2 * i + 1];   // (vy, vz, u, t)
     R acc[3];
-    getacc3<R, J>(a, p.x, p.y, p.z, ut.x, ut.y, a.zone[i], acc);
+    getacc3<R, J>(a, p.x, p.y, p.z, q.z, q.w, a.zone[i], acc);
     p.w = p.w - a.fac * acc[0];
     q.x = q.x - a.fac * acc[1];
     q.y = q.y - a.fac * acc[2];
-    a.xv[i] = p;
-    a.vw[i] = q;
+    a.rec[2 * i] = p;
+    a.rec[2 * i + 1] = q;
 }
 
 template <typename R, int J>
@@ -677,9 +682,10 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_integrals_kernel(P3Args<R> a, in
     if (i < a.n) {
         const int z = a.zone[i];
         if (z >= 1) {
-            const R4 p = a.xv[i];
-            const typename Vec2<R>::T q2 = a.vw[i];
-            const typename Vec2<R>::T ut = a.ut[i];
+            const R4 p = a.rec[2 * i];
+            const R4 q4 = a.rec[2 * i + 1];
+            const typename Vec2<R>::T q2 = mk2(q4.x, q4.y);
+            const typename Vec2<R>::T ut = mk2(q4.z, q4.w);
             R vx = p.w, vy = q2.x, vz = q2.y;
             if (centred) {
                 R acc[3];
@@ -760,8 +766,9 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_key_kernel(P3Args<R> a, int kshi
     const int z = a.zone[i];
     unsigned k = kmax;
     if (z >= 1 && z <= a.nz) {
-        const typename Vec4<R>::T p = a.xv[i];
-        const typename Vec2<R>::T ut = a.ut[i];
+        const typename Vec4<R>::T p = a.rec[2 * i];
+        const typename Vec4<R>::T q = a.rec[2 * i + 1];
+        const typename Vec2<R>::T ut = mk2(q.z, q.w);
         R w[P3NW<J>::v];
         k = (unsigned)(cellJ<R, J>(ut.x, ut.y, p.z, a, false, w) >> kshift);
     }
@@ -774,21 +781,19 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_permute_kernel(P3PermArgs<R> p) 
     const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= p.n) return;
     const unsigned j = p.perm[i];
-    p.xv_out[i] = p.xv_in[j];
-    p.vw_out[i] = p.vw_in[j];
-    p.ut_out[i] = p.ut_in[j];
+    p.rec_out[2 * i] = p.rec_in[2 * (long long)j];
+    p.rec_out[2 * i + 1] = p.rec_in[2 * (long long)j + 1];
     p.zout[i] = p.zin[j];
     p.iout[i] = p.iin[j];
 }
 
 // load: planar host arrays staged on the device (x, y, z, vx in s4 and vy, vz
-// in s2, n values each, one after another) -> interleaved storage; ids = load
-// order
+// in s2, n values each, one after another) -> particle records ((u, t) set
+// by p3_ut_kernel afterwards); ids = load order
 template <typename R>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_interleave_kernel(const R *__restrict__ s4,
                                                                   const R *__restrict__ s2,
-                                                                  typename Vec4<R>::T *__restrict__ xv,
-                                                                  typename Vec2<R>::T *__restrict__ vw,
+                                                                  typename Vec4<R>::T *__restrict__ rec,
                                                                   unsigned *__restrict__ id,
                                                                   long long n) {
     const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
@@ -798,25 +803,22 @@ __global__ __launch_bounds__(P3_PBLOCK) void p3_interleave_kernel(const R *__res
     a.y = s4[n + i];
     a.z = s4[2 * n + i];
     a.w = s4[3 * n + i];
-    typename Vec2<R>::T b;
-    b.x = s2[i];
-    b.y = s2[n + i];
-    xv[i] = a;
-    vw[i] = b;
+    rec[2 * i] = a;
+    rec[2 * i + 1] = mk4(s2[i], s2[n + i], (R)0, (R)0);
     id[i] = (unsigned)i;
 }
 
 // unload: interleaved storage -> planar arrays in load order, zones widened
 template <typename R>
 __global__ __launch_bounds__(P3_PBLOCK) void p3_deinterleave_kernel(
-    const typename Vec4<R>::T *__restrict__ xv, const typename Vec2<R>::T *__restrict__ vw,
-    const uint8_t *__restrict__ zone, const unsigned *__restrict__ id, R *__restrict__ d4,
-    R *__restrict__ d2, int *__restrict__ iz, long long n) {
+    const typename Vec4<R>::T *__restrict__ rec, const uint8_t *__restrict__ zone,
+    const unsigned *__restrict__ id, R *__restrict__ d4, R *__restrict__ d2,
+    int *__restrict__ iz, long long n) {
     const long long i = (long long)blockIdx.x * P3_PBLOCK + threadIdx.x;
     if (i >= n) return;
     const long long p = id[i];
-    const typename Vec4<R>::T a = xv[i];
-    const typename Vec2<R>::T b = vw[i];
+    const typename Vec4<R>::T a = rec[2 * i];
+    const typename Vec4<R>::T b = rec[2 * i + 1];
     d4[p] = a.x;
     d4[n + p] = a.y;
     d4[2 * n + p] = a.z;
@@ -1252,22 +1254,21 @@ hipError_t launch_p3_key(const P3Args<R> &a, int kshift, unsigned kmax, unsigned
 }
 
 template <typename R>
-hipError_t launch_p3_interleave(const R *s4, const R *s2, typename Vec4<R>::T *xv,
-                                typename Vec2<R>::T *vw, unsigned *id, long long n,
-                                hipStream_t s) {
+hipError_t launch_p3_interleave(const R *s4, const R *s2, typename Vec4<R>::T *rec,
+                                unsigned *id, long long n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(p3_interleave_kernel<R>, dim3(p3_blocks(n)), dim3(P3_PBLOCK), 0, s, s4,
-                       s2, xv, vw, id, n);
+                       s2, rec, id, n);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_p3_deinterleave(const typename Vec4<R>::T *xv, const typename Vec2<R>::T *vw,
-                                  const uint8_t *zone, const unsigned *id, R *d4, R *d2,
-                                  int *iz, long long n, hipStream_t s) {
+hipError_t launch_p3_deinterleave(const typename Vec4<R>::T *rec, const uint8_t *zone,
+                                  const unsigned *id, R *d4, R *d2, int *iz, long long n,
+                                  hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(p3_deinterleave_kernel<R>, dim3(p3_blocks(n)), dim3(P3_PBLOCK), 0, s, xv,
-                       vw, zone, id, d4, d2, iz, n);
+    hipLaunchKernelGGL(p3_deinterleave_kernel<R>, dim3(p3_blocks(n)), dim3(P3_PBLOCK), 0, s,
+                       rec, zone, id, d4, d2, iz, n);
     return hipGetLastError();
 }
 
@@ -1348,10 +1349,8 @@ hipError_t launch_p3_green(double *G, const double *twd, const double *twzd, con
                                          unsigned *, hipStream_t);                     \
     template hipError_t launch_p3_permute<R>(const P3PermArgs<R> &, hipStream_t);          \
     template hipError_t launch_p3_interleave<R>(const R *, const R *, typename Vec4<R>::T *,  \
-                                                typename Vec2<R>::T *, unsigned *, long long, \
-                                                hipStream_t);                                 \
-    template hipError_t launch_p3_deinterleave<R>(const typename Vec4<R>::T *,               \
-                                                  const typename Vec2<R>::T *, const uint8_t *, \
+                                                unsigned *, long long, hipStream_t);          \
+    template hipError_t launch_p3_deinterleave<R>(const typename Vec4<R>::T *, const uint8_t *, \
                                                   const unsigned *, R *, R *, int *, long long, \
                                                   hipStream_t);
 P3_INST(float)

@@ -206,7 +206,6 @@ extern "C" int galhip_p3dset(const galhip_p3d_params *p, galhip_ctx **out) {
 }
 
 void p3_free(galhip_ctx *c) {
-    dfree(c->vw); dfree(c->vw2);
     dfree(c->A3); dfree(c->M3); dfree(c->W3); dfree(c->Z3); dfree(c->fld4);
     dfree(c->twz); dfree(c->twzd);
     dfree(c->slot_m_d); dfree(c->slot_p_d); dfree(c->slot_of_d);
@@ -252,8 +251,8 @@ int p3_set_green(galhip_ctx *c, const double *G) {
 template <typename R> static P3Args<R> p3_args(galhip_ctx *c);
 
 static void p3_free_particles(galhip_ctx *c) {
-    dfree(c->xv); dfree(c->vw); dfree(c->ut); dfree(c->zone); dfree(c->id);
-    dfree(c->xv2); dfree(c->vw2); dfree(c->ut2); dfree(c->zone2); dfree(c->id2);
+    dfree(c->xv); dfree(c->ut); dfree(c->zone); dfree(c->id);
+    dfree(c->xv2); dfree(c->ut2); dfree(c->zone2); dfree(c->id2);
     dfree(c->key); dfree(c->key2); dfree(c->perm); dfree(c->perm2); dfree(c->sort_tmp);
     c->sort_tmp_bytes = 0;
     c->n = c->cap = 0;
@@ -264,10 +263,10 @@ static int p3_ensure(galhip_ctx *c, long long n) {
     if (n <= c->cap) { c->n = n; return 0; }
     p3_free_particles(c);
     const size_t es = esize(c), m = n > 0 ? n : 1;
-    int rc = dalloc(&c->xv, 4 * es * m) | dalloc(&c->vw, 2 * es * m) | dalloc(&c->zone, m) |
-             dalloc(&c->ut, 2 * es * m) | dalloc(&c->ut2, 2 * es * m) |
-             dalloc(&c->id, sizeof(unsigned) * m) | dalloc(&c->xv2, 4 * es * m) |
-             dalloc(&c->vw2, 2 * es * m) | dalloc(&c->zone2, m) |
+    // particle records, 8 values each (see P3Args), and the sort's second copy
+    int rc = dalloc(&c->xv, 8 * es * m) | dalloc(&c->zone, m) |
+             dalloc(&c->id, sizeof(unsigned) * m) | dalloc(&c->xv2, 8 * es * m) |
+             dalloc(&c->zone2, m) |
              dalloc(&c->id2, sizeof(unsigned) * m) | dalloc(&c->key, sizeof(unsigned) * m) |
              dalloc(&c->key2, sizeof(unsigned) * m) | dalloc(&c->perm, sizeof(unsigned) * m) |
              dalloc(&c->perm2, sizeof(unsigned) * m);
@@ -309,9 +308,9 @@ extern "C" int galhip_restrt3(galhip_ctx *c, int64_t n, const void *x, const voi
     }
     HIPCHK(hipMemcpy(c->counters, cnt, sizeof cnt, hipMemcpyHostToDevice));
     if (n > 0) {
-        // the planar arrays through the second particle buffers (free outside
-        // a sort), interleaved on the device
-        unsigned char *t4 = (unsigned char *)c->xv2, *t2 = (unsigned char *)c->vw2;
+        // the planar arrays through the second record buffer (free outside a
+        // sort, 8 n values), interleaved on the device
+        unsigned char *t4 = (unsigned char *)c->xv2, *t2 = t4 + 4 * es * n;
         const void *s4[4] = {x, y, z, vx};
         const void *s2[2] = {vy, vz};
         for (int k = 0; k < 4; k++)
@@ -320,11 +319,10 @@ extern "C" int galhip_restrt3(galhip_ctx *c, int64_t n, const void *x, const voi
             HIPCHK(hipMemcpy(t2 + k * es * n, s2[k], es * n, hipMemcpyHostToDevice));
         if (c->prec == GALHIP_FP32)
             HIPCHK(launch_p3_interleave<float>((const float *)t4, (const float *)t2,
-                                               (float4 *)c->xv, (float2 *)c->vw, c->id, n, c->s));
+                                               (float4 *)c->xv, c->id, n, c->s));
         else
             HIPCHK(launch_p3_interleave<double>((const double *)t4, (const double *)t2,
-                                                (double4 *)c->xv, (double2 *)c->vw, c->id, n,
-                                                c->s));
+                                                (double4 *)c->xv, c->id, n, c->s));
         HIPCHK(hipMemcpy(c->zone, zz.data(), n, hipMemcpyHostToDevice));
     }
     c->qbits = qbits_for(n);
@@ -346,15 +344,14 @@ extern "C" int galhip_unload3(galhip_ctx *c, void *x, void *y, void *z, void *vx
     // planar, in load order, into the second particle buffers and the sort
     // keys (all free outside a sort), then straight to the caller's arrays
     const size_t es = esize(c);
-    unsigned char *t4 = (unsigned char *)c->xv2, *t2 = (unsigned char *)c->vw2;
+    unsigned char *t4 = (unsigned char *)c->xv2, *t2 = t4 + 4 * es * n;
     int *zs = (int *)c->key;
     if (c->prec == GALHIP_FP32)
-        HIPCHK(launch_p3_deinterleave<float>((const float4 *)c->xv, (const float2 *)c->vw, c->zone,
+        HIPCHK(launch_p3_deinterleave<float>((const float4 *)c->xv, c->zone,
                                              c->id, (float *)t4, (float *)t2, zs, n, c->s));
     else
-        HIPCHK(launch_p3_deinterleave<double>((const double4 *)c->xv, (const double2 *)c->vw,
-                                              c->zone, c->id, (double *)t4, (double *)t2, zs, n,
-                                              c->s));
+        HIPCHK(launch_p3_deinterleave<double>((const double4 *)c->xv, c->zone, c->id,
+                                              (double *)t4, (double *)t2, zs, n, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     void *d4[4] = {x, y, z, vx};
     void *d2[2] = {vy, vz};
@@ -382,12 +379,8 @@ template <typename R> static int p3_sort_impl(galhip_ctx *c) {
     HIPCHK(rocprim::radix_sort_pairs(c->sort_tmp, tmp, c->key, c->key2, c->perm,
                                               c->perm2, (int)n, 0, end_bit, c->s));
     P3PermArgs<R> p;
-    p.xv_in = (const typename Vec4<R>::T *)c->xv;
-    p.xv_out = (typename Vec4<R>::T *)c->xv2;
-    p.vw_in = (const typename Vec2<R>::T *)c->vw;
-    p.vw_out = (typename Vec2<R>::T *)c->vw2;
-    p.ut_in = (const typename Vec2<R>::T *)c->ut;
-    p.ut_out = (typename Vec2<R>::T *)c->ut2;
+    p.rec_in = (const typename Vec4<R>::T *)c->xv;
+    p.rec_out = (typename Vec4<R>::T *)c->xv2;
     p.zin = c->zone;
     p.zout = c->zone2;
     p.iin = c->id;
@@ -396,8 +389,6 @@ template <typename R> static int p3_sort_impl(galhip_ctx *c) {
     p.n = n;
     HIPCHK(launch_p3_permute<R>(p, c->s));
     std::swap(c->xv, c->xv2);
-    std::swap(c->vw, c->vw2);
-    std::swap(c->ut, c->ut2);
     std::swap(c->zone, c->zone2);
     std::swap(c->id, c->id2);
     c->steps_since_sort = 0;
@@ -417,9 +408,7 @@ int p3_unload(galhip_ctx *, void *, void *, void *, void *, int32_t *) {
 template <typename R> static P3Args<R> p3_args(galhip_ctx *c) {
     P3Args<R> a;
     memset(&a, 0, sizeof a);
-    a.xv = (typename Vec4<R>::T *)c->xv;
-    a.vw = (typename Vec2<R>::T *)c->vw;
-    a.ut = (typename Vec2<R>::T *)c->ut;
+    a.rec = (typename Vec4<R>::T *)c->xv;
     a.zone = c->zone;
     a.n = c->n;
     a.fld = (const typename Vec4<R>::T *)c->fld4;
