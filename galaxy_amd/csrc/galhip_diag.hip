@@ -34,7 +34,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 template <typename R>
 __global__ __launch_bounds__(LG_BLOCK) void lgsp_part_kernel(
-    const typename Vec4<R>::T *__restrict__ xv, const uint8_t *__restrict__ zone,
+    const typename Vec4<R>::T *__restrict__ xv, int ps, const uint8_t *__restrict__ zone,
     long long n, int nz, int offanal, int nsect, int np, int nm, double inv_lscale,
     double *__restrict__ part) {
     __shared__ double acc[LG_WAVES][2 * LG_MAXQ];
@@ -54,7 +54,8 @@ __global__ __launch_bounds__(LG_BLOCK) void lgsp_part_kernel(
         if (i < n) {
             const int z = zone[i];
             const bool inc = z >= 1 && (z <= nz || (offanal && z == nz + 1));
-            const double x = (double)xv[i].x, y = (double)xv[i].y;
+            const typename Vec4<R>::T p = xv[i * ps];   // x, y lead the particle's data
+            const double x = (double)p.x, y = (double)p.y;
             const double rc = sqrt(x * x + y * y);
             if (inc && rc > 0.0) {
                 const double cx = x / rc, sy = y / rc;
@@ -139,12 +140,12 @@ __global__ __launch_bounds__(LG_BLOCK) void lgsp_sum_kernel(const double *__rest
 }
 
 template <typename R>
-hipError_t launch_lgspia(const void *xv, const uint8_t *zone, long long n, int nz, int offanal,
-                         int nsect, int np, int nm, double lscale, double *part, int nb,
-                         double *out, hipStream_t s) {
+hipError_t launch_lgspia(const void *xv, int ps, const uint8_t *zone, long long n, int nz,
+                         int offanal, int nsect, int np, int nm, double lscale, double *part,
+                         int nb, double *out, hipStream_t s) {
     if (np * nm > LG_MAXQ) return hipErrorInvalidValue;
     hipLaunchKernelGGL(lgsp_part_kernel<R>, dim3(nb), dim3(LG_BLOCK), 0, s,
-                       (const typename Vec4<R>::T *)xv, zone, n, nz, offanal, nsect, np, nm,
+                       (const typename Vec4<R>::T *)xv, ps, zone, n, nz, offanal, nsect, np, nm,
                        1.0 / lscale, part);
     hipLaunchKernelGGL(lgsp_sum_kernel, dim3(1), dim3(LG_BLOCK), 0, s, (const double *)part, nb,
                        np * nm, out);
@@ -182,17 +183,20 @@ extern "C" int galhip_lgspia(galhip_ctx *c, int32_t np, int32_t nm, double lscal
     if (!rc) {
         hipError_t e = hipSuccess;
         const size_t esz = c->prec == GALHIP_FP32 ? sizeof(float4) : sizeof(double4);
+        // 4-vectors per particle: (x, y, vx, vy) on p2d, a two-vector record on p3d
+        const int ps = c->p3d ? 2 : 1;
         for (int m = 0; m < c->nmem && e == hipSuccess; m++) {
             const long long off = c->ens ? c->m_off[m] : 0;
             const long long cnt = c->ens ? c->m_off[m + 1] - off : c->n;
-            const void *xv = (const char *)c->xv + (size_t)off * esz;
+            const void *xv = (const char *)c->xv + (size_t)off * ps * esz;
             const uint8_t *zn = c->zone + off;
             e = c->prec == GALHIP_FP32
-                    ? launch_lgspia<float>(xv, zn, cnt, c->nz, c->offanal, c->nsect, np, nm,
-                                           lscale, part, lgsp_blocks(cnt), dout + 2 * nq * m, c->s)
-                    : launch_lgspia<double>(xv, zn, cnt, c->nz, c->offanal, c->nsect, np, nm,
-                                            lscale, part, lgsp_blocks(cnt), dout + 2 * nq * m,
-                                            c->s);
+                    ? launch_lgspia<float>(xv, ps, zn, cnt, c->nz, c->offanal, c->nsect, np,
+                                           nm, lscale, part, lgsp_blocks(cnt),
+                                           dout + 2 * nq * m, c->s)
+                    : launch_lgspia<double>(xv, ps, zn, cnt, c->nz, c->offanal, c->nsect, np,
+                                            nm, lscale, part, lgsp_blocks(cnt),
+                                            dout + 2 * nq * m, c->s);
         }
         if (e == hipSuccess)
             e = hipMemcpyAsync(out, dout, sizeof(double) * (size_t)c->nmem * 2 * nq,

@@ -186,6 +186,53 @@ __device__ __forceinline__ long long quant<float>(float w, int qbits) {
     return __double2ll_rn(ldexp((double)w, qbits));
 }
 
+// The N weights of one deposit as fixed point, the values of quant, for an
+// active lane (an inactive lane's values are left undefined).  FP32: the
+// exact shift of quant<float>; only a wavefront in which some active lane
+// has a weight outside the shift's range (zero, negative, very small, or not
+// below 2) takes the FP64 rounding, behind a wave-uniform branch, so that the
+// others do not issue it.
+template <typename R, int N>
+__device__ __forceinline__ void quant_n(const R *w, int qbits, bool active, long long *v) {
+#pragma unroll
+    for (int q = 0; q < N; q++) v[q] = active ? quant<R>(w[q], qbits) : 0;
+}
+template <int N>
+__device__ __forceinline__ void quant_n_f32(const float *w, int qbits, bool active,
+                                            long long *v) {
+    bool slow = false;
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        const unsigned b = __float_as_uint(w[q]);
+        const unsigned sh = (b >> 23) + (unsigned)(qbits - 150);
+        const unsigned long long m = (unsigned long long)((b & 0x7fffffu) | 0x800000u);
+        v[q] = (long long)(m << (sh & 63u));
+        slow = slow || sh > 39u;
+    }
+    if (__ballot(active && slow)) {
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            const unsigned sh = (__float_as_uint(w[q]) >> 23) + (unsigned)(qbits - 150);
+            if (sh > 39u) v[q] = __double2ll_rn(ldexp((double)w[q], qbits));
+        }
+    }
+}
+template <>
+__device__ __forceinline__ void quant_n<float, 4>(const float *w, int qbits, bool active,
+                                                  long long *v) {
+    quant_n_f32<4>(w, qbits, active, v);
+}
+template <>
+__device__ __forceinline__ void quant_n<float, 8>(const float *w, int qbits, bool active,
+                                                  long long *v) {
+    quant_n_f32<8>(w, qbits, active, v);
+}
+template <>
+__device__ __forceinline__ void quant_n<float, 27>(const float *w, int qbits, bool active,
+                                                   long long *v) {
+    quant_n_f32<27>(w, qbits, active, v);
+}
+
 // Fixed radial tables (the halo force, its potential, the force's
 // derivative in a model parameter) are the cubic Hermite interpolants of
 // (val, der) on r_k = k dr, held per interval k as the coefficients of

@@ -352,8 +352,7 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
         }
     }
     long long v[NW];
-#pragma unroll
-    for (int q = 0; q < NW; q++) v[q] = on ? quant<R>(w[q], a.qbits) : 0;
+    quant_n<R, NW>(w, a.qbits, on, v);
     bool direct = on && !pend;
     for (int pass = 0;; pass++) {
         int *const sb = s_b + 1 + 6 * (pass & 1);

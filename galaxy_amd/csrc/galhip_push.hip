@@ -561,19 +561,21 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
         long long *trep = tile[lane & (TILE_REP - 1)];
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
-            if (!pend[k] || dc_zone(dcell[k]) != zt) continue;
             const int ring = dc_ring(dcell[k]);
             int d = dc_spoke(dcell[k]) - sref;
             if (d < -half) d += na;
             else if (d >= half) d -= na;
-            if (ring < r0 || ring - r0 + 1 >= th || d < d0 || d - d0 + 1 >= tw) continue;
+            if (!pend[k] || dc_zone(dcell[k]) != zt || ring < r0 || ring - r0 + 1 >= th ||
+                d < d0 || d - d0 + 1 >= tw)
+                continue;
+            long long qv[4];
+            quant_n<R, 4>(dw[k], qbits, true, qv);
             const int b = (ring - r0) * tw + (d - d0);
             const int ti[4] = {b, b + 1, b + tw, b + tw + 1};
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const long long qv = quant<R>(dw[k][q], qbits);
-                if (qv != 0) atomicAdd((unsigned long long *)&trep[ti[q]], (unsigned long long)qv);
-            }
+            for (int q = 0; q < 4; q++)
+                if (qv[q] != 0)
+                    atomicAdd((unsigned long long *)&trep[ti[q]], (unsigned long long)qv[q]);
             pend[k] = false;
         }
         const bool last = pass + 1 >= GH_PUSH_PASSES;
@@ -604,15 +606,15 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
     for (int k = 0; k < PPT; k++) {
         const int z = dc_zone(dcell[k]), ring = dc_ring(dcell[k]);
         if (z == 0 || (ring + 1 < nr && !pend[k])) continue;
+        long long qv[4];
+        quant_n<R, 4>(dw[k], qbits, true, qv);
         atomicMin(&s_jr[z - 1], ring);
         atomicMax(&s_kr[z - 1], ring);
         int ic[4];
         corners_rs(ring, dc_spoke(dcell[k]), na, ic);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const long long qv = quant<R>(dw[k][q], qbits);
-            if (qv != 0) deposit_global(a.mfx, nz, a.mesh, z, ic[q], qv, mo);
-        }
+        for (int q = 0; q < 4; q++)
+            if (qv[q] != 0) deposit_global(a.mfx, nz, a.mesh, z, ic[q], qv[q], mo);
     }
     __syncthreads();
     if (tid < nz && s_kr[tid] >= 0) {
@@ -665,7 +667,7 @@ __global__ __launch_bounds__(GH_BLOCK) void massgn_kernel(PushArgs<R> a) {
     }
     const int key = on ? ((z - 1) * (a.nr + 1) + ring) * a.na + spoke : -1;
     long long qv[4];
-    for (int q = 0; q < 4; q++) qv[q] = on ? quant<R>(w[q], a.qbits) : 0;
+    quant_n<R, 4>(w, a.qbits, on, qv);
     unsigned long long pend = __ballot(key >= 0);
     while (pend) {
         const int leader = __ffsll((long long)pend) - 1;
