@@ -502,16 +502,19 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
         const int sref = (int)(kref & 0xFFFF);
         int *const bx = s_box + 4 * (pass & 1);
         if (tid == 0) s_ref[(pass + 1) & 1] = ~0ull;   // last read in the previous pass
-        // ring and spoke-offset box of the zone's pending deposits
+        // ring and spoke-offset box of the zone's pending deposits; dd: the
+        // deposits' spoke offsets from the reference spoke
+        int dd[PPT];
         {
             int rmin = INT_MAX, rmax = -1, dmin = INT_MAX, dmax = INT_MIN;
 #pragma unroll
             for (int k = 0; k < PPT; k++) {
+                int d = dc_spoke(dcell[k]) - sref;
+                if (d < -half) d += na;
+                else if (d >= half) d -= na;
+                dd[k] = d;
                 if (pend[k] && dc_zone(dcell[k]) == zt) {
                     const int ring = dc_ring(dcell[k]);
-                    int d = dc_spoke(dcell[k]) - sref;
-                    if (d < -half) d += na;
-                    else if (d >= half) d -= na;
                     rmin = min(rmin, ring);
                     rmax = max(rmax, ring);
                     dmin = min(dmin, d);
@@ -561,10 +564,7 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
         long long *trep = tile[lane & (TILE_REP - 1)];
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
-            const int ring = dc_ring(dcell[k]);
-            int d = dc_spoke(dcell[k]) - sref;
-            if (d < -half) d += na;
-            else if (d >= half) d -= na;
+            const int ring = dc_ring(dcell[k]), d = dd[k];
             if (!pend[k] || dc_zone(dcell[k]) != zt || ring < r0 || ring - r0 + 1 >= th ||
                 d < d0 || d - d0 + 1 >= tw)
                 continue;
