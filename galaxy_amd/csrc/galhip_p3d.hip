@@ -239,12 +239,15 @@ __device__ __forceinline__ void getacc3(const P3Args<R> &a, R ox, R oy, R oz, R 
 #pragma unroll
         for (int q = 0; q < NW; q++) {
             // a corner outside the field array (the last ring with t == na;
-            // for jmass 3 also the first ring's bottom plane) reads 0
-            const R4 f = (ic[q] >= 0 && ic[q] < a.mesh) ? a.fld[ic[q]]
-                                                        : mk4((R)0, (R)0, (R)0, (R)0);
-            sx = sx + f.x * w[q];
-            sy = sy + f.y * w[q];
-            sz = sz + f.z * w[q];
+            // for jmass 3 also the first ring's bottom plane) reads 0: cell 0
+            // is read instead and its value multiplied by 0 (unconditional
+            // loads, no per-corner branch)
+            const bool in = ic[q] >= 0 && ic[q] < a.mesh;
+            const R4 f = a.fld[in ? ic[q] : 0];
+            const R wq = in ? w[q] : (R)0;
+            sx = sx + f.x * wq;
+            sy = sy + f.y * wq;
+            sz = sz + f.z * wq;
         }
         if (a.fixforce) {
             R r2 = (R)0;
