@@ -10,6 +10,7 @@
 #include "galhip_ctx.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -35,8 +36,7 @@ hipError_t launch_integrals(const PushArgs<R> &, const R *, const double4 *, int
                             double *, double *, hipStream_t);
 template <typename R>
 hipError_t launch_key(const typename Vec2<R>::T *, const uint8_t *, long long,
-                      int, int, int, int, unsigned, unsigned *, unsigned *,
-                      hipStream_t);
+                      int, int, int, int, unsigned, unsigned short *, hipStream_t);
 template <typename R>
 hipError_t launch_key_ens(const typename Vec2<R>::T *, const uint8_t *, int, int, int, int, int,
                           unsigned, const int4 *, unsigned *, unsigned *, hipStream_t);
@@ -405,10 +405,18 @@ static int ensure_particles(galhip_ctx *c, long long n) {
     rc |= dalloc(&c->key, sizeof(unsigned) * n); rc |= dalloc(&c->key2, sizeof(unsigned) * n);
     rc |= dalloc(&c->perm, sizeof(unsigned) * n); rc |= dalloc(&c->perm2, sizeof(unsigned) * n);
     if (rc) { free_particles(c); return rc; }
-    size_t tmp = 0;
+    // scratch of the two sorts: 32-bit keys with stored values (ensembles),
+    // 16-bit keys with the identity as values (one grid)
+    size_t tmp = 0, tmp16 = 0;
     hipError_t e = rocprim::radix_sort_pairs(
         nullptr, tmp, c->key, c->key2, c->perm, c->perm2, (int)n, 0, 32, c->s);
+    if (e == hipSuccess)
+        e = rocprim::radix_sort_pairs(nullptr, tmp16, (unsigned short *)c->key,
+                                      (unsigned short *)c->key2,
+                                      rocprim::counting_iterator<unsigned>(0u), c->perm2,
+                                      (int)n, 0, 16, c->s);
     if (e != hipSuccess) return gh_fail(GALHIP_EHIP, "radix sort sizing: %s", hipGetErrorString(e));
+    if (tmp16 > tmp) tmp = tmp16;
     rc = dalloc(&c->sort_tmp, tmp);
     if (c->comp) rc |= dalloc(&c->xlo, 2 * es * n) | dalloc(&c->xlo2, 2 * es * n);
     if (rc) { free_particles(c); return rc; }
@@ -439,13 +447,19 @@ template <typename R> static int sort_impl(galhip_ctx *c) {
                                  c->na, c->nz, kbits, kmax, c->blk_d, c->key, c->perm, c->s));
         end_bit = kbits + mbits;
     } else {
+        // one grid: 16-bit keys, values the identity (no index array)
         HIPCHK(launch_key<R>((const typename Vec2<R>::T *)c->ut, c->zone, n, c->nr,
-                             c->na, c->nz, kshift, kmax, c->key, c->perm, c->s));
+                             c->na, c->nz, kshift, kmax, (unsigned short *)c->key, c->s));
     }
     size_t tmp = c->sort_tmp_bytes;
-    HIPCHK(rocprim::radix_sort_pairs(c->sort_tmp, tmp, c->key, c->key2,
-                                              c->perm, c->perm2, (int)n, 0,
-                                              end_bit, c->s));
+    if (c->ens)
+        HIPCHK(rocprim::radix_sort_pairs(c->sort_tmp, tmp, c->key, c->key2, c->perm, c->perm2,
+                                         (int)n, 0, end_bit, c->s));
+    else
+        HIPCHK(rocprim::radix_sort_pairs(c->sort_tmp, tmp, (unsigned short *)c->key,
+                                         (unsigned short *)c->key2,
+                                         rocprim::counting_iterator<unsigned>(0u), c->perm2,
+                                         (int)n, 0, end_bit, c->s));
     PermArgs<R> p;
     p.xv_in = (const typename Vec4<R>::T *)c->xv;
     p.xv_out = (typename Vec4<R>::T *)c->xv2;

@@ -951,8 +951,7 @@ template <typename R>
 __global__ __launch_bounds__(GH_BLOCK) void key_kernel(
     const typename Vec2<R>::T *__restrict__ ut,
     const uint8_t *__restrict__ zone, long long n, int nr, int na, int nz,
-    int kshift, unsigned kmax, unsigned *__restrict__ key,
-    unsigned *__restrict__ idx) {
+    int kshift, unsigned kmax, unsigned short *__restrict__ key) {
     long long i = (long long)blockIdx.x * GH_BLOCK + threadIdx.x;
     if (i >= n) return;
     unsigned k = kmax;
@@ -965,8 +964,7 @@ __global__ __launch_bounds__(GH_BLOCK) void key_kernel(
         k = (unsigned)(ring * na + spoke) >> kshift;
         if (k >= kmax) k = kmax - 1;
     }
-    key[i] = k;
-    idx[i] = (unsigned)i;
+    key[i] = (unsigned short)k;
 }
 
 // Ensemble sort key: (member << kbits) | the member's cell key (off-grid
@@ -1158,11 +1156,10 @@ hipError_t launch_tmcent(const PushArgs<R> &a, R fac, hipStream_t s) {
 template <typename R>
 hipError_t launch_key(const typename Vec2<R>::T *ut, const uint8_t *zone,
                       long long n, int nr, int na, int nz, int kshift,
-                      unsigned kmax, unsigned *key, unsigned *idx,
-                      hipStream_t s) {
+                      unsigned kmax, unsigned short *key, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(key_kernel<R>, dim3(nblocks(n, GH_BLOCK)), dim3(GH_BLOCK),
-                       0, s, ut, zone, n, nr, na, nz, kshift, kmax, key, idx);
+                       0, s, ut, zone, n, nr, na, nz, kshift, kmax, key);
     return hipGetLastError();
 }
 
@@ -1219,8 +1216,8 @@ hipError_t launch_permute_all(const PermArgs<R> &p, hipStream_t s) {
                                             double *, double *, hipStream_t);    \
     template hipError_t launch_key<R>(const typename Vec2<R>::T *,           \
                                       const uint8_t *, long long, int, int,   \
-                                      int, int, unsigned, unsigned *,         \
-                                      unsigned *, hipStream_t);               \
+                                      int, int, unsigned, unsigned short *,   \
+                                      hipStream_t);                           \
     template hipError_t launch_key_ens<R>(const typename Vec2<R>::T *, const uint8_t *, \
                                           int, int, int, int, int, unsigned,          \
                                           const int4 *, unsigned *, unsigned *,       \
