@@ -34,6 +34,7 @@ template <typename R> __device__ __forceinline__ R wave_sum(R v) {
 constexpr int SOLVE_BLOCK = 1024;
 constexpr int ANA_SPLIT = 1;   // blocks per ring in the analysis (harmonics)
 constexpr int SYN_SPLIT = 1;   // blocks per ring in the synthesis (spokes)
+constexpr int SYN_TPW = 3;     // contraction tasks per wavefront in flight together
 
 // grid (nr, ANA_SPLIT, nmem): block (ir, h, mem) forms the ring's combined
 // mass and the harmonics m = h, h + ANA_SPLIT, ... of grid mem (ensemble
@@ -162,35 +163,66 @@ __global__ __launch_bounds__(SOLVE_BLOCK) void synth_kernel(SynArgs<R> a) {
                                        ir - 1];
         }
     }
-    for (int task = wave; task < (PRE ? 0 : ntypes * a.ng); task += nw) {
-        const int type = task / a.ng, m = task - type * a.ng;
-        const bool azim = type == 1;
-        const bool valid = !a.lg[m] && (!azim || (m >= 1 && m <= maz));
-        R sc = 0, ss = 0;
-        if (valid) {
-            const bool pair = m >= 1 && m < mmax - 1;
+    // The contraction: one wavefront per (type, harmonic) dot product over
+    // the source rings, SYN_TPW of them per wavefront at a time with all
+    // their loads issued before the sums (the load latency, not the
+    // arithmetic, bounds this phase); each lane's terms are added in source
+    // order as one task at a time would.
+    const int ntask = PRE ? 0 : ntypes * a.ng;
+    for (int t0 = wave; t0 < ntask; t0 += SYN_TPW * nw) {
+        R sc[SYN_TPW], ss[SYN_TPW];
+        bool ok[SYN_TPW], azim[SYN_TPW], pair[SYN_TPW];
+        const R *g[SYN_TPW], *mc[SYN_TPW], *ms[SYN_TPW];
+#pragma unroll
+        for (int u = 0; u < SYN_TPW; u++) {
+            const int task = t0 + u * nw;
+            const int type = task / a.ng, m = task - type * a.ng;
+            azim[u] = type == 1;
+            ok[u] = task < ntask && !a.lg[m] && (!azim[u] || (m >= 1 && m <= maz));
+            pair[u] = m >= 1 && m < mmax - 1;
             const int off = m == 0 ? 0 : 2 * m - 1;
-            const R *g = a.G + (((size_t)type * a.ng + m) * nr + (ir - 1)) * nr;
-            const R *mc = coef + (size_t)off * nr;
-            const R *ms = coef + (size_t)(off + 1) * nr;
-            for (int is = jrs + lane; is <= krs; is += 64) {
-                R gv = g[is - 1];
-                R c = mc[is - 1];
-                R s = pair ? ms[is - 1] : (R)0;
-                if (!azim) {
-                    sc += gv * c;
-                    ss += gv * s;
-                } else {
-                    sc += gv * s;
-                    ss -= gv * c;
-                }
-            }
-            sc = wave_sum(sc);
-            ss = wave_sum(ss);
+            g[u] = a.G + (((size_t)type * a.ng + m) * nr + (ir - 1)) * nr;
+            mc[u] = coef + (size_t)off * nr;
+            ms[u] = coef + (size_t)(off + 1) * nr;
+            sc[u] = 0;
+            ss[u] = 0;
         }
-        if (lane == 0) {
-            W[type][0][m] = sc;
-            W[type][1][m] = ss;
+        for (int base = jrs; base <= krs; base += 4 * 64) {
+            R gv[SYN_TPW][4], cv[SYN_TPW][4], sv[SYN_TPW][4];
+#pragma unroll
+            for (int u = 0; u < SYN_TPW; u++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int is = base + lane + 64 * r;
+                    const bool in = ok[u] && is <= krs;
+                    gv[u][r] = in ? g[u][is - 1] : (R)0;
+                    cv[u][r] = in ? mc[u][is - 1] : (R)0;
+                    sv[u][r] = (in && pair[u]) ? ms[u][is - 1] : (R)0;
+                }
+#pragma unroll
+            for (int u = 0; u < SYN_TPW; u++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (base + lane + 64 * r > krs) continue;
+                    if (!azim[u]) {
+                        sc[u] += gv[u][r] * cv[u][r];
+                        ss[u] += gv[u][r] * sv[u][r];
+                    } else {
+                        sc[u] += gv[u][r] * sv[u][r];
+                        ss[u] -= gv[u][r] * cv[u][r];
+                    }
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < SYN_TPW; u++) {
+            const int task = t0 + u * nw;
+            if (task >= ntask) break;                 // wave-uniform
+            const R c = wave_sum(sc[u]), sn = wave_sum(ss[u]);
+            if (lane == 0) {
+                const int type = task / a.ng, m = task - type * a.ng;
+                W[type][0][m] = ok[u] ? c : (R)0;
+                W[type][1][m] = ok[u] ? sn : (R)0;
+            }
         }
     }
     __syncthreads();
