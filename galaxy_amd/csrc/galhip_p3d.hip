@@ -1040,15 +1040,22 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_synth_kernel(P3SolveArgs<R> a) {
     const size_t row = ((size_t)(ir - 1) * ngz + plane) * na;
     for (int j = tid; j < na; j += P3_BLOCK) {
         R F0 = 0, F1 = 0, F2 = 0, F3 = 0;
+        // k = (m j) mod na, advanced with m (the slots come in ascending m)
+        int mcur = 0, k = 0;
         for (int s = 0; s < nm; s++) {
             const int m = sm[s];
+            if (m < mcur) {
+                mcur = 0;
+                k = 0;
+            }
+            for (; mcur < m; mcur++) {
+                k += j;
+                if (k >= na) k -= na;
+            }
             R b;
             if (m == 0) b = (R)1;
             else if (m == mmax - 1) b = (j & 1) ? (R)-1 : (R)1;
-            else {
-                const int k = (int)(((long long)m * j) % na);
-                b = sp[s] ? (R)-2 * tws[2 * k + 1] : (R)2 * tws[2 * k];
-            }
+            else b = sp[s] ? (R)-2 * tws[2 * k + 1] : (R)2 * tws[2 * k];
             F0 += zc[0][s] * b;
             F1 += zc[1][s] * b;
             F2 += zc[2][s] * b;
