@@ -47,10 +47,34 @@ constexpr int P3_MAX_NA = 2048;
 constexpr int P3_MAX_NM = 128;   // compact components (2 ng)
 constexpr int P3_MAX_NR = 1024;
 
+// Wavefront sum by DPP row exchanges (pairs, quads, half-rows, rows of 16
+// lanes), then the four rows' sums read from lanes 0, 16, 32, 48; every lane
+// gets the total.  The solve kernels' many short dot products were waiting on
+// the LDS-routed shuffles of a butterfly.
+template <int CTRL> __device__ __forceinline__ float p3_dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL> __device__ __forceinline__ double p3_dpp(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float p3_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double p3_lane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 template <typename R> __device__ __forceinline__ R p3_wave_sum(R v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += p3_dpp<0xB1>(v);    // quad_perm 1,0,3,2
+    v += p3_dpp<0x4E>(v);    // quad_perm 2,3,0,1
+    v += p3_dpp<0x141>(v);   // row_half_mirror
+    v += p3_dpp<0x140>(v);   // row_mirror
+    return (p3_lane(v, 0) + p3_lane(v, 16)) + (p3_lane(v, 32) + p3_lane(v, 48));
 }
 
 // (u, t) of a position as weight() forms them (weight.f:210-218): the FP32
@@ -889,6 +913,25 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_analyse_kernel(P3SolveArgs<R> a) 
 }
 
 // grid (nr): the doubled vertical transform of each component column
+// The vertical twiddles (cos, sin of 2 pi k / P) in LDS with one pad entry
+// after every 256 bytes: the transforms read them at index p k mod P across
+// consecutive k (or q), a stride that puts many lanes on one bank when it is
+// a multiple of a large power of two; the pad spreads those over the banks.
+template <typename R> constexpr int tz_shift() { return sizeof(R) == 4 ? 5 : 4; }
+template <typename R> __host__ __device__ constexpr size_t tz_words(size_t P) {
+    return 2 * (P + (P >> tz_shift<R>()) + 1);
+}
+template <typename R> __device__ __forceinline__ int tz_at(int idx) {
+    return 2 * (idx + (idx >> tz_shift<R>()));
+}
+template <typename R>
+__device__ __forceinline__ void tz_stage(const R *twz, int P, R *tz) {
+    for (int e = threadIdx.x; e < P; e += P3_BLOCK) {
+        tz[tz_at<R>(e)] = twz[2 * e];
+        tz[tz_at<R>(e) + 1] = twz[2 * e + 1];
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(P3_BLOCK) void p3_zfwd_kernel(P3SolveArgs<R> a) {
     extern __shared__ unsigned char p3_lds[];
@@ -900,7 +943,7 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_zfwd_kernel(P3SolveArgs<R> a) {
     if (jrs < 1 || ir < jrs || ir > krs) return;
     const R *src = a.A + (size_t)(ir - 1) * nm * ngz;
     for (int e = tid; e < nm * ngz; e += P3_BLOCK) col[e] = src[e];
-    for (int e = tid; e < 2 * P; e += P3_BLOCK) tz[e] = a.twz[e];
+    tz_stage<R>(a.twz, P, tz);
     __syncthreads();
     R *dst = a.M + (size_t)(ir - 1) * nm * P;
     for (int o = tid; o < nm * (ngz + 1); o += P3_BLOCK) {
@@ -909,8 +952,9 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_zfwd_kernel(P3SolveArgs<R> a) {
         R sa = 0, sb = 0;
         int idx = 0;
         for (int p = 0; p < ngz; p++) {
-            sa += c[p] * tz[2 * idx];
-            sb += c[p] * tz[2 * idx + 1];
+            const int t = tz_at<R>(idx);
+            sa += c[p] * tz[t];
+            sb += c[p] * tz[t + 1];
             idx += k;
             if (idx >= P) idx -= P;
         }
@@ -994,7 +1038,7 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_zinv_kernel(P3SolveArgs<R> a) {
     if (jrs < 1 || ir < jrf || ir > krf) return;
     const R *src = a.W + ((size_t)type * a.nr + (ir - 1)) * nm * P;
     for (int e = tid; e < nm * P; e += P3_BLOCK) h[e] = src[e];
-    for (int e = tid; e < 2 * P; e += P3_BLOCK) tz[e] = a.twz[e];
+    tz_stage<R>(a.twz, P, tz);
     __syncthreads();
     R *dst = a.Z + ((size_t)type * a.nr + (ir - 1)) * ngz * nm;
     for (int o = tid; o < nm * ngz; o += P3_BLOCK) {
@@ -1003,7 +1047,8 @@ __global__ __launch_bounds__(P3_BLOCK) void p3_zinv_kernel(P3SolveArgs<R> a) {
         R v = c[0] + ((q & 1) ? -c[P - 1] : c[P - 1]);
         int idx = q;
         for (int k = 1; k < ngz; k++) {
-            v += (R)2 * (c[2 * k - 1] * tz[2 * idx] - c[2 * k] * tz[2 * idx + 1]);
+            const int t = tz_at<R>(idx);
+            v += (R)2 * (c[2 * k - 1] * tz[t] - c[2 * k] * tz[t + 1]);
             idx += q;
             if (idx >= P) idx -= P;
         }
@@ -1309,8 +1354,8 @@ hipError_t launch_p3_integrals(const P3Args<R> &a, int with_pot, const double4 *
 template <typename R> bool p3_solve_fits(const P3SolveArgs<R> &a) {
     const size_t P = 2 * (size_t)a.ngz;
     return a.na <= P3_MAX_NA && a.nm <= P3_MAX_NM && a.nr <= P3_MAX_NR &&
-           ((size_t)a.nm * a.ngz + 2 * P) * sizeof(R) <= 48 * 1024 &&
-           ((size_t)a.nm * P + 2 * P) * sizeof(R) <= 48 * 1024;
+           ((size_t)a.nm * a.ngz + tz_words<R>(P)) * sizeof(R) <= 48 * 1024 &&
+           ((size_t)a.nm * P + tz_words<R>(P)) * sizeof(R) <= 48 * 1024;
 }
 
 template <typename R> hipError_t launch_p3_solve(const P3SolveArgs<R> &a, hipStream_t s) {
@@ -1320,13 +1365,13 @@ template <typename R> hipError_t launch_p3_solve(const P3SolveArgs<R> &a, hipStr
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(p3_zfwd_kernel<R>, dim3(a.nr), dim3(P3_BLOCK),
-                       ((size_t)a.nm * a.ngz + 2 * P) * sizeof(R), s, a);
+                       ((size_t)a.nm * a.ngz + tz_words<R>(P)) * sizeof(R), s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(p3_conv_kernel<R>, dim3(a.ng, a.ngz + 1, a.ntypes), dim3(P3_BLOCK),
                        0, s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(p3_zinv_kernel<R>, dim3(a.nr, a.ntypes), dim3(P3_BLOCK),
-                       ((size_t)a.nm * P + 2 * P) * sizeof(R), s, a);
+                       ((size_t)a.nm * P + tz_words<R>(P)) * sizeof(R), s, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(p3_synth_kernel<R>, dim3(a.ngz, a.nr), dim3(P3_BLOCK), 0, s, a);
     return hipGetLastError();
