@@ -81,6 +81,36 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
            (__builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48));
 }
 
+// Floating-point wavefront sum by DPP row exchanges (pairs, quads,
+// half-rows, rows of 16 lanes), then the four rows' sums read from lanes 0,
+// 16, 32, 48; every lane gets the total.  Short dot products in the solve
+// kernels were waiting on the LDS-routed shuffles of a butterfly.
+template <int CTRL> __device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename R> __device__ __forceinline__ R wave_sum_dpp(R v) {
+    v += dpp_mov<0xB1>(v);    // quad_perm 1,0,3,2
+    v += dpp_mov<0x4E>(v);    // quad_perm 2,3,0,1
+    v += dpp_mov<0x141>(v);   // row_half_mirror
+    v += dpp_mov<0x140>(v);   // row_mirror
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
 // ---- per-particle geometry ------------------------------------------------
 
 // Grid coordinates of a position as weight() forms them

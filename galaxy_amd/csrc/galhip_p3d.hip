@@ -47,35 +47,7 @@ constexpr int P3_MAX_NA = 2048;
 constexpr int P3_MAX_NM = 128;   // compact components (2 ng)
 constexpr int P3_MAX_NR = 1024;
 
-// Wavefront sum by DPP row exchanges (pairs, quads, half-rows, rows of 16
-// lanes), then the four rows' sums read from lanes 0, 16, 32, 48; every lane
-// gets the total.  The solve kernels' many short dot products were waiting on
-// the LDS-routed shuffles of a butterfly.
-template <int CTRL> __device__ __forceinline__ float p3_dpp(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
-}
-template <int CTRL> __device__ __forceinline__ double p3_dpp(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ float p3_lane(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ __forceinline__ double p3_lane(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <typename R> __device__ __forceinline__ R p3_wave_sum(R v) {
-    v += p3_dpp<0xB1>(v);    // quad_perm 1,0,3,2
-    v += p3_dpp<0x4E>(v);    // quad_perm 2,3,0,1
-    v += p3_dpp<0x141>(v);   // row_half_mirror
-    v += p3_dpp<0x140>(v);   // row_mirror
-    return (p3_lane(v, 0) + p3_lane(v, 16)) + (p3_lane(v, 32) + p3_lane(v, 48));
-}
+template <typename R> __device__ __forceinline__ R p3_wave_sum(R v) { return wave_sum_dpp(v); }
 
 // (u, t) of a position as weight() forms them (weight.f:210-218): the FP32
 // logarithm and angle of the p2d grid, t = 0 on the axis.

@@ -22,11 +22,7 @@ namespace gh {
 constexpr int MAX_NA = 2048;
 constexpr int MAX_NG = 64;
 
-template <typename R> __device__ __forceinline__ R wave_sum(R v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+template <typename R> __device__ __forceinline__ R wave_sum(R v) { return wave_sum_dpp(v); }
 
 // One block of SOLVE_BLOCK threads (16 wavefronts) per ring: the dot
 // products over rings and spokes are short, so many wavefronts per CU are
