@@ -47,7 +47,7 @@ constexpr int PPT = GH_PPT;            // particles per thread
 constexpr int PB = GH_PUSH_BLOCK;      // threads per block
 // minimum wavefronts per SIMD of push_kernel (launch bounds), FP32 and FP64
 #ifndef GH_PUSH_WAVES32
-#define GH_PUSH_WAVES32 5
+#define GH_PUSH_WAVES32 6
 #endif
 #ifndef GH_PUSH_WAVES64
 #define GH_PUSH_WAVES64 4
@@ -126,9 +126,10 @@ __device__ __forceinline__ int wave_max(int v) { return wave_max_i32(v); }
 // ENS: an ensemble launch, the block's particles and member from a.blk.
 // COMP: compensated positions (a.xlo holds the low words).
 template <typename R, bool ENS, bool COMP>
-// Wavefronts per SIMD, measured: FP32 at least 5 (96 VGPRs, no spill; 4
-// waves at the unconstrained 102 VGPRs is 8 % slower, 6 waves spill); FP64
-// at least 4 (128 VGPRs, a few spills, 4 % faster than 3 waves without).
+// Wavefronts per SIMD, measured: FP32 at least 6 (80 VGPRs, 4 spilled;
+// 1.8 % faster than 5 waves at 86 VGPRs without spills; the LDS of six
+// blocks per CU just fits); FP64 at least 4 (128 VGPRs, a few spills, 4 %
+// faster than 3 waves without).
 __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVES64) void push_kernel(PushArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     typedef typename Vec4<R>::T R4;
