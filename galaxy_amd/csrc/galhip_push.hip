@@ -360,6 +360,15 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
             fr[k] = (R)fixforce_at((double)rr[k], kf[k], a.nfr, a.fr_inv_dr, a.fr_c, 0);
     }
 
+    // FP32: the states of particles 1 .. PPT-1 read once more (L2 hits):
+    // held in registers across the gathers above, one of them would be
+    // spilled to scratch at 6 wavefronts per SIMD, a store and a reload
+    // through memory (FP64 keeps them: the re-read spills more there)
+    if constexpr (sizeof(R) == 4) {
+#pragma unroll
+        for (int k = 1; k < PPT; k++) st[k] = xvb[min(k * PB + tid, nblk - 1)];
+    }
+
     // deposit state kept per particle: ring, spoke, zone (0 = none), weights
     // dcell: zone << 28 | ring << 16 | spoke (zone 0: no deposit)
     int dcell[PPT];
