@@ -63,8 +63,8 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE of the all-zones launch (10M pushes), each scaled
 # by what the same counter reads for a 1 GiB copy kernel (FETCH_SIZE x 2,
 # WRITE_SIZE x 1); profiles/push_kernel_p2d-256x512-n1e7-3zones_traffic.txt.
-PUSH_HBM_READ_B = 29.6
-PUSH_HBM_WRITE_B = 28.7
+PUSH_HBM_READ_B = 26.9
+PUSH_HBM_WRITE_B = 24.7
 # VALU instructions per push of push_kernel<float> (rocprofv3 SQ_INSTS_VALU of
 # the all-zones launch x 64 lanes / 10M pushes;
 # profiles/push_kernel_p2d-256x512-n1e7-3zones_sq_counters.txt) and MI355X's
@@ -78,7 +78,7 @@ P3_PUSH_HBM_READ_B = 36.0
 P3_PUSH_HBM_WRITE_B = 39.3
 # the same for push_kernel<float, true> on the 64-disc ensemble (mean of the
 # dispatches in profiles/push_kernel_ens-64x128x256-n8.6e7_traffic.txt)
-ENS_PUSH_HBM_READ_B = 26.9
+ENS_PUSH_HBM_READ_B = 28.6
 ENS_PUSH_HBM_WRITE_B = 26.0
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # algorithmic grid bytes of one C2 step besides the particles: the F_R and
