@@ -81,6 +81,17 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
            (__builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48));
 }
 
+// n / d for 0 <= n < 2^16 and 1 <= d < 2^16 without the integer division
+// sequence: n times the single-precision reciprocal of d is within 2^-6 of
+// the quotient, so its truncation is off by at most one, which two exact
+// integer tests correct.
+__device__ __forceinline__ int div_small(int n, int d) {
+    int q = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
+    q -= q * d > n ? 1 : 0;
+    q += (q + 1) * d <= n ? 1 : 0;
+    return q;
+}
+
 // Floating-point wavefront sum by DPP row exchanges (pairs, quads,
 // half-rows, rows of 16 lanes), then the four rows' sums read from lanes 0,
 // 16, 32, 48; every lane gets the total.  Short dot products in the solve

@@ -316,7 +316,7 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
                                                int *s_kr) {
     constexpr int NW = P3NW<J>::v;
     constexpr int LO = J == 3 ? 1 : 0;   // corner cells below the central one
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
     // reference key: thread order from the middle thread, then the cell
     const unsigned long long rkey =
         ((unsigned long long)((tid - P3_PBLOCK / 2) & (P3_PBLOCK - 1)) << 40) |
@@ -421,18 +421,33 @@ __device__ __forceinline__ void deposit3_block(const P3Args<R> &a, bool on, int 
             pend = false;
         }
         __syncthreads();
-        // flush: one wave per (ring, plane) row of the window; window spokes
-        // q0 + dq lie in [-1, na], folded back onto the ring
-        for (int row = wave; row < (use ? NR * NP : 0); row += P3_PBLOCK / 64) {
-            const int dr = row / NP, dp = row - dr * NP;
-            const int base = ((r0 + dr) * a.ngz + (p0 + dp)) * a.na;
-            const long long *t = tile + row * NQ;
-            for (int dq = lane; dq < NQ; dq += 64) {
-                const long long tv = t[dq];
-                int sp = q0 + dq;
-                sp = sp >= a.na ? sp - a.na : sp < 0 ? sp + a.na : sp;
-                if (tv != 0)
-                    atomicAdd((unsigned long long *)&a.mfx[tz - 1][base + sp], (unsigned long long)tv);
+        // flush: the window's cells over all threads in storage order (a
+        // row's cells are consecutive spokes, so a wave's atomics coalesce),
+        // one global atomic per non-zero cell; the (ring, plane, spoke)
+        // offsets of cell e advanced by the block size with carries instead
+        // of divided out per cell; window spokes q0 + dq lie in [-1, na],
+        // folded back onto the ring
+        if (use) {
+            const int t1 = div_small(P3_PBLOCK, NQ), sq = P3_PBLOCK - t1 * NQ;
+            const int sr = div_small(t1, NP), spl = t1 - sr * NP;
+            const int u1 = div_small(tid, NQ);
+            int dq = tid - u1 * NQ, dr = div_small(u1, NP);
+            int dp = u1 - dr * NP;
+            for (int e = tid; e < ncell; e += P3_PBLOCK) {
+                const long long tv = tile[e];
+                if (tv != 0) {
+                    int sp = q0 + dq;
+                    sp = sp >= a.na ? sp - a.na : sp < 0 ? sp + a.na : sp;
+                    atomicAdd((unsigned long long *)&a.mfx[tz - 1][((r0 + dr) * a.ngz + (p0 + dp)) * a.na + sp],
+                              (unsigned long long)tv);
+                }
+                dq += sq;
+                const int c1 = dq >= NQ ? 1 : 0;
+                dq -= c1 * NQ;
+                dp += spl + c1;
+                const int c2 = dp >= NP ? 1 : 0;
+                dp -= c2 * NP;
+                dr += sr + c2;
             }
         }
         if (pass + 1 >= P3_PASSES) {

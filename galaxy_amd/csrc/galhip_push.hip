@@ -126,10 +126,9 @@ __device__ __forceinline__ int wave_max(int v) { return wave_max_i32(v); }
 // ENS: an ensemble launch, the block's particles and member from a.blk.
 // COMP: compensated positions (a.xlo holds the low words).
 template <typename R, bool ENS, bool COMP>
-// Wavefronts per SIMD, measured: FP32 at least 6 (80 VGPRs, 4 spilled;
-// 1.8 % faster than 5 waves at 86 VGPRs without spills; the LDS of six
-// blocks per CU just fits); FP64 at least 4 (128 VGPRs, a few spills, 4 %
-// faster than 3 waves without).
+// Wavefronts per SIMD, measured: FP32 at least 6 (80 VGPRs at most; faster
+// than 5 waves at 86 VGPRs; the LDS of six blocks per CU just fits); FP64
+// at least 4 (128 VGPRs, a few spills, 4 % faster than 3 waves without).
 __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVES64) void push_kernel(PushArgs<R> a) {
     typedef typename Vec2<R>::T R2;
     typedef typename Vec4<R>::T R4;
@@ -591,22 +590,21 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
         const bool last = pass + 1 >= GH_PUSH_PASSES;
         if (!last) offer_ref((pass + 1) & 1);
         __syncthreads();
-        // flush: one global atomic per touched cell
-        for (int row = 0; row < th; row++) {
-            const int ring = r0 + row;
-            for (int col = tid; col < tw; col += PB) {
-                const int j = row * tw + col;
-                long long v = 0;
+        // flush: one global atomic per touched cell, the tile's cells over
+        // all threads in storage order (a row's cells are consecutive
+        // spokes, so a wave's atomics coalesce)
+        for (int j = tid; j < tlen; j += PB) {
+            long long v = 0;
 #pragma unroll
-                for (int rep = 0; rep < TILE_REP; rep++) v += tile[rep][j];
-                if (v != 0) {
-                    int spoke = sref + d0 + col;
-                    if (spoke < 0) spoke += na;
-                    else if (spoke >= na) spoke -= na;
-                    if (spoke >= na) spoke -= na;
-                    atomicAdd((unsigned long long *)&a.mfx[zt - 1][mo + ring * na + spoke],
-                              (unsigned long long)v);
-                }
+            for (int rep = 0; rep < TILE_REP; rep++) v += tile[rep][j];
+            if (v != 0) {
+                const int row = div_small(j, tw), col = j - row * tw;
+                int spoke = sref + d0 + col;
+                if (spoke < 0) spoke += na;
+                else if (spoke >= na) spoke -= na;
+                if (spoke >= na) spoke -= na;
+                atomicAdd((unsigned long long *)&a.mfx[zt - 1][mo + (r0 + row) * na + spoke],
+                          (unsigned long long)v);
             }
         }
         if (last) break;
