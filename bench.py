@@ -40,7 +40,8 @@ all members per second, push-kernel roofline at 32 B per push, the oracle
 on the largest member's disc as the CPU baseline.
 
 --workload C3 measures the 3-D polar grid instead (200 x 256 x 125 grid,
-N = 5e7, m <= 8, sech^2 vertical profile, 200 steps by default) and prints
+N = 5e7, m <= 8, sech^2 vertical profile, 210 steps by default, re-sorted
+every 7) and prints
 its own line: push-kernel roofline at 48 B per push ((x, y, z, vx) and
 (vy, vz) read and written), the p3d oracle as the CPU baseline.
 """
@@ -63,23 +64,23 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE of the all-zones launch (10M pushes), each scaled
 # by what the same counter reads for a 1 GiB copy kernel (FETCH_SIZE x 2,
 # WRITE_SIZE x 1); profiles/push_kernel_p2d-256x512-n1e7-3zones_traffic.txt.
-PUSH_HBM_READ_B = 26.9
-PUSH_HBM_WRITE_B = 24.7
+PUSH_HBM_READ_B = 27.0
+PUSH_HBM_WRITE_B = 24.8
 # VALU instructions per push of push_kernel<float> (rocprofv3 SQ_INSTS_VALU of
 # the all-zones launch x 64 lanes / 10M pushes;
 # profiles/push_kernel_p2d-256x512-n1e7-3zones_sq_counters.txt) and MI355X's
 # VALU issue rate: 256 CUs x 64 lanes x 2.4 GHz (one wave64 instruction per
 # CU per cycle, non-packed).
-PUSH_VALU_PER_PUSH = 486.0
+PUSH_VALU_PER_PUSH = 484.0
 # HBM bytes per particle of p3_push_kernel<float> on the 3-D workload (mean
 # of the dispatches in profiles/p3_push_kernel_p3d-200x256x125-n5e7_traffic.txt,
 # scaled by the same copy calibration)
-P3_PUSH_HBM_READ_B = 36.0
-P3_PUSH_HBM_WRITE_B = 39.3
+P3_PUSH_HBM_READ_B = 36.4
+P3_PUSH_HBM_WRITE_B = 41.5
 # the same for push_kernel<float, true> on the 64-disc ensemble (mean of the
 # dispatches in profiles/push_kernel_ens-64x128x256-n8.6e7_traffic.txt)
-ENS_PUSH_HBM_READ_B = 28.6
-ENS_PUSH_HBM_WRITE_B = 26.0
+ENS_PUSH_HBM_READ_B = 29.5
+ENS_PUSH_HBM_WRITE_B = 26.1
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # algorithmic grid bytes of one C2 step besides the particles: the F_R and
 # F_phi tables in FP32 (2 x 17 harmonics x 256 x 256 x 4 B, read once by the
@@ -672,9 +673,9 @@ def main():
         return
     if args.workload == "C3":
         if args.steps == 500:
-            args.steps = 200
+            args.steps = 210
         if args.sort_interval is None:
-            args.sort_interval = 5
+            args.sort_interval = 7
         main_p3d(args)
         return
     if args.sort_interval is None:
