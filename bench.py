@@ -40,8 +40,8 @@ all members per second, push-kernel roofline at 32 B per push, the oracle
 on the largest member's disc as the CPU baseline.
 
 --workload C3 measures the 3-D polar grid instead (200 x 256 x 125 grid,
-N = 5e7, m <= 8, sech^2 vertical profile, 210 steps by default, re-sorted
-every 7) and prints
+N = 5e7, m <= 8, sech^2 vertical profile, 200 steps by default, re-sorted
+every 8) and prints
 its own line: push-kernel roofline at 48 B per push ((x, y, z, vx) and
 (vy, vz) read and written), the p3d oracle as the CPU baseline.
 """
@@ -76,7 +76,7 @@ PUSH_VALU_PER_PUSH = 484.0
 # of the dispatches in profiles/p3_push_kernel_p3d-200x256x125-n5e7_traffic.txt,
 # scaled by the same copy calibration)
 P3_PUSH_HBM_READ_B = 36.4
-P3_PUSH_HBM_WRITE_B = 41.5
+P3_PUSH_HBM_WRITE_B = 41.6
 # the same for push_kernel<float, true> on the 64-disc ensemble (mean of the
 # dispatches in profiles/push_kernel_ens-64x128x256-n8.6e7_traffic.txt)
 ENS_PUSH_HBM_READ_B = 29.5
@@ -673,9 +673,9 @@ def main():
         return
     if args.workload == "C3":
         if args.steps == 500:
-            args.steps = 210
+            args.steps = 200
         if args.sort_interval is None:
-            args.sort_interval = 7
+            args.sort_interval = 8
         main_p3d(args)
         return
     if args.sort_interval is None:

@@ -279,7 +279,7 @@ __device__ __forceinline__ int p3_wave_max(int v) { return wave_max_i32(v); }
 #define GH_P3_TILE 4096
 #endif
 #ifndef GH_P3_PASSES
-#define GH_P3_PASSES 3
+#define GH_P3_PASSES 8
 #endif
 constexpr int P3_TILE = GH_P3_TILE;       // cells of a block's LDS deposit window
 constexpr int P3_PASSES = GH_P3_PASSES;   // window passes before the direct fallback
