@@ -65,7 +65,7 @@ HBM_PEAK_GBS = 8000.0
 # by what the same counter reads for a 1 GiB copy kernel (FETCH_SIZE x 2,
 # WRITE_SIZE x 1); profiles/push_kernel_p2d-256x512-n1e7-3zones_traffic.txt.
 PUSH_HBM_READ_B = 27.0
-PUSH_HBM_WRITE_B = 24.8
+PUSH_HBM_WRITE_B = 24.7
 # VALU instructions per push of push_kernel<float> (rocprofv3 SQ_INSTS_VALU of
 # the all-zones launch x 64 lanes / 10M pushes;
 # profiles/push_kernel_p2d-256x512-n1e7-3zones_sq_counters.txt) and MI355X's
@@ -80,7 +80,7 @@ P3_PUSH_HBM_WRITE_B = 41.6
 # the same for push_kernel<float, true> on the 64-disc ensemble (mean of the
 # dispatches in profiles/push_kernel_ens-64x128x256-n8.6e7_traffic.txt)
 ENS_PUSH_HBM_READ_B = 29.5
-ENS_PUSH_HBM_WRITE_B = 26.1
+ENS_PUSH_HBM_WRITE_B = 25.9
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # algorithmic grid bytes of one C2 step besides the particles: the F_R and
 # F_phi tables in FP32 (2 x 17 harmonics x 256 x 256 x 4 B, read once by the

@@ -56,7 +56,7 @@ constexpr int TILE_CAP = GH_TILE_CAP;  // LDS tile, cells of a ring x spoke wind
 constexpr int TILE_REP = GH_TILE_REP;  // replicas (lane & (REP-1)) against conflicts
 constexpr int FF_CAP = 256;            // staged fixed-force table entries
 #ifndef GH_PUSH_PASSES
-#define GH_PUSH_PASSES 2               // deposit window passes before global atomics
+#define GH_PUSH_PASSES 3               // deposit window passes before global atomics
 #endif
 
 // fields of a packed deposit cell (zone << 28 | ring << 16 | spoke)
@@ -504,7 +504,8 @@ __global__ __launch_bounds__(PB, sizeof(R) == 4 ? GH_PUSH_WAVES32 : GH_PUSH_WAVE
     };
     offer_ref(0);
     __syncthreads();
-    for (int pass = 0;; pass++) {
+#pragma unroll
+    for (int pass = 0; pass < GH_PUSH_PASSES; pass++) {
         const unsigned long long kref = s_ref[pass & 1];
         if (kref == ~0ull) break;                  // nothing pending (uniform)
         const int zt = (int)((kref >> 28) & 0xF), rref = (int)((kref >> 16) & 0xFFF);
