@@ -79,8 +79,8 @@ P3_PUSH_HBM_READ_B = 36.4
 P3_PUSH_HBM_WRITE_B = 41.6
 # the same for push_kernel<float, true> on the 64-disc ensemble (mean of the
 # dispatches in profiles/push_kernel_ens-64x128x256-n8.6e7_traffic.txt)
-ENS_PUSH_HBM_READ_B = 29.5
-ENS_PUSH_HBM_WRITE_B = 25.9
+ENS_PUSH_HBM_READ_B = 29.7
+ENS_PUSH_HBM_WRITE_B = 26.3
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 # algorithmic grid bytes of one C2 step besides the particles: the F_R and
 # F_phi tables in FP32 (2 x 17 harmonics x 256 x 256 x 4 B, read once by the
@@ -664,7 +664,7 @@ def main():
         if args.steps == 500:
             args.steps = 300
         if args.sort_interval is None:
-            args.sort_interval = 15
+            args.sort_interval = 20
         if args.graph is None:
             args.graph = 1
         if args.contraction is None:
@@ -679,7 +679,7 @@ def main():
         main_p3d(args)
         return
     if args.sort_interval is None:
-        args.sort_interval = 16
+        args.sort_interval = 20
     from galaxy_amd import configs
     w = configs.C2
     n = args.n or w.n
